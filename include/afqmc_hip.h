@@ -1,0 +1,146 @@
+/*
+ * afqmc_hip.h -- C ABI of the gfx950 phaseless-AFQMC step (libafqmc_hip.so).
+ *
+ * Plain C: device pointers, sizes and scalars only.  Complex arrays are
+ * interleaved (re, im) float64 pairs, the memory layout of complex128; every
+ * array is dense row-major with the shape given beside it.  A context borrows
+ * the static tensors passed to afqmc_set_static for its whole lifetime and
+ * borrows walker arrays only for the duration of a call.  All work is
+ * enqueued on the context's stream; nothing blocks the host except
+ * afqmc_sync / afqmc_get_flags.  A context is not re-entrant; distinct
+ * contexts may be used concurrently.
+ *
+ * Reference interfaces each entry point replaces (ipie v0.7.1 source tree):
+ *   afqmc_greens_function  SingleDet.calc_greens_function
+ *                          ipie/trial_wavefunction/single_det.py:180-184 ->
+ *                          ipie/estimators/greens_function_single_det.py:62-154
+ *   afqmc_overlap          SingleDet.calc_overlap, ipie/propagation/overlap.py:29-59
+ *   afqmc_force_bias       SingleDet.calc_force_bias, ipie/propagation/force_bias.py:79-121
+ *   afqmc_one_body         PhaselessBase.propagate_walkers_one_body,
+ *                          ipie/propagation/phaseless_base.py:274-280, operations.py:26-55
+ *   afqmc_two_body         PhaselessBase.propagate_walkers_two_body,
+ *                          phaseless_base.py:282-310 (+ apply_VHS, phaseless_generic.py:35-97)
+ *   afqmc_update_weight    PhaselessBase.update_weight, phaseless_base.py:340-380
+ *   afqmc_propagate        PhaselessBase.propagate_walkers, phaseless_base.py:312-338
+ *   afqmc_local_energy     local_energy_batch, ipie/estimators/local_energy_batch.py:36-75
+ *                          (local_energy_sd.py:106-155, 358-403, 987-1080)
+ *   afqmc_reortho          UHFWalkers.reortho, ipie/walkers/uhf_walkers.py:108-161
+ *   afqmc_draw_normals     the xi draw, phaseless_base.py:296
+ */
+#ifndef AFQMC_HIP_H
+#define AFQMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define AFQMC_OK 0
+#define AFQMC_ERR_INVALID_ARG 1  /* null pointer, bad size, call before set_static */
+#define AFQMC_ERR_SHAPE 2        /* sizes outside what the kernels support */
+#define AFQMC_ERR_UNSUPPORTED 3  /* precision mode / feature not available */
+#define AFQMC_ERR_HIP 4          /* a HIP runtime call or launch failed */
+#define AFQMC_ERR_ALLOC 5        /* device allocation failed */
+
+/* device-side flags (afqmc_get_flags), OR-ed since the last clear */
+#define AFQMC_FLAG_SINGULAR 1    /* an overlap matrix was exactly singular */
+#define AFQMC_FLAG_NONFINITE 2   /* a non-finite weight was produced */
+#define AFQMC_FLAG_CHOLESKY 4    /* reortho met a numerically rank-deficient walker */
+
+/* precision modes */
+#define AFQMC_PREC_FP64 0
+
+typedef struct afqmc_ctx afqmc_ctx;
+
+typedef struct afqmc_dims {
+  int32_t nbasis;    /* N */
+  int32_t nchol;     /* X (= nfields for real Cholesky vectors) */
+  int32_t nalpha;    /* n_a, 1..64 */
+  int32_t nbeta;     /* n_b, 0..64 */
+  int32_t nwalkers;  /* maximum W per call */
+  int32_t exp_nmax;  /* Taylor order of exp(VHS), ipie default 6 */
+  int32_t rhf;       /* walkers.rhf: propagate alpha only, beta == alpha */
+  int32_t precision; /* AFQMC_PREC_* */
+} afqmc_dims;
+
+typedef struct afqmc_static {
+  const double* chol_packed; /* [N(N+1)/2, X] real, numpy.triu_indices order */
+  const double* rchola;      /* [X, n_a N] real, half-rotated Cholesky */
+  const double* rcholb;      /* [X, n_b N] real */
+  const double* rH1a;        /* [n_a, N] complex */
+  const double* rH1b;        /* [n_b, N] complex */
+  const double* expH1;       /* [2, N, N] real, exp(-dt/2 H1_shifted) per spin */
+  const double* mf_shift;    /* [X] complex */
+  const double* psia;        /* [N, n_a] complex, trial orbitals */
+  const double* psib;        /* [N, n_b] complex */
+  double dt;
+  double ecore;
+} afqmc_static;
+
+/* counters returned by afqmc_propagate / afqmc_two_body */
+typedef struct afqmc_counters {
+  uint64_t nfb_trig; /* force-bias entries capped so far (device accumulated) */
+} afqmc_counters;
+
+int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims);
+void afqmc_destroy(afqmc_ctx* ctx);
+/* hipStream_t passed as void*; NULL = the default stream */
+int afqmc_set_stream(afqmc_ctx* ctx, void* stream);
+int afqmc_set_static(afqmc_ctx* ctx, const afqmc_static* s);
+const char* afqmc_last_error(const afqmc_ctx* ctx);
+int afqmc_sync(afqmc_ctx* ctx);
+/* synchronises, returns the OR of device flags since the last clear */
+int afqmc_get_flags(afqmc_ctx* ctx, int32_t* flags, int clear);
+/* total force-bias caps accumulated on the device (synchronises) */
+int afqmc_get_nfb(afqmc_ctx* ctx, uint64_t* nfb);
+
+/* Ghalf_s = (phi_s^T psi_s^*)^{-1} phi_s^T and ovlp = prod_s det(phi_s^T psi_s^*) e^{-log_shift}.
+ * phia [W, N, n_a], phib [W, N, n_b], Ghalfa [W, n_a, N], Ghalfb [W, n_b, N],
+ * ovlp [W] complex; log_shift [W] real or NULL. */
+int afqmc_greens_function(afqmc_ctx* ctx, int32_t nwalkers, const double* phia, const double* phib,
+                          double* ghalfa, double* ghalfb, double* ovlp, const double* log_shift);
+int afqmc_overlap(afqmc_ctx* ctx, int32_t nwalkers, const double* phia, const double* phib, double* ovlp,
+                  const double* log_shift);
+/* vbias [W, X] complex */
+int afqmc_force_bias(afqmc_ctx* ctx, int32_t nwalkers, const double* ghalfa, const double* ghalfb,
+                     double* vbias);
+/* phi_s <- expH1[s] phi_s, in place */
+int afqmc_one_body(afqmc_ctx* ctx, int32_t nwalkers, double* phia, double* phib);
+/* Two-body step from the current Ghalf: force bias, xbar (capped), fields,
+ * cmf / cfb [W] complex, VHS and the Taylor exponential applied in place.
+ * xi [W, X] real injects the fields; NULL draws them from the counter-based
+ * generator (seed, counter).  xi_out [W, X] (optional) receives the fields
+ * used; vbias_out [W, X] complex (optional) the force bias. */
+int afqmc_two_body(afqmc_ctx* ctx, int32_t nwalkers, double* phia, double* phib, const double* ghalfa,
+                   const double* ghalfb, const double* xi, uint64_t seed, uint64_t counter, double* cmf,
+                   double* cfb, double* xi_out, double* vbias_out);
+/* weight [W] real, hybrid_energy [W] complex (in: previous), walker_ovlp [W] complex (out) */
+int afqmc_update_weight(afqmc_ctx* ctx, int32_t nwalkers, const double* ovlp, const double* ovlp_new,
+                        const double* cfb, const double* cmf, double eshift, double* weight,
+                        double* hybrid_energy, double* walker_ovlp);
+/* One full phaseless step (GF, one body, two body, one body, overlap, weights).
+ * Ghalf receives the Green's function of the walkers at the start of the step;
+ * ovlp [W] complex receives the new overlap.  energy_prev [W, 3] complex
+ * (optional) receives the local energy of the incoming walkers (the state the
+ * step starts from), computed from the same Green's function. */
+int afqmc_propagate(afqmc_ctx* ctx, int32_t nwalkers, double* phia, double* phib, double* ghalfa,
+                    double* ghalfb, const double* xi, uint64_t seed, uint64_t counter, double eshift,
+                    double* weight, double* ovlp, double* hybrid_energy, const double* log_shift,
+                    double* energy_prev);
+/* energy [W, 3] complex = (E1 + E2, E1, E2) from Ghalf. */
+int afqmc_local_energy(afqmc_ctx* ctx, int32_t nwalkers, const double* ghalfa, const double* ghalfb,
+                       double* energy);
+/* QR re-orthogonalisation in place; ovlp [W] complex is divided by detR [W]
+ * real; log_detR [W] real is incremented; detR_shift [W] real or NULL. */
+int afqmc_reortho(afqmc_ctx* ctx, int32_t nwalkers, double* phia, double* phib, double* ovlp, double* detR,
+                  double* log_detR, const double* detR_shift);
+/* n standard normals (Philox4x32-10, Box-Muller) for stream `counter`. */
+int afqmc_draw_normals(afqmc_ctx* ctx, double* out, int64_t n, uint64_t seed, uint64_t counter);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AFQMC_HIP_H */

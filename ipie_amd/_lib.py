@@ -1,0 +1,323 @@
+"""ctypes binding of libafqmc_hip.so (the C ABI declared in include/afqmc_hip.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+Tensors are passed as raw device pointers; every wrapper checks dtype, device,
+contiguity and shape before the call.
+"""
+
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libafqmc_hip.so")
+
+AFQMC_OK = 0
+STATUS_NAMES = {
+    1: "AFQMC_ERR_INVALID_ARG",
+    2: "AFQMC_ERR_SHAPE",
+    3: "AFQMC_ERR_UNSUPPORTED",
+    4: "AFQMC_ERR_HIP",
+    5: "AFQMC_ERR_ALLOC",
+}
+FLAG_SINGULAR = 1
+FLAG_NONFINITE = 2
+FLAG_CHOLESKY = 4
+PREC_FP64 = 0
+
+# Every symbol the header declares, with (restype, argtypes).
+_dp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_u64 = ctypes.c_uint64
+_i64 = ctypes.c_int64
+_f64 = ctypes.c_double
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [
+        ("nbasis", _i32),
+        ("nchol", _i32),
+        ("nalpha", _i32),
+        ("nbeta", _i32),
+        ("nwalkers", _i32),
+        ("exp_nmax", _i32),
+        ("rhf", _i32),
+        ("precision", _i32),
+    ]
+
+
+class Static(ctypes.Structure):
+    _fields_ = [
+        ("chol_packed", _dp),
+        ("rchola", _dp),
+        ("rcholb", _dp),
+        ("rH1a", _dp),
+        ("rH1b", _dp),
+        ("expH1", _dp),
+        ("mf_shift", _dp),
+        ("psia", _dp),
+        ("psib", _dp),
+        ("dt", _f64),
+        ("ecore", _f64),
+    ]
+
+
+SIGNATURES = {
+    "afqmc_create": (ctypes.c_int, [ctypes.POINTER(_dp), ctypes.c_int, ctypes.POINTER(Dims)]),
+    "afqmc_destroy": (None, [_dp]),
+    "afqmc_set_stream": (ctypes.c_int, [_dp, _dp]),
+    "afqmc_set_static": (ctypes.c_int, [_dp, ctypes.POINTER(Static)]),
+    "afqmc_last_error": (ctypes.c_char_p, [_dp]),
+    "afqmc_sync": (ctypes.c_int, [_dp]),
+    "afqmc_get_flags": (ctypes.c_int, [_dp, ctypes.POINTER(_i32), ctypes.c_int]),
+    "afqmc_get_nfb": (ctypes.c_int, [_dp, ctypes.POINTER(_u64)]),
+    "afqmc_greens_function": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "afqmc_overlap": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _dp]),
+    "afqmc_force_bias": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp]),
+    "afqmc_one_body": (ctypes.c_int, [_dp, _i32, _dp, _dp]),
+    "afqmc_two_body": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _dp, _dp, _u64, _u64, _dp, _dp, _dp, _dp]),
+    "afqmc_update_weight": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _dp, _f64, _dp, _dp, _dp]),
+    "afqmc_propagate": (
+        ctypes.c_int,
+        [_dp, _i32, _dp, _dp, _dp, _dp, _dp, _u64, _u64, _f64, _dp, _dp, _dp, _dp, _dp],
+    ),
+    "afqmc_local_energy": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp]),
+    "afqmc_reortho": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "afqmc_draw_normals": (ctypes.c_int, [_dp, _dp, _i64, _u64, _u64]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libafqmc_hip.so once; raise if it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m ipie_amd.build` (hipcc, gfx950). "
+                "There is no CPU fallback."
+            )
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class AfqmcError(RuntimeError):
+    pass
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _check_tensor(name, t, dtype, shape, device):
+    if t is None:
+        raise ValueError(f"{name} is required")
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor, got {type(t)}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if t.device != device:
+        raise ValueError(f"{name} must live on {device}, got {t.device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+    return t
+
+
+class Context:
+    """Owns one afqmc_ctx (dims, workspace, borrowed static tensors) on one device."""
+
+    def __init__(self, nbasis, nchol, nalpha, nbeta, nwalkers, exp_nmax=6, rhf=False, device=None):
+        self.lib = load()
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.dims = Dims(nbasis, nchol, nalpha, nbeta, nwalkers, exp_nmax, int(bool(rhf)), PREC_FP64)
+        h = _dp()
+        with torch.cuda.device(self.device):
+            st = self.lib.afqmc_create(ctypes.byref(h), self.device.index or 0, ctypes.byref(self.dims))
+        if st != AFQMC_OK:
+            raise AfqmcError(f"afqmc_create failed: {STATUS_NAMES.get(st, st)} (dims={self.describe()})")
+        self.h = h
+        self._static_refs = None
+
+    def describe(self):
+        d = self.dims
+        return f"N={d.nbasis} X={d.nchol} na={d.nalpha} nb={d.nbeta} W={d.nwalkers} rhf={d.rhf}"
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.afqmc_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    # ------------------------------------------------------------------ plumbing
+    def _call(self, name, *args):
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        self.lib.afqmc_set_stream(self.h, ctypes.c_void_p(stream))
+        st = getattr(self.lib, name)(self.h, *args)
+        if st != AFQMC_OK:
+            msg = self.lib.afqmc_last_error(self.h)
+            raise AfqmcError(f"{name}: {STATUS_NAMES.get(st, st)}: {msg.decode() if msg else ''}")
+
+    def set_static(self, *, chol_packed, rchola, rcholb, rH1a, rH1b, expH1, mf_shift, psia, psib, dt, ecore):
+        d = self.dims
+        dev = self.device
+        f64, c128 = torch.float64, torch.complex128
+        N, X, na, nb = d.nbasis, d.nchol, d.nalpha, d.nbeta
+        _check_tensor("chol_packed", chol_packed, f64, (N * (N + 1) // 2, X), dev)
+        _check_tensor("rchola", rchola, f64, (X, na * N), dev)
+        _check_tensor("rcholb", rcholb, f64, (X, nb * N), dev)
+        _check_tensor("rH1a", rH1a, c128, (na, N), dev)
+        _check_tensor("rH1b", rH1b, c128, (nb, N), dev)
+        _check_tensor("expH1", expH1, f64, (2, N, N), dev)
+        _check_tensor("mf_shift", mf_shift, c128, (X,), dev)
+        _check_tensor("psia", psia, c128, (N, na), dev)
+        _check_tensor("psib", psib, c128, (N, nb), dev)
+        s = Static(
+            ptr(chol_packed), ptr(rchola), ptr(rcholb), ptr(rH1a), ptr(rH1b), ptr(expH1), ptr(mf_shift),
+            ptr(psia), ptr(psib), float(dt), float(ecore),
+        )
+        st = self.lib.afqmc_set_static(self.h, ctypes.byref(s))
+        if st != AFQMC_OK:
+            raise AfqmcError(f"afqmc_set_static: {STATUS_NAMES.get(st, st)}: {self.lib.afqmc_last_error(self.h)}")
+        # the context borrows these: keep them alive
+        self._static_refs = (chol_packed, rchola, rcholb, rH1a, rH1b, expH1, mf_shift, psia, psib)
+
+    def _walker_shapes(self, nw):
+        d = self.dims
+        return (nw, d.nbasis, d.nalpha), (nw, d.nbasis, d.nbeta), (nw, d.nalpha, d.nbasis), (nw, d.nbeta, d.nbasis)
+
+    def _chk_phi(self, phia, phib):
+        nw = phia.shape[0]
+        sa, sb, _, _ = self._walker_shapes(nw)
+        _check_tensor("phia", phia, torch.complex128, sa, self.device)
+        _check_tensor("phib", phib, torch.complex128, sb, self.device)
+        return nw
+
+    def _chk_ghalf(self, ga, gb, nw):
+        _, _, ga_s, gb_s = self._walker_shapes(nw)
+        _check_tensor("Ghalfa", ga, torch.complex128, ga_s, self.device)
+        _check_tensor("Ghalfb", gb, torch.complex128, gb_s, self.device)
+
+    def _chk_vec(self, name, t, nw, dtype):
+        return _check_tensor(name, t, dtype, (nw,), self.device)
+
+    # ------------------------------------------------------------------ entry points
+    def greens_function(self, phia, phib, ghalfa, ghalfb, ovlp, log_shift=None):
+        nw = self._chk_phi(phia, phib)
+        self._chk_ghalf(ghalfa, ghalfb, nw)
+        self._chk_vec("ovlp", ovlp, nw, torch.complex128)
+        if log_shift is not None:
+            self._chk_vec("log_shift", log_shift, nw, torch.float64)
+        self._call("afqmc_greens_function", nw, ptr(phia), ptr(phib), ptr(ghalfa), ptr(ghalfb), ptr(ovlp),
+                   ptr(log_shift))
+
+    def overlap(self, phia, phib, ovlp, log_shift=None):
+        nw = self._chk_phi(phia, phib)
+        self._chk_vec("ovlp", ovlp, nw, torch.complex128)
+        if log_shift is not None:
+            self._chk_vec("log_shift", log_shift, nw, torch.float64)
+        self._call("afqmc_overlap", nw, ptr(phia), ptr(phib), ptr(ovlp), ptr(log_shift))
+
+    def force_bias(self, ghalfa, ghalfb, vbias):
+        nw = ghalfa.shape[0]
+        self._chk_ghalf(ghalfa, ghalfb, nw)
+        _check_tensor("vbias", vbias, torch.complex128, (nw, self.dims.nchol), self.device)
+        self._call("afqmc_force_bias", nw, ptr(ghalfa), ptr(ghalfb), ptr(vbias))
+
+    def one_body(self, phia, phib):
+        nw = self._chk_phi(phia, phib)
+        self._call("afqmc_one_body", nw, ptr(phia), ptr(phib))
+
+    def two_body(self, phia, phib, ghalfa, ghalfb, cmf, cfb, xi=None, seed=0, counter=0, xi_out=None,
+                 vbias_out=None):
+        nw = self._chk_phi(phia, phib)
+        self._chk_ghalf(ghalfa, ghalfb, nw)
+        self._chk_vec("cmf", cmf, nw, torch.complex128)
+        self._chk_vec("cfb", cfb, nw, torch.complex128)
+        X = self.dims.nchol
+        if xi is not None:
+            _check_tensor("xi", xi, torch.float64, (nw, X), self.device)
+        if xi_out is not None:
+            _check_tensor("xi_out", xi_out, torch.float64, (nw, X), self.device)
+        if vbias_out is not None:
+            _check_tensor("vbias_out", vbias_out, torch.complex128, (nw, X), self.device)
+        self._call("afqmc_two_body", nw, ptr(phia), ptr(phib), ptr(ghalfa), ptr(ghalfb), ptr(xi), seed, counter,
+                   ptr(cmf), ptr(cfb), ptr(xi_out), ptr(vbias_out))
+
+    def update_weight(self, ovlp, ovlp_new, cfb, cmf, eshift, weight, hybrid_energy, walker_ovlp):
+        nw = ovlp.shape[0]
+        for nm, t in (("ovlp", ovlp), ("ovlp_new", ovlp_new), ("cfb", cfb), ("cmf", cmf),
+                      ("hybrid_energy", hybrid_energy), ("walker_ovlp", walker_ovlp)):
+            self._chk_vec(nm, t, nw, torch.complex128)
+        self._chk_vec("weight", weight, nw, torch.float64)
+        self._call("afqmc_update_weight", nw, ptr(ovlp), ptr(ovlp_new), ptr(cfb), ptr(cmf), float(eshift),
+                   ptr(weight), ptr(hybrid_energy), ptr(walker_ovlp))
+
+    def propagate(self, phia, phib, ghalfa, ghalfb, weight, ovlp, hybrid_energy, eshift, xi=None, seed=0,
+                  counter=0, log_shift=None, energy_prev=None):
+        nw = self._chk_phi(phia, phib)
+        self._chk_ghalf(ghalfa, ghalfb, nw)
+        self._chk_vec("weight", weight, nw, torch.float64)
+        self._chk_vec("ovlp", ovlp, nw, torch.complex128)
+        self._chk_vec("hybrid_energy", hybrid_energy, nw, torch.complex128)
+        if xi is not None:
+            _check_tensor("xi", xi, torch.float64, (nw, self.dims.nchol), self.device)
+        if log_shift is not None:
+            self._chk_vec("log_shift", log_shift, nw, torch.float64)
+        if energy_prev is not None:
+            _check_tensor("energy_prev", energy_prev, torch.complex128, (nw, 3), self.device)
+        self._call("afqmc_propagate", nw, ptr(phia), ptr(phib), ptr(ghalfa), ptr(ghalfb), ptr(xi), seed, counter,
+                   float(eshift), ptr(weight), ptr(ovlp), ptr(hybrid_energy), ptr(log_shift), ptr(energy_prev))
+
+    def local_energy(self, ghalfa, ghalfb, energy):
+        nw = ghalfa.shape[0]
+        self._chk_ghalf(ghalfa, ghalfb, nw)
+        _check_tensor("energy", energy, torch.complex128, (nw, 3), self.device)
+        self._call("afqmc_local_energy", nw, ptr(ghalfa), ptr(ghalfb), ptr(energy))
+
+    def reortho(self, phia, phib, ovlp, detR, log_detR, detR_shift=None):
+        nw = self._chk_phi(phia, phib)
+        self._chk_vec("ovlp", ovlp, nw, torch.complex128)
+        self._chk_vec("detR", detR, nw, torch.float64)
+        self._chk_vec("log_detR", log_detR, nw, torch.float64)
+        if detR_shift is not None:
+            self._chk_vec("detR_shift", detR_shift, nw, torch.float64)
+        self._call("afqmc_reortho", nw, ptr(phia), ptr(phib), ptr(ovlp), ptr(detR), ptr(log_detR),
+                   ptr(detR_shift))
+
+    def draw_normals(self, out, seed, counter):
+        _check_tensor("out", out, torch.float64, None, self.device)
+        self._call("afqmc_draw_normals", ptr(out), out.numel(), seed, counter)
+
+    def flags(self, clear=True):
+        f = _i32(0)
+        st = self.lib.afqmc_get_flags(self.h, ctypes.byref(f), int(clear))
+        if st != AFQMC_OK:
+            raise AfqmcError(f"afqmc_get_flags: {STATUS_NAMES.get(st, st)}")
+        return int(f.value)
+
+    def nfb(self):
+        v = _u64(0)
+        st = self.lib.afqmc_get_nfb(self.h, ctypes.byref(v))
+        if st != AFQMC_OK:
+            raise AfqmcError(f"afqmc_get_nfb: {STATUS_NAMES.get(st, st)}")
+        return int(v.value)
+
+    def sync(self):
+        st = self.lib.afqmc_sync(self.h)
+        if st != AFQMC_OK:
+            raise AfqmcError(f"afqmc_sync: {self.lib.afqmc_last_error(self.h)}")

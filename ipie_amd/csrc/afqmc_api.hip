@@ -1,0 +1,613 @@
+// C ABI of libafqmc_hip.so: context, workspace and step orchestration.
+// Declared in include/afqmc_hip.h.  Every entry point validates its inputs on
+// the host, enqueues kernels on the context's stream and returns a status;
+// nothing here blocks the host except afqmc_sync / afqmc_get_flags /
+// afqmc_get_nfb.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "afqmc_internal.hpp"
+#include "../../include/afqmc_hip.h"
+
+using namespace afqmc;
+
+struct afqmc_ctx {
+  afqmc_dims d;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  afqmc_static s{};
+  bool have_static = false;
+  char err[512] = {0};
+  int ncol = 0;  // columns of the concatenated walker buffers (n_a + n_b, or n_a)
+  int fb_ksplit = 1;
+  int xs_a = 1, xs_b = 1;
+  // workspace (device)
+  double* vbias_part = nullptr;  // [ksplit, W, X] complex
+  double* xshifted = nullptr;    // [W, X] complex
+  double* wvec = nullptr;        // 6 x [W] complex: cmf, cfb, ecoul, ovlp_old, ovlp_new, spare
+  double* exx_a = nullptr;       // [xs_a, W] complex
+  double* exx_b = nullptr;       // [xs_b, W] complex
+  double* vhs = nullptr;         // [W, N, N] complex
+  double* buf[4] = {nullptr, nullptr, nullptr, nullptr};  // P, Ta, Tb, Q: [W, N, ncol] complex
+  double* qr_work = nullptr;     // [W, N, nmax] complex
+  int* tri = nullptr;            // [2, P] packed-index rows / cols
+  int* flags = nullptr;
+  unsigned long long* nfb = nullptr;
+  bool ws_ready = false;
+  char describe_dims[128] = {0};
+};
+
+namespace {
+
+int fail(afqmc_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof(c->err), fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+#define HIP_TRY(c, expr)                                                                     \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess) return fail((c), AFQMC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+int check_ready(afqmc_ctx* c, int32_t nw) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (!c->have_static) return fail(c, AFQMC_ERR_INVALID_ARG, "afqmc_set_static has not been called");
+  if (nw < 1 || nw > c->d.nwalkers)
+    return fail(c, AFQMC_ERR_SHAPE, "nwalkers=%d outside [1, %d]", nw, c->d.nwalkers);
+  c->err[0] = 0;
+  return AFQMC_OK;
+}
+
+bool two_spins(const afqmc_ctx* c) { return c->d.nbeta > 0 && !c->d.rhf; }
+
+double* wv(afqmc_ctx* c, int i) { return c->wvec + 2 * (int64_t)i * c->d.nwalkers; }
+enum { WV_CMF = 0, WV_CFB = 1, WV_ECOUL = 2, WV_OVLP_OLD = 3, WV_OVLP_NEW = 4 };
+
+hipError_t dmalloc(void** p, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  return hipMalloc(p, bytes);
+}
+
+// Workspace is allocated on first use by an entry point that needs it (the
+// Green's function and overlap need none).
+int ensure_ws(afqmc_ctx* c) {
+  if (c->ws_ready) return AFQMC_OK;
+  const afqmc_dims& d = c->d;
+  if (d.nchol < 1) return fail(c, AFQMC_ERR_SHAPE, "this entry point needs nchol >= 1");
+  const int64_t W = d.nwalkers, N = d.nbasis, X = d.nchol, P = N * (N + 1) / 2;
+  const int nmax = d.nalpha > d.nbeta ? d.nalpha : d.nbeta;
+  bool ok = true;
+  ok = ok && dmalloc((void**)&c->vbias_part, 16 * (size_t)c->fb_ksplit * W * X) == hipSuccess;
+  ok = ok && dmalloc((void**)&c->xshifted, 16 * (size_t)W * X) == hipSuccess;
+  ok = ok && dmalloc((void**)&c->wvec, 16 * (size_t)6 * W) == hipSuccess;
+  ok = ok && dmalloc((void**)&c->exx_a, 16 * (size_t)c->xs_a * W) == hipSuccess;
+  ok = ok && dmalloc((void**)&c->exx_b, 16 * (size_t)c->xs_b * W) == hipSuccess;
+  ok = ok && dmalloc((void**)&c->vhs, 16 * (size_t)W * N * N) == hipSuccess;
+  for (int i = 0; i < 4; ++i) ok = ok && dmalloc((void**)&c->buf[i], 16 * (size_t)W * N * c->ncol) == hipSuccess;
+  ok = ok && dmalloc((void**)&c->qr_work, 16 * (size_t)W * N * nmax) == hipSuccess;
+  ok = ok && dmalloc((void**)&c->tri, sizeof(int) * 2 * (size_t)P) == hipSuccess;
+  if (!ok) return fail(c, AFQMC_ERR_ALLOC, "workspace allocation failed (%s)", c->describe_dims);
+  std::vector<int> tri(2 * P);
+  int64_t p = 0;
+  for (int i = 0; i < N; ++i)
+    for (int j = i; j < N; ++j, ++p) {
+      tri[p] = i;
+      tri[P + p] = j;
+    }
+  HIP_TRY(c, hipMemcpy(c->tri, tri.data(), sizeof(int) * 2 * P, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemset(c->wvec, 0, 16 * (size_t)6 * W));
+  c->ws_ready = true;
+  return AFQMC_OK;
+}
+
+// Static tensors needed by the propagation / energy entry points.
+int need_static(afqmc_ctx* c) {
+  const bool two = c->d.nbeta > 0 && !c->d.rhf;
+  if (!c->s.chol_packed || !c->s.rchola || !c->s.rH1a || !c->s.expH1 || !c->s.mf_shift)
+    return fail(c, AFQMC_ERR_INVALID_ARG, "static Hamiltonian tensors were not set");
+  if (two && (!c->s.rcholb || !c->s.rH1b)) return fail(c, AFQMC_ERR_INVALID_ARG, "beta static tensors were not set");
+  if (!(c->s.dt > 0.0)) return fail(c, AFQMC_ERR_INVALID_ARG, "dt must be positive");
+  return ensure_ws(c);
+}
+
+// ---------------------------------------------------------------- phases
+int do_gf(afqmc_ctx* c, int nw, const double* phia, const double* phib, double* ga, double* gb, double* ovlp,
+          const double* log_shift, bool want_ghalf) {
+  GfArgs a;
+  a.nwalkers = nw;
+  a.nbasis = c->d.nbasis;
+  a.nalpha = c->d.nalpha;
+  a.nbeta = c->d.nbeta;
+  a.rhf = c->d.rhf;
+  a.phia = phia;
+  a.phib = phib;
+  a.psia = c->s.psia;
+  a.psib = c->s.psib;
+  a.ghalfa = ga;
+  a.ghalfb = gb;
+  a.ovlp = ovlp;
+  a.log_shift = log_shift;
+  a.flags = c->flags;
+  HIP_TRY(c, launch_gf(a, want_ghalf, c->stream));
+  return AFQMC_OK;
+}
+
+int do_force_bias(afqmc_ctx* c, int nw, const double* ga, const double* gb) {
+  const int N = c->d.nbasis, na = c->d.nalpha, nb = c->d.nbeta;
+  ForceBiasArgs a;
+  a.nwalkers = nw;
+  a.nchol = c->d.nchol;
+  a.ksplit = c->fb_ksplit;
+  a.nseg = 1;
+  a.seg[0] = NtSegment{ga, (int64_t)na * N, c->s.rchola, (int64_t)na * N, na * N};
+  a.scale = 1.0;
+  if (two_spins(c)) {
+    a.nseg = 2;
+    a.seg[1] = NtSegment{gb, (int64_t)nb * N, c->s.rcholb, (int64_t)nb * N, nb * N};
+  } else if (c->d.rhf && nb > 0) {
+    a.scale = 2.0;
+  }
+  a.vbias_part = c->vbias_part;
+  HIP_TRY(c, launch_force_bias(a, c->stream));
+  return AFQMC_OK;
+}
+
+int do_fields(afqmc_ctx* c, int nw, bool want_fields, const double* xi, uint64_t seed, uint64_t counter,
+              double* xi_out, double* vbias_out) {
+  FieldsArgs a;
+  a.nwalkers = nw;
+  a.nchol = c->d.nchol;
+  a.ksplit = c->fb_ksplit;
+  a.vbias_part = c->vbias_part;
+  a.mf_shift = c->s.mf_shift;
+  a.xi = xi;
+  a.seed = seed;
+  a.counter = counter;
+  a.sqrt_dt = sqrt(c->s.dt);
+  a.fbbound = 1.0;
+  a.xi_out = xi_out;
+  a.xshifted = c->xshifted;
+  a.cmf = wv(c, WV_CMF);
+  a.cfb = wv(c, WV_CFB);
+  a.ecoul = wv(c, WV_ECOUL);
+  a.vbias_out = vbias_out;
+  a.nfb = c->nfb;
+  a.want_fields = want_fields ? 1 : 0;
+  HIP_TRY(c, launch_fields(a, c->stream));
+  return AFQMC_OK;
+}
+
+int do_vhs(afqmc_ctx* c, int nw) {
+  VhsArgs a;
+  a.nwalkers = nw;
+  a.nbasis = c->d.nbasis;
+  a.nchol = c->d.nchol;
+  a.xshifted = c->xshifted;
+  a.chol_packed = c->s.chol_packed;
+  const int P = c->d.nbasis * (c->d.nbasis + 1) / 2;
+  a.tri_i = c->tri;
+  a.tri_j = c->tri + P;
+  a.sqrt_dt = sqrt(c->s.dt);
+  a.vhs = c->vhs;
+  HIP_TRY(c, launch_vhs(a, c->stream));
+  return AFQMC_OK;
+}
+
+// C[w] (ldc, c_stride) = alpha * A * B[w] (ldb, b_stride), ncol columns.
+int nn(afqmc_ctx* c, int nw, const double* A, bool acplx, int64_t a_stride, const double* B, int64_t b_stride,
+       int64_t ldb, double* C, int64_t c_stride, int64_t ldc, int ncol, double alpha, double* accum,
+       const double* accum_init) {
+  BatchedNNArgs a;
+  a.nwalkers = nw;
+  a.nrows = c->d.nbasis;
+  a.ncol = ncol;
+  a.A = A;
+  a.a_complex = acplx;
+  a.a_stride = a_stride;
+  a.B = B;
+  a.b_stride = b_stride;
+  a.ldb = ldb;
+  a.C = C;
+  a.c_stride = c_stride;
+  a.ldc = ldc;
+  a.alpha = alpha;
+  a.accum = accum;
+  a.accum_init = accum_init;
+  HIP_TRY(c, launch_batched_nn(a, c->stream));
+  return AFQMC_OK;
+}
+
+// phi_s -> P (one body applied when with_expH1, plain copy otherwise)
+int gather(afqmc_ctx* c, int nw, const double* phia, const double* phib, bool with_expH1) {
+  const int N = c->d.nbasis, na = c->d.nalpha, nb = c->d.nbeta, nc = c->ncol;
+  const int64_t NN = (int64_t)N * N;
+  double* P = c->buf[0];
+  if (with_expH1) {
+    int r = nn(c, nw, c->s.expH1, false, 0, phia, (int64_t)N * na, na, P, (int64_t)N * nc, nc, na, 1.0, nullptr,
+               nullptr);
+    if (r) return r;
+    if (two_spins(c))
+      return nn(c, nw, c->s.expH1 + NN, false, 0, phib, (int64_t)N * nb, nb, P + 2 * na, (int64_t)N * nc, nc, nb,
+                1.0, nullptr, nullptr);
+    return AFQMC_OK;
+  }
+  HIP_TRY(c, hipMemcpy2DAsync(P, 16 * nc, phia, 16 * na, 16 * na, (size_t)nw * N, hipMemcpyDeviceToDevice,
+                              c->stream));
+  if (two_spins(c))
+    HIP_TRY(c, hipMemcpy2DAsync(P + 2 * na, 16 * nc, phib, 16 * nb, 16 * nb, (size_t)nw * N,
+                                hipMemcpyDeviceToDevice, c->stream));
+  return AFQMC_OK;
+}
+
+// src (concatenated) -> phi_s
+int scatter(afqmc_ctx* c, int nw, const double* src, double* phia, double* phib, bool with_expH1) {
+  const int N = c->d.nbasis, na = c->d.nalpha, nb = c->d.nbeta, nc = c->ncol;
+  const int64_t NN = (int64_t)N * N;
+  if (with_expH1) {
+    int r = nn(c, nw, c->s.expH1, false, 0, src, (int64_t)N * nc, nc, phia, (int64_t)N * na, na, na, 1.0, nullptr,
+               nullptr);
+    if (r) return r;
+    if (two_spins(c))
+      return nn(c, nw, c->s.expH1 + NN, false, 0, src + 2 * na, (int64_t)N * nc, nc, phib, (int64_t)N * nb, nb, nb,
+                1.0, nullptr, nullptr);
+    return AFQMC_OK;
+  }
+  HIP_TRY(c, hipMemcpy2DAsync(phia, 16 * na, src, 16 * nc, 16 * na, (size_t)nw * N, hipMemcpyDeviceToDevice,
+                              c->stream));
+  if (two_spins(c))
+    HIP_TRY(c, hipMemcpy2DAsync(phib, 16 * nb, src + 2 * na, 16 * nc, 16 * nb, (size_t)nw * N,
+                                hipMemcpyDeviceToDevice, c->stream));
+  return AFQMC_OK;
+}
+
+// Taylor series of exp(VHS) applied to P; returns the buffer holding the result.
+int taylor(afqmc_ctx* c, int nw, const double** result) {
+  const int N = c->d.nbasis, nc = c->ncol;
+  const int64_t NN = (int64_t)N * N, stride = (int64_t)N * nc;
+  double* P = c->buf[0];
+  double* Q = c->buf[3];
+  if (c->d.exp_nmax <= 0) {
+    *result = P;
+    return AFQMC_OK;
+  }
+  const double* prev = P;
+  for (int k = 1; k <= c->d.exp_nmax; ++k) {
+    double* out = (k & 1) ? c->buf[1] : c->buf[2];
+    int r = nn(c, nw, c->vhs, true, NN, prev, stride, nc, out, stride, nc, nc, 1.0 / k, Q, k == 1 ? P : nullptr);
+    if (r) return r;
+    prev = out;
+  }
+  *result = Q;
+  return AFQMC_OK;
+}
+
+int do_exchange(afqmc_ctx* c, int nw, const double* ga, const double* gb) {
+  const int N = c->d.nbasis;
+  ExchangeArgs a;
+  a.nwalkers = nw;
+  a.nchol = c->d.nchol;
+  a.nbasis = N;
+  a.nocc = c->d.nalpha;
+  a.ghalf = ga;
+  a.rchol = c->s.rchola;
+  a.xsplit = c->xs_a;
+  a.exx_part = c->exx_a;
+  HIP_TRY(c, launch_exchange(a, c->stream));
+  if (two_spins(c)) {
+    a.nocc = c->d.nbeta;
+    a.ghalf = gb;
+    a.rchol = c->s.rcholb;
+    a.xsplit = c->xs_b;
+    a.exx_part = c->exx_b;
+    HIP_TRY(c, launch_exchange(a, c->stream));
+  }
+  return AFQMC_OK;
+}
+
+int do_energy(afqmc_ctx* c, int nw, const double* ga, const double* gb, double* energy) {
+  EnergyArgs a;
+  a.nwalkers = nw;
+  a.nbasis = c->d.nbasis;
+  a.nalpha = c->d.nalpha;
+  a.nbeta = c->d.nbeta;
+  a.rhf = c->d.rhf;
+  a.ghalfa = ga;
+  a.ghalfb = gb;
+  a.rH1a = c->s.rH1a;
+  a.rH1b = c->s.rH1b;
+  a.ecoul = wv(c, WV_ECOUL);
+  a.exx_part_a = c->exx_a;
+  a.xs_a = c->xs_a;
+  a.exx_part_b = two_spins(c) ? c->exx_b : nullptr;
+  a.xs_b = c->xs_b;
+  a.ecore = c->s.ecore;
+  a.energy = energy;
+  HIP_TRY(c, launch_energy(a, c->stream));
+  return AFQMC_OK;
+}
+
+int do_two_body_core(afqmc_ctx* c, int nw, const double* ga, const double* gb, const double* xi, uint64_t seed,
+                     uint64_t counter, double* xi_out, double* vbias_out, bool energy_too, double* energy) {
+  int r = do_force_bias(c, nw, ga, gb);
+  if (r) return r;
+  r = do_fields(c, nw, true, xi, seed, counter, xi_out, vbias_out);
+  if (r) return r;
+  if (energy_too) {
+    r = do_exchange(c, nw, ga, gb);
+    if (r) return r;
+    r = do_energy(c, nw, ga, gb, energy);
+    if (r) return r;
+  }
+  return do_vhs(c, nw);
+}
+
+}  // namespace
+
+extern "C" {
+
+int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims) {
+  if (!out || !dims) return AFQMC_ERR_INVALID_ARG;
+  *out = nullptr;
+  const afqmc_dims d = *dims;
+  if (d.nbasis < 1 || d.nchol < 0 || d.nalpha < 1 || d.nbeta < 0 || d.nwalkers < 1 || d.exp_nmax < 0)
+    return AFQMC_ERR_SHAPE;
+  if (d.nalpha > 64 || d.nbeta > 64 || d.nalpha > d.nbasis || d.nbeta > d.nbasis) return AFQMC_ERR_SHAPE;
+  if (d.precision != AFQMC_PREC_FP64) return AFQMC_ERR_UNSUPPORTED;
+  if (hipSetDevice(device) != hipSuccess) return AFQMC_ERR_HIP;
+  afqmc_ctx* c = new afqmc_ctx();
+  c->d = d;
+  c->device = device;
+  const bool two = d.nbeta > 0 && !d.rhf;
+  c->ncol = two ? d.nalpha + d.nbeta : d.nalpha;
+  const int64_t W = d.nwalkers, N = d.nbasis, X = d.nchol > 0 ? d.nchol : 1;
+  // force-bias split-K: enough workgroups to fill the chip, >= 64 K chunks each
+  {
+    const int64_t tiles = ceil_div(W, 64) * ceil_div(X, 128);
+    const int64_t kch = ceil_div((int64_t)d.nalpha * N, 16) + (two ? ceil_div((int64_t)d.nbeta * N, 16) : 0);
+    int64_t ks = ceil_div(1024, tiles);
+    if (ks > kch / 64) ks = kch / 64;
+    if (ks > 8) ks = 8;
+    if (ks < 1) ks = 1;
+    c->fb_ksplit = (int)ks;
+  }
+  c->xs_a = exchange_xsplit(d.nwalkers, (int)X, d.nalpha);
+  c->xs_b = two ? exchange_xsplit(d.nwalkers, (int)X, d.nbeta) : 1;
+  snprintf(c->describe_dims, sizeof(c->describe_dims), "N=%d X=%d na=%d nb=%d W=%d", d.nbasis, d.nchol, d.nalpha,
+           d.nbeta, d.nwalkers);
+  if (dmalloc((void**)&c->flags, sizeof(int)) != hipSuccess ||
+      dmalloc((void**)&c->nfb, sizeof(unsigned long long)) != hipSuccess) {
+    afqmc_destroy(c);
+    return AFQMC_ERR_ALLOC;
+  }
+  if (hipMemset(c->flags, 0, sizeof(int)) != hipSuccess ||
+      hipMemset(c->nfb, 0, sizeof(unsigned long long)) != hipSuccess) {
+    afqmc_destroy(c);
+    return AFQMC_ERR_HIP;
+  }
+  *out = c;
+  return AFQMC_OK;
+}
+
+void afqmc_destroy(afqmc_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  else (void)hipDeviceSynchronize();
+  void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
+                  c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete c;
+}
+
+int afqmc_set_stream(afqmc_ctx* c, void* stream) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  c->stream = (hipStream_t)stream;
+  return AFQMC_OK;
+}
+
+int afqmc_set_static(afqmc_ctx* c, const afqmc_static* s) {
+  if (!c || !s) return AFQMC_ERR_INVALID_ARG;
+  const bool two = c->d.nbeta > 0 && !c->d.rhf;
+  // The trial orbitals are always required; the Hamiltonian tensors only by the
+  // entry points that use them (checked there).
+  if (!s->psia) return fail(c, AFQMC_ERR_INVALID_ARG, "null psia");
+  if (c->d.nbeta > 0 && !s->psib && two) return fail(c, AFQMC_ERR_INVALID_ARG, "null psib");
+  c->s = *s;
+  c->have_static = true;
+  return AFQMC_OK;
+}
+
+const char* afqmc_last_error(const afqmc_ctx* c) { return c ? c->err : "null context"; }
+
+int afqmc_sync(afqmc_ctx* c) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_get_flags(afqmc_ctx* c, int32_t* flags, int clear) {
+  if (!c || !flags) return AFQMC_ERR_INVALID_ARG;
+  int h = 0;
+  HIP_TRY(c, hipMemcpyAsync(&h, c->flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *flags = h;
+  if (clear) HIP_TRY(c, hipMemsetAsync(c->flags, 0, sizeof(int), c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_get_nfb(afqmc_ctx* c, uint64_t* nfb) {
+  if (!c || !nfb) return AFQMC_ERR_INVALID_ARG;
+  unsigned long long h = 0;
+  HIP_TRY(c, hipMemcpyAsync(&h, c->nfb, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *nfb = h;
+  return AFQMC_OK;
+}
+
+int afqmc_greens_function(afqmc_ctx* c, int32_t nw, const double* phia, const double* phib, double* ga,
+                          double* gb, double* ovlp, const double* log_shift) {
+  int r = check_ready(c, nw);
+  if (r) return r;
+  if (!phia || !ga || !ovlp || (two_spins(c) && (!phib || !gb)))
+    return fail(c, AFQMC_ERR_INVALID_ARG, "null walker array");
+  return do_gf(c, nw, phia, phib, ga, gb, ovlp, log_shift, true);
+}
+
+int afqmc_overlap(afqmc_ctx* c, int32_t nw, const double* phia, const double* phib, double* ovlp,
+                  const double* log_shift) {
+  int r = check_ready(c, nw);
+  if (r) return r;
+  if (!phia || !ovlp || (two_spins(c) && !phib)) return fail(c, AFQMC_ERR_INVALID_ARG, "null walker array");
+  return do_gf(c, nw, phia, phib, nullptr, nullptr, ovlp, log_shift, false);
+}
+
+int afqmc_force_bias(afqmc_ctx* c, int32_t nw, const double* ga, const double* gb, double* vbias) {
+  int r = check_ready(c, nw);
+  if (r) return r;
+  r = need_static(c);
+  if (r) return r;
+  if (!ga || !vbias || (two_spins(c) && !gb)) return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  r = do_force_bias(c, nw, ga, gb);
+  if (r) return r;
+  return do_fields(c, nw, false, nullptr, 0, 0, nullptr, vbias);
+}
+
+int afqmc_one_body(afqmc_ctx* c, int32_t nw, double* phia, double* phib) {
+  int r = check_ready(c, nw);
+  if (r) return r;
+  r = need_static(c);
+  if (r) return r;
+  if (!phia || (two_spins(c) && !phib)) return fail(c, AFQMC_ERR_INVALID_ARG, "null walker array");
+  r = gather(c, nw, phia, phib, true);
+  if (r) return r;
+  return scatter(c, nw, c->buf[0], phia, phib, false);
+}
+
+int afqmc_two_body(afqmc_ctx* c, int32_t nw, double* phia, double* phib, const double* ga, const double* gb,
+                   const double* xi, uint64_t seed, uint64_t counter, double* cmf, double* cfb, double* xi_out,
+                   double* vbias_out) {
+  int r = check_ready(c, nw);
+  if (r) return r;
+  r = need_static(c);
+  if (r) return r;
+  if (!phia || !ga || !cmf || !cfb || (two_spins(c) && (!phib || !gb)))
+    return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  r = do_two_body_core(c, nw, ga, gb, xi, seed, counter, xi_out, vbias_out, false, nullptr);
+  if (r) return r;
+  HIP_TRY(c, hipMemcpyAsync(cmf, wv(c, WV_CMF), 16 * (size_t)nw, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(cfb, wv(c, WV_CFB), 16 * (size_t)nw, hipMemcpyDeviceToDevice, c->stream));
+  r = gather(c, nw, phia, phib, false);
+  if (r) return r;
+  const double* res = nullptr;
+  r = taylor(c, nw, &res);
+  if (r) return r;
+  return scatter(c, nw, res, phia, phib, false);
+}
+
+int afqmc_update_weight(afqmc_ctx* c, int32_t nw, const double* ovlp, const double* ovlp_new, const double* cfb,
+                        const double* cmf, double eshift, double* weight, double* hybrid_energy,
+                        double* walker_ovlp) {
+  int r = check_ready(c, nw);
+  if (r) return r;
+  if (!(c->s.dt > 0.0)) return fail(c, AFQMC_ERR_INVALID_ARG, "dt must be positive");
+  if (!ovlp || !ovlp_new || !cfb || !cmf || !weight || !hybrid_energy || !walker_ovlp)
+    return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  WeightArgs a;
+  a.nwalkers = nw;
+  a.ovlp_old = ovlp;
+  a.ovlp_new = ovlp_new;
+  a.cfb = cfb;
+  a.cmf = cmf;
+  a.dt = c->s.dt;
+  a.eshift = eshift;
+  a.ebound = sqrt(2.0 / c->s.dt);
+  a.weight = weight;
+  a.hybrid_energy = hybrid_energy;
+  a.walker_ovlp = walker_ovlp;
+  a.flags = c->flags;
+  HIP_TRY(c, launch_update_weight(a, c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_propagate(afqmc_ctx* c, int32_t nw, double* phia, double* phib, double* ga, double* gb,
+                    const double* xi, uint64_t seed, uint64_t counter, double eshift, double* weight,
+                    double* ovlp, double* hybrid_energy, const double* log_shift, double* energy_prev) {
+  int r = check_ready(c, nw);
+  if (r) return r;
+  r = need_static(c);
+  if (r) return r;
+  if (!phia || !ga || !weight || !ovlp || !hybrid_energy || (two_spins(c) && (!phib || !gb)))
+    return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  r = do_gf(c, nw, phia, phib, ga, gb, wv(c, WV_OVLP_OLD), log_shift, true);
+  if (r) return r;
+  r = gather(c, nw, phia, phib, true);
+  if (r) return r;
+  r = do_two_body_core(c, nw, ga, gb, xi, seed, counter, nullptr, nullptr, energy_prev != nullptr, energy_prev);
+  if (r) return r;
+  const double* res = nullptr;
+  r = taylor(c, nw, &res);
+  if (r) return r;
+  r = scatter(c, nw, res, phia, phib, true);
+  if (r) return r;
+  r = do_gf(c, nw, phia, phib, nullptr, nullptr, wv(c, WV_OVLP_NEW), log_shift, false);
+  if (r) return r;
+  return afqmc_update_weight(c, nw, wv(c, WV_OVLP_OLD), wv(c, WV_OVLP_NEW), wv(c, WV_CFB), wv(c, WV_CMF), eshift,
+                             weight, hybrid_energy, ovlp);
+}
+
+int afqmc_local_energy(afqmc_ctx* c, int32_t nw, const double* ga, const double* gb, double* energy) {
+  int r = check_ready(c, nw);
+  if (r) return r;
+  r = need_static(c);
+  if (r) return r;
+  if (!ga || !energy || (two_spins(c) && !gb)) return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  r = do_force_bias(c, nw, ga, gb);
+  if (r) return r;
+  r = do_fields(c, nw, false, nullptr, 0, 0, nullptr, nullptr);
+  if (r) return r;
+  r = do_exchange(c, nw, ga, gb);
+  if (r) return r;
+  return do_energy(c, nw, ga, gb, energy);
+}
+
+int afqmc_reortho(afqmc_ctx* c, int32_t nw, double* phia, double* phib, double* ovlp, double* detR,
+                  double* log_detR, const double* detR_shift) {
+  int r = check_ready(c, nw);
+  if (r) return r;
+  r = ensure_ws(c);
+  if (r) return r;
+  if (!phia || !detR || (c->d.nbeta > 0 && !phib)) return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  ReorthoArgs a;
+  a.nwalkers = nw;
+  a.nbasis = c->d.nbasis;
+  a.nalpha = c->d.nalpha;
+  a.nbeta = c->d.nbeta;
+  a.phia = phia;
+  a.phib = phib;
+  a.work = c->qr_work;
+  a.ovlp = ovlp;
+  a.detR = detR;
+  a.log_detR = log_detR;
+  a.detR_shift = detR_shift;
+  a.flags = c->flags;
+  HIP_TRY(c, launch_reortho(a, c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_draw_normals(afqmc_ctx* c, double* out, int64_t n, uint64_t seed, uint64_t counter) {
+  if (!c || !out || n < 0) return AFQMC_ERR_INVALID_ARG;
+  HIP_TRY(c, launch_draw_normals(out, n, seed, counter, c->stream));
+  return AFQMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,175 @@
+// Internal (non-exported) declarations shared by the kernel translation units.
+#pragma once
+
+#include "afqmc_common.hpp"
+
+namespace afqmc {
+
+enum : int {
+  AFQMC_FLAG_SINGULAR = 1,    // an overlap matrix had an exactly zero pivot
+  AFQMC_FLAG_NONFINITE = 2,   // a non-finite overlap / weight was produced
+  AFQMC_FLAG_CHOLESKY = 4,    // a Gram matrix was not positive definite in reortho
+};
+
+// ---------------------------------------------------------------- GF / overlap
+struct GfArgs {
+  int nwalkers, nbasis, nalpha, nbeta, rhf;
+  const double* phia;   // [W, N, na] complex
+  const double* phib;   // [W, N, nb] complex
+  const double* psia;   // [N, na] complex
+  const double* psib;   // [N, nb] complex
+  double* ghalfa;       // [W, na, N] complex (GF only)
+  double* ghalfb;       // [W, nb, N] complex (GF only)
+  double* ovlp;         // [W] complex
+  const double* log_shift;  // [W] or null
+  int* flags;
+};
+size_t gf_lds_bytes(int nalpha, int nbeta);
+hipError_t launch_gf(const GfArgs& a, bool want_ghalf, hipStream_t stream);
+
+// ---------------------------------------------------------------- GEMMs
+// Rows of a complex operand A (K contiguous) times rows of a real operand B
+// (K contiguous): C[r, c] = sum_k A[r, k] B[c, k], summed over up to two
+// (A, B, K) segments.
+struct NtSegment {
+  const double* A;   // complex, row r at A + 2 * r * lda
+  int64_t lda;       // in complex elements
+  const double* B;   // real, row c at B + c * ldb
+  int64_t ldb;       // in doubles
+  int K;
+};
+
+struct ForceBiasArgs {
+  int nwalkers, nchol, nseg, ksplit;
+  NtSegment seg[2];
+  double scale;        // 1, or 2 for RHF walkers
+  double* vbias_part;  // [ksplit, W, X] complex
+};
+hipError_t launch_force_bias(const ForceBiasArgs& a, hipStream_t stream);
+
+struct VhsArgs {
+  int nwalkers, nbasis, nchol;
+  const double* xshifted;     // [W, X] complex
+  const double* chol_packed;  // [P, X] real
+  const int* tri_i;           // [P] row of packed index p
+  const int* tri_j;           // [P] column of packed index p
+  double sqrt_dt;
+  double* vhs;                // [W, N, N] complex (full, symmetric)
+};
+hipError_t launch_vhs(const VhsArgs& a, hipStream_t stream);
+
+// Batched C[w] = A[w] * B[w] with A [N, N] (real shared or complex per walker)
+// and B [N, ncol] complex (columns contiguous).  Epilogue:
+//   out = alpha * (A B);  if accum: accum_buf += out
+struct BatchedNNArgs {
+  int nwalkers, nrows, ncol;
+  const double* A;       // real [N, N] (shared) or complex [W, N, N]
+  bool a_complex;
+  int64_t a_stride;      // walker stride of A in elements of its type (0 = shared)
+  const double* B;       // complex, walker w at B + 2 * w * b_stride, row k at + 2 * k * ldb
+  int64_t b_stride, ldb;
+  double* C;             // complex output, same addressing as B (c_stride, ldc)
+  int64_t c_stride, ldc;
+  double alpha;
+  double* accum;         // optional complex accumulator, same addressing as C
+  const double* accum_init;  // optional: accum = accum_init + out (else accum += out)
+};
+hipError_t launch_batched_nn(const BatchedNNArgs& a, hipStream_t stream);
+
+// ---------------------------------------------------------------- exchange
+struct ExchangeArgs {
+  int nwalkers, nchol, nbasis, nocc;
+  const double* ghalf;  // [W, n, N] complex
+  const double* rchol;  // [X, n, N] real
+  int xsplit;
+  double* exx_part;     // [xsplit, W] complex, sum_x tr(T_x T_x) (no 1/2)
+};
+hipError_t launch_exchange(const ExchangeArgs& a, hipStream_t stream);
+int exchange_xsplit(int nwalkers, int nchol, int nocc);
+
+// ---------------------------------------------------------------- elementwise
+struct FieldsArgs {
+  int nwalkers, nchol, ksplit;
+  const double* vbias_part;   // [ksplit, W, X] complex
+  const double* mf_shift;     // [X] complex
+  const double* xi;           // [W, X] real, or null -> Philox draw
+  uint64_t seed, counter;
+  double sqrt_dt, fbbound;
+  double* xi_out;             // optional: the fields used, [W, X]
+  double* xshifted;           // [W, X] complex
+  double* cmf;                // [W] complex
+  double* cfb;                // [W] complex
+  double* ecoul;              // [W] complex: 1/2 sum_x vbias^2
+  double* vbias_out;          // optional [W, X] complex (summed vbias)
+  unsigned long long* nfb;    // number of capped force-bias entries (accumulated)
+  int want_fields;            // 0: only vbias sum + Coulomb
+};
+hipError_t launch_fields(const FieldsArgs& a, hipStream_t stream);
+
+struct WeightArgs {
+  int nwalkers;
+  const double* ovlp_old;  // [W] complex
+  const double* ovlp_new;  // [W] complex
+  const double* cfb;
+  const double* cmf;
+  double dt, eshift, ebound;
+  double* weight;          // [W] real
+  double* hybrid_energy;   // [W] complex (in: previous, out: new)
+  double* walker_ovlp;     // [W] complex (out: ovlp_new)
+  int* flags;
+};
+hipError_t launch_update_weight(const WeightArgs& a, hipStream_t stream);
+
+struct EnergyArgs {
+  int nwalkers, nbasis, nalpha, nbeta, rhf;
+  const double* ghalfa;
+  const double* ghalfb;
+  const double* rH1a;     // [na, N] complex
+  const double* rH1b;
+  const double* ecoul;    // [W] complex
+  const double* exx_part_a;  // [xs_a, W] complex
+  int xs_a;
+  const double* exx_part_b;  // [xs_b, W] complex (null when rhf / nbeta == 0)
+  int xs_b;
+  double ecore;
+  double* energy;         // [W, 3] complex
+};
+hipError_t launch_energy(const EnergyArgs& a, hipStream_t stream);
+
+hipError_t launch_draw_normals(double* out, int64_t n, uint64_t seed, uint64_t counter, hipStream_t stream);
+
+// ---------------------------------------------------------------- reortho
+struct ReorthoArgs {
+  int nwalkers, nbasis, nalpha, nbeta;
+  double* phia;
+  double* phib;
+  double* work;          // [W, N, max(na, nb)] complex scratch
+  double* ovlp;          // [W] complex, divided by detR
+  double* detR;          // [W] real out
+  double* log_detR;      // [W] real, += log detR
+  const double* detR_shift;  // [W] or null
+  int* flags;
+};
+hipError_t launch_reortho(const ReorthoArgs& a, hipStream_t stream);
+
+// Raise a kernel's dynamic-LDS limit when it needs more than the 64 KiB default
+// (gfx950 has 160 KiB per CU).
+inline hipError_t allow_lds(const void* kernel, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  if (bytes > 160 * 1024) return hipErrorInvalidValue;
+  return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+// XCD-aware remap: consecutive logical ids land on the same XCD (round-robin
+// dispatch puts hardware id b on XCD b % 8).
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  constexpr int NX = 8;
+  const int per = (nblocks + NX - 1) / NX;
+  int tall = nblocks % NX;
+  if (tall == 0) tall = NX;
+  const int xcd = bid % NX, local = bid / NX;
+  if (xcd < tall) return xcd * per + local;
+  return tall * per + (xcd - tall) * (per - 1) + local;
+}
+
+}  // namespace afqmc

@@ -1,0 +1,194 @@
+// Per-walker elementwise / reduction kernels of the step:
+//   fields      xbar, force-bias cap, xi, xshifted, cmf, cfb (+ Coulomb)
+//               (ipie/propagation/phaseless_base.py:282-310, 360-368;
+//                Coulomb: ipie/estimators/local_energy_sd.py:158-201)
+//   weights     hybrid energy, phaseless importance weight
+//               (phaseless_base.py:340-358, 370-380)
+//   energy      e1b + Coulomb - exchange assembly
+//               (ipie/estimators/local_energy_sd.py:358-403, 106-155)
+//   normals     counter-based N(0, 1) fields (Philox4x32-10 + Box-Muller)
+#include "afqmc_internal.hpp"
+
+namespace afqmc {
+
+namespace {
+
+constexpr int FLD_THREADS = 256;
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), key = seed, counter = (index, stream).
+__device__ __forceinline__ uint4 philox4x32(uint4 ctr, uint2 key) {
+  constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(M0, ctr.x), lo0 = M0 * ctr.x;
+    const uint32_t hi1 = __umulhi(M1, ctr.z), lo1 = M1 * ctr.z;
+    ctr = make_uint4(hi1 ^ ctr.y ^ key.x, lo1, hi0 ^ ctr.w ^ key.y, lo0);
+    key.x += W0;
+    key.y += W1;
+  }
+  return ctr;
+}
+
+// Two standard normals for pair index p of stream `counter`.
+__device__ __forceinline__ double2 normal_pair(uint64_t seed, uint64_t counter, uint64_t p) {
+  const uint4 c = make_uint4((uint32_t)p, (uint32_t)(p >> 32), (uint32_t)counter, (uint32_t)(counter >> 32));
+  const uint4 r = philox4x32(c, make_uint2((uint32_t)seed, (uint32_t)(seed >> 32)));
+  // 53-bit uniforms in (0, 1]
+  const uint64_t a = ((uint64_t)r.x << 21) ^ ((uint64_t)r.y >> 11);
+  const uint64_t b = ((uint64_t)r.z << 21) ^ ((uint64_t)r.w >> 11);
+  const double u1 = ((double)(a & ((1ull << 53) - 1)) + 1.0) * (1.0 / 9007199254740992.0);
+  const double u2 = (double)(b & ((1ull << 53) - 1)) * (1.0 / 9007199254740992.0);
+  const double rad = sqrt(-2.0 * log(u1));
+  double s, co;
+  sincospi(2.0 * u2, &s, &co);
+  return make_double2(rad * co, rad * s);
+}
+
+__device__ __forceinline__ double draw_normal(uint64_t seed, uint64_t counter, uint64_t idx) {
+  const double2 v = normal_pair(seed, counter, idx >> 1);
+  return (idx & 1) ? v.y : v.x;
+}
+
+__global__ void __launch_bounds__(FLD_THREADS) fields_kernel(FieldsArgs a) {
+  __shared__ double red[FLD_THREADS / 64];
+  const int w = blockIdx.x;
+  const int X = a.nchol;
+  const int64_t plane = (int64_t)a.nwalkers * X;
+  double cmf_r = 0, cmf_i = 0, cfb_r = 0, cfb_i = 0, ec_r = 0, ec_i = 0;
+  unsigned long long ncap = 0;
+  for (int x = threadIdx.x; x < X; x += FLD_THREADS) {
+    const int64_t idx = (int64_t)w * X + x;
+    cplx vb = ld(a.vbias_part, idx);
+    for (int z = 1; z < a.ksplit; ++z) vb = cadd(vb, ld(a.vbias_part, z * plane + idx));
+    if (a.vbias_out) st(a.vbias_out, idx, vb);
+    ec_r += vb.re * vb.re - vb.im * vb.im;
+    ec_i += 2.0 * vb.re * vb.im;
+    if (!a.want_fields) continue;
+    const cplx mf = ld(a.mf_shift, x);
+    // xbar = -sqrt(dt) (i vbias - mf_shift)
+    cplx xb = cmake(a.sqrt_dt * (mf.re + vb.im), a.sqrt_dt * (mf.im - vb.re));
+    const double axb = cabs(xb);
+    if (axb > a.fbbound) {
+      ++ncap;
+      if (axb > 1e-13) xb = cmake(xb.re / axb, xb.im / axb);
+    }
+    const double xi = a.xi ? a.xi[idx] : draw_normal(a.seed, a.counter, (uint64_t)idx);
+    if (a.xi_out) a.xi_out[idx] = xi;
+    const cplx xs = cmake(xi - xb.re, -xb.im);
+    st(a.xshifted, idx, xs);
+    cmf_r += xs.re * mf.re - xs.im * mf.im;
+    cmf_i += xs.re * mf.im + xs.im * mf.re;
+    cfb_r += xi * xb.re - 0.5 * (xb.re * xb.re - xb.im * xb.im);
+    cfb_i += xi * xb.im - xb.re * xb.im;
+  }
+  ec_r = block_sum(ec_r, red);
+  ec_i = block_sum(ec_i, red);
+  if (a.want_fields) {
+    cmf_r = block_sum(cmf_r, red);
+    cmf_i = block_sum(cmf_i, red);
+    cfb_r = block_sum(cfb_r, red);
+    cfb_i = block_sum(cfb_i, red);
+    if (ncap) atomicAdd(a.nfb, ncap);
+  }
+  if (threadIdx.x == 0) {
+    if (a.ecoul) st(a.ecoul, w, cmake(0.5 * ec_r, 0.5 * ec_i));
+    if (a.want_fields) {
+      st(a.cmf, w, cmake(-a.sqrt_dt * cmf_r, -a.sqrt_dt * cmf_i));
+      st(a.cfb, w, cmake(cfb_r, cfb_i));
+    }
+  }
+}
+
+__global__ void update_weight_kernel(WeightArgs a) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= a.nwalkers) return;
+  const cplx o_old = ld(a.ovlp_old, w), o_new = ld(a.ovlp_new, w);
+  const cplx ratio = cdiv(o_new, o_old);
+  const cplx lr = clog(ratio);
+  const cplx cfb = ld(a.cfb, w), cmf = ld(a.cmf, w);
+  cplx eh = cmake(-(lr.re + cfb.re + cmf.re) / a.dt, -(lr.im + cfb.im + cmf.im) / a.dt);
+  if (fabs(a.eshift) >= 1e-10) {
+    const double emax = a.eshift + a.ebound, emin = a.eshift - a.ebound;
+    eh.re = fmin(fmax(eh.re, emin), emax);
+  }
+  const cplx prev = ld(a.hybrid_energy, w);
+  const cplx arg = cmake(-a.dt * (0.5 * (eh.re + prev.re) - a.eshift), -a.dt * 0.5 * (eh.im + prev.im));
+  const double magn = exp(arg.re);  // |exp(arg)|
+  const double dtheta = -a.dt * eh.im - cfb.im;
+  double cosf = cos(dtheta);
+  if (cosf < 0.0) cosf = 0.0;
+  const double wnew = a.weight[w] * magn * cosf;
+  a.weight[w] = wnew;
+  st(a.hybrid_energy, w, eh);
+  st(a.walker_ovlp, w, o_new);
+  if (!isfinite(wnew) && a.flags) atomicOr(a.flags, AFQMC_FLAG_NONFINITE);
+}
+
+__global__ void __launch_bounds__(FLD_THREADS) energy_kernel(EnergyArgs a) {
+  __shared__ double red[FLD_THREADS / 64];
+  const int w = blockIdx.x;
+  const int N = a.nbasis;
+  double er = 0.0, ei = 0.0;
+  const int nspin = (a.rhf || a.nbeta == 0) ? 1 : 2;
+  for (int s = 0; s < nspin; ++s) {
+    const int n = s == 0 ? a.nalpha : a.nbeta;
+    const double* g = (s == 0 ? a.ghalfa : a.ghalfb) + 2 * (int64_t)w * n * N;
+    const double* h = s == 0 ? a.rH1a : a.rH1b;
+    for (int e = threadIdx.x; e < n * N; e += FLD_THREADS) {
+      const cplx gv = ld(g, e), hv = ld(h, e);
+      er += gv.re * hv.re - gv.im * hv.im;
+      ei += gv.re * hv.im + gv.im * hv.re;
+    }
+  }
+  er = block_sum(er, red);
+  ei = block_sum(ei, red);
+  if (threadIdx.x == 0) {
+    const double f = a.rhf ? 2.0 : 1.0;
+    const cplx e1 = cmake(f * er + a.ecore, f * ei);
+    cplx ex = cmake(0.0, 0.0);
+    for (int z = 0; z < a.xs_a; ++z) ex = cadd(ex, ld(a.exx_part_a, (int64_t)z * a.nwalkers + w));
+    if (a.exx_part_b)
+      for (int z = 0; z < a.xs_b; ++z) ex = cadd(ex, ld(a.exx_part_b, (int64_t)z * a.nwalkers + w));
+    // UHF: exx = 1/2 (sum_a + sum_b); RHF: exx = 2 * 1/2 * sum_a
+    const double fx = a.rhf ? 1.0 : 0.5;
+    ex = cscale(ex, fx);
+    const cplx e2 = csub(ld(a.ecoul, w), ex);
+    st(a.energy, 3 * (int64_t)w + 0, cadd(e1, e2));
+    st(a.energy, 3 * (int64_t)w + 1, e1);
+    st(a.energy, 3 * (int64_t)w + 2, e2);
+  }
+}
+
+__global__ void normals_kernel(double* out, int64_t n, uint64_t seed, uint64_t counter) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (2 * p >= n) return;
+  const double2 v = normal_pair(seed, counter, (uint64_t)p);
+  out[2 * p] = v.x;
+  if (2 * p + 1 < n) out[2 * p + 1] = v.y;
+}
+
+}  // namespace
+
+hipError_t launch_fields(const FieldsArgs& a, hipStream_t stream) {
+  hipLaunchKernelGGL(fields_kernel, dim3(a.nwalkers), dim3(FLD_THREADS), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_weight(const WeightArgs& a, hipStream_t stream) {
+  hipLaunchKernelGGL(update_weight_kernel, dim3((unsigned)ceil_div(a.nwalkers, 256)), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(const EnergyArgs& a, hipStream_t stream) {
+  hipLaunchKernelGGL(energy_kernel, dim3(a.nwalkers), dim3(FLD_THREADS), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_draw_normals(double* out, int64_t n, uint64_t seed, uint64_t counter, hipStream_t stream) {
+  const int64_t pairs = (n + 1) / 2;
+  hipLaunchKernelGGL(normals_kernel, dim3((unsigned)ceil_div(pairs, 256)), dim3(256), 0, stream, out, n, seed,
+                     counter);
+  return hipGetLastError();
+}
+
+}  // namespace afqmc

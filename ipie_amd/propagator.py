@@ -1,0 +1,182 @@
+"""Phaseless propagator (drop-in for ipie's PhaselessGeneric with GenericRealChol).
+
+Mirrors the operator API of ipie/propagation/phaseless_base.py:242-388 and
+phaseless_generic.py:28-111: ``build(hamiltonian, trial, walkers,
+mpi_handler, verbose)``, ``propagate_walkers(walkers, hamiltonian, trial,
+eshift)``, the one-/two-body halves, ``update_weight``, ``timer`` with the six
+PropagatorTimer fields, ``nfb_trig``, ``dt``, ``sqrt_dt``, ``ebound``,
+``fbbound``, ``mf_shift`` and ``expH1``.
+
+``propagate_walkers`` is one C-ABI call (afqmc_propagate): GF -> exp(-dt/2 H1)
+-> force bias -> fields -> VHS -> Taylor exp -> exp(-dt/2 H1) -> overlap ->
+weights, all enqueued on the current stream with no host synchronisation.
+
+Auxiliary fields: ``xi_source`` (callable ``(nwalkers, nfields) -> [W, X]``
+float64, numpy or torch) injects them exactly like ipie's
+``xp.random.normal(0, 1, X * W).reshape(W, X)`` (phaseless_base.py:296);
+without it the fields come from the device generator (Philox4x32-10, key
+``seed``, one counter value per call).
+"""
+
+import time
+
+import numpy
+import scipy.linalg
+import torch
+
+from ipie_amd import engine as _engine
+
+
+class PropagatorTimer:
+    """Restates ipie/propagation/continuous_base.py:4-11."""
+
+    def __init__(self):
+        self.tfbias = 0.0
+        self.tovlp = 0.0
+        self.tupdate = 0.0
+        self.tgf = 0.0
+        self.tvhs = 0.0
+        self.tgemm = 0.0
+
+
+def construct_mean_field_shift(hamiltonian, trial):
+    """mf = i L^T Re vec(Ga + Gb) - L^T Im vec(Ga + Gb) (phaseless_base.py:153-172)."""
+    g = (trial.G[0] + trial.G[1]).ravel()
+    return 1j * hamiltonian.chol.T.dot(g.real) - hamiltonian.chol.T.dot(g.imag)
+
+
+def construct_one_body_propagator(hamiltonian, mf_shift, dt):
+    """expH1[s] = expm(-dt/2 (h1e_mod[s] - i sum_x L_x mf_x)) (phaseless_base.py:27-74)."""
+    nb = hamiltonian.nbasis
+    shift = 1j * hamiltonian.chol.dot(mf_shift).reshape(nb, nb)
+    H1 = hamiltonian.h1e_mod - numpy.array([shift, shift])
+    return numpy.array([scipy.linalg.expm(-0.5 * dt * H1[0]), scipy.linalg.expm(-0.5 * dt * H1[1])])
+
+
+class PhaselessGeneric:
+    def __init__(self, time_step, exp_nmax=6, verbose=False, seed=7):
+        self.dt = time_step
+        self.verbose = verbose
+        self.sqrt_dt = self.dt**0.5
+        self.isqrt_dt = 1j * self.sqrt_dt
+        self.exp_nmax = exp_nmax
+        self.nhe_trig = 0
+        self.ebound = (2.0 / self.dt) ** 0.5
+        self.fbbound = 1.0
+        self.timer = PropagatorTimer()
+        self.mpi_handler = None
+        self.vbias = None
+        self.xi_source = None
+        self.seed = int(seed)
+        self.counter = 0
+        self.timing = False
+        self.last_xi = None
+        self.keep_xi = False
+        self._engine = None
+
+    # ------------------------------------------------------------------ set-up
+    def build(self, hamiltonian, trial=None, walkers=None, mpi_handler=None, verbose=False):
+        """Mean-field shift, one-body propagator and device context (phaseless_base.py:256-272)."""
+        if walkers is None:
+            raise ValueError("the MI355X propagator needs the walkers at build time (device context)")
+        if not trial.half_rotated:
+            trial.half_rotate(hamiltonian)
+        self.mf_shift = construct_mean_field_shift(hamiltonian, trial)
+        self.expH1 = construct_one_body_propagator(hamiltonian, self.mf_shift, self.dt)
+        eng = _engine.Engine(trial, walkers, nchol=hamiltonian.nchol, exp_nmax=self.exp_nmax)
+        eng.set_hamiltonian(hamiltonian, self.expH1, self.mf_shift, self.dt)
+        walkers._engine = eng
+        self._engine = eng
+        self.mpi_handler = mpi_handler
+
+    def cast_to_cupy(self, verbose=False):
+        return None
+
+    @property
+    def nfb_trig(self):
+        return 0 if self._engine is None else self._engine.ctx.nfb()
+
+    # ------------------------------------------------------------------ fields
+    def _next_xi(self, walkers, hamiltonian):
+        if self.xi_source is None:
+            self.counter += 1
+            return None
+        xi = self.xi_source(walkers.nwalkers, hamiltonian.nfields)
+        if not isinstance(xi, torch.Tensor):
+            xi = torch.from_numpy(numpy.ascontiguousarray(xi, dtype=numpy.float64))
+        xi = xi.to(device=walkers.device, dtype=torch.float64).contiguous()
+        self.last_xi = xi
+        return xi
+
+    def _eng(self, walkers):
+        eng = walkers._engine
+        if eng is None or eng is not self._engine:
+            raise RuntimeError("walkers are not bound to this propagator: call build(hamiltonian, trial, walkers)")
+        return eng
+
+    # ------------------------------------------------------------------ step
+    def propagate_walkers(self, walkers, hamiltonian, trial, eshift, energy_prev=None):
+        """One phaseless step (phaseless_base.py:312-338).
+
+        ``energy_prev`` (optional [W, 3] complex128 device tensor) receives the
+        local energy of the incoming walkers from the step's own Green's function.
+        """
+        eng = self._eng(walkers)
+        xi = self._next_xi(walkers, hamiltonian)
+        if self.timing:
+            torch.cuda.synchronize(walkers.device)
+            t0 = time.time()
+        eng.ctx.propagate(
+            walkers.phia, walkers.phib, walkers.Ghalfa, walkers.Ghalfb, walkers.weight, walkers.ovlp,
+            walkers.hybrid_energy, float(numpy.real(eshift)), xi=xi, seed=self.seed, counter=self.counter,
+            log_shift=walkers.log_shift, energy_prev=energy_prev,
+        )
+        if self.timing:
+            torch.cuda.synchronize(walkers.device)
+            self.timer.tgemm += time.time() - t0
+
+    def propagate_walkers_one_body(self, walkers):
+        """phi_s <- expH1[s] phi_s (phaseless_base.py:274-280)."""
+        t0 = time.time()
+        self._eng(walkers).ctx.one_body(walkers.phia, walkers.phib)
+        if self.timing:
+            torch.cuda.synchronize(walkers.device)
+            self.timer.tgemm += time.time() - t0
+
+    def propagate_walkers_two_body(self, walkers, hamiltonian, trial):
+        """Force bias, fields and exp(VHS) from walkers.Ghalf (phaseless_base.py:282-310); returns (cmf, cfb)."""
+        eng = self._eng(walkers)
+        xi = self._next_xi(walkers, hamiltonian)
+        nw = walkers.nwalkers
+        cmf = torch.empty(nw, dtype=torch.complex128, device=walkers.device)
+        cfb = torch.empty_like(cmf)
+        self.vbias = torch.empty(nw, hamiltonian.nchol, dtype=torch.complex128, device=walkers.device)
+        xi_out = None
+        if self.keep_xi and xi is None:
+            xi_out = torch.empty(nw, hamiltonian.nchol, dtype=torch.float64, device=walkers.device)
+        t0 = time.time()
+        eng.ctx.two_body(walkers.phia, walkers.phib, walkers.Ghalfa, walkers.Ghalfb, cmf, cfb, xi=xi, seed=self.seed,
+                         counter=self.counter, xi_out=xi_out, vbias_out=self.vbias)
+        if xi_out is not None:
+            self.last_xi = xi_out
+        if self.timing:
+            torch.cuda.synchronize(walkers.device)
+            self.timer.tvhs += time.time() - t0
+        return cmf, cfb
+
+    def update_weight(self, walkers, ovlp, ovlp_new, cfb, cmf, eshift):
+        """Hybrid energy and phaseless weight update (phaseless_base.py:340-380)."""
+        self._eng(walkers).ctx.update_weight(ovlp, ovlp_new, cfb, cmf, float(numpy.real(eshift)), walkers.weight,
+                                             walkers.hybrid_energy, walkers.ovlp)
+
+    def propagate_walkers_stepwise(self, walkers, hamiltonian, trial, eshift):
+        """The same step as separate entry points, in the reference's phase order."""
+        ovlp = trial.calc_greens_function(walkers)
+        self.propagate_walkers_one_body(walkers)
+        cmf, cfb = self.propagate_walkers_two_body(walkers, hamiltonian, trial)
+        self.propagate_walkers_one_body(walkers)
+        ovlp_new = trial.calc_overlap(walkers)
+        self.update_weight(walkers, ovlp, ovlp_new, cfb, cmf, eshift)
+
+
+Phaseless = {"generic": PhaselessGeneric}
