@@ -5,7 +5,8 @@
 // 16x16 fragment convention:
 //   A operand (16 x 4):  lane l supplies A[l % 16][k(l / 16)]
 //   B operand (4 x 16):  lane l supplies B[k(l / 16)][l % 16]
-//   accumulator (16x16): lane l holds D[4 * (l / 16) + r][l % 16], r = 0..3
+//   accumulator (16x16): lane l holds D[4 * r + l / 16][l % 16], r = 0..3
+//   (register r of lane group q holds row 4r + q; measured on gfx950)
 // The k index of a lane group only has to agree between A and B, so loaders
 // give lane group q the contiguous k range [4q, 4q + 4) of a 16-wide K chunk
 // and issue the four MFMAs of that chunk with s = 0..3 (k = 4q + s).

@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(EX_THREADS) exchange_kernel(ExchangeArgs a, Ex
           if (col >= 0 && col < n) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              const int row = rt * 16 + 4 * q + r;
+              const int row = rt * 16 + 4 * r + q;
               if (row < rows_valid) S[row * (n + 1) + col] = cmake(re[m][r], im[m][r]);
             }
           }

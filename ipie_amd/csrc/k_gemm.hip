@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(64 * FB_WM * FB_WN) force_bias_kernel(ForceBia
     for (int j = 0; j < FB_RN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int w = arow0 + 16 * i + 4 * (lane >> 4) + r;
+        const int w = arow0 + 16 * i + 4 * r + (lane >> 4);
         const int x = bcol0 + 16 * j + (lane & 15);
         if (w < a.nwalkers && x < a.nchol)
           st(out, (int64_t)w * a.nchol + x, cmake(a.scale * re[i][j][r], a.scale * im[i][j][r]));
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(64 * VH_WM * VH_WN) vhs_kernel(VhsArgs a) {
     for (int i = 0; i < VH_RM; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int w = arow0 + 16 * i + 4 * (lane >> 4) + r;
+        const int w = arow0 + 16 * i + 4 * r + (lane >> 4);
         if (w >= a.nwalkers) continue;
         // i sqrt(dt) (re + i im) = sqrt(dt) (-im + i re)
         const cplx v = cmake(-a.sqrt_dt * im[i][j][r], a.sqrt_dt * re[i][j][r]);
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(64 * NN_WAVES) batched_nn_kernel(BatchedNNArgs
     if (col >= a.ncol) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = row0 + 4 * q + r;
+      const int row = row0 + 4 * r + q;
       if (row >= a.nrows) continue;
       const int64_t off = (int64_t)row * a.ldc + col;
       const cplx v = cmake(a.alpha * re[j][r], a.alpha * im[j][r]);

@@ -45,6 +45,7 @@ class Pair:
         from ipie_amd.trial import SingleDet
         from ipie_amd.walkers import UHFWalkers
 
+        self.torch = torch
         self.nelec = nelec
         self.N = h1e.shape[-1]
         self.W = nwalkers
@@ -61,7 +62,6 @@ class Pair:
         self.prop.build(self.ham, self.trial, self.walkers)
         if init is not None:
             self.set_phi(init[:, :, : nelec[0]], init[:, :, nelec[0]:])
-        self.torch = torch
 
     def set_phi(self, pa, pb):
         t = self.torch
