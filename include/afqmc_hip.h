@@ -49,6 +49,20 @@ extern "C" {
 #define AFQMC_FLAG_NONFINITE 2   /* a non-finite weight was produced */
 #define AFQMC_FLAG_CHOLESKY 4    /* reortho met a numerically rank-deficient walker */
 
+/* phases timed by afqmc_profile (HIP events on the context's stream) */
+#define AFQMC_PHASE_GF 0          /* Green's function (PropagatorTimer.tgf) */
+#define AFQMC_PHASE_ONE_BODY 1    /* exp(-dt/2 H1) products and walker copies (tgemm) */
+#define AFQMC_PHASE_FORCE_BIAS 2  /* force-bias contraction (tfbias) */
+#define AFQMC_PHASE_FIELDS 3      /* xbar, fields, cmf, cfb, Coulomb (tfbias) */
+#define AFQMC_PHASE_VHS 4         /* VHS contraction + symmetric unpack (tvhs) */
+#define AFQMC_PHASE_TAYLOR 5      /* Taylor series of exp(VHS) (tgemm) */
+#define AFQMC_PHASE_OVERLAP 6     /* overlap of the propagated walkers (tovlp) */
+#define AFQMC_PHASE_WEIGHT 7      /* hybrid energy and weights (tupdate) */
+#define AFQMC_PHASE_EXCHANGE 8    /* fused exchange contraction (local energy) */
+#define AFQMC_PHASE_ENERGY 9      /* one-body energy and energy assembly */
+#define AFQMC_PHASE_REORTHO 10    /* QR re-orthogonalisation */
+#define AFQMC_NPHASES 11
+
 /* precision modes */
 #define AFQMC_PREC_FP64 0
 
@@ -78,11 +92,6 @@ typedef struct afqmc_static {
   double dt;
   double ecore;
 } afqmc_static;
-
-/* counters returned by afqmc_propagate / afqmc_two_body */
-typedef struct afqmc_counters {
-  uint64_t nfb_trig; /* force-bias entries capped so far (device accumulated) */
-} afqmc_counters;
 
 int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims);
 void afqmc_destroy(afqmc_ctx* ctx);
@@ -136,6 +145,12 @@ int afqmc_local_energy(afqmc_ctx* ctx, int32_t nwalkers, const double* ghalfa, c
  * real; log_detR [W] real is incremented; detR_shift [W] real or NULL. */
 int afqmc_reortho(afqmc_ctx* ctx, int32_t nwalkers, double* phia, double* phib, double* ovlp, double* detR,
                   double* log_detR, const double* detR_shift);
+/* Per-phase timing: afqmc_profile(ctx, 1) clears the totals and starts
+ * recording HIP events around every phase; afqmc_profile_read synchronises
+ * and returns accumulated milliseconds and launch counts per phase
+ * (arrays of AFQMC_NPHASES). */
+int afqmc_profile(afqmc_ctx* ctx, int enable);
+int afqmc_profile_read(afqmc_ctx* ctx, double* ms, int64_t* launches);
 /* n standard normals (Philox4x32-10, Box-Muller) for stream `counter`. */
 int afqmc_draw_normals(afqmc_ctx* ctx, double* out, int64_t n, uint64_t seed, uint64_t counter);
 

@@ -85,7 +85,12 @@ SIGNATURES = {
     "afqmc_local_energy": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp]),
     "afqmc_reortho": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _dp, _dp, _dp]),
     "afqmc_draw_normals": (ctypes.c_int, [_dp, _dp, _i64, _u64, _u64]),
+    "afqmc_profile": (ctypes.c_int, [_dp, ctypes.c_int]),
+    "afqmc_profile_read": (ctypes.c_int, [_dp, ctypes.POINTER(_f64), ctypes.POINTER(_i64)]),
 }
+
+PHASES = ["gf", "one_body", "force_bias", "fields", "vhs", "taylor", "overlap", "weight", "exchange", "energy",
+          "reortho"]
 
 _lib = None
 
@@ -316,6 +321,21 @@ class Context:
         if st != AFQMC_OK:
             raise AfqmcError(f"afqmc_get_nfb: {STATUS_NAMES.get(st, st)}")
         return int(v.value)
+
+    def profile(self, enable=True):
+        st = self.lib.afqmc_profile(self.h, int(bool(enable)))
+        if st != AFQMC_OK:
+            raise AfqmcError(f"afqmc_profile: {STATUS_NAMES.get(st, st)}")
+
+    def profile_read(self):
+        """{phase: (milliseconds, launches)} accumulated since profile(True) (synchronises)."""
+        n = len(PHASES)
+        ms = (_f64 * n)()
+        cnt = (_i64 * n)()
+        st = self.lib.afqmc_profile_read(self.h, ms, cnt)
+        if st != AFQMC_OK:
+            raise AfqmcError(f"afqmc_profile_read: {self.lib.afqmc_last_error(self.h)}")
+        return {PHASES[i]: (float(ms[i]), int(cnt[i])) for i in range(n)}
 
     def sync(self):
         st = self.lib.afqmc_sync(self.h)

@@ -14,6 +14,11 @@
 
 using namespace afqmc;
 
+struct PendingEvent {
+  int phase;
+  hipEvent_t start, stop;
+};
+
 struct afqmc_ctx {
   afqmc_dims d;
   int device = 0;
@@ -38,6 +43,12 @@ struct afqmc_ctx {
   unsigned long long* nfb = nullptr;
   bool ws_ready = false;
   char describe_dims[128] = {0};
+  // per-phase HIP-event timing (afqmc_profile / afqmc_profile_read)
+  bool prof = false;
+  std::vector<hipEvent_t> ev_free;
+  std::vector<PendingEvent> ev_pending;
+  double prof_ms[AFQMC_NPHASES] = {0};
+  int64_t prof_count[AFQMC_NPHASES] = {0};
 };
 
 namespace {
@@ -57,6 +68,37 @@ int fail(afqmc_ctx* c, int code, const char* fmt, ...) {
     hipError_t _e = (expr);                                                                  \
     if (_e != hipSuccess) return fail((c), AFQMC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
   } while (0)
+
+hipEvent_t take_event(afqmc_ctx* c) {
+  if (!c->ev_free.empty()) {
+    hipEvent_t e = c->ev_free.back();
+    c->ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Records a start/stop event pair around one phase when profiling is on.
+struct PhaseScope {
+  afqmc_ctx* c;
+  int phase;
+  hipEvent_t start = nullptr;
+  PhaseScope(afqmc_ctx* ctx, int ph) : c(ctx), phase(ph) {
+    if (c->prof) {
+      start = take_event(c);
+      if (start) (void)hipEventRecord(start, c->stream);
+    }
+  }
+  ~PhaseScope() {
+    if (!start) return;
+    hipEvent_t stop = take_event(c);
+    if (!stop) return;
+    (void)hipEventRecord(stop, c->stream);
+    c->ev_pending.push_back(PendingEvent{phase, start, stop});
+  }
+};
 
 int check_ready(afqmc_ctx* c, int32_t nw) {
   if (!c) return AFQMC_ERR_INVALID_ARG;
@@ -137,6 +179,7 @@ int do_gf(afqmc_ctx* c, int nw, const double* phia, const double* phib, double* 
   a.ovlp = ovlp;
   a.log_shift = log_shift;
   a.flags = c->flags;
+  PhaseScope ps(c, want_ghalf ? AFQMC_PHASE_GF : AFQMC_PHASE_OVERLAP);
   HIP_TRY(c, launch_gf(a, want_ghalf, c->stream));
   return AFQMC_OK;
 }
@@ -157,6 +200,7 @@ int do_force_bias(afqmc_ctx* c, int nw, const double* ga, const double* gb) {
     a.scale = 2.0;
   }
   a.vbias_part = c->vbias_part;
+  PhaseScope ps(c, AFQMC_PHASE_FORCE_BIAS);
   HIP_TRY(c, launch_force_bias(a, c->stream));
   return AFQMC_OK;
 }
@@ -182,6 +226,7 @@ int do_fields(afqmc_ctx* c, int nw, bool want_fields, const double* xi, uint64_t
   a.vbias_out = vbias_out;
   a.nfb = c->nfb;
   a.want_fields = want_fields ? 1 : 0;
+  PhaseScope ps(c, AFQMC_PHASE_FIELDS);
   HIP_TRY(c, launch_fields(a, c->stream));
   return AFQMC_OK;
 }
@@ -198,6 +243,7 @@ int do_vhs(afqmc_ctx* c, int nw) {
   a.tri_j = c->tri + P;
   a.sqrt_dt = sqrt(c->s.dt);
   a.vhs = c->vhs;
+  PhaseScope ps(c, AFQMC_PHASE_VHS);
   HIP_TRY(c, launch_vhs(a, c->stream));
   return AFQMC_OK;
 }
@@ -228,6 +274,7 @@ int nn(afqmc_ctx* c, int nw, const double* A, bool acplx, int64_t a_stride, cons
 
 // phi_s -> P (one body applied when with_expH1, plain copy otherwise)
 int gather(afqmc_ctx* c, int nw, const double* phia, const double* phib, bool with_expH1) {
+  PhaseScope ps(c, AFQMC_PHASE_ONE_BODY);
   const int N = c->d.nbasis, na = c->d.nalpha, nb = c->d.nbeta, nc = c->ncol;
   const int64_t NN = (int64_t)N * N;
   double* P = c->buf[0];
@@ -250,6 +297,7 @@ int gather(afqmc_ctx* c, int nw, const double* phia, const double* phib, bool wi
 
 // src (concatenated) -> phi_s
 int scatter(afqmc_ctx* c, int nw, const double* src, double* phia, double* phib, bool with_expH1) {
+  PhaseScope ps(c, AFQMC_PHASE_ONE_BODY);
   const int N = c->d.nbasis, na = c->d.nalpha, nb = c->d.nbeta, nc = c->ncol;
   const int64_t NN = (int64_t)N * N;
   if (with_expH1) {
@@ -271,6 +319,7 @@ int scatter(afqmc_ctx* c, int nw, const double* src, double* phia, double* phib,
 
 // Taylor series of exp(VHS) applied to P; returns the buffer holding the result.
 int taylor(afqmc_ctx* c, int nw, const double** result) {
+  PhaseScope ps(c, AFQMC_PHASE_TAYLOR);
   const int N = c->d.nbasis, nc = c->ncol;
   const int64_t NN = (int64_t)N * N, stride = (int64_t)N * nc;
   double* P = c->buf[0];
@@ -301,13 +350,17 @@ int do_exchange(afqmc_ctx* c, int nw, const double* ga, const double* gb) {
   a.rchol = c->s.rchola;
   a.xsplit = c->xs_a;
   a.exx_part = c->exx_a;
-  HIP_TRY(c, launch_exchange(a, c->stream));
+  {
+    PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
+    HIP_TRY(c, launch_exchange(a, c->stream));
+  }
   if (two_spins(c)) {
     a.nocc = c->d.nbeta;
     a.ghalf = gb;
     a.rchol = c->s.rcholb;
     a.xsplit = c->xs_b;
     a.exx_part = c->exx_b;
+    PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
     HIP_TRY(c, launch_exchange(a, c->stream));
   }
   return AFQMC_OK;
@@ -331,6 +384,7 @@ int do_energy(afqmc_ctx* c, int nw, const double* ga, const double* gb, double* 
   a.xs_b = c->xs_b;
   a.ecore = c->s.ecore;
   a.energy = energy;
+  PhaseScope ps(c, AFQMC_PHASE_ENERGY);
   HIP_TRY(c, launch_energy(a, c->stream));
   return AFQMC_OK;
 }
@@ -406,6 +460,11 @@ void afqmc_destroy(afqmc_ctx* c) {
                   c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
+  for (const PendingEvent& pe : c->ev_pending) {
+    (void)hipEventDestroy(pe.start);
+    (void)hipEventDestroy(pe.stop);
+  }
   delete c;
 }
 
@@ -535,6 +594,7 @@ int afqmc_update_weight(afqmc_ctx* c, int32_t nw, const double* ovlp, const doub
   a.hybrid_energy = hybrid_energy;
   a.walker_ovlp = walker_ovlp;
   a.flags = c->flags;
+  PhaseScope ps(c, AFQMC_PHASE_WEIGHT);
   HIP_TRY(c, launch_update_weight(a, c->stream));
   return AFQMC_OK;
 }
@@ -600,6 +660,7 @@ int afqmc_reortho(afqmc_ctx* c, int32_t nw, double* phia, double* phib, double* 
   a.log_detR = log_detR;
   a.detR_shift = detR_shift;
   a.flags = c->flags;
+  PhaseScope ps(c, AFQMC_PHASE_REORTHO);
   HIP_TRY(c, launch_reortho(a, c->stream));
   return AFQMC_OK;
 }
@@ -607,6 +668,35 @@ int afqmc_reortho(afqmc_ctx* c, int32_t nw, double* phia, double* phib, double* 
 int afqmc_draw_normals(afqmc_ctx* c, double* out, int64_t n, uint64_t seed, uint64_t counter) {
   if (!c || !out || n < 0) return AFQMC_ERR_INVALID_ARG;
   HIP_TRY(c, launch_draw_normals(out, n, seed, counter, c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_profile(afqmc_ctx* c, int enable) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  c->prof = enable != 0;
+  for (int i = 0; i < AFQMC_NPHASES; ++i) {
+    c->prof_ms[i] = 0.0;
+    c->prof_count[i] = 0;
+  }
+  return AFQMC_OK;
+}
+
+int afqmc_profile_read(afqmc_ctx* c, double* ms, int64_t* launches) {
+  if (!c || !ms || !launches) return AFQMC_ERR_INVALID_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (const PendingEvent& pe : c->ev_pending) {
+    float t = 0.0f;
+    HIP_TRY(c, hipEventElapsedTime(&t, pe.start, pe.stop));
+    c->prof_ms[pe.phase] += t;
+    c->prof_count[pe.phase] += 1;
+    c->ev_free.push_back(pe.start);
+    c->ev_free.push_back(pe.stop);
+  }
+  c->ev_pending.clear();
+  for (int i = 0; i < AFQMC_NPHASES; ++i) {
+    ms[i] = c->prof_ms[i];
+    launches[i] = c->prof_count[i];
+  }
   return AFQMC_OK;
 }
 
