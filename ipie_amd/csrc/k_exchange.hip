@@ -149,10 +149,176 @@ __global__ void __launch_bounds__(EX_THREADS) exchange_kernel(ExchangeArgs a, Ex
   }
 }
 
-ExPlan make_plan(int nocc) {
+// Tiled variant with compile-time geometry: RT x CT output tiles of 16x16
+// dealt round-robin to NW waves, double-buffered LDS staging with the next K
+// chunk fetched into registers while the current one is multiplied (one
+// barrier per chunk).
+template <int RT, int CT, int NW>
+__global__ void __launch_bounds__(64 * NW) exchange_tiled_kernel(ExchangeArgs a, ExPlan pl) {
+  constexpr int THREADS = 64 * NW;
+  constexpr int R = RT * 16, C = CT * 16;
+  constexpr int RS = R + 1, CS = C + 2;
+  constexpr int NT = RT * CT;
+  constexpr int MAXT = (NT + NW - 1) / NW;
+  constexpr int STAGE = 2 * EX_KC * RS + EX_KC * CS;  // doubles per staging buffer
+  constexpr int A_PER = (R * EX_KC + THREADS - 1) / THREADS;
+  constexpr int B_PER = (C * EX_KC + THREADS - 1) / THREADS;
+  extern __shared__ double lds[];
+  __shared__ double red[NW];
+  const int n = a.nocc, N = a.nbasis;
+  const int rows_valid = pl.wb * n, cols_valid = pl.xb * n;
+  cplx* S = reinterpret_cast<cplx*>(lds);
+
+  const int nwb = (int)ceil_div(a.nwalkers, pl.wb);
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int bw = lin % nwb, split = lin / nwb;
+  const int nxb = (int)ceil_div(a.nchol, pl.xb);
+  const int xb_begin = (int)((int64_t)nxb * split / a.xsplit);
+  const int xb_end = (int)((int64_t)nxb * (split + 1) / a.xsplit);
+
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
+  const int w0 = bw * pl.wb;
+  const int64_t arow_base = (int64_t)w0 * n;
+  const int64_t arow_limit = (int64_t)a.nwalkers * n;
+  const int64_t brow_limit = (int64_t)a.nchol * n;
+
+  int a_off[MAXT], b_off[MAXT];
+#pragma unroll
+  for (int m = 0; m < MAXT; ++m) {
+    const int t = wave + NW * m;
+    a_off[m] = (t / CT) * 16 + l16;
+    b_off[m] = (t % CT) * 16 + l16;
+  }
+
+  double wacc_r = 0.0, wacc_i = 0.0;
+  const int nch = (int)ceil_div(N, EX_KC);
+  cplx pa[A_PER];
+  double pb[B_PER];
+
+  for (int xblk = xb_begin; xblk < xb_end; ++xblk) {
+    const int x0 = xblk * pl.xb;
+    const int64_t brow_base = (int64_t)x0 * n;
+    auto fetch = [&](int c) {
+      const int k0 = c * EX_KC;
+#pragma unroll
+      for (int i = 0; i < A_PER; ++i) {
+        const int e = tid + i * THREADS;
+        const int row = e / EX_KC, k = e % EX_KC;
+        const int64_t ar = arow_base + row;
+        pa[i] = cmake(0.0, 0.0);
+        if (e < R * EX_KC && row < rows_valid && ar < arow_limit && k0 + k < N) pa[i] = ld(a.ghalf, ar * N + k0 + k);
+      }
+#pragma unroll
+      for (int i = 0; i < B_PER; ++i) {
+        const int e = tid + i * THREADS;
+        const int col = e / EX_KC, k = e % EX_KC;
+        const int64_t br = brow_base + col;
+        pb[i] = 0.0;
+        if (e < C * EX_KC && col < cols_valid && br < brow_limit && k0 + k < N) pb[i] = a.rchol[br * N + k0 + k];
+      }
+    };
+    auto stash = [&](int buf) {
+      cplx* As = reinterpret_cast<cplx*>(lds + buf * STAGE);
+      double* Bs = lds + buf * STAGE + 2 * EX_KC * RS;
+#pragma unroll
+      for (int i = 0; i < A_PER; ++i) {
+        const int e = tid + i * THREADS;
+        if (e < R * EX_KC) As[(e % EX_KC) * RS + e / EX_KC] = pa[i];
+      }
+#pragma unroll
+      for (int i = 0; i < B_PER; ++i) {
+        const int e = tid + i * THREADS;
+        if (e < C * EX_KC) Bs[(e % EX_KC) * CS + e / EX_KC] = pb[i];
+      }
+    };
+    v4d re[MAXT], im[MAXT];
+#pragma unroll
+    for (int m = 0; m < MAXT; ++m) {
+      re[m] = zero4d();
+      im[m] = zero4d();
+    }
+    fetch(0);
+    __syncthreads();  // previous x block's epilogue is done with the aliased area
+    stash(0);
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+      if (c + 1 < nch) fetch(c + 1);
+      const cplx* As = reinterpret_cast<const cplx*>(lds + (c & 1) * STAGE);
+      const double* Bs = lds + (c & 1) * STAGE + 2 * EX_KC * RS;
+#pragma unroll 1
+      for (int s = 0; s < 4; ++s) {
+        const int k = 4 * q + s;
+#pragma unroll
+        for (int m = 0; m < MAXT; ++m) {
+          if (wave + NW * m < NT) {
+            const cplx av = As[k * RS + a_off[m]];
+            const double bv = Bs[k * CS + b_off[m]];
+            re[m] = mfma64(av.re, bv, re[m]);
+            im[m] = mfma64(av.im, bv, im[m]);
+          }
+        }
+      }
+      if (c + 1 < nch) stash((c + 1) & 1);
+      __syncthreads();
+    }
+    for (int xl = 0; xl < pl.xb; ++xl) {
+      if (x0 + xl >= a.nchol) break;
+      const int cbeg = xl * n;
+#pragma unroll
+      for (int m = 0; m < MAXT; ++m) {
+        if (wave + NW * m < NT) {
+          const int col = b_off[m] - cbeg;
+          if (col >= 0 && col < n) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = a_off[m] - l16 + 4 * r + q;
+              if (row < rows_valid) S[row * (n + 1) + col] = cmake(re[m][r], im[m][r]);
+            }
+          }
+        }
+      }
+      __syncthreads();
+      const int nn = n * n;
+      for (int e = tid; e < pl.wb * nn; e += THREADS) {
+        const int wl = e % pl.wb, ij = e / pl.wb, i = ij / n, j = ij % n;
+        const cplx tij = S[(wl * n + j) * (n + 1) + i];
+        const cplx tji = S[(wl * n + i) * (n + 1) + j];
+        wacc_r += tij.re * tji.re - tij.im * tji.im;
+        wacc_i += tij.re * tji.im + tij.im * tji.re;
+      }
+      __syncthreads();
+    }
+  }
+  for (int wl = 0; wl < pl.wb; ++wl) {
+    const bool mine = (tid % pl.wb) == wl;
+    const double vr = block_sum(mine ? wacc_r : 0.0, red);
+    const double vi = block_sum(mine ? wacc_i : 0.0, red);
+    const int w = w0 + wl;
+    if (tid == 0 && w < a.nwalkers) st(a.exx_part, (int64_t)split * a.nwalkers + w, cmake(vr, vi));
+  }
+}
+
+template <int RT, int CT, int NW>
+size_t tiled_lds_bytes(const ExPlan& pl, int nocc) {
+  constexpr int STAGE = 2 * EX_KC * (RT * 16 + 1) + EX_KC * (CT * 16 + 2);
+  const size_t stage = sizeof(double) * 2 * (size_t)STAGE;
+  const size_t epi = sizeof(cplx) * (size_t)(pl.wb * nocc) * (nocc + 1);
+  return stage > epi ? stage : epi;
+}
+
+template <int RT, int CT, int NW>
+hipError_t launch_tiled(const ExchangeArgs& a, const ExPlan& pl, dim3 grid, hipStream_t stream) {
+  const size_t lds = tiled_lds_bytes<RT, CT, NW>(pl, a.nocc);
+  hipError_t e = allow_lds((const void*)exchange_tiled_kernel<RT, CT, NW>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((exchange_tiled_kernel<RT, CT, NW>), grid, dim3(64 * NW), lds, stream, a, pl);
+  return hipGetLastError();
+}
+
+ExPlan make_plan(int nocc, int max_tiles) {
   // Choose walkers-per-tile (wb, a power of two dividing the block size) and
   // vectors-per-tile (xb) minimising padding, with at most 5 row tiles, 10
-  // column tiles and 32 tiles total (8 accumulator tiles per wave).
+  // column tiles and max_tiles tiles in total.
   ExPlan best{1, 1, 1, 1};
   double best_score = -1.0;
   for (int wb = 1; wb <= 8; wb *= 2) {
@@ -162,7 +328,7 @@ ExPlan make_plan(int nocc) {
       const int C = round16(xb * nocc);
       if (C / 16 > 10) break;
       const int tiles = (R / 16) * (C / 16);
-      if (tiles > 32) break;
+      if (tiles > max_tiles) break;
       const double eff = (double)(wb * nocc) * (xb * nocc) / ((double)R * C);
       // prefer efficient tiles, then bigger tiles (more reuse)
       const double score = eff * 1000.0 + tiles;
@@ -175,6 +341,15 @@ ExPlan make_plan(int nocc) {
   return best;
 }
 
+// Plan used by the launcher: the tiled kernel when its geometry is compiled in.
+bool tiled_geometry(const ExPlan& pl) { return pl.RT >= 1 && pl.RT <= 5 && pl.CT >= 8 && pl.CT <= 10; }
+
+ExPlan choose_plan(int nocc) {
+  const ExPlan big = make_plan(nocc, 50);
+  if (tiled_geometry(big)) return big;
+  return make_plan(nocc, 32);
+}
+
 size_t ex_lds_bytes(const ExPlan& pl, int nocc) {
   const size_t stage = sizeof(double) * (size_t)(2 * EX_KC * (pl.RT * 16 + 1) + EX_KC * (pl.CT * 16 + 2));
   const size_t epi = sizeof(cplx) * (size_t)(pl.wb * nocc) * (nocc + 1);
@@ -184,7 +359,7 @@ size_t ex_lds_bytes(const ExPlan& pl, int nocc) {
 }  // namespace
 
 int exchange_xsplit(int nwalkers, int nchol, int nocc) {
-  const ExPlan pl = make_plan(nocc);
+  const ExPlan pl = choose_plan(nocc);
   const int nwb = (int)ceil_div(nwalkers, pl.wb);
   const int nxb = (int)ceil_div(nchol, pl.xb);
   // aim for >= 2048 workgroups, never more splits than x blocks
@@ -197,9 +372,19 @@ int exchange_xsplit(int nwalkers, int nchol, int nocc) {
 hipError_t launch_exchange(const ExchangeArgs& a, hipStream_t stream) {
   if (a.nocc <= 0) return hipSuccess;
   if (a.nocc > 80) return hipErrorInvalidValue;
-  const ExPlan pl = make_plan(a.nocc);
+  const ExPlan pl = choose_plan(a.nocc);
   const int nwb = (int)ceil_div(a.nwalkers, pl.wb);
   const dim3 grid(nwb * a.xsplit);
+  if (tiled_geometry(pl)) {
+#define AFQMC_TILED(rt, ct) \
+  if (pl.RT == rt && pl.CT == ct) return launch_tiled<rt, ct, (rt * ct > 32 ? 8 : 4)>(a, pl, grid, stream);
+    AFQMC_TILED(1, 8) AFQMC_TILED(1, 9) AFQMC_TILED(1, 10)
+    AFQMC_TILED(2, 8) AFQMC_TILED(2, 9) AFQMC_TILED(2, 10)
+    AFQMC_TILED(3, 8) AFQMC_TILED(3, 9) AFQMC_TILED(3, 10)
+    AFQMC_TILED(4, 8) AFQMC_TILED(4, 9) AFQMC_TILED(4, 10)
+    AFQMC_TILED(5, 8) AFQMC_TILED(5, 9) AFQMC_TILED(5, 10)
+#undef AFQMC_TILED
+  }
   const size_t lds = ex_lds_bytes(pl, a.nocc);
   const int ntiles = pl.RT * pl.CT;
   hipError_t e = allow_lds(ntiles <= 16 ? (const void*)exchange_kernel<4> : (const void*)exchange_kernel<8>, lds);
