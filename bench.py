@@ -24,7 +24,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "walker-steps/s at N=400,X=2400: FP64 vs mixed+surrogate at <=0.1 mHa bias"
-PEAK = {"f64_mfma_tflops": 78.6, "hbm_gbs": 8000.0}
+PEAK = {"f64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
 REORTHO_EVERY = 5
 
 
@@ -56,14 +56,10 @@ def exchange_flops_per_launch(N, X, n, W):
     return float(X) * W * (4 * n * n * N + 8 * n * n)
 
 
-def build_device(w, scale):
-    import torch
-
+def build_host(w, scale):
     from ipie_amd import Generic, synthetic
     from ipie_amd.hamiltonian import GenericRealChol
-    from ipie_amd.propagator import PhaselessGeneric
     from ipie_amd.trial import SingleDet
-    from ipie_amd.walkers import UHFWalkers
 
     t0 = time.time()
     h1e, chol, ecore = synthetic.make_hamiltonian(w.nbasis, w.nchol, w.decay, seed=w.seed)
@@ -77,14 +73,24 @@ def build_device(w, scale):
     trial = SingleDet(psi, (w.nalpha, w.nbeta), w.nbasis)
     trial.half_rotate(ham)
     trial.calculate_energy(system, ham)
-    walkers = UHFWalkers(psi, w.nalpha, w.nbeta, w.nbasis, w.nwalkers)
-    walkers.build(trial)
-    prop = PhaselessGeneric(w.dt, seed=w.seed)
-    prop.build(ham, trial, walkers)
+    log(f"[bench] host set-up in {time.time() - t0:.1f}s, E_trial={trial.energy.real:.6f}")
+    return dict(h1e=h1e, chol=chol, ecore=ecore, psi=psi, system=system, ham=ham, trial=trial)
+
+
+def build_device(host, w, precision):
+    import torch
+
+    from ipie_amd.propagator import PhaselessGeneric
+    from ipie_amd.walkers import UHFWalkers
+
+    t0 = time.time()
+    walkers = UHFWalkers(host["psi"], w.nalpha, w.nbeta, w.nbasis, w.nwalkers)
+    walkers.build(host["trial"])
+    prop = PhaselessGeneric(w.dt, seed=w.seed, precision=precision)
+    prop.build(host["ham"], host["trial"], walkers)
     torch.cuda.synchronize()
-    log(f"[bench] host set-up + upload in {time.time() - t0:.1f}s, E_trial={trial.energy.real:.6f}")
-    return dict(h1e=h1e, chol=chol, ecore=ecore, psi=psi, system=system, ham=ham, trial=trial, walkers=walkers,
-                prop=prop)
+    log(f"[bench] {precision}: device context + upload in {time.time() - t0:.1f}s")
+    return dict(host, walkers=walkers, prop=prop)
 
 
 def run_gpu(st, w, steps, warmup):
@@ -171,6 +177,8 @@ def main():
     ap.add_argument("--workload", default="n400")
     ap.add_argument("--nwalkers", type=int, default=None)
     ap.add_argument("--cholesky-scale", default="normalised", choices=["normalised", "as-specified"])
+    ap.add_argument("--precision", default="fp64,mixed", help="comma list of fp64, mixed; the last listed "
+                    "mixed mode is the headline value")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-walkers", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -191,31 +199,44 @@ def main():
     if args.nwalkers:
         w = dataclasses.replace(w, nwalkers=args.nwalkers)
     scale = 1.0 if args.cholesky_scale == "as-specified" else math.sqrt(12.0 / (w.nbasis * w.decay))
-    st = build_device(w, scale)
-    elapsed, prof, emean, finite = run_gpu(st, w, args.steps, args.warmup)
+    host = build_host(w, scale)
     K = args.steps
-    value = w.nwalkers * K / elapsed
-    ms_per_step = 1e3 * elapsed / K
-    # dominant kernel: exchange (one launch per spin per step)
-    ex_ms, ex_n = prof["exchange"]
-    fl = 0.5 * (exchange_flops_per_launch(w.nbasis, w.nchol, w.nalpha, w.nwalkers)
-                + exchange_flops_per_launch(w.nbasis, w.nchol, w.nbeta, w.nwalkers))
-    ex_avg_s = ex_ms / max(ex_n, 1) / 1e3
-    achieved = fl / ex_avg_s / 1e12
-    phases = {k: round(v[0] / K, 3) for k, v in prof.items()}
+    modes = {}
+    for precision in args.precision.split(","):
+        st = build_device(host, w, precision)
+        elapsed, prof, emean, finite = run_gpu(st, w, K, args.warmup)
+        ex_ms, ex_n = prof["exchange"]
+        fl = 0.5 * (exchange_flops_per_launch(w.nbasis, w.nchol, w.nalpha, w.nwalkers)
+                    + exchange_flops_per_launch(w.nbasis, w.nchol, w.nbeta, w.nwalkers))
+        ex_avg_s = ex_ms / max(ex_n, 1) / 1e3
+        modes[precision] = dict(
+            value=w.nwalkers * K / elapsed,
+            ms_per_step=1e3 * elapsed / K,
+            exchange_avg_launch_ms=ex_avg_s * 1e3,
+            exchange_tflops=fl / ex_avg_s / 1e12,
+            exchange_flops_per_launch=fl,
+            phases_ms_per_step={k: round(v[0] / K, 3) for k, v in prof.items()},
+            weighted_energy_last_step=emean,
+            finite=finite,
+        )
+        del st
+        torch.cuda.empty_cache()
+    head = "mixed" if "mixed" in modes else next(iter(modes))
+    m = modes[head]
+    peak = PEAK["f64_mfma_tflops"] if head == "fp64" else PEAK["bf16_mfma_tflops"] / 3.0
     step_fl = step_flops(w.nbasis, w.nchol, w.nalpha, w.nbeta)
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": m["value"],
         "unit": "walker-steps/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if head == "fp64" else "f64 + bf16x3 (exchange, fp32 accumulate)",
         "data": "synthetic",
         "config": {
             "workload": f"{w.name}: N={w.nbasis} X={w.nchol} decay={w.decay} nalpha={w.nalpha} nbeta={w.nbeta} "
@@ -226,29 +247,34 @@ def main():
             "nbeta": w.nbeta,
             "nwalkers": w.nwalkers,
             "cholesky_scale": args.cholesky_scale,
-            "precision": "fp64",
+            "precision": head,
             "parallelism": "single GPU (walkers batched)",
         },
         "roofline": {
-            "kernel": "exchange_kernel (fused exchange contraction, one launch per spin)",
+            "kernel": "exchange (fused exchange contraction, one launch per spin)"
+                      + (" split-BF16 x3" if head == "mixed" else " FP64"),
             "bound": "mfma",
-            "achieved": achieved,
-            "peak": PEAK["f64_mfma_tflops"],
+            "achieved": m["exchange_tflops"],
+            "peak": peak,
+            "peak_note": "FP64 MFMA 78.6 TFLOP/s" if head == "fp64" else
+                         "dense BF16 MFMA 2.5 PFLOP/s / 3 products per algorithmic flop",
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK["f64_mfma_tflops"],
+            "frac": m["exchange_tflops"] / peak,
             "traffic": None,
-            "avg_launch_ms": ex_avg_s * 1e3,
-            "flops_per_launch": fl,
+            "avg_launch_ms": m["exchange_avg_launch_ms"],
+            "flops_per_launch": m["exchange_flops_per_launch"],
         },
-        "step_tflops": step_fl * value / 1e12,
-        "step_frac_of_f64_peak": step_fl * value / 1e12 / PEAK["f64_mfma_tflops"],
-        "phases_ms_per_step": phases,
-        "fp64_walker_steps_per_s": value,
-        "weighted_energy_last_step": emean,
-        "finite": finite,
+        "step_tflops": step_fl * m["value"] / 1e12,
+        "modes": modes,
     }
+    if "fp64" in modes:
+        out["fp64_walker_steps_per_s"] = modes["fp64"]["value"]
+        if head != "fp64":
+            out["speedup_vs_fp64"] = m["value"] / modes["fp64"]["value"]
+            out["energy_diff_vs_fp64_last_step_Ha"] = m["weighted_energy_last_step"] - modes["fp64"][
+                "weighted_energy_last_step"]
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(st, w, args.cpu_walkers, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(host, w, args.cpu_walkers, args.cpu_steps)
         log(f"[bench] cpu baseline {out['cpu_baseline']['value']:.3f} walker-steps/s")
     else:
         out["cpu_baseline"] = None

@@ -64,7 +64,9 @@ extern "C" {
 #define AFQMC_NPHASES 11
 
 /* precision modes */
-#define AFQMC_PREC_FP64 0
+#define AFQMC_PREC_FP64 0   /* every contraction in FP64 */
+#define AFQMC_PREC_MIXED 1  /* exchange as split-BF16 products with FP32 accumulation (FP64
+                               trace sum); LU, inverse, overlaps, weights and QR stay FP64 */
 
 typedef struct afqmc_ctx afqmc_ctx;
 

@@ -25,6 +25,8 @@ FLAG_SINGULAR = 1
 FLAG_NONFINITE = 2
 FLAG_CHOLESKY = 4
 PREC_FP64 = 0
+PREC_MIXED = 1
+PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
 
 # Every symbol the header declares, with (restype, argtypes).
 _dp = ctypes.c_void_p
@@ -143,10 +145,13 @@ def _check_tensor(name, t, dtype, shape, device):
 class Context:
     """Owns one afqmc_ctx (dims, workspace, borrowed static tensors) on one device."""
 
-    def __init__(self, nbasis, nchol, nalpha, nbeta, nwalkers, exp_nmax=6, rhf=False, device=None):
+    def __init__(self, nbasis, nchol, nalpha, nbeta, nwalkers, exp_nmax=6, rhf=False, device=None, precision="fp64"):
         self.lib = load()
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
-        self.dims = Dims(nbasis, nchol, nalpha, nbeta, nwalkers, exp_nmax, int(bool(rhf)), PREC_FP64)
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        self.precision = precision
+        self.dims = Dims(nbasis, nchol, nalpha, nbeta, nwalkers, exp_nmax, int(bool(rhf)), PRECISIONS[precision])
         h = _dp()
         with torch.cuda.device(self.device):
             st = self.lib.afqmc_create(ctypes.byref(h), self.device.index or 0, ctypes.byref(self.dims))

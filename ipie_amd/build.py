@@ -11,7 +11,15 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libafqmc_hip.so")
-SOURCES = ["afqmc_api.hip", "k_greens.hip", "k_gemm.hip", "k_exchange.hip", "k_fields.hip", "k_qr.hip"]
+SOURCES = [
+    "afqmc_api.hip",
+    "k_greens.hip",
+    "k_gemm.hip",
+    "k_exchange.hip",
+    "k_exchange_bf16.hip",
+    "k_fields.hip",
+    "k_qr.hip",
+]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -43,6 +43,14 @@ struct afqmc_ctx {
   unsigned long long* nfb = nullptr;
   bool ws_ready = false;
   char describe_dims[128] = {0};
+  // split-BF16 exchange (AFQMC_PREC_MIXED)
+  int kpad = 0;
+  bool bx_ok[2] = {false, false};
+  int bx_wb[2] = {1, 1}, bx_xb[2] = {1, 1}, bx_rt[2] = {0, 0}, bx_ct[2] = {0, 0}, bx_xs[2] = {8, 8};
+  uint16_t* gplanes = nullptr;  // 4 x [W * nmax, kpad]
+  uint16_t* lplanes[2] = {nullptr, nullptr};  // 2 x [X * n_s, kpad] per spin
+  bool lplanes_ready = false;
+  int xs_used[2] = {1, 1};  // partial-sum count of the last exchange per spin
   // per-phase HIP-event timing (afqmc_profile / afqmc_profile_read)
   bool prof = false;
   std::vector<hipEvent_t> ev_free;
@@ -131,8 +139,18 @@ int ensure_ws(afqmc_ctx* c) {
   ok = ok && dmalloc((void**)&c->vbias_part, 16 * (size_t)c->fb_ksplit * W * X) == hipSuccess;
   ok = ok && dmalloc((void**)&c->xshifted, 16 * (size_t)W * X) == hipSuccess;
   ok = ok && dmalloc((void**)&c->wvec, 16 * (size_t)6 * W) == hipSuccess;
-  ok = ok && dmalloc((void**)&c->exx_a, 16 * (size_t)c->xs_a * W) == hipSuccess;
-  ok = ok && dmalloc((void**)&c->exx_b, 16 * (size_t)c->xs_b * W) == hipSuccess;
+  const int xa = c->xs_a > c->bx_xs[0] ? c->xs_a : c->bx_xs[0];
+  const int xbm = c->xs_b > c->bx_xs[1] ? c->xs_b : c->bx_xs[1];
+  ok = ok && dmalloc((void**)&c->exx_a, 16 * (size_t)xa * W) == hipSuccess;
+  ok = ok && dmalloc((void**)&c->exx_b, 16 * (size_t)xbm * W) == hipSuccess;
+  if (c->bx_ok[0] || c->bx_ok[1]) {
+    ok = ok && dmalloc((void**)&c->gplanes, 2 * (size_t)4 * W * nmax * c->kpad) == hipSuccess;
+    for (int s = 0; s < 2; ++s)
+      if (c->bx_ok[s]) {
+        const int n = s == 0 ? d.nalpha : d.nbeta;
+        ok = ok && dmalloc((void**)&c->lplanes[s], 2 * (size_t)2 * X * n * c->kpad) == hipSuccess;
+      }
+  }
   ok = ok && dmalloc((void**)&c->vhs, 16 * (size_t)W * N * N) == hipSuccess;
   for (int i = 0; i < 4; ++i) ok = ok && dmalloc((void**)&c->buf[i], 16 * (size_t)W * N * c->ncol) == hipSuccess;
   ok = ok && dmalloc((void**)&c->qr_work, 16 * (size_t)W * N * nmax) == hipSuccess;
@@ -339,30 +357,59 @@ int taylor(afqmc_ctx* c, int nw, const double** result) {
   return AFQMC_OK;
 }
 
-int do_exchange(afqmc_ctx* c, int nw, const double* ga, const double* gb) {
+int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g) {
   const int N = c->d.nbasis;
+  const int n = spin == 0 ? c->d.nalpha : c->d.nbeta;
+  const double* rchol = spin == 0 ? c->s.rchola : c->s.rcholb;
+  double* part = spin == 0 ? c->exx_a : c->exx_b;
+  if (c->d.precision == AFQMC_PREC_MIXED && c->bx_ok[spin]) {
+    if (!c->lplanes_ready) {
+      for (int s = 0; s < 2; ++s)
+        if (c->bx_ok[s]) {
+          const int ns = s == 0 ? c->d.nalpha : c->d.nbeta;
+          HIP_TRY(c, launch_split_real(s == 0 ? c->s.rchola : c->s.rcholb, (int64_t)c->d.nchol * ns, N, c->kpad,
+                                       c->lplanes[s], c->stream));
+        }
+      c->lplanes_ready = true;
+    }
+    PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
+    HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes, c->stream));
+    ExchangeBf16Args b;
+    b.nwalkers = nw;
+    b.nchol = c->d.nchol;
+    b.nocc = n;
+    b.kpad = c->kpad;
+    b.wb = c->bx_wb[spin];
+    b.xb = c->bx_xb[spin];
+    b.rt_ = c->bx_rt[spin];
+    b.ct_ = c->bx_ct[spin];
+    b.xsplit = c->bx_xs[spin];
+    b.gplanes = c->gplanes;
+    b.lplanes = c->lplanes[spin];
+    b.exx_part = part;
+    HIP_TRY(c, launch_exchange_bf16(b, c->stream));
+    c->xs_used[spin] = b.xsplit;
+    return AFQMC_OK;
+  }
   ExchangeArgs a;
   a.nwalkers = nw;
   a.nchol = c->d.nchol;
   a.nbasis = N;
-  a.nocc = c->d.nalpha;
-  a.ghalf = ga;
-  a.rchol = c->s.rchola;
-  a.xsplit = c->xs_a;
-  a.exx_part = c->exx_a;
-  {
-    PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
-    HIP_TRY(c, launch_exchange(a, c->stream));
-  }
-  if (two_spins(c)) {
-    a.nocc = c->d.nbeta;
-    a.ghalf = gb;
-    a.rchol = c->s.rcholb;
-    a.xsplit = c->xs_b;
-    a.exx_part = c->exx_b;
-    PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
-    HIP_TRY(c, launch_exchange(a, c->stream));
-  }
+  a.nocc = n;
+  a.ghalf = g;
+  a.rchol = rchol;
+  a.xsplit = spin == 0 ? c->xs_a : c->xs_b;
+  a.exx_part = part;
+  PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
+  HIP_TRY(c, launch_exchange(a, c->stream));
+  c->xs_used[spin] = a.xsplit;
+  return AFQMC_OK;
+}
+
+int do_exchange(afqmc_ctx* c, int nw, const double* ga, const double* gb) {
+  int r = exchange_spin(c, nw, 0, ga);
+  if (r) return r;
+  if (two_spins(c)) return exchange_spin(c, nw, 1, gb);
   return AFQMC_OK;
 }
 
@@ -379,9 +426,9 @@ int do_energy(afqmc_ctx* c, int nw, const double* ga, const double* gb, double* 
   a.rH1b = c->s.rH1b;
   a.ecoul = wv(c, WV_ECOUL);
   a.exx_part_a = c->exx_a;
-  a.xs_a = c->xs_a;
+  a.xs_a = c->xs_used[0];
   a.exx_part_b = two_spins(c) ? c->exx_b : nullptr;
-  a.xs_b = c->xs_b;
+  a.xs_b = c->xs_used[1];
   a.ecore = c->s.ecore;
   a.energy = energy;
   PhaseScope ps(c, AFQMC_PHASE_ENERGY);
@@ -415,7 +462,7 @@ int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims) {
   if (d.nbasis < 1 || d.nchol < 0 || d.nalpha < 1 || d.nbeta < 0 || d.nwalkers < 1 || d.exp_nmax < 0)
     return AFQMC_ERR_SHAPE;
   if (d.nalpha > 64 || d.nbeta > 64 || d.nalpha > d.nbasis || d.nbeta > d.nbasis) return AFQMC_ERR_SHAPE;
-  if (d.precision != AFQMC_PREC_FP64) return AFQMC_ERR_UNSUPPORTED;
+  if (d.precision != AFQMC_PREC_FP64 && d.precision != AFQMC_PREC_MIXED) return AFQMC_ERR_UNSUPPORTED;
   if (hipSetDevice(device) != hipSuccess) return AFQMC_ERR_HIP;
   afqmc_ctx* c = new afqmc_ctx();
   c->d = d;
@@ -437,6 +484,19 @@ int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims) {
   c->xs_b = two ? exchange_xsplit(d.nwalkers, (int)X, d.nbeta) : 1;
   snprintf(c->describe_dims, sizeof(c->describe_dims), "N=%d X=%d na=%d nb=%d W=%d", d.nbasis, d.nchol, d.nalpha,
            d.nbeta, d.nwalkers);
+  c->kpad = (d.nbasis + 31) / 32 * 32;
+  if (d.precision == AFQMC_PREC_MIXED) {
+    for (int s = 0; s < (two ? 2 : 1); ++s) {
+      const int n = s == 0 ? d.nalpha : d.nbeta;
+      c->bx_ok[s] = bf16_exchange_plan(n, &c->bx_wb[s], &c->bx_xb[s], &c->bx_rt[s], &c->bx_ct[s]);
+      if (!c->bx_ok[s]) continue;
+      const int64_t nwb = ceil_div(W, c->bx_wb[s]), nxb = ceil_div(X, c->bx_xb[s]);
+      int64_t xs = ceil_div(ceil_div(2048, nwb), 8) * 8;
+      if (xs < 8) xs = 8;
+      if (xs > nxb) xs = nxb;
+      c->bx_xs[s] = (int)xs;
+    }
+  }
   if (dmalloc((void**)&c->flags, sizeof(int)) != hipSuccess ||
       dmalloc((void**)&c->nfb, sizeof(unsigned long long)) != hipSuccess) {
     afqmc_destroy(c);
@@ -457,7 +517,8 @@ void afqmc_destroy(afqmc_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   else (void)hipDeviceSynchronize();
   void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
-                  c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb};
+                  c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb, c->gplanes, c->lplanes[0],
+                  c->lplanes[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
@@ -483,6 +544,7 @@ int afqmc_set_static(afqmc_ctx* c, const afqmc_static* s) {
   if (c->d.nbeta > 0 && !s->psib && two) return fail(c, AFQMC_ERR_INVALID_ARG, "null psib");
   c->s = *s;
   c->have_static = true;
+  c->lplanes_ready = false;
   return AFQMC_OK;
 }
 

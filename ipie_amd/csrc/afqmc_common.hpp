@@ -7,6 +7,9 @@
 //   B operand (4 x 16):  lane l supplies B[k(l / 16)][l % 16]
 //   accumulator (16x16): lane l holds D[4 * r + l / 16][l % 16], r = 0..3
 //   (register r of lane group q holds row 4r + q; measured on gfx950)
+// The FP32-output instructions (v_mfma_f32_16x16x4f32, _16x16x32_bf16) hold
+// D[4 * (l / 16) + r][l % 16] instead; the bf16 one takes 8 k values per lane:
+// A[l % 16][8 (l / 16) + j], B[8 (l / 16) + j][l % 16], j = 0..7.
 // The k index of a lane group only has to agree between A and B, so loaders
 // give lane group q the contiguous k range [4q, 4q + 4) of a 16-wide K chunk
 // and issue the four MFMAs of that chunk with s = 0..3 (k = 4q + s).

@@ -19,15 +19,16 @@ class SingularOverlapError(numpy.linalg.LinAlgError):
 
 
 class Engine:
-    def __init__(self, trial, walkers, nchol=0, exp_nmax=6):
+    def __init__(self, trial, walkers, nchol=0, exp_nmax=6, precision="fp64"):
         self.trial = trial
+        self.precision = precision
         self.nchol = nchol
         self.exp_nmax = exp_nmax
         self.device = walkers.device
         self.rhf = bool(walkers.rhf)
         self.ctx = _lib.Context(
             walkers.nbasis, nchol, walkers.nup, walkers.ndown, walkers.nwalkers, exp_nmax=exp_nmax, rhf=walkers.rhf,
-            device=walkers.device,
+            device=walkers.device, precision=precision,
         )
         self.static = {}
         self.check_singular = True

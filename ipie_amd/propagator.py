@@ -11,6 +11,10 @@ PropagatorTimer fields, ``nfb_trig``, ``dt``, ``sqrt_dt``, ``ebound``,
 -> force bias -> fields -> VHS -> Taylor exp -> exp(-dt/2 H1) -> overlap ->
 weights, all enqueued on the current stream with no host synchronisation.
 
+``precision``: "fp64" (every contraction in FP64) or "mixed" (the exchange of
+the local energy as split-BF16 products with FP32 accumulation; Green's
+function, overlaps, weights, propagation and QR stay FP64).
+
 Auxiliary fields: ``xi_source`` (callable ``(nwalkers, nfields) -> [W, X]``
 float64, numpy or torch) injects them exactly like ipie's
 ``xp.random.normal(0, 1, X * W).reshape(W, X)`` (phaseless_base.py:296);
@@ -54,8 +58,9 @@ def construct_one_body_propagator(hamiltonian, mf_shift, dt):
 
 
 class PhaselessGeneric:
-    def __init__(self, time_step, exp_nmax=6, verbose=False, seed=7):
+    def __init__(self, time_step, exp_nmax=6, verbose=False, seed=7, precision="fp64"):
         self.dt = time_step
+        self.precision = precision
         self.verbose = verbose
         self.sqrt_dt = self.dt**0.5
         self.isqrt_dt = 1j * self.sqrt_dt
@@ -83,7 +88,8 @@ class PhaselessGeneric:
             trial.half_rotate(hamiltonian)
         self.mf_shift = construct_mean_field_shift(hamiltonian, trial)
         self.expH1 = construct_one_body_propagator(hamiltonian, self.mf_shift, self.dt)
-        eng = _engine.Engine(trial, walkers, nchol=hamiltonian.nchol, exp_nmax=self.exp_nmax)
+        eng = _engine.Engine(trial, walkers, nchol=hamiltonian.nchol, exp_nmax=self.exp_nmax,
+                             precision=self.precision)
         eng.set_hamiltonian(hamiltonian, self.expH1, self.mf_shift, self.dt)
         walkers._engine = eng
         self._engine = eng
