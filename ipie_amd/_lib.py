@@ -88,6 +88,8 @@ SIGNATURES = {
     "afqmc_reortho": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _dp, _dp, _dp]),
     "afqmc_draw_normals": (ctypes.c_int, [_dp, _dp, _i64, _u64, _u64]),
     "afqmc_profile": (ctypes.c_int, [_dp, ctypes.c_int]),
+    "afqmc_surrogate_configure": (ctypes.c_int, [_dp, ctypes.c_int, _f64, ctypes.c_int, ctypes.c_int, _f64, _u64]),
+    "afqmc_surrogate_stats": (ctypes.c_int, [_dp, ctypes.POINTER(_f64), ctypes.c_int]),
     "afqmc_profile_read": (ctypes.c_int, [_dp, ctypes.POINTER(_f64), ctypes.POINTER(_i64)]),
 }
 
@@ -326,6 +328,18 @@ class Context:
         if st != AFQMC_OK:
             raise AfqmcError(f"afqmc_get_nfb: {STATUS_NAMES.get(st, st)}")
         return int(v.value)
+
+    def surrogate_configure(self, enable=True, fraction=0.125, warmup=100, train_iters=4, lr=1e-3, seed=11):
+        self._call("afqmc_surrogate_configure", int(bool(enable)), float(fraction), int(warmup), int(train_iters),
+                   float(lr), int(seed))
+
+    def surrogate_stats(self):
+        out = (_f64 * 6)()
+        st = self.lib.afqmc_surrogate_stats(self.h, out, 6)
+        if st != AFQMC_OK:
+            raise AfqmcError(f"afqmc_surrogate_stats: {self.lib.afqmc_last_error(self.h)}")
+        keys = ["steps", "warmup", "last_k", "train_loss", "resid2", "target_var"]
+        return dict(zip(keys, [float(v) for v in out]))
 
     def profile(self, enable=True):
         st = self.lib.afqmc_profile(self.h, int(bool(enable)))

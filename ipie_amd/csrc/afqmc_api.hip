@@ -6,7 +6,10 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <cstring>
 
+#include <cmath>
+#include <random>
 #include <vector>
 
 #include "afqmc_internal.hpp"
@@ -51,6 +54,35 @@ struct afqmc_ctx {
   uint16_t* lplanes[2] = {nullptr, nullptr};  // 2 x [X * n_s, kpad] per spin
   bool lplanes_ready = false;
   int xs_used[2] = {1, 1};  // partial-sum count of the last exchange per spin
+  int64_t exx_cap = 0;      // capacity (complex entries) of each exchange partial buffer
+  double* gcomp = nullptr;  // [W, nmax, N] complex: Ghalf of sampled walkers (FP64 exchange)
+  // learned exchange control variate
+  struct {
+    bool enabled = false;
+    double fraction = 0.125;
+    int warmup = 100, train_iters = 4, online_iters = 1;
+    float lr = 1e-3f;
+    int64_t steps = 0;
+    int adam_step = 0;
+    int last_k = 0;
+    std::mt19937_64 rng;
+    float* feat = nullptr;
+    double* pred = nullptr;
+    double* est = nullptr;
+    double* target = nullptr;
+    double* resid = nullptr;
+    int* walker_of = nullptr;
+    double* xbar = nullptr;  // [2, W]: mean, max
+    double* e1 = nullptr;    // [W] complex
+    float* master = nullptr;
+    float* m = nullptr;
+    float* v = nullptr;
+    uint16_t* wbf = nullptr;
+    float* bias = nullptr;
+    float* grad = nullptr;
+    float* loss = nullptr;
+    SurrogateState* state = nullptr;
+  } sur;
   // per-phase HIP-event timing (afqmc_profile / afqmc_profile_read)
   bool prof = false;
   std::vector<hipEvent_t> ev_free;
@@ -139,10 +171,11 @@ int ensure_ws(afqmc_ctx* c) {
   ok = ok && dmalloc((void**)&c->vbias_part, 16 * (size_t)c->fb_ksplit * W * X) == hipSuccess;
   ok = ok && dmalloc((void**)&c->xshifted, 16 * (size_t)W * X) == hipSuccess;
   ok = ok && dmalloc((void**)&c->wvec, 16 * (size_t)6 * W) == hipSuccess;
-  const int xa = c->xs_a > c->bx_xs[0] ? c->xs_a : c->bx_xs[0];
-  const int xbm = c->xs_b > c->bx_xs[1] ? c->xs_b : c->bx_xs[1];
-  ok = ok && dmalloc((void**)&c->exx_a, 16 * (size_t)xa * W) == hipSuccess;
-  ok = ok && dmalloc((void**)&c->exx_b, 16 * (size_t)xbm * W) == hipSuccess;
+  // partial sums: at most one per Cholesky vector per walker, whatever the split
+  c->exx_cap = (int64_t)X * W;
+  ok = ok && dmalloc((void**)&c->exx_a, 16 * (size_t)c->exx_cap) == hipSuccess;
+  ok = ok && dmalloc((void**)&c->exx_b, 16 * (size_t)c->exx_cap) == hipSuccess;
+  ok = ok && dmalloc((void**)&c->gcomp, 16 * (size_t)W * nmax * N) == hipSuccess;
   if (c->bx_ok[0] || c->bx_ok[1]) {
     ok = ok && dmalloc((void**)&c->gplanes, 2 * (size_t)4 * W * nmax * c->kpad) == hipSuccess;
     for (int s = 0; s < 2; ++s)
@@ -244,6 +277,8 @@ int do_fields(afqmc_ctx* c, int nw, bool want_fields, const double* xi, uint64_t
   a.vbias_out = vbias_out;
   a.nfb = c->nfb;
   a.want_fields = want_fields ? 1 : 0;
+  a.xbar_mean = c->sur.enabled ? c->sur.xbar : nullptr;
+  a.xbar_max = c->sur.enabled ? c->sur.xbar + c->d.nwalkers : nullptr;
   PhaseScope ps(c, AFQMC_PHASE_FIELDS);
   HIP_TRY(c, launch_fields(a, c->stream));
   return AFQMC_OK;
@@ -357,7 +392,9 @@ int taylor(afqmc_ctx* c, int nw, const double** result) {
   return AFQMC_OK;
 }
 
-int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g) {
+// Exchange partial sums of nw walkers for one spin.  With walker_of, the nw
+// walkers are the sampled ones (walker_of[t], t < nw) of the Ghalf array g.
+int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g, const int* walker_of) {
   const int N = c->d.nbasis;
   const int n = spin == 0 ? c->d.nalpha : c->d.nbeta;
   const double* rchol = spin == 0 ? c->s.rchola : c->s.rcholb;
@@ -373,7 +410,7 @@ int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g) {
       c->lplanes_ready = true;
     }
     PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
-    HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes, c->stream));
+    HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes, c->stream, walker_of, n));
     ExchangeBf16Args b;
     b.nwalkers = nw;
     b.nchol = c->d.nchol;
@@ -383,13 +420,23 @@ int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g) {
     b.xb = c->bx_xb[spin];
     b.rt_ = c->bx_rt[spin];
     b.ct_ = c->bx_ct[spin];
-    b.xsplit = c->bx_xs[spin];
+    const int64_t nwb = ceil_div(nw, b.wb), nxb = ceil_div(c->d.nchol, b.xb);
+    int64_t xs = ceil_div(ceil_div(2048, nwb), 8) * 8;
+    if (xs > nxb) xs = nxb;
+    if (xs < 1) xs = 1;
+    b.xsplit = (int)xs;
     b.gplanes = c->gplanes;
     b.lplanes = c->lplanes[spin];
     b.exx_part = part;
+    if ((int64_t)b.xsplit * nw > c->exx_cap) return fail(c, AFQMC_ERR_SHAPE, "exchange partial buffer too small");
     HIP_TRY(c, launch_exchange_bf16(b, c->stream));
     c->xs_used[spin] = b.xsplit;
     return AFQMC_OK;
+  }
+  PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
+  if (walker_of) {
+    HIP_TRY(c, launch_sur_gather(g, walker_of, nw, (int64_t)n * N, c->gcomp, c->stream));
+    g = c->gcomp;
   }
   ExchangeArgs a;
   a.nwalkers = nw;
@@ -398,22 +445,23 @@ int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g) {
   a.nocc = n;
   a.ghalf = g;
   a.rchol = rchol;
-  a.xsplit = spin == 0 ? c->xs_a : c->xs_b;
+  a.xsplit = exchange_xsplit(nw, c->d.nchol, n);
   a.exx_part = part;
-  PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
+  if ((int64_t)a.xsplit * nw > c->exx_cap) return fail(c, AFQMC_ERR_SHAPE, "exchange partial buffer too small");
   HIP_TRY(c, launch_exchange(a, c->stream));
   c->xs_used[spin] = a.xsplit;
   return AFQMC_OK;
 }
 
-int do_exchange(afqmc_ctx* c, int nw, const double* ga, const double* gb) {
-  int r = exchange_spin(c, nw, 0, ga);
+int do_exchange(afqmc_ctx* c, int nw, const double* ga, const double* gb, const int* walker_of = nullptr) {
+  int r = exchange_spin(c, nw, 0, ga, walker_of);
   if (r) return r;
-  if (two_spins(c)) return exchange_spin(c, nw, 1, gb);
+  if (two_spins(c)) return exchange_spin(c, nw, 1, gb, walker_of);
   return AFQMC_OK;
 }
 
-int do_energy(afqmc_ctx* c, int nw, const double* ga, const double* gb, double* energy) {
+int do_energy(afqmc_ctx* c, int nw, const double* ga, const double* gb, double* energy,
+              const double* exx_est = nullptr, double* e1_out = nullptr) {
   EnergyArgs a;
   a.nwalkers = nw;
   a.nbasis = c->d.nbasis;
@@ -431,21 +479,114 @@ int do_energy(afqmc_ctx* c, int nw, const double* ga, const double* gb, double* 
   a.xs_b = c->xs_used[1];
   a.ecore = c->s.ecore;
   a.energy = energy;
+  a.exx_est = exx_est;
+  a.e1_out = e1_out;
   PhaseScope ps(c, AFQMC_PHASE_ENERGY);
   HIP_TRY(c, launch_energy(a, c->stream));
   return AFQMC_OK;
 }
 
+int64_t gcd64(int64_t a, int64_t b) {
+  while (b) {
+    const int64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+int64_t inv_mod(int64_t a, int64_t m) {
+  int64_t g = m, x = 0, x1 = 1, a1 = a % m;
+  while (a1) {
+    const int64_t q = g / a1;
+    int64_t t = g - q * a1;
+    g = a1;
+    a1 = t;
+    t = x - q * x1;
+    x = x1;
+    x1 = t;
+  }
+  x %= m;
+  return x < 0 ? x + m : x;
+}
+
+// Local energy of nw walkers from their Ghalf with the learned exchange control
+// variate (requires the force bias + fields of this Green's function).
+int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb, const double* ovlp,
+                        double* energy) {
+  auto& S = c->sur;
+  const bool warm = S.steps < S.warmup;
+  // one-body energy for the features (the energy output is rewritten below)
+  int r = do_energy(c, nw, ga, gb, energy, S.pred, S.e1);
+  if (r) return r;
+  SurrogateFeatureArgs fa{nw, S.e1, wv(c, WV_ECOUL), S.xbar, S.xbar + c->d.nwalkers, ovlp, S.feat};
+  HIP_TRY(c, launch_sur_features(fa, c->stream));
+  if (S.steps > 0) {
+    SurrogateForwardArgs f{nw, S.feat, S.wbf, S.bias, S.state, S.pred};
+    HIP_TRY(c, launch_sur_forward(f, c->stream));
+  } else {
+    HIP_TRY(c, hipMemsetAsync(S.pred, 0, 16 * (size_t)nw, c->stream));
+  }
+  int k = nw;
+  int64_t pa = 1, pb = 0;
+  if (!warm) {
+    k = (int)std::lround(S.fraction * nw);
+    if (k < 1) k = 1;
+    if (k > nw) k = nw;
+    std::uniform_int_distribution<int64_t> da(1, nw > 1 ? nw - 1 : 1), db(0, nw - 1);
+    do {
+      pa = da(S.rng);
+    } while (gcd64(pa, nw) != 1);
+    pb = db(S.rng);
+  }
+  S.last_k = k;
+  HIP_TRY(c, launch_sur_select(nw, k, inv_mod(pa, nw), pb, S.walker_of, c->stream));
+  r = do_exchange(c, k, ga, gb, S.walker_of);
+  if (r) return r;
+  SurrogateCombineArgs cb;
+  cb.nwalkers = nw;
+  cb.k = k;
+  cb.perm_a = pa;
+  cb.perm_b = pb;
+  cb.pred = S.pred;
+  cb.part_a = c->exx_a;
+  cb.xs_a = c->xs_used[0];
+  cb.part_b = two_spins(c) ? c->exx_b : nullptr;
+  cb.xs_b = c->xs_used[1];
+  cb.est = S.est;
+  cb.target = S.target;
+  cb.resid = S.resid;
+  HIP_TRY(c, launch_sur_combine(cb, c->stream));
+  const int iters = warm ? S.train_iters : S.online_iters;
+  if (warm) {
+    SurrogateStatsArgs sa{k, S.feat, S.walker_of, S.target, S.state};
+    HIP_TRY(c, launch_sur_stats(sa, c->stream));
+  }
+  for (int it = 0; it < iters; ++it) {
+    SurrogateTrainArgs ta{k, S.feat, S.walker_of, S.target, S.master, S.state, S.grad, S.loss};
+    HIP_TRY(c, launch_sur_grad(ta, c->stream));
+    SurrogateAdamArgs ad{sur_grad_groups(k), ++S.adam_step, S.lr, S.grad, S.master, S.m, S.v, S.wbf, S.bias};
+    HIP_TRY(c, launch_sur_adam(ad, c->stream));
+  }
+  S.steps += 1;
+  return do_energy(c, nw, ga, gb, energy, S.est, nullptr);
+}
+
 int do_two_body_core(afqmc_ctx* c, int nw, const double* ga, const double* gb, const double* xi, uint64_t seed,
-                     uint64_t counter, double* xi_out, double* vbias_out, bool energy_too, double* energy) {
+                     uint64_t counter, double* xi_out, double* vbias_out, bool energy_too, double* energy,
+                     const double* ovlp = nullptr) {
   int r = do_force_bias(c, nw, ga, gb);
   if (r) return r;
   r = do_fields(c, nw, true, xi, seed, counter, xi_out, vbias_out);
   if (r) return r;
   if (energy_too) {
-    r = do_exchange(c, nw, ga, gb);
-    if (r) return r;
-    r = do_energy(c, nw, ga, gb, energy);
+    if (c->sur.enabled && ovlp) {
+      r = do_energy_surrogate(c, nw, ga, gb, ovlp, energy);
+    } else {
+      r = do_exchange(c, nw, ga, gb);
+      if (r) return r;
+      r = do_energy(c, nw, ga, gb, energy);
+    }
     if (r) return r;
   }
   return do_vhs(c, nw);
@@ -518,7 +659,9 @@ void afqmc_destroy(afqmc_ctx* c) {
   else (void)hipDeviceSynchronize();
   void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
                   c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb, c->gplanes, c->lplanes[0],
-                  c->lplanes[1]};
+                  c->lplanes[1], c->gcomp, c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
+                  c->sur.resid, c->sur.walker_of, c->sur.xbar, c->sur.e1, c->sur.master, c->sur.m,
+                  c->sur.v, c->sur.wbf, c->sur.bias, c->sur.grad, c->sur.loss, c->sur.state};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
@@ -674,7 +817,8 @@ int afqmc_propagate(afqmc_ctx* c, int32_t nw, double* phia, double* phib, double
   if (r) return r;
   r = gather(c, nw, phia, phib, true);
   if (r) return r;
-  r = do_two_body_core(c, nw, ga, gb, xi, seed, counter, nullptr, nullptr, energy_prev != nullptr, energy_prev);
+  r = do_two_body_core(c, nw, ga, gb, xi, seed, counter, nullptr, nullptr, energy_prev != nullptr, energy_prev,
+                       wv(c, WV_OVLP_OLD));
   if (r) return r;
   const double* res = nullptr;
   r = taylor(c, nw, &res);
@@ -759,6 +903,100 @@ int afqmc_profile_read(afqmc_ctx* c, double* ms, int64_t* launches) {
     ms[i] = c->prof_ms[i];
     launches[i] = c->prof_count[i];
   }
+  return AFQMC_OK;
+}
+
+int afqmc_surrogate_configure(afqmc_ctx* c, int enable, double fraction, int warmup_steps, int train_iters,
+                              double lr, uint64_t seed) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (!enable) {
+    c->sur.enabled = false;
+    return AFQMC_OK;
+  }
+  if (!(fraction > 0.0 && fraction <= 1.0) || warmup_steps < 0 || train_iters < 0 || !(lr > 0.0))
+    return fail(c, AFQMC_ERR_INVALID_ARG, "bad surrogate parameters");
+  int r = check_ready(c, 1);
+  if (r) return r;
+  r = ensure_ws(c);
+  if (r) return r;
+  auto& S = c->sur;
+  const int64_t W = c->d.nwalkers;
+  const int groups = sur_grad_groups((int)W);
+  if (!S.master) {
+    bool ok = true;
+    ok = ok && dmalloc((void**)&S.feat, sizeof(float) * W * SUR_FEATURES) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.pred, 16 * W) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.est, 16 * W) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.target, 16 * W) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.resid, 8 * W) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.walker_of, sizeof(int) * W) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.xbar, 16 * W) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.e1, 16 * W) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.master, sizeof(float) * SUR_NPARAM) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.m, sizeof(float) * SUR_NPARAM) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.v, sizeof(float) * SUR_NPARAM) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.wbf, sizeof(uint16_t) * SUR_NPARAM) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.bias, sizeof(float) * (2 * SUR_HIDDEN + 2)) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.grad, sizeof(float) * (size_t)groups * SUR_NPARAM) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.loss, sizeof(float) * (size_t)groups) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.state, sizeof(SurrogateState)) == hipSuccess;
+    if (!ok) return fail(c, AFQMC_ERR_ALLOC, "surrogate allocation failed");
+  }
+  S.enabled = true;
+  S.fraction = fraction;
+  S.warmup = warmup_steps;
+  S.train_iters = train_iters;
+  S.lr = (float)lr;
+  S.steps = 0;
+  S.adam_step = 0;
+  S.rng.seed(seed);
+  // weights ~ N(0, 1/fan_in), biases 0 (host RNG, then upload)
+  std::vector<float> P(SUR_NPARAM, 0.0f);
+  std::normal_distribution<float> nd(0.0f, 1.0f);
+  const int H = SUR_HIDDEN, F = SUR_FEATURES;
+  for (int i = 0; i < H * F; ++i) P[i] = nd(S.rng) / std::sqrt((float)F);
+  for (int i = 0; i < H * H; ++i) P[H * F + i] = nd(S.rng) / std::sqrt((float)H);
+  for (int i = 0; i < 2 * H; ++i) P[H * F + H * H + i] = nd(S.rng) / std::sqrt((float)H);
+  std::vector<uint16_t> Pb(SUR_NPARAM, 0);
+  for (int i = 0; i < SUR_NPARAM; ++i) {
+    uint32_t u;
+    std::memcpy(&u, &P[i], 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    Pb[i] = (uint16_t)(u >> 16);
+  }
+  HIP_TRY(c, hipMemcpyAsync(S.master, P.data(), sizeof(float) * SUR_NPARAM, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(S.wbf, Pb.data(), sizeof(uint16_t) * SUR_NPARAM, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemsetAsync(S.bias, 0, sizeof(float) * (2 * SUR_HIDDEN + 2), c->stream));
+  HIP_TRY(c, hipMemsetAsync(S.m, 0, sizeof(float) * SUR_NPARAM, c->stream));
+  HIP_TRY(c, hipMemsetAsync(S.v, 0, sizeof(float) * SUR_NPARAM, c->stream));
+  HIP_TRY(c, hipMemsetAsync(S.state, 0, sizeof(SurrogateState), c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_surrogate_stats(afqmc_ctx* c, double* out, int n) {
+  if (!c || !out || n < 6) return AFQMC_ERR_INVALID_ARG;
+  auto& S = c->sur;
+  out[0] = (double)S.steps;
+  out[1] = (double)S.warmup;
+  out[2] = (double)S.last_k;
+  out[3] = out[4] = out[5] = 0.0;
+  if (!S.enabled || S.steps == 0) return AFQMC_OK;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const int groups = sur_grad_groups(S.last_k);
+  std::vector<float> loss(groups);
+  std::vector<double> resid(S.last_k);
+  SurrogateState st;
+  HIP_TRY(c, hipMemcpy(loss.data(), S.loss, sizeof(float) * groups, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(resid.data(), S.resid, 8 * (size_t)S.last_k, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(&st, S.state, sizeof(st), hipMemcpyDeviceToHost));
+  double l = 0.0, rr = 0.0;
+  for (float v : loss) l += v;
+  for (double v : resid) rr += v;
+  out[3] = l / S.last_k;                         // training loss per sample (normalised units)
+  out[4] = rr / S.last_k;                        // mean |exact - prediction|^2 on the sample (Ha^2)
+  const double m = st.count > 0 ? st.sum[SUR_FEATURES] / st.count : 0.0;
+  out[5] = st.count > 0 ? st.sumsq[SUR_FEATURES] / st.count - m * m : 0.0;  // target variance (Re)
   return AFQMC_OK;
 }
 

@@ -95,7 +95,8 @@ struct ExchangeBf16Args {
   double* exx_part;         // [xsplit, W] complex
 };
 bool bf16_exchange_plan(int nocc, int* wb, int* xb, int* rt, int* ct);
-hipError_t launch_split_ghalf(const double* g, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream);
+hipError_t launch_split_ghalf(const double* g, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream,
+                              const int* walker_of = nullptr, int nocc = 1);
 hipError_t launch_split_real(const double* x, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream);
 hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream);
 
@@ -115,6 +116,8 @@ struct FieldsArgs {
   double* vbias_out;          // optional [W, X] complex (summed vbias)
   unsigned long long* nfb;    // number of capped force-bias entries (accumulated)
   int want_fields;            // 0: only vbias sum + Coulomb
+  double* xbar_mean;          // optional [W]: mean |xbar| over x (after the cap)
+  double* xbar_max;           // optional [W]: max |xbar|
 };
 hipError_t launch_fields(const FieldsArgs& a, hipStream_t stream);
 
@@ -145,10 +148,89 @@ struct EnergyArgs {
   int xs_b;
   double ecore;
   double* energy;         // [W, 3] complex
+  const double* exx_est;  // optional [W] complex: exchange sum to use instead of the partials
+  double* e1_out;         // optional [W] complex: one-body energy
 };
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t stream);
 
 hipError_t launch_draw_normals(double* out, int64_t n, uint64_t seed, uint64_t counter, hipStream_t stream);
+
+// ---------------------------------------------------------------- surrogate
+constexpr int SUR_HIDDEN = 128;
+constexpr int SUR_FEATURES = 9;
+constexpr int SUR_NPARAM = SUR_HIDDEN * SUR_FEATURES + SUR_HIDDEN * SUR_HIDDEN + 4 * SUR_HIDDEN + 2;
+
+struct SurrogateState {
+  double sum[SUR_FEATURES + 2], sumsq[SUR_FEATURES + 2], count;
+  float mean[SUR_FEATURES + 2], inv_std[SUR_FEATURES + 2];
+};
+
+struct SurrogateFeatureArgs {
+  int nwalkers;
+  const double* e1;        // [W] complex
+  const double* ecoul;     // [W] complex
+  const double* xbar_mean; // [W]
+  const double* xbar_max;  // [W]
+  const double* ovlp;      // [W] complex
+  float* feat;             // [W][F]
+};
+struct SurrogateStatsArgs {
+  int count;
+  const float* feat;
+  const int* walker_of;    // sample -> walker (null: identity)
+  const double* target;    // [count][2]
+  SurrogateState* state;
+};
+struct SurrogateForwardArgs {
+  int nwalkers;
+  const float* feat;
+  const uint16_t* wbf;     // bf16 weights W1 | W2 | W3
+  const float* bias;       // b1 | b2 | b3
+  const SurrogateState* state;
+  double* pred;            // [W] complex
+};
+struct SurrogateTrainArgs {
+  int count;
+  const float* feat;
+  const int* walker_of;
+  const double* target;    // [count][2]
+  const float* master;     // [NPARAM]
+  const SurrogateState* state;
+  float* grad;             // [groups][NPARAM]
+  float* loss;             // [groups]
+};
+struct SurrogateAdamArgs {
+  int ngroups, step;
+  float lr;
+  const float* grad;
+  float* master;
+  float* m;
+  float* v;
+  uint16_t* wbf;
+  float* bias;
+};
+struct SurrogateCombineArgs {
+  int nwalkers, k;
+  int64_t perm_a, perm_b;  // sample position of walker w: (a w + b) mod W
+  const double* pred;      // [W] complex
+  const double* part_a;    // [xs_a, k] complex
+  int xs_a;
+  const double* part_b;    // [xs_b, k] complex or null
+  int xs_b;
+  double* est;             // [W] complex
+  double* target;          // [k][2] (exact values of the samples) or null
+  double* resid;           // [k] |exact - g|^2 or null
+};
+hipError_t launch_sur_features(const SurrogateFeatureArgs& a, hipStream_t s);
+hipError_t launch_sur_stats(const SurrogateStatsArgs& a, hipStream_t s);
+hipError_t launch_sur_forward(const SurrogateForwardArgs& a, hipStream_t s);
+int sur_grad_groups(int count);
+hipError_t launch_sur_grad(const SurrogateTrainArgs& a, hipStream_t s);
+hipError_t launch_sur_adam(const SurrogateAdamArgs& a, hipStream_t s);
+hipError_t launch_sur_combine(const SurrogateCombineArgs& a, hipStream_t s);
+hipError_t launch_sur_select(int nwalkers, int k, int64_t inv_a, int64_t b, int* walker_of, hipStream_t s);
+hipError_t launch_sur_gather(const double* g, const int* walker_of, int k, int64_t per_walker, double* out,
+                             hipStream_t s);
 
 // ---------------------------------------------------------------- reortho
 struct ReorthoArgs {

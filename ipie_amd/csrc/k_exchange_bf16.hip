@@ -49,16 +49,18 @@ __device__ __forceinline__ v4f mfma_bf16(v8bf a, v8bf b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Ghalf [W*n, N] complex -> 4 bf16 planes [W*n, Kp].
+// Ghalf [W*n, N] complex -> 4 bf16 planes [rows, Kp]; with walker_of, plane
+// row t*n + j takes Ghalf row walker_of[t]*n + j (sampled walkers, compacted).
 __global__ void split_ghalf_kernel(const double* __restrict__ g, int64_t rows, int N, int Kp,
-                                   uint16_t* __restrict__ planes) {
+                                   uint16_t* __restrict__ planes, const int* __restrict__ walker_of, int nocc) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= rows * Kp) return;
   const int64_t r = idx / Kp;
   const int k = (int)(idx % Kp);
   uint16_t rh = 0, rl = 0, ih = 0, il = 0;
   if (k < N) {
-    const cplx v = ld(g, r * N + k);
+    const int64_t src = walker_of ? (int64_t)walker_of[r / nocc] * nocc + r % nocc : r;
+    const cplx v = ld(g, src * N + k);
     split_bf16(v.re, rh, rl);
     split_bf16(v.im, ih, il);
   }
@@ -257,7 +259,8 @@ hipError_t launch_bf16(const ExchangeBf16Args& a, hipStream_t stream) {
 }  // namespace
 
 // Geometry for the split-BF16 exchange: wb walkers x n rows and xb vectors x n
-// columns padded to an even number (<= 10) of 16-row tiles; wb a power of two.
+// columns padded to an even number (<= 10) of 16-row tiles (every even pair
+// is instantiated); wb a power of two.
 bool bf16_exchange_plan(int nocc, int* wb, int* xb, int* rt, int* ct) {
   double best_score = -1.0, best_eff = 0.0;
   for (int w = 1; w <= 8; w *= 2) {
@@ -283,10 +286,11 @@ bool bf16_exchange_plan(int nocc, int* wb, int* xb, int* rt, int* ct) {
   return best_score > 0.0 && best_eff > 0.7;
 }
 
-hipError_t launch_split_ghalf(const double* g, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream) {
+hipError_t launch_split_ghalf(const double* g, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream,
+                              const int* walker_of, int nocc) {
   const int64_t total = rows * Kp;
   hipLaunchKernelGGL(split_ghalf_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream, g, rows, N, Kp,
-                     planes);
+                     planes, walker_of, nocc);
   return hipGetLastError();
 }
 
@@ -300,8 +304,9 @@ hipError_t launch_split_real(const double* x, int64_t rows, int N, int Kp, uint1
 hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream) {
 #define AFQMC_BX(rt, ct) \
   if (a.rt_ == rt && a.ct_ == ct) return launch_bf16<rt, ct>(a, stream);
-  AFQMC_BX(10, 10) AFQMC_BX(8, 8) AFQMC_BX(8, 10) AFQMC_BX(10, 8) AFQMC_BX(6, 10) AFQMC_BX(10, 6)
-  AFQMC_BX(6, 6) AFQMC_BX(4, 10) AFQMC_BX(4, 8) AFQMC_BX(2, 10) AFQMC_BX(6, 8) AFQMC_BX(8, 6)
+#define AFQMC_BX_ROW(rt) AFQMC_BX(rt, 2) AFQMC_BX(rt, 4) AFQMC_BX(rt, 6) AFQMC_BX(rt, 8) AFQMC_BX(rt, 10)
+  AFQMC_BX_ROW(2) AFQMC_BX_ROW(4) AFQMC_BX_ROW(6) AFQMC_BX_ROW(8) AFQMC_BX_ROW(10)
+#undef AFQMC_BX_ROW
 #undef AFQMC_BX
   return hipErrorInvalidValue;
 }

@@ -55,6 +55,7 @@ __global__ void __launch_bounds__(FLD_THREADS) fields_kernel(FieldsArgs a) {
   const int X = a.nchol;
   const int64_t plane = (int64_t)a.nwalkers * X;
   double cmf_r = 0, cmf_i = 0, cfb_r = 0, cfb_i = 0, ec_r = 0, ec_i = 0;
+  double xb_sum = 0.0, xb_max = 0.0;
   unsigned long long ncap = 0;
   for (int x = threadIdx.x; x < X; x += FLD_THREADS) {
     const int64_t idx = (int64_t)w * X + x;
@@ -63,7 +64,6 @@ __global__ void __launch_bounds__(FLD_THREADS) fields_kernel(FieldsArgs a) {
     if (a.vbias_out) st(a.vbias_out, idx, vb);
     ec_r += vb.re * vb.re - vb.im * vb.im;
     ec_i += 2.0 * vb.re * vb.im;
-    if (!a.want_fields) continue;
     const cplx mf = ld(a.mf_shift, x);
     // xbar = -sqrt(dt) (i vbias - mf_shift)
     cplx xb = cmake(a.sqrt_dt * (mf.re + vb.im), a.sqrt_dt * (mf.im - vb.re));
@@ -72,6 +72,10 @@ __global__ void __launch_bounds__(FLD_THREADS) fields_kernel(FieldsArgs a) {
       ++ncap;
       if (axb > 1e-13) xb = cmake(xb.re / axb, xb.im / axb);
     }
+    const double axc = axb > a.fbbound ? 1.0 : axb;
+    xb_sum += axc;
+    xb_max = fmax(xb_max, axc);
+    if (!a.want_fields) continue;
     const double xi = a.xi ? a.xi[idx] : draw_normal(a.seed, a.counter, (uint64_t)idx);
     if (a.xi_out) a.xi_out[idx] = xi;
     const cplx xs = cmake(xi - xb.re, -xb.im);
@@ -83,12 +87,26 @@ __global__ void __launch_bounds__(FLD_THREADS) fields_kernel(FieldsArgs a) {
   }
   ec_r = block_sum(ec_r, red);
   ec_i = block_sum(ec_i, red);
+  if (a.xbar_mean) {
+    xb_sum = block_sum(xb_sum, red);
+    // block max: wave shuffles then the per-wave maxima in order
+    for (int off = 32; off > 0; off >>= 1) xb_max = fmax(xb_max, __shfl_xor(xb_max, off, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = xb_max;
+    __syncthreads();
+    double m = 0.0;
+    for (int i = 0; i < FLD_THREADS / 64; ++i) m = fmax(m, red[i]);
+    if (threadIdx.x == 0) {
+      a.xbar_mean[w] = xb_sum / X;
+      a.xbar_max[w] = m;
+    }
+  }
   if (a.want_fields) {
     cmf_r = block_sum(cmf_r, red);
     cmf_i = block_sum(cmf_i, red);
     cfb_r = block_sum(cfb_r, red);
     cfb_i = block_sum(cfb_i, red);
-    if (ncap) atomicAdd(a.nfb, ncap);
+    if (ncap && a.nfb) atomicAdd(a.nfb, ncap);
   }
   if (threadIdx.x == 0) {
     if (a.ecoul) st(a.ecoul, w, cmake(0.5 * ec_r, 0.5 * ec_i));
@@ -145,10 +163,15 @@ __global__ void __launch_bounds__(FLD_THREADS) energy_kernel(EnergyArgs a) {
   if (threadIdx.x == 0) {
     const double f = a.rhf ? 2.0 : 1.0;
     const cplx e1 = cmake(f * er + a.ecore, f * ei);
+    if (a.e1_out) st(a.e1_out, w, e1);
     cplx ex = cmake(0.0, 0.0);
-    for (int z = 0; z < a.xs_a; ++z) ex = cadd(ex, ld(a.exx_part_a, (int64_t)z * a.nwalkers + w));
-    if (a.exx_part_b)
-      for (int z = 0; z < a.xs_b; ++z) ex = cadd(ex, ld(a.exx_part_b, (int64_t)z * a.nwalkers + w));
+    if (a.exx_est) {
+      ex = ld(a.exx_est, w);
+    } else {
+      for (int z = 0; z < a.xs_a; ++z) ex = cadd(ex, ld(a.exx_part_a, (int64_t)z * a.nwalkers + w));
+      if (a.exx_part_b)
+        for (int z = 0; z < a.xs_b; ++z) ex = cadd(ex, ld(a.exx_part_b, (int64_t)z * a.nwalkers + w));
+    }
     // UHF: exx = 1/2 (sum_a + sum_b); RHF: exx = 2 * 1/2 * sum_a
     const double fx = a.rhf ? 1.0 : 0.5;
     ex = cscale(ex, fx);

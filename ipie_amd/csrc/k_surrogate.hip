@@ -1,0 +1,363 @@
+// Learned control variate for the exchange energy.
+//
+// Per walker w, cheap exact features f_w (one-body energy, Coulomb energy,
+// mean and max |xbar|, log|overlap|, overlap phase) feed a small MLP
+// g(f) = W3 silu(W2 silu(W1 f + b1) + b2) + b3 (2 hidden layers of H units,
+// BF16 weights for inference, FP32 accumulation) that predicts the walker's
+// exchange sum E_x(w) = sum_x tr(T_x T_x) (complex, both spins).  The exact
+// exchange is evaluated on a subset S of k walkers in which every walker has
+// inclusion probability k / W, and the estimate
+//     E_x_hat(w) = g(w) + [w in S] (W / k) (E_x(w) - g(w))
+// is unbiased for every walker whatever g is (Horvitz-Thompson).  S is the
+// image of [0, k) under w = a^{-1} (t - b) mod W for a random a coprime to W
+// and random b, drawn afresh every step.
+//
+// Training: Adam on the mean squared error of normalised targets, on exact
+// samples only (all walkers during warm-up, the subset afterwards); inputs and
+// targets are normalised with running statistics accumulated during warm-up.
+// Every reduction has a fixed order, so a run is reproducible.
+#include "afqmc_internal.hpp"
+
+namespace afqmc {
+
+namespace {
+
+constexpr int SG_H = SUR_HIDDEN;
+constexpr int SG_F = SUR_FEATURES;
+constexpr int SG_SAMPLES_PER_WG = 32;
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+__device__ __forceinline__ uint16_t f2bf_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+// raw features [W][F]
+__global__ void sur_features_kernel(SurrogateFeatureArgs a) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= a.nwalkers) return;
+  const cplx e1 = ld(a.e1, w), ej = ld(a.ecoul, w), o = ld(a.ovlp, w);
+  const double ao = cabs(o);
+  float* f = a.feat + (int64_t)w * SG_F;
+  f[0] = (float)e1.re;
+  f[1] = (float)e1.im;
+  f[2] = (float)ej.re;
+  f[3] = (float)ej.im;
+  f[4] = (float)a.xbar_mean[w];
+  f[5] = (float)a.xbar_max[w];
+  f[6] = (float)(ao > 0.0 ? log(ao) : -700.0);
+  f[7] = (float)(ao > 0.0 ? o.re / ao : 1.0);
+  f[8] = (float)(ao > 0.0 ? o.im / ao : 0.0);
+}
+
+// Exact exchange sum of sampled walker t (both spins, summed over the partials).
+__device__ __forceinline__ cplx exact_sum(const SurrogateCombineArgs& a, int t) {
+  cplx ex = cmake(0.0, 0.0);
+  for (int z = 0; z < a.xs_a; ++z) ex = cadd(ex, ld(a.part_a, (int64_t)z * a.k + t));
+  if (a.part_b)
+    for (int z = 0; z < a.xs_b; ++z) ex = cadd(ex, ld(a.part_b, (int64_t)z * a.k + t));
+  return ex;
+}
+
+// Running statistics (FP64 sums) of the features and the exact targets of
+// this batch; one workgroup, fixed reduction order.
+__global__ void sur_stats_kernel(SurrogateStatsArgs a) {
+  __shared__ double red[256 / 64];
+  const int nq = SG_F + 2;
+  for (int qd = 0; qd < nq; ++qd) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int t = threadIdx.x; t < a.count; t += blockDim.x) {
+      const int w = a.walker_of ? a.walker_of[t] : t;
+      double v;
+      if (qd < SG_F) v = a.feat[(int64_t)w * SG_F + qd];
+      else v = qd == SG_F ? a.target[2 * t] : a.target[2 * t + 1];
+      s1 += v;
+      s2 += v * v;
+    }
+    s1 = block_sum(s1, red);
+    s2 = block_sum(s2, red);
+    if (threadIdx.x == 0) {
+      a.state->sum[qd] += s1;
+      a.state->sumsq[qd] += s2;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a.state->count += a.count;
+    const double n = a.state->count;
+    for (int qd = 0; qd < nq; ++qd) {
+      const double m = a.state->sum[qd] / n;
+      const double var = fmax(a.state->sumsq[qd] / n - m * m, 0.0);
+      a.state->mean[qd] = (float)m;
+      a.state->inv_std[qd] = (float)(1.0 / sqrt(var + 1e-12));
+    }
+  }
+}
+
+// Inference: pred[w] = denormalised g(f_w), BF16 weights, FP32 accumulation.
+// One workgroup of H threads per group of walkers.
+__global__ void __launch_bounds__(SG_H) sur_forward_kernel(SurrogateForwardArgs a) {
+  __shared__ float h1[SG_H];
+  __shared__ float red[2][SG_H / 64];
+  const int j = threadIdx.x;
+  const SurrogateState* sst = a.state;
+  const uint16_t* w1 = a.wbf;                    // [H][F]
+  const uint16_t* w2 = w1 + SG_H * SG_F;         // [H][H]
+  const uint16_t* w3 = w2 + SG_H * SG_H;         // [2][H]
+  const float* bias = a.bias;                    // b1 [H], b2 [H], b3 [2] (FP32)
+  for (int w = blockIdx.x; w < a.nwalkers; w += gridDim.x) {
+    const float* f = a.feat + (int64_t)w * SG_F;
+    float z1 = bias[j];
+#pragma unroll
+    for (int q = 0; q < SG_F; ++q) z1 += bf2f(w1[j * SG_F + q]) * ((f[q] - sst->mean[q]) * sst->inv_std[q]);
+    __syncthreads();
+    h1[j] = silu(z1);
+    __syncthreads();
+    float z2 = bias[SG_H + j];
+    for (int k = 0; k < SG_H; ++k) z2 += bf2f(w2[j * SG_H + k]) * h1[k];
+    const float h2 = silu(z2);
+    float o0 = bf2f(w3[j]) * h2, o1 = bf2f(w3[SG_H + j]) * h2;
+    for (int off = 32; off > 0; off >>= 1) {
+      o0 += __shfl_xor(o0, off, 64);
+      o1 += __shfl_xor(o1, off, 64);
+    }
+    if ((j & 63) == 0) {
+      red[0][j >> 6] = o0;
+      red[1][j >> 6] = o1;
+    }
+    __syncthreads();
+    if (j == 0) {
+      float s0 = bias[2 * SG_H], s1 = bias[2 * SG_H + 1];
+      for (int i = 0; i < SG_H / 64; ++i) {
+        s0 += red[0][i];
+        s1 += red[1][i];
+      }
+      const double re = sst->mean[SG_F] + (double)s0 / sst->inv_std[SG_F];
+      const double im = sst->mean[SG_F + 1] + (double)s1 / sst->inv_std[SG_F + 1];
+      st(a.pred, w, cmake(re, im));
+    }
+  }
+}
+
+// Gradient of the batch loss, partial per workgroup: grad[wg][P].
+// Loss = mean_t sum_o (g_o(f_t) - y_t,o)^2 on normalised targets.
+__global__ void __launch_bounds__(SG_H) sur_grad_kernel(SurrogateTrainArgs a) {
+  extern __shared__ float sm[];
+  float* W2s = sm;                    // [H][H] master weights
+  float* dW2 = W2s + SG_H * SG_H;     // [H][H] gradient accumulator
+  float* h1 = dW2 + SG_H * SG_H;      // [H]
+  float* dz2 = h1 + SG_H;             // [H]
+  float* xin = dz2 + SG_H;            // [F]
+  __shared__ float red[2][SG_H / 64];
+  __shared__ float dout[2];
+  const int j = threadIdx.x;
+  float loss_acc = 0.0f;
+  const float* P = a.master;
+  const float* W1 = P;                       // [H][F]
+  const float* W2 = W1 + SG_H * SG_F;        // [H][H]
+  const float* W3 = W2 + SG_H * SG_H;        // [2][H]
+  const float* b1 = W3 + 2 * SG_H;           // [H]
+  const float* b2 = b1 + SG_H;               // [H]
+  const float* b3 = b2 + SG_H;               // [2]
+  const SurrogateState* sst = a.state;
+  for (int e = j; e < SG_H * SG_H; e += SG_H) {
+    W2s[e] = W2[e];
+    dW2[e] = 0.0f;
+  }
+  float gW1[SG_F];
+#pragma unroll
+  for (int q = 0; q < SG_F; ++q) gW1[q] = 0.0f;
+  float gb1 = 0.0f, gb2 = 0.0f, gW3a = 0.0f, gW3b = 0.0f, gb3a = 0.0f, gb3b = 0.0f;
+  float w1row[SG_F];
+#pragma unroll
+  for (int q = 0; q < SG_F; ++q) w1row[q] = W1[j * SG_F + q];
+  const float w3a = W3[j], w3b = W3[SG_H + j];
+  __syncthreads();
+  const int t0 = blockIdx.x * SG_SAMPLES_PER_WG;
+  const int t1 = min(t0 + SG_SAMPLES_PER_WG, a.count);
+  const float inv_count = 1.0f / (float)a.count;
+  for (int t = t0; t < t1; ++t) {
+    const int w = a.walker_of ? a.walker_of[t] : t;
+    if (j < SG_F) xin[j] = (a.feat[(int64_t)w * SG_F + j] - sst->mean[j]) * sst->inv_std[j];
+    __syncthreads();
+    float z1 = b1[j];
+#pragma unroll
+    for (int q = 0; q < SG_F; ++q) z1 += w1row[q] * xin[q];
+    h1[j] = silu(z1);
+    __syncthreads();
+    float z2 = b2[j];
+    for (int k = 0; k < SG_H; ++k) z2 += W2s[j * SG_H + k] * h1[k];
+    const float h2 = silu(z2);
+    float o0 = w3a * h2, o1 = w3b * h2;
+    for (int off = 32; off > 0; off >>= 1) {
+      o0 += __shfl_xor(o0, off, 64);
+      o1 += __shfl_xor(o1, off, 64);
+    }
+    if ((j & 63) == 0) {
+      red[0][j >> 6] = o0;
+      red[1][j >> 6] = o1;
+    }
+    __syncthreads();
+    if (j == 0) {
+      float s0 = b3[0], s1 = b3[1];
+      for (int i = 0; i < SG_H / 64; ++i) {
+        s0 += red[0][i];
+        s1 += red[1][i];
+      }
+      const float y0 = (float)((a.target[2 * t] - sst->mean[SG_F]) * sst->inv_std[SG_F]);
+      const float y1 = (float)((a.target[2 * t + 1] - sst->mean[SG_F + 1]) * sst->inv_std[SG_F + 1]);
+      dout[0] = 2.0f * (s0 - y0) * inv_count;
+      dout[1] = 2.0f * (s1 - y1) * inv_count;
+      loss_acc += (s0 - y0) * (s0 - y0) + (s1 - y1) * (s1 - y1);
+    }
+    __syncthreads();
+    const float d0 = dout[0], d1 = dout[1];
+    gW3a += d0 * h2;
+    gW3b += d1 * h2;
+    if (j == 0) {
+      gb3a += d0;
+      gb3b += d1;
+    }
+    const float g2 = (w3a * d0 + w3b * d1) * dsilu(z2);
+    dz2[j] = g2;
+    gb2 += g2;
+    for (int k = 0; k < SG_H; ++k) dW2[j * SG_H + k] += g2 * h1[k];
+    __syncthreads();
+    float dh1 = 0.0f;
+    for (int r = 0; r < SG_H; ++r) dh1 += W2s[r * SG_H + j] * dz2[r];
+    const float g1 = dh1 * dsilu(z1);
+    gb1 += g1;
+#pragma unroll
+    for (int q = 0; q < SG_F; ++q) gW1[q] += g1 * xin[q];
+    __syncthreads();
+  }
+  float* G = a.grad + (int64_t)blockIdx.x * SUR_NPARAM;
+#pragma unroll
+  for (int q = 0; q < SG_F; ++q) G[j * SG_F + q] = gW1[q];
+  for (int k = 0; k < SG_H; ++k) G[SG_H * SG_F + j * SG_H + k] = dW2[j * SG_H + k];
+  G[SG_H * SG_F + SG_H * SG_H + j] = gW3a;
+  G[SG_H * SG_F + SG_H * SG_H + SG_H + j] = gW3b;
+  G[SG_H * SG_F + SG_H * SG_H + 2 * SG_H + j] = gb1;
+  G[SG_H * SG_F + SG_H * SG_H + 3 * SG_H + j] = gb2;
+  if (j == 0) {
+    G[SUR_NPARAM - 2] = gb3a;
+    G[SUR_NPARAM - 1] = gb3b;
+    a.loss[blockIdx.x] = loss_acc;
+  }
+}
+
+// Sum the per-workgroup gradients (fixed order), Adam step, refresh the BF16
+// inference copy of the weights and the FP32 biases.
+__global__ void sur_adam_kernel(SurrogateAdamArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= SUR_NPARAM) return;
+  float g = 0.0f;
+  for (int i = 0; i < a.ngroups; ++i) g += a.grad[(int64_t)i * SUR_NPARAM + p];
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float m = b1 * a.m[p] + (1.0f - b1) * g;
+  const float v = b2 * a.v[p] + (1.0f - b2) * g * g;
+  a.m[p] = m;
+  a.v[p] = v;
+  const float mh = m / (1.0f - powf(b1, (float)a.step));
+  const float vh = v / (1.0f - powf(b2, (float)a.step));
+  const float w = a.master[p] - a.lr * mh / (sqrtf(vh) + eps);
+  a.master[p] = w;
+  constexpr int NW = SG_H * SG_F + SG_H * SG_H + 2 * SG_H;  // weights, then biases
+  if (p < NW) a.wbf[p] = f2bf_rne(w);
+  else a.bias[p - NW] = w;
+}
+
+// Horvitz-Thompson combination: est[w] = g(w) + [w in S] (W / k) (exact - g(w)).
+__global__ void sur_combine_kernel(SurrogateCombineArgs a) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= a.nwalkers) return;
+  const cplx g = ld(a.pred, w);
+  cplx est = g;
+  const int64_t pos = ((int64_t)a.perm_a * w + a.perm_b) % a.nwalkers;
+  if (pos < a.k) {
+    const cplx ex = exact_sum(a, (int)pos);
+    const double s = (double)a.nwalkers / (double)a.k;
+    est = cadd(g, cscale(csub(ex, g), s));
+    if (a.target) {
+      a.target[2 * pos] = ex.re;
+      a.target[2 * pos + 1] = ex.im;
+    }
+    if (a.resid) {
+      const cplx r = csub(ex, g);
+      a.resid[pos] = r.re * r.re + r.im * r.im;
+    }
+  }
+  st(a.est, w, est);
+}
+
+// walker index of sample t: w = a^{-1} (t - b) mod W
+__global__ void sur_select_kernel(int nwalkers, int k, int64_t inv_a, int64_t b, int* walker_of) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k) return;
+  int64_t v = ((int64_t)t - b) % nwalkers;
+  if (v < 0) v += nwalkers;
+  walker_of[t] = (int)((inv_a * v) % nwalkers);
+}
+
+// Ghalf rows of the sampled walkers, compacted: out[t] = g[walker_of[t]] ([n, N] complex each).
+__global__ void sur_gather_kernel(const double* __restrict__ g, const int* walker_of, int k, int64_t per_walker,
+                                  double* __restrict__ out) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)k * per_walker) return;
+  const int t = (int)(idx / per_walker);
+  const int64_t e = idx % per_walker;
+  const int64_t src = (int64_t)walker_of[t] * per_walker + e;
+  reinterpret_cast<double2*>(out)[idx] = reinterpret_cast<const double2*>(g)[src];
+}
+
+}  // namespace
+
+hipError_t launch_sur_features(const SurrogateFeatureArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(sur_features_kernel, dim3((unsigned)ceil_div(a.nwalkers, 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_sur_stats(const SurrogateStatsArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(sur_stats_kernel, dim3(1), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_sur_forward(const SurrogateForwardArgs& a, hipStream_t s) {
+  const int groups = a.nwalkers < 1024 ? a.nwalkers : 1024;
+  hipLaunchKernelGGL(sur_forward_kernel, dim3(groups), dim3(SG_H), 0, s, a);
+  return hipGetLastError();
+}
+int sur_grad_groups(int count) { return (int)ceil_div(count, SG_SAMPLES_PER_WG); }
+hipError_t launch_sur_grad(const SurrogateTrainArgs& a, hipStream_t s) {
+  const size_t lds = sizeof(float) * (2 * SG_H * SG_H + 2 * SG_H + SG_F);
+  hipError_t e = allow_lds((const void*)sur_grad_kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(sur_grad_kernel, dim3(sur_grad_groups(a.count)), dim3(SG_H), lds, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_sur_adam(const SurrogateAdamArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(sur_adam_kernel, dim3((unsigned)ceil_div(SUR_NPARAM, 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_sur_combine(const SurrogateCombineArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(sur_combine_kernel, dim3((unsigned)ceil_div(a.nwalkers, 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_sur_select(int nwalkers, int k, int64_t inv_a, int64_t b, int* walker_of, hipStream_t s) {
+  hipLaunchKernelGGL(sur_select_kernel, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, s, nwalkers, k, inv_a, b,
+                     walker_of);
+  return hipGetLastError();
+}
+hipError_t launch_sur_gather(const double* g, const int* walker_of, int k, int64_t per_walker, double* out,
+                             hipStream_t s) {
+  const int64_t total = (int64_t)k * per_walker;
+  hipLaunchKernelGGL(sur_gather_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s, g, walker_of, k,
+                     per_walker, out);
+  return hipGetLastError();
+}
+
+}  // namespace afqmc

@@ -15,6 +15,12 @@ weights, all enqueued on the current stream with no host synchronisation.
 the local energy as split-BF16 products with FP32 accumulation; Green's
 function, overlaps, weights, propagation and QR stay FP64).
 
+``surrogate``: None, or True / a dict (fraction, warmup, train_iters, lr,
+seed) enabling the learned exchange control variate for the ``energy_prev``
+output of ``propagate_walkers`` (an unbiased estimate per walker; exact on a
+random subset of walkers, see afqmc_surrogate_configure).  ``local_energy_batch``
+is always exact.
+
 Auxiliary fields: ``xi_source`` (callable ``(nwalkers, nfields) -> [W, X]``
 float64, numpy or torch) injects them exactly like ipie's
 ``xp.random.normal(0, 1, X * W).reshape(W, X)`` (phaseless_base.py:296);
@@ -58,9 +64,10 @@ def construct_one_body_propagator(hamiltonian, mf_shift, dt):
 
 
 class PhaselessGeneric:
-    def __init__(self, time_step, exp_nmax=6, verbose=False, seed=7, precision="fp64"):
+    def __init__(self, time_step, exp_nmax=6, verbose=False, seed=7, precision="fp64", surrogate=None):
         self.dt = time_step
         self.precision = precision
+        self.surrogate = surrogate
         self.verbose = verbose
         self.sqrt_dt = self.dt**0.5
         self.isqrt_dt = 1j * self.sqrt_dt
@@ -91,12 +98,19 @@ class PhaselessGeneric:
         eng = _engine.Engine(trial, walkers, nchol=hamiltonian.nchol, exp_nmax=self.exp_nmax,
                              precision=self.precision)
         eng.set_hamiltonian(hamiltonian, self.expH1, self.mf_shift, self.dt)
+        if self.surrogate:
+            opts = dict(fraction=0.125, warmup=100, train_iters=4, lr=1e-3, seed=self.seed)
+            opts.update(self.surrogate if isinstance(self.surrogate, dict) else {})
+            eng.ctx.surrogate_configure(True, **opts)
         walkers._engine = eng
         self._engine = eng
         self.mpi_handler = mpi_handler
 
     def cast_to_cupy(self, verbose=False):
         return None
+
+    def surrogate_stats(self):
+        return self._engine.ctx.surrogate_stats()
 
     @property
     def nfb_trig(self):
