@@ -49,7 +49,7 @@ struct afqmc_ctx {
   // split-BF16 exchange (AFQMC_PREC_MIXED)
   int kpad = 0;
   bool bx_ok[2] = {false, false};
-  int bx_wb[2] = {1, 1}, bx_xb[2] = {1, 1}, bx_rt[2] = {0, 0}, bx_ct[2] = {0, 0}, bx_xs[2] = {8, 8};
+  int bx_wb[2] = {1, 1}, bx_xb[2] = {1, 1};
   uint16_t* gplanes = nullptr;  // 4 x [W * nmax, kpad]
   uint16_t* lplanes[2] = {nullptr, nullptr};  // 2 x [X * n_s, kpad] per spin
   bool lplanes_ready = false;
@@ -418,8 +418,6 @@ int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g, const int* wa
     b.kpad = c->kpad;
     b.wb = c->bx_wb[spin];
     b.xb = c->bx_xb[spin];
-    b.rt_ = c->bx_rt[spin];
-    b.ct_ = c->bx_ct[spin];
     const int64_t nwb = ceil_div(nw, b.wb), nxb = ceil_div(c->d.nchol, b.xb);
     int64_t xs = ceil_div(ceil_div(2048, nwb), 8) * 8;
     if (xs > nxb) xs = nxb;
@@ -629,13 +627,7 @@ int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims) {
   if (d.precision == AFQMC_PREC_MIXED) {
     for (int s = 0; s < (two ? 2 : 1); ++s) {
       const int n = s == 0 ? d.nalpha : d.nbeta;
-      c->bx_ok[s] = bf16_exchange_plan(n, &c->bx_wb[s], &c->bx_xb[s], &c->bx_rt[s], &c->bx_ct[s]);
-      if (!c->bx_ok[s]) continue;
-      const int64_t nwb = ceil_div(W, c->bx_wb[s]), nxb = ceil_div(X, c->bx_xb[s]);
-      int64_t xs = ceil_div(ceil_div(2048, nwb), 8) * 8;
-      if (xs < 8) xs = 8;
-      if (xs > nxb) xs = nxb;
-      c->bx_xs[s] = (int)xs;
+      c->bx_ok[s] = bf16_exchange_plan(n, &c->bx_wb[s], &c->bx_xb[s]);
     }
   }
   if (dmalloc((void**)&c->flags, sizeof(int)) != hipSuccess ||

@@ -89,12 +89,12 @@ int exchange_xsplit(int nwalkers, int nchol, int nocc);
 
 struct ExchangeBf16Args {
   int nwalkers, nchol, nocc, kpad;  // kpad = N rounded up to 32
-  int wb, xb, rt_, ct_, xsplit;
+  int wb, xb, xsplit;  // walkers and Cholesky vectors per workgroup tile, Cholesky ranges
   const uint16_t* gplanes;  // 4 x [W*n, kpad] bf16: re_hi, re_lo, im_hi, im_lo
   const uint16_t* lplanes;  // 2 x [X*n, kpad] bf16: hi, lo
   double* exx_part;         // [xsplit, W] complex
 };
-bool bf16_exchange_plan(int nocc, int* wb, int* xb, int* rt, int* ct);
+bool bf16_exchange_plan(int nocc, int* wb, int* xb);
 hipError_t launch_split_ghalf(const double* g, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream,
                               const int* walker_of = nullptr, int nocc = 1);
 hipError_t launch_split_real(const double* x, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream);

@@ -12,10 +12,13 @@
 // padding, so a 32-wide K chunk of a row is one aligned 64-byte segment:
 //   Ghalf: 4 planes (re_hi, re_lo, im_hi, im_lo), rows = (walker, j)
 //   rchol: 2 planes (hi, lo), rows = (x, i)
-// Workgroup: 4 waves in a 2 x 2 grid, each wave owns (RT/2) x (CT/2) output
-// tiles of 16 x 16; double-buffered LDS staging; one barrier per K chunk.
-// Workgroups that run together on one XCD cover 4 walker blocks x 8 Cholesky
-// ranges, so both operand tiles are shared through that XCD's L2.
+// Workgroup: 8 waves in a 4 x 2 grid, each wave owns 2 x 5 output tiles of
+// 16 x 16, so one workgroup covers 128 rows (wb walkers) by 160 columns (xb
+// Cholesky vectors); two waves share each SIMD, so one wave's LDS and barrier
+// waits are covered by the other's MFMAs.  Double-buffered LDS staging, one
+// barrier per K chunk.  Workgroups that run together on one XCD cover a few
+// walker blocks x 8 Cholesky ranges, so both operand tiles are shared through
+// that XCD's L2.
 #include "afqmc_internal.hpp"
 
 namespace afqmc {
@@ -23,9 +26,25 @@ namespace afqmc {
 namespace {
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+// native 16-byte vector: plain loads and stores, kept in registers (a uint4
+// struct copy lowers to a memcpy that pins the staging arrays to scratch)
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
-constexpr int BX_THREADS = 256;
+// 8 waves in a BX_GR x BX_GC grid, each owning BX_MR x BX_MC tiles of 16 x 16
+constexpr int BX_GR = 4, BX_GC = 2, BX_MR = 2, BX_MC = 5;
+constexpr int BX_R = 16 * BX_GR * BX_MR;  // rows of a workgroup tile (walker orbitals)
+constexpr int BX_C = 16 * BX_GC * BX_MC;  // columns (Cholesky-vector orbitals)
+constexpr int BX_THREADS = 64 * BX_GR * BX_GC;
 constexpr int BX_KC = 32;
+// one K chunk in 16-byte vectors: A planes [4][BX_R][4], B planes [2][BX_C][4]
+constexpr int BX_A_VEC = 4 * BX_R * 4;
+constexpr int BX_B_VEC = 2 * BX_C * 4;
+constexpr int BX_STAGE_VEC = BX_A_VEC + BX_B_VEC;
+constexpr int BX_A_PER = BX_A_VEC / BX_THREADS;
+constexpr int BX_B_PER = (BX_B_VEC + BX_THREADS - 1) / BX_THREADS;
+static_assert(BX_A_VEC % BX_THREADS == 0, "A staging is split evenly over the threads");
+static_assert((BX_B_VEC % BX_THREADS) % 64 == 0, "the partial B vector round is wave-uniform");
+constexpr size_t BX_LDS_STAGES = 2 * 16 * (size_t)BX_STAGE_VEC;  // double-buffered staging, bytes
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
   uint32_t u = __float_as_uint(f);
@@ -39,14 +58,20 @@ __device__ __forceinline__ void split_bf16(double x, uint16_t& hi, uint16_t& lo)
   lo = f2bf((float)(x - (double)bf2f(hi)));
 }
 
-__device__ __forceinline__ v8bf as_v8bf(uint4 u) {
-  v8bf r;
-  __builtin_memcpy(&r, &u, 16);
-  return r;
-}
+__device__ __forceinline__ v8bf as_v8bf(v4u u) { return __builtin_bit_cast(v8bf, u); }
 
 __device__ __forceinline__ v4f mfma_bf16(v8bf a, v8bf b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// LDS slot of 16-byte vector `seg` (0..3) of staged row `row`.  Rows are 64
+// bytes, so without a swizzle rows r and r + 4 of one fragment read (16 rows,
+// same segment) fall on the same banks; XOR-ing the segment with bits 2..3 of
+// the row spreads the 16 rows over all 64 banks.
+__device__ __forceinline__ int lds_vec(int row, int seg) { return row * 4 + (seg ^ ((row >> 2) & 3)); }
+
+__device__ __forceinline__ v8bf lds_frag(const uint16_t* base, int row, int seg) {
+  return as_v8bf(reinterpret_cast<const v4u*>(base)[lds_vec(row, seg)]);
 }
 
 // Ghalf [W*n, N] complex -> 4 bf16 planes [rows, Kp]; with walker_of, plane
@@ -84,117 +109,131 @@ __global__ void split_real_kernel(const double* __restrict__ x, int64_t rows, in
   planes[rows * Kp + idx] = l;
 }
 
-template <int RT, int CT>
+
+// Source offsets (bf16 units, K chunk 0) of the B vectors thread tid stages
+// for Cholesky block xblk; rows past the end are clamped to the last row.
+__device__ __forceinline__ void bx_set_b(int64_t (&b_src)[BX_B_PER], int tid, int xblk, int xb, int n, int64_t brows,
+                                         int64_t bplane, int Kp) {
+#pragma unroll
+  for (int i = 0; i < BX_B_PER; ++i) {
+    const int v = (tid + i * BX_THREADS) % BX_B_VEC;  // threads past the end load a duplicate, never stored
+    const int p = v / (BX_C * 4), rem = v % (BX_C * 4);
+    int64_t r = (int64_t)xblk * xb * n + rem / 4;
+    if (r > brows - 1) r = brows - 1;
+    b_src[i] = p * bplane + r * Kp + (rem % 4) * 8;
+  }
+}
+
+__device__ __forceinline__ void bx_fetch(v4u (&pa)[BX_A_PER], v4u (&pb)[BX_B_PER], const uint16_t* gp,
+                                         const uint16_t* lp, const int64_t (&a_src)[BX_A_PER],
+                                         const int64_t (&b_src)[BX_B_PER], int c) {
+  const int64_t k0 = (int64_t)c * BX_KC;
+#pragma unroll
+  for (int i = 0; i < BX_A_PER; ++i) pa[i] = *reinterpret_cast<const v4u*>(gp + a_src[i] + k0);
+#pragma unroll
+  for (int i = 0; i < BX_B_PER; ++i) pb[i] = *reinterpret_cast<const v4u*>(lp + b_src[i] + k0);
+}
+
+__device__ __forceinline__ void bx_stash(uint4* lds_u4, const v4u (&pa)[BX_A_PER], const v4u (&pb)[BX_B_PER],
+                                         int tid, int buf) {
+  v4u* base = reinterpret_cast<v4u*>(lds_u4) + buf * BX_STAGE_VEC;
+#pragma unroll
+  for (int i = 0; i < BX_A_PER; ++i) {
+    const int v = tid + i * BX_THREADS;
+    base[lds_vec(v >> 2, v & 3)] = pa[i];
+  }
+#pragma unroll
+  for (int i = 0; i < BX_B_PER; ++i) {
+    const int v = tid + i * BX_THREADS;
+    if (v < BX_B_VEC) base[BX_A_VEC + lds_vec(v >> 2, v & 3)] = pb[i];
+  }
+}
+
+
 __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16Args a) {
-  constexpr int R = RT * 16, C = CT * 16;
-  constexpr int WR = RT / 2, WC = CT / 2;  // tiles per wave
-  // staging (uint16 units): A planes [4][R][32], B planes [2][C][32]
-  constexpr int A_ELEMS = 4 * R * BX_KC, B_ELEMS = 2 * C * BX_KC;
-  constexpr int STAGE = A_ELEMS + B_ELEMS;
-  constexpr int A_VEC = A_ELEMS / 8, B_VEC = B_ELEMS / 8;  // 16-byte vectors
-  constexpr int A_PER = (A_VEC + BX_THREADS - 1) / BX_THREADS;
-  constexpr int B_PER = (B_VEC + BX_THREADS - 1) / BX_THREADS;
   extern __shared__ uint4 lds_u4[];
-  uint16_t* lds = reinterpret_cast<uint16_t*>(lds_u4);
   __shared__ double red[BX_THREADS / 64];
-  // epilogue staging aliases the buffers: S[row][col] complex float, stride n + 1
+  // epilogue staging aliases the stage buffers: S[row][col] complex float, stride n + 1
   float2* S = reinterpret_cast<float2*>(lds_u4);
 
-  const int n = a.nocc, Kp = a.kpad;
-  const int rows_valid = a.wb * n, cols_valid = a.xb * n;
-  const int nwb = (int)ceil_div(a.nwalkers, a.wb);
+  const int n = a.nocc, Kp = a.kpad, wb = a.wb, xb = a.xb;
+  const int rows_valid = wb * n;
+  const int nwb = (int)ceil_div(a.nwalkers, wb);
   const int lin = xcd_remap(blockIdx.x, gridDim.x);
   const int split = lin % a.xsplit, bw = lin / a.xsplit;
-  const int nxb = (int)ceil_div(a.nchol, a.xb);
+  const int nxb = (int)ceil_div(a.nchol, xb);
   const int xb_begin = (int)((int64_t)nxb * split / a.xsplit);
   const int xb_end = (int)((int64_t)nxb * (split + 1) / a.xsplit);
   if (bw >= nwb) return;
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
-  const int wr = wave & 1, wc = wave >> 1;
-  const int w0 = bw * a.wb;
-  const int64_t arow_base = (int64_t)w0 * n, arow_limit = (int64_t)a.nwalkers * n;
-  const int64_t brow_limit = (int64_t)a.nchol * n;
-  const int64_t aplane = arow_limit * Kp, bplane = brow_limit * Kp;
+  const int wr = wave % BX_GR, wc = wave / BX_GR;
+  const int64_t arows = (int64_t)a.nwalkers * n, brows = (int64_t)a.nchol * n;
+  const int64_t aplane = arows * Kp, bplane = brows * Kp;
   const int nch = Kp / BX_KC;
 
-  double wacc_r = 0.0, wacc_i = 0.0;
-  uint4 pa[A_PER], pb[B_PER];
+  // Source offsets (bf16 units, K chunk 0) of the vectors this thread stages.
+  // Rows past the end of an array are clamped to its last row: padding rows
+  // and columns of the tile only feed outputs that the epilogue never reads.
+  int64_t a_src[BX_A_PER], b_src[BX_B_PER];
+#pragma unroll
+  for (int i = 0; i < BX_A_PER; ++i) {
+    const int v = tid + i * BX_THREADS;
+    const int p = v / (BX_R * 4), rem = v % (BX_R * 4);
+    int64_t r = (int64_t)bw * wb * n + rem / 4;
+    if (r > arows - 1) r = arows - 1;
+    a_src[i] = p * aplane + r * Kp + (rem % 4) * 8;
+  }
+  v4u pa[BX_A_PER], pb[BX_B_PER];
 
+  // trace epilogue: walker wl of the block is summed by threads [wl G, (wl + 1) G)
+  const int G = BX_THREADS / wb;
+  const int my_wl = tid / G, my_idx = tid % G;
+  const int G_i = G % n, G_j = (G / n) % n, G_v = G / (n * n);  // G = (G_v n + G_j) n + G_i
+  // vectors staged per epilogue round: as many as the stage buffers hold
+  const int xg = max(1, min(xb, (int)((int64_t)BX_LDS_STAGES / ((int64_t)rows_valid * (n + 1) * sizeof(float2)))));
+  double wacc_r = 0.0, wacc_i = 0.0;
+
+  if (xb_begin < xb_end) {
+    bx_set_b(b_src, tid, xb_begin, xb, n, brows, bplane, Kp);
+    bx_fetch(pa, pb, a.gplanes, a.lplanes, a_src, b_src, 0);
+  }
   for (int xblk = xb_begin; xblk < xb_end; ++xblk) {
-    const int x0 = xblk * a.xb;
-    const int64_t brow_base = (int64_t)x0 * n;
-    auto fetch = [&](int c) {
-      const int k0 = c * BX_KC;
+    const int x0 = xblk * xb;
+    bx_stash(lds_u4, pa, pb, tid, 0);
+    __syncthreads();
+    v4f re[BX_MR][BX_MC], im[BX_MR][BX_MC];
 #pragma unroll
-      for (int i = 0; i < A_PER; ++i) {
-        const int v = tid + i * BX_THREADS;  // vector index: plane, row, 4 vectors per row
-        pa[i] = make_uint4(0, 0, 0, 0);
-        if (v < A_VEC) {
-          const int p = v / (R * 4), rem = v % (R * 4), row = rem / 4, seg = rem % 4;
-          const int64_t ar = arow_base + row;
-          if (row < rows_valid && ar < arow_limit)
-            pa[i] = *reinterpret_cast<const uint4*>(a.gplanes + p * aplane + ar * Kp + k0 + seg * 8);
-        }
-      }
+    for (int i = 0; i < BX_MR; ++i)
 #pragma unroll
-      for (int i = 0; i < B_PER; ++i) {
-        const int v = tid + i * BX_THREADS;
-        pb[i] = make_uint4(0, 0, 0, 0);
-        if (v < B_VEC) {
-          const int p = v / (C * 4), rem = v % (C * 4), col = rem / 4, seg = rem % 4;
-          const int64_t br = brow_base + col;
-          if (col < cols_valid && br < brow_limit)
-            pb[i] = *reinterpret_cast<const uint4*>(a.lplanes + p * bplane + br * Kp + k0 + seg * 8);
-        }
-      }
-    };
-    auto stash = [&](int buf) {
-      uint4* base = lds_u4 + buf * (STAGE / 8);
-#pragma unroll
-      for (int i = 0; i < A_PER; ++i) {
-        const int v = tid + i * BX_THREADS;
-        if (v < A_VEC) base[v] = pa[i];
-      }
-#pragma unroll
-      for (int i = 0; i < B_PER; ++i) {
-        const int v = tid + i * BX_THREADS;
-        if (v < B_VEC) base[A_VEC + v] = pb[i];
-      }
-    };
-    v4f re[WR][WC], im[WR][WC];
-#pragma unroll
-    for (int i = 0; i < WR; ++i)
-#pragma unroll
-      for (int j = 0; j < WC; ++j) {
+      for (int j = 0; j < BX_MC; ++j) {
         re[i][j] = zero4f();
         im[i][j] = zero4f();
       }
-    fetch(0);
-    __syncthreads();
-    stash(0);
-    __syncthreads();
     for (int c = 0; c < nch; ++c) {
-      if (c + 1 < nch) fetch(c + 1);
-      const uint16_t* As = lds + (c & 1) * STAGE;
-      const uint16_t* Bs = As + A_ELEMS;
-      // B fragments of the wave's WC column tiles (hi, lo)
-      v8bf bh[WC], bl[WC];
+      // next chunk, or after the last one the first chunk of the next x block
+      // (stashed after the epilogue)
+      const bool more = c + 1 < nch, next_blk = !more && xblk + 1 < xb_end;
+      if (next_blk) bx_set_b(b_src, tid, xblk + 1, xb, n, brows, bplane, Kp);
+      if (more || next_blk) bx_fetch(pa, pb, a.gplanes, a.lplanes, a_src, b_src, more ? c + 1 : 0);
+      const uint16_t* As = reinterpret_cast<const uint16_t*>(lds_u4 + (c & 1) * BX_STAGE_VEC);
+      const uint16_t* Bs = As + 8 * BX_A_VEC;
+      v8bf bh[BX_MC], bl[BX_MC];
 #pragma unroll
-      for (int j = 0; j < WC; ++j) {
-        const int col = (wc * WC + j) * 16 + l16;
-        bh[j] = as_v8bf(*reinterpret_cast<const uint4*>(Bs + (0 * C + col) * BX_KC + 8 * q));
-        bl[j] = as_v8bf(*reinterpret_cast<const uint4*>(Bs + (1 * C + col) * BX_KC + 8 * q));
+      for (int j = 0; j < BX_MC; ++j) {
+        const int col = (wc * BX_MC + j) * 16 + l16;
+        bh[j] = lds_frag(Bs, col, q);
+        bl[j] = lds_frag(Bs, BX_C + col, q);
       }
 #pragma unroll
-      for (int i = 0; i < WR; ++i) {
-        const int row = (wr * WR + i) * 16 + l16;
-        const v8bf arh = as_v8bf(*reinterpret_cast<const uint4*>(As + (0 * R + row) * BX_KC + 8 * q));
-        const v8bf arl = as_v8bf(*reinterpret_cast<const uint4*>(As + (1 * R + row) * BX_KC + 8 * q));
-        const v8bf aih = as_v8bf(*reinterpret_cast<const uint4*>(As + (2 * R + row) * BX_KC + 8 * q));
-        const v8bf ail = as_v8bf(*reinterpret_cast<const uint4*>(As + (3 * R + row) * BX_KC + 8 * q));
+      for (int i = 0; i < BX_MR; ++i) {
+        const int row = (wr * BX_MR + i) * 16 + l16;
+        const v8bf arh = lds_frag(As, row, q);
+        const v8bf arl = lds_frag(As, BX_R + row, q);
+        const v8bf aih = lds_frag(As, 2 * BX_R + row, q);
+        const v8bf ail = lds_frag(As, 3 * BX_R + row, q);
 #pragma unroll
-        for (int j = 0; j < WC; ++j) {
+        for (int j = 0; j < BX_MC; ++j) {
           re[i][j] = mfma_bf16(arh, bh[j], re[i][j]);
           re[i][j] = mfma_bf16(arh, bl[j], re[i][j]);
           re[i][j] = mfma_bf16(arl, bh[j], re[i][j]);
@@ -203,87 +242,79 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
           im[i][j] = mfma_bf16(ail, bh[j], im[i][j]);
         }
       }
-      if (c + 1 < nch) stash((c + 1) & 1);
+      if (c + 1 < nch) bx_stash(lds_u4, pa, pb, tid, (c + 1) & 1);
       __syncthreads();
     }
-    // epilogue, one Cholesky vector at a time (accumulator row = 4 q + r)
-    for (int xl = 0; xl < a.xb; ++xl) {
-      if (x0 + xl >= a.nchol) break;
-      const int cbeg = xl * n;
+    // epilogue: the block's vectors in groups of xg, staged as S[v][row][col]
+    // (row stride n + 1), then sum_ij T_ij T_ji per walker; the element index
+    // e = (v, jj, ii) of each thread advances by G with carries, no division
+    for (int g0 = 0; g0 < xb; g0 += xg) {
+      const int gn = min(min(xg, xb - g0), a.nchol - (x0 + g0));
+      if (gn <= 0) break;
 #pragma unroll
-      for (int j = 0; j < WC; ++j) {
-        const int col = (wc * WC + j) * 16 + l16 - cbeg;
-        if (col >= 0 && col < n) {
+      for (int j = 0; j < BX_MC; ++j) {
+        const int colg = (wc * BX_MC + j) * 16 + l16;
+        const int v = colg / n - g0, col = colg % n;
+        if (v >= 0 && v < gn) {
+          float2* Sv = S + (int64_t)v * rows_valid * (n + 1) + col;
 #pragma unroll
-          for (int i = 0; i < WR; ++i)
+          for (int i = 0; i < BX_MR; ++i)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              const int row = (wr * WR + i) * 16 + 4 * q + r;
-              if (row < rows_valid) S[row * (n + 1) + col] = make_float2(re[i][j][r], im[i][j][r]);
+              const int row = (wr * BX_MR + i) * 16 + 4 * q + r;
+              if (row < rows_valid) Sv[row * (n + 1)] = make_float2(re[i][j][r], im[i][j][r]);
             }
         }
       }
       __syncthreads();
-      const int nn = n * n;
-      for (int e = tid; e < a.wb * nn; e += BX_THREADS) {
-        const int wl = e % a.wb, ij = e / a.wb, ii = ij / n, jj = ij % n;
-        const float2 tij = S[(wl * n + jj) * (n + 1) + ii];
-        const float2 tji = S[(wl * n + ii) * (n + 1) + jj];
-        wacc_r += (double)tij.x * tji.x - (double)tij.y * tji.y;
-        wacc_i += (double)tij.x * tji.y + (double)tij.y * tji.x;
+      if (my_wl < wb) {
+        int ii = my_idx % n, jj = (my_idx / n) % n, v = my_idx / (n * n);
+        const int total = gn * n * n;
+        for (int e = my_idx; e < total; e += G) {
+          const float2* Sw = S + ((int64_t)v * rows_valid + my_wl * n) * (n + 1);
+          const float2 tij = Sw[jj * (n + 1) + ii];
+          const float2 tji = Sw[ii * (n + 1) + jj];
+          wacc_r += (double)tij.x * tji.x - (double)tij.y * tji.y;
+          wacc_i += (double)tij.x * tji.y + (double)tij.y * tji.x;
+          ii += G_i;
+          jj += G_j;
+          v += G_v;
+          if (ii >= n) {
+            ii -= n;
+            ++jj;
+          }
+          if (jj >= n) {
+            jj -= n;
+            ++v;
+          }
+        }
       }
       __syncthreads();
     }
   }
-  for (int wl = 0; wl < a.wb; ++wl) {
-    const bool mine = (tid % a.wb) == wl;
+  for (int wl = 0; wl < wb; ++wl) {
+    const bool mine = my_wl == wl;
     const double vr = block_sum(mine ? wacc_r : 0.0, red);
     const double vi = block_sum(mine ? wacc_i : 0.0, red);
-    const int w = w0 + wl;
+    const int w = bw * wb + wl;
     if (tid == 0 && w < a.nwalkers) st(a.exx_part, (int64_t)split * a.nwalkers + w, cmake(vr, vi));
   }
 }
 
-template <int RT, int CT>
-hipError_t launch_bf16(const ExchangeBf16Args& a, hipStream_t stream) {
-  constexpr int STAGE_BYTES = 2 * (4 * RT * 16 * BX_KC + 2 * CT * 16 * BX_KC);
-  const size_t epi = sizeof(float2) * (size_t)(a.wb * a.nocc) * (a.nocc + 1);
-  const size_t lds = (size_t)2 * STAGE_BYTES > epi ? (size_t)2 * STAGE_BYTES : epi;
-  hipError_t e = allow_lds((const void*)exchange_bf16_kernel<RT, CT>, lds);
-  if (e != hipSuccess) return e;
-  const int nwb = (int)ceil_div(a.nwalkers, a.wb);
-  hipLaunchKernelGGL((exchange_bf16_kernel<RT, CT>), dim3(nwb * a.xsplit), dim3(BX_THREADS), lds, stream, a);
-  return hipGetLastError();
+size_t bf16_lds_bytes(int wb, int nocc) {
+  const size_t epi = sizeof(float2) * (size_t)(wb * nocc) * (nocc + 1);  // one vector
+  return BX_LDS_STAGES > epi ? BX_LDS_STAGES : epi;
 }
 
 }  // namespace
 
-// Geometry for the split-BF16 exchange: wb walkers x n rows and xb vectors x n
-// columns padded to an even number (<= 10) of 16-row tiles (every even pair
-// is instantiated); wb a power of two.
-bool bf16_exchange_plan(int nocc, int* wb, int* xb, int* rt, int* ct) {
-  double best_score = -1.0, best_eff = 0.0;
-  for (int w = 1; w <= 8; w *= 2) {
-    const int R = (w * nocc + 15) / 16;
-    if (R > 10) break;
-    const int RT = R + (R & 1);
-    for (int x = 1; x <= 32; ++x) {
-      const int Cn = (x * nocc + 15) / 16;
-      if (Cn > 10) break;
-      const int CT = Cn + (Cn & 1);
-      const double eff = (double)(w * nocc) * (x * nocc) / (256.0 * RT * CT);
-      const double score = eff * 1000.0 + RT * CT;
-      if (score > best_score) {
-        best_score = score;
-        best_eff = eff;
-        *wb = w;
-        *xb = x;
-        *rt = RT;
-        *ct = CT;
-      }
-    }
-  }
-  return best_score > 0.0 && best_eff > 0.7;
+// Geometry for the split-BF16 exchange: as many walkers (wb) and Cholesky
+// vectors (xb) as fit the 128 x 160 workgroup tile.
+bool bf16_exchange_plan(int nocc, int* wb, int* xb) {
+  if (nocc < 1 || nocc > BX_R || nocc > BX_C) return false;
+  *wb = BX_R / nocc;
+  *xb = BX_C / nocc;
+  return bf16_lds_bytes(*wb, nocc) <= 160 * 1024;
 }
 
 hipError_t launch_split_ghalf(const double* g, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream,
@@ -302,13 +333,15 @@ hipError_t launch_split_real(const double* x, int64_t rows, int N, int Kp, uint1
 }
 
 hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream) {
-#define AFQMC_BX(rt, ct) \
-  if (a.rt_ == rt && a.ct_ == ct) return launch_bf16<rt, ct>(a, stream);
-#define AFQMC_BX_ROW(rt) AFQMC_BX(rt, 2) AFQMC_BX(rt, 4) AFQMC_BX(rt, 6) AFQMC_BX(rt, 8) AFQMC_BX(rt, 10)
-  AFQMC_BX_ROW(2) AFQMC_BX_ROW(4) AFQMC_BX_ROW(6) AFQMC_BX_ROW(8) AFQMC_BX_ROW(10)
-#undef AFQMC_BX_ROW
-#undef AFQMC_BX
-  return hipErrorInvalidValue;
+  if (a.nocc < 1 || a.wb < 1 || a.xb < 1 || a.wb * a.nocc > BX_R || a.xb * a.nocc > BX_C || a.kpad % BX_KC != 0 ||
+      a.xsplit < 1)
+    return hipErrorInvalidValue;
+  const size_t lds = bf16_lds_bytes(a.wb, a.nocc);
+  hipError_t e = allow_lds((const void*)exchange_bf16_kernel, lds);
+  if (e != hipSuccess) return e;
+  const int nwb = (int)ceil_div(a.nwalkers, a.wb);
+  hipLaunchKernelGGL(exchange_bf16_kernel, dim3(nwb * a.xsplit), dim3(BX_THREADS), lds, stream, a);
+  return hipGetLastError();
 }
 
 }  // namespace afqmc
