@@ -189,7 +189,6 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
   // trace epilogue: walker wl of the block is summed by threads [wl G, (wl + 1) G)
   const int G = BX_THREADS / wb;
   const int my_wl = tid / G, my_idx = tid % G;
-  const int G_i = G % n, G_j = (G / n) % n, G_v = G / (n * n);  // G = (G_v n + G_j) n + G_i
   // vectors staged per epilogue round: as many as the stage buffers hold
   const int xg = max(1, min(xb, (int)((int64_t)BX_LDS_STAGES / ((int64_t)rows_valid * (n + 1) * sizeof(float2)))));
   double wacc_r = 0.0, wacc_i = 0.0;
@@ -246,8 +245,7 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
       __syncthreads();
     }
     // epilogue: the block's vectors in groups of xg, staged as S[v][row][col]
-    // (row stride n + 1), then sum_ij T_ij T_ji per walker; the element index
-    // e = (v, jj, ii) of each thread advances by G with carries, no division
+    // (row stride n + 1), then sum_ij T_ij T_ji per walker
     for (int g0 = 0; g0 < xb; g0 += xg) {
       const int gn = min(min(xg, xb - g0), a.nchol - (x0 + g0));
       if (gn <= 0) break;
@@ -268,25 +266,42 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
       }
       __syncthreads();
       if (my_wl < wb) {
-        int ii = my_idx % n, jj = (my_idx / n) % n, v = my_idx / (n * n);
-        const int total = gn * n * n;
-        for (int e = my_idx; e < total; e += G) {
+        // sum_ij S_ij S_ji = sum_r S_rr^2 + 2 sum_{r<c} S_rc S_cr.  Each unordered
+        // pair {r, c} is visited once from r at distance d = c - r mod n,
+        // d = 1 .. (n-1)/2, plus d = n/2 from r < n/2 when n is even.  Task
+        // (v, r, h) of a thread walks half h of row r's distances.
+        const int ntask = gn * n * 2;
+        for (int task = my_idx; task < ntask; task += G) {
+          const int h = task & 1, vr = task >> 1, v = vr / n, r = vr % n;
+          const int cnt = (n - 1) / 2 + ((n % 2 == 0 && r < n / 2) ? 1 : 0);
+          const int half = (cnt + 1) / 2;
+          const int d0 = h * half, d1 = h ? cnt : half;
           const float2* Sw = S + ((int64_t)v * rows_valid + my_wl * n) * (n + 1);
-          const float2 tij = Sw[jj * (n + 1) + ii];
-          const float2 tji = Sw[ii * (n + 1) + jj];
-          wacc_r += (double)tij.x * tji.x - (double)tij.y * tji.y;
-          wacc_i += (double)tij.x * tji.y + (double)tij.y * tji.x;
-          ii += G_i;
-          jj += G_j;
-          v += G_v;
-          if (ii >= n) {
-            ii -= n;
-            ++jj;
+          if (h == 0) {
+            const float2 t = Sw[r * (n + 1) + r];
+            wacc_r += (double)t.x * t.x - (double)t.y * t.y;
+            wacc_i += 2.0 * (double)t.x * t.y;
           }
-          if (jj >= n) {
-            jj -= n;
-            ++v;
+          int c = r + 1 + d0;
+          if (c >= n) c -= n;
+          const float2* p1 = Sw + r * (n + 1) + c;  // S[r][c]
+          const float2* p2 = Sw + c * (n + 1) + r;  // S[c][r]
+          double orr = 0.0, oii = 0.0;
+          for (int d = d0; d < d1; ++d) {
+            const float2 a = *p1, b = *p2;
+            orr += (double)a.x * b.x - (double)a.y * b.y;
+            oii += (double)a.x * b.y + (double)a.y * b.x;
+            ++c;
+            ++p1;
+            p2 += n + 1;
+            if (c == n) {
+              c = 0;
+              p1 -= n;
+              p2 -= n * (n + 1);
+            }
           }
+          wacc_r += 2.0 * orr;
+          wacc_i += 2.0 * oii;
         }
       }
       __syncthreads();
