@@ -220,8 +220,10 @@ def main():
     K = args.steps
     mode_list = [m for m in args.modes.split(",") if m]
     # every mode runs the same number of steps before its timed block, so all
-    # modes time the same stretch of one trajectory (the propagation is FP64 in
-    # every mode and the fields come from the same counter-based stream)
+    # modes time the same stretch of one trajectory (the fields come from the
+    # same counter-based stream; the mixed modes' split-BF16 force bias and VHS
+    # move the walkers by ~1e-6 relative per step, so the trajectories stay
+    # close and the per-step energy difference measures the precision bias)
     warmup = args.warmup
     if any("surrogate" in m for m in mode_list):
         warmup = max(warmup, SURROGATE_WARMUP + 5)
@@ -281,8 +283,9 @@ def main():
     mx = modes[ex_mode]
     peak = PEAK["f64_mfma_tflops"] if ex_mode == "fp64" else PEAK["bf16_mfma_tflops"] / 3.0
     step_fl = step_flops(w.nbasis, w.nchol, w.nalpha, w.nbeta)
-    dtype = {"fp64": "f64", "mixed": "f64 + bf16x3 exchange (fp32 accumulate)",
-             "mixed+surrogate": "f64 + bf16x3 exchange on 1/8 of walkers + bf16-weight MLP control variate"}[head]
+    dtype = {"fp64": "f64",
+             "mixed": "f64 (LU, inverse, Taylor, weights, QR) + bf16x3 force bias, VHS, exchange (fp32 accumulate)",
+             "mixed+surrogate": "mixed, exchange on 1/8 of walkers + bf16-weight MLP control variate"}[head]
     out = {
         "metric": METRIC,
         "value": m["value"],

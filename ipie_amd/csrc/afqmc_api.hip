@@ -46,13 +46,17 @@ struct afqmc_ctx {
   unsigned long long* nfb = nullptr;
   bool ws_ready = false;
   char describe_dims[128] = {0};
-  // split-BF16 exchange (AFQMC_PREC_MIXED)
-  int kpad = 0;
+  // split-BF16 operand planes (AFQMC_PREC_MIXED)
+  int kpad = 0;  // N rounded up to 32
+  int xpad = 0;  // X rounded up to 32
   bool bx_ok[2] = {false, false};
   int bx_wb[2] = {1, 1}, bx_xb[2] = {1, 1};
-  uint16_t* gplanes = nullptr;  // 4 x [W * nmax, kpad]
-  uint16_t* lplanes[2] = {nullptr, nullptr};  // 2 x [X * n_s, kpad] per spin
-  bool lplanes_ready = false;
+  bool bf16_fb = false;                       // force bias from split-BF16 planes
+  uint16_t* gplanes[2] = {nullptr, nullptr};  // per spin: 4 x [W * n_s, kpad] (Ghalf)
+  uint16_t* lplanes[2] = {nullptr, nullptr};  // per spin: 2 x [X * n_s, kpad] (half-rotated Cholesky)
+  uint16_t* xplanes = nullptr;                // 4 x [W, xpad] (shifted fields)
+  uint16_t* cplanes = nullptr;                // 2 x [P, xpad] (packed Cholesky)
+  bool planes_ready = false;                  // lplanes / cplanes hold the current static tensors
   int xs_used[2] = {1, 1};  // partial-sum count of the last exchange per spin
   int64_t exx_cap = 0;      // capacity (complex entries) of each exchange partial buffer
   double* gcomp = nullptr;  // [W, nmax, N] complex: Ghalf of sampled walkers (FP64 exchange)
@@ -176,13 +180,15 @@ int ensure_ws(afqmc_ctx* c) {
   ok = ok && dmalloc((void**)&c->exx_a, 16 * (size_t)c->exx_cap) == hipSuccess;
   ok = ok && dmalloc((void**)&c->exx_b, 16 * (size_t)c->exx_cap) == hipSuccess;
   ok = ok && dmalloc((void**)&c->gcomp, 16 * (size_t)W * nmax * N) == hipSuccess;
-  if (c->bx_ok[0] || c->bx_ok[1]) {
-    ok = ok && dmalloc((void**)&c->gplanes, 2 * (size_t)4 * W * nmax * c->kpad) == hipSuccess;
-    for (int s = 0; s < 2; ++s)
-      if (c->bx_ok[s]) {
-        const int n = s == 0 ? d.nalpha : d.nbeta;
-        ok = ok && dmalloc((void**)&c->lplanes[s], 2 * (size_t)2 * X * n * c->kpad) == hipSuccess;
-      }
+  for (int s = 0; s < 2; ++s)
+    if (c->bx_ok[s]) {
+      const int n = s == 0 ? d.nalpha : d.nbeta;
+      ok = ok && dmalloc((void**)&c->gplanes[s], 2 * (size_t)4 * W * n * c->kpad) == hipSuccess;
+      ok = ok && dmalloc((void**)&c->lplanes[s], 2 * (size_t)2 * X * n * c->kpad) == hipSuccess;
+    }
+  if (d.precision == AFQMC_PREC_MIXED) {
+    ok = ok && dmalloc((void**)&c->xplanes, 2 * (size_t)4 * W * c->xpad) == hipSuccess;
+    ok = ok && dmalloc((void**)&c->cplanes, 2 * (size_t)2 * P * c->xpad) == hipSuccess;
   }
   ok = ok && dmalloc((void**)&c->vhs, 16 * (size_t)W * N * N) == hipSuccess;
   for (int i = 0; i < 4; ++i) ok = ok && dmalloc((void**)&c->buf[i], 16 * (size_t)W * N * c->ncol) == hipSuccess;
@@ -212,6 +218,25 @@ int need_static(afqmc_ctx* c) {
   return ensure_ws(c);
 }
 
+// Split-BF16 planes of the static tensors (half-rotated and packed Cholesky),
+// made on first use after afqmc_set_static.
+int ensure_planes(afqmc_ctx* c) {
+  if (c->planes_ready) return AFQMC_OK;
+  const int N = c->d.nbasis;
+  for (int s = 0; s < 2; ++s)
+    if (c->bx_ok[s]) {
+      const int n = s == 0 ? c->d.nalpha : c->d.nbeta;
+      HIP_TRY(c, launch_split_real(s == 0 ? c->s.rchola : c->s.rcholb, (int64_t)c->d.nchol * n, N, c->kpad,
+                                   c->lplanes[s], c->stream));
+    }
+  if (c->cplanes) {
+    const int64_t P = (int64_t)N * (N + 1) / 2;
+    HIP_TRY(c, launch_split_real(c->s.chol_packed, P, c->d.nchol, c->xpad, c->cplanes, c->stream));
+  }
+  c->planes_ready = true;
+  return AFQMC_OK;
+}
+
 // ---------------------------------------------------------------- phases
 int do_gf(afqmc_ctx* c, int nw, const double* phia, const double* phib, double* ga, double* gb, double* ovlp,
           const double* log_shift, bool want_ghalf) {
@@ -235,7 +260,40 @@ int do_gf(afqmc_ctx* c, int nw, const double* phia, const double* phib, double* 
   return AFQMC_OK;
 }
 
+// Force bias from split-BF16 planes: one GEMM per spin, K split in
+// BF16_FB_SPLIT ranges, partial sums [spin * BF16_FB_SPLIT + z][W][X] summed
+// by the fields kernel.
+int do_force_bias_bf16(afqmc_ctx* c, int nw, const double* ga, const double* gb) {
+  const int N = c->d.nbasis;
+  int r = ensure_planes(c);
+  if (r) return r;
+  PhaseScope ps(c, AFQMC_PHASE_FORCE_BIAS);
+  const int nspin = two_spins(c) ? 2 : 1;
+  for (int s = 0; s < nspin; ++s) {
+    const int n = s == 0 ? c->d.nalpha : c->d.nbeta;
+    HIP_TRY(c, launch_split_ghalf(s == 0 ? ga : gb, (int64_t)nw * n, N, c->kpad, c->gplanes[s], c->stream));
+    Bf16GemmArgs b{};
+    b.a = c->gplanes[s];
+    b.a_plane = (int64_t)nw * n * c->kpad;
+    b.lda = (int64_t)n * c->kpad;
+    b.m = nw;
+    b.b = c->lplanes[s];
+    b.b_plane = (int64_t)c->d.nchol * n * c->kpad;
+    b.ldb = (int64_t)n * c->kpad;
+    b.nn = c->d.nchol;
+    b.kchunks = (int)(b.lda / 32);
+    b.ksplit = BF16_FB_SPLIT;
+    b.epilogue = BF16_EPI_PARTIAL;
+    b.out = c->vbias_part;
+    b.z_offset = s * BF16_FB_SPLIT;
+    b.scale = (c->d.rhf && c->d.nbeta > 0) ? 2.0 : 1.0;
+    HIP_TRY(c, launch_bf16_gemm(b, c->stream));
+  }
+  return AFQMC_OK;
+}
+
 int do_force_bias(afqmc_ctx* c, int nw, const double* ga, const double* gb) {
+  if (c->bf16_fb) return do_force_bias_bf16(c, nw, ga, gb);
   const int N = c->d.nbasis, na = c->d.nalpha, nb = c->d.nbeta;
   ForceBiasArgs a;
   a.nwalkers = nw;
@@ -285,6 +343,33 @@ int do_fields(afqmc_ctx* c, int nw, bool want_fields, const double* xi, uint64_t
 }
 
 int do_vhs(afqmc_ctx* c, int nw) {
+  if (c->cplanes) {  // split-BF16: VHS[w] = i sqrt(dt) sum_x xshifted[w, x] L_x
+    int r = ensure_planes(c);
+    if (r) return r;
+    PhaseScope ps(c, AFQMC_PHASE_VHS);
+    const int N = c->d.nbasis;
+    const int64_t P = (int64_t)N * (N + 1) / 2;
+    HIP_TRY(c, launch_split_ghalf(c->xshifted, nw, c->d.nchol, c->xpad, c->xplanes, c->stream));
+    Bf16GemmArgs b{};
+    b.a = c->xplanes;
+    b.a_plane = (int64_t)nw * c->xpad;
+    b.lda = c->xpad;
+    b.m = nw;
+    b.b = c->cplanes;
+    b.b_plane = P * c->xpad;
+    b.ldb = c->xpad;
+    b.nn = P;
+    b.kchunks = c->xpad / 32;
+    b.ksplit = 1;
+    b.epilogue = BF16_EPI_VHS;
+    b.out = c->vhs;
+    b.scale = sqrt(c->s.dt);
+    b.tri_i = c->tri;
+    b.tri_j = c->tri + P;
+    b.nbasis = N;
+    HIP_TRY(c, launch_bf16_gemm(b, c->stream));
+    return AFQMC_OK;
+  }
   VhsArgs a;
   a.nwalkers = nw;
   a.nbasis = c->d.nbasis;
@@ -400,17 +485,10 @@ int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g, const int* wa
   const double* rchol = spin == 0 ? c->s.rchola : c->s.rcholb;
   double* part = spin == 0 ? c->exx_a : c->exx_b;
   if (c->d.precision == AFQMC_PREC_MIXED && c->bx_ok[spin]) {
-    if (!c->lplanes_ready) {
-      for (int s = 0; s < 2; ++s)
-        if (c->bx_ok[s]) {
-          const int ns = s == 0 ? c->d.nalpha : c->d.nbeta;
-          HIP_TRY(c, launch_split_real(s == 0 ? c->s.rchola : c->s.rcholb, (int64_t)c->d.nchol * ns, N, c->kpad,
-                                       c->lplanes[s], c->stream));
-        }
-      c->lplanes_ready = true;
-    }
+    int r = ensure_planes(c);
+    if (r) return r;
     PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
-    HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes, c->stream, walker_of, n));
+    HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, walker_of, n));
     ExchangeBf16Args b;
     b.nwalkers = nw;
     b.nchol = c->d.nchol;
@@ -423,7 +501,7 @@ int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g, const int* wa
     if (xs > nxb) xs = nxb;
     if (xs < 1) xs = 1;
     b.xsplit = (int)xs;
-    b.gplanes = c->gplanes;
+    b.gplanes = c->gplanes[spin];
     b.lplanes = c->lplanes[spin];
     b.exx_part = part;
     if ((int64_t)b.xsplit * nw > c->exx_cap) return fail(c, AFQMC_ERR_SHAPE, "exchange partial buffer too small");
@@ -629,6 +707,9 @@ int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims) {
       const int n = s == 0 ? d.nalpha : d.nbeta;
       c->bx_ok[s] = bf16_exchange_plan(n, &c->bx_wb[s], &c->bx_xb[s]);
     }
+    c->xpad = (d.nchol + 31) / 32 * 32;
+    c->bf16_fb = d.nchol > 0 && c->bx_ok[0] && (!two || c->bx_ok[1]);
+    if (c->bf16_fb) c->fb_ksplit = BF16_FB_SPLIT * (two ? 2 : 1);
   }
   if (dmalloc((void**)&c->flags, sizeof(int)) != hipSuccess ||
       dmalloc((void**)&c->nfb, sizeof(unsigned long long)) != hipSuccess) {
@@ -650,8 +731,8 @@ void afqmc_destroy(afqmc_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   else (void)hipDeviceSynchronize();
   void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
-                  c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb, c->gplanes, c->lplanes[0],
-                  c->lplanes[1], c->gcomp, c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
+                  c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
+                  c->lplanes[1], c->xplanes, c->cplanes, c->gcomp, c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
                   c->sur.resid, c->sur.walker_of, c->sur.xbar, c->sur.e1, c->sur.master, c->sur.m,
                   c->sur.v, c->sur.wbf, c->sur.bias, c->sur.grad, c->sur.loss, c->sur.state};
   for (void* p : ptrs)
@@ -679,7 +760,7 @@ int afqmc_set_static(afqmc_ctx* c, const afqmc_static* s) {
   if (c->d.nbeta > 0 && !s->psib && two) return fail(c, AFQMC_ERR_INVALID_ARG, "null psib");
   c->s = *s;
   c->have_static = true;
-  c->lplanes_ready = false;
+  c->planes_ready = false;
   return AFQMC_OK;
 }
 

@@ -100,6 +100,26 @@ hipError_t launch_split_ghalf(const double* g, int64_t rows, int N, int Kp, uint
 hipError_t launch_split_real(const double* x, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream);
 hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream);
 
+// C = A B^T on split-BF16 planes: A 4 planes [m][lda] (re hi, re lo, im hi,
+// im lo), B 2 planes [nn][ldb] (hi, lo), K = kchunks * 32 (zero padded).
+enum { BF16_EPI_PARTIAL = 0, BF16_EPI_VHS = 1 };
+constexpr int BF16_FB_SPLIT = 8;  // K ranges per spin of the split-BF16 force bias
+struct Bf16GemmArgs {
+  const uint16_t* a;
+  int64_t a_plane, lda, m;
+  const uint16_t* b;
+  int64_t b_plane, ldb, nn;
+  int kchunks, ksplit;
+  int epilogue;
+  double* out;       // PARTIAL: [z_offset + ksplit][m][nn] complex; VHS: [m][N][N] complex
+  int z_offset;
+  double scale;      // PARTIAL: out = scale C; VHS: out = i scale C
+  const int* tri_i;  // VHS: packed index -> (i, j)
+  const int* tri_j;
+  int nbasis;
+};
+hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream);
+
 // ---------------------------------------------------------------- elementwise
 struct FieldsArgs {
   int nwalkers, nchol, ksplit;

@@ -150,6 +150,38 @@ __device__ __forceinline__ void bx_stash(uint4* lds_u4, const v4u (&pa)[BX_A_PER
 }
 
 
+// One K chunk from LDS stage `stage`: the wave's BX_MR x BX_MC tiles,
+// re += Re(A) B and im += Im(A) B, each as hi*hi + hi*lo + lo*hi.
+__device__ __forceinline__ void bx_chunk(const uint4* stage, int wr, int wc, int l16, int q,
+                                         v4f (&re)[BX_MR][BX_MC], v4f (&im)[BX_MR][BX_MC]) {
+  const uint16_t* As = reinterpret_cast<const uint16_t*>(stage);
+  const uint16_t* Bs = As + 8 * BX_A_VEC;
+  v8bf bh[BX_MC], bl[BX_MC];
+#pragma unroll
+  for (int j = 0; j < BX_MC; ++j) {
+    const int col = (wc * BX_MC + j) * 16 + l16;
+    bh[j] = lds_frag(Bs, col, q);
+    bl[j] = lds_frag(Bs, BX_C + col, q);
+  }
+#pragma unroll
+  for (int i = 0; i < BX_MR; ++i) {
+    const int row = (wr * BX_MR + i) * 16 + l16;
+    const v8bf arh = lds_frag(As, row, q);
+    const v8bf arl = lds_frag(As, BX_R + row, q);
+    const v8bf aih = lds_frag(As, 2 * BX_R + row, q);
+    const v8bf ail = lds_frag(As, 3 * BX_R + row, q);
+#pragma unroll
+    for (int j = 0; j < BX_MC; ++j) {
+      re[i][j] = mfma_bf16(arh, bh[j], re[i][j]);
+      re[i][j] = mfma_bf16(arh, bl[j], re[i][j]);
+      re[i][j] = mfma_bf16(arl, bh[j], re[i][j]);
+      im[i][j] = mfma_bf16(aih, bh[j], im[i][j]);
+      im[i][j] = mfma_bf16(aih, bl[j], im[i][j]);
+      im[i][j] = mfma_bf16(ail, bh[j], im[i][j]);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16Args a) {
   extern __shared__ uint4 lds_u4[];
   __shared__ double red[BX_THREADS / 64];
@@ -215,32 +247,7 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
       const bool more = c + 1 < nch, next_blk = !more && xblk + 1 < xb_end;
       if (next_blk) bx_set_b(b_src, tid, xblk + 1, xb, n, brows, bplane, Kp);
       if (more || next_blk) bx_fetch(pa, pb, a.gplanes, a.lplanes, a_src, b_src, more ? c + 1 : 0);
-      const uint16_t* As = reinterpret_cast<const uint16_t*>(lds_u4 + (c & 1) * BX_STAGE_VEC);
-      const uint16_t* Bs = As + 8 * BX_A_VEC;
-      v8bf bh[BX_MC], bl[BX_MC];
-#pragma unroll
-      for (int j = 0; j < BX_MC; ++j) {
-        const int col = (wc * BX_MC + j) * 16 + l16;
-        bh[j] = lds_frag(Bs, col, q);
-        bl[j] = lds_frag(Bs, BX_C + col, q);
-      }
-#pragma unroll
-      for (int i = 0; i < BX_MR; ++i) {
-        const int row = (wr * BX_MR + i) * 16 + l16;
-        const v8bf arh = lds_frag(As, row, q);
-        const v8bf arl = lds_frag(As, BX_R + row, q);
-        const v8bf aih = lds_frag(As, 2 * BX_R + row, q);
-        const v8bf ail = lds_frag(As, 3 * BX_R + row, q);
-#pragma unroll
-        for (int j = 0; j < BX_MC; ++j) {
-          re[i][j] = mfma_bf16(arh, bh[j], re[i][j]);
-          re[i][j] = mfma_bf16(arh, bl[j], re[i][j]);
-          re[i][j] = mfma_bf16(arl, bh[j], re[i][j]);
-          im[i][j] = mfma_bf16(aih, bh[j], im[i][j]);
-          im[i][j] = mfma_bf16(aih, bl[j], im[i][j]);
-          im[i][j] = mfma_bf16(ail, bh[j], im[i][j]);
-        }
-      }
+      bx_chunk(lds_u4 + (c & 1) * BX_STAGE_VEC, wr, wc, l16, q, re, im);
       if (c + 1 < nch) bx_stash(lds_u4, pa, pb, tid, (c + 1) & 1);
       __syncthreads();
     }
@@ -316,6 +323,91 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
   }
 }
 
+// C = A B^T for split-BF16 operand planes (A: 4 planes, complex; B: 2 planes,
+// real), K chunk range of split z, 128 x 160 tile per workgroup.  Epilogues:
+//   BF16_EPI_PARTIAL  out[z_offset + z][row][col] = scale C     (force bias)
+//   BF16_EPI_VHS      out[w][i][j] = out[w][j][i] = i scale C[w][p],
+//                     (i, j) = (tri_i[p], tri_j[p])              (VHS)
+// Workgroups that run together share the B tile (row blocks vary fastest).
+__global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
+  extern __shared__ uint4 lds_u4[];
+  const int mb = (int)ceil_div(a.m, BX_R), nb = (int)ceil_div(a.nn, BX_C);
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int rb = lin % mb, cb = (lin / mb) % nb, z = lin / mb / nb;
+  if (z >= a.ksplit) return;
+  const int c0 = (int)((int64_t)a.kchunks * z / a.ksplit), c1 = (int)((int64_t)a.kchunks * (z + 1) / a.ksplit);
+
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
+  const int wr = wave % BX_GR, wc = wave / BX_GR;
+  // staged rows past the end are clamped to the last row (their outputs are not written)
+  int64_t a_src[BX_A_PER], b_src[BX_B_PER];
+#pragma unroll
+  for (int i = 0; i < BX_A_PER; ++i) {
+    const int v = tid + i * BX_THREADS;
+    const int p = v / (BX_R * 4), rem = v % (BX_R * 4);
+    int64_t r = (int64_t)rb * BX_R + rem / 4;
+    if (r > a.m - 1) r = a.m - 1;
+    a_src[i] = p * a.a_plane + r * a.lda + (rem % 4) * 8;
+  }
+#pragma unroll
+  for (int i = 0; i < BX_B_PER; ++i) {
+    const int v = (tid + i * BX_THREADS) % BX_B_VEC;
+    const int p = v / (BX_C * 4), rem = v % (BX_C * 4);
+    int64_t r = (int64_t)cb * BX_C + rem / 4;
+    if (r > a.nn - 1) r = a.nn - 1;
+    b_src[i] = p * a.b_plane + r * a.ldb + (rem % 4) * 8;
+  }
+  v4f re[BX_MR][BX_MC], im[BX_MR][BX_MC];
+#pragma unroll
+  for (int i = 0; i < BX_MR; ++i)
+#pragma unroll
+    for (int j = 0; j < BX_MC; ++j) {
+      re[i][j] = zero4f();
+      im[i][j] = zero4f();
+    }
+  v4u pa[BX_A_PER], pb[BX_B_PER];
+  if (c0 < c1) {
+    bx_fetch(pa, pb, a.a, a.b, a_src, b_src, c0);
+    bx_stash(lds_u4, pa, pb, tid, 0);
+    __syncthreads();
+  }
+  for (int c = c0; c < c1; ++c) {
+    const bool more = c + 1 < c1;
+    if (more) bx_fetch(pa, pb, a.a, a.b, a_src, b_src, c + 1);
+    bx_chunk(lds_u4 + ((c - c0) & 1) * BX_STAGE_VEC, wr, wc, l16, q, re, im);
+    if (more) bx_stash(lds_u4, pa, pb, tid, (c + 1 - c0) & 1);
+    __syncthreads();
+  }
+  const int64_t NN = (int64_t)a.nbasis * a.nbasis;
+#pragma unroll
+  for (int j = 0; j < BX_MC; ++j) {
+    const int64_t col = (int64_t)cb * BX_C + (wc * BX_MC + j) * 16 + l16;
+    if (col >= a.nn) continue;
+    int ti = 0, tj = 0;
+    if (a.epilogue == BF16_EPI_VHS) {
+      ti = a.tri_i[col];
+      tj = a.tri_j[col];
+    }
+#pragma unroll
+    for (int i = 0; i < BX_MR; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = (int64_t)rb * BX_R + (wr * BX_MR + i) * 16 + 4 * q + r;
+        if (row >= a.m) continue;
+        const double vr = a.scale * re[i][j][r], vi = a.scale * im[i][j][r];
+        if (a.epilogue == BF16_EPI_VHS) {
+          // i scale (re + i im) = scale (-im + i re)
+          const cplx v = cmake(-vi, vr);
+          double* base = a.out + 2 * row * NN;
+          st(base, (int64_t)ti * a.nbasis + tj, v);
+          if (ti != tj) st(base, (int64_t)tj * a.nbasis + ti, v);
+        } else {
+          st(a.out, ((int64_t)(a.z_offset + z) * a.m + row) * a.nn + col, cmake(vr, vi));
+        }
+      }
+  }
+}
+
 size_t bf16_lds_bytes(int wb, int nocc) {
   const size_t epi = sizeof(float2) * (size_t)(wb * nocc) * (nocc + 1);  // one vector
   return BX_LDS_STAGES > epi ? BX_LDS_STAGES : epi;
@@ -356,6 +448,18 @@ hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream) {
   if (e != hipSuccess) return e;
   const int nwb = (int)ceil_div(a.nwalkers, a.wb);
   hipLaunchKernelGGL(exchange_bf16_kernel, dim3(nwb * a.xsplit), dim3(BX_THREADS), lds, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream) {
+  if (a.m < 1 || a.nn < 1 || a.kchunks < 1 || a.ksplit < 1 || a.lda % BX_KC != 0 || a.ldb % BX_KC != 0 ||
+      (int64_t)a.kchunks * BX_KC > a.lda || (int64_t)a.kchunks * BX_KC > a.ldb)
+    return hipErrorInvalidValue;
+  if (a.epilogue == BF16_EPI_VHS && (int64_t)a.nbasis * (a.nbasis + 1) / 2 != a.nn) return hipErrorInvalidValue;
+  hipError_t e = allow_lds((const void*)bf16_gemm_kernel, BX_LDS_STAGES);
+  if (e != hipSuccess) return e;
+  const int64_t grid = ceil_div(a.m, BX_R) * ceil_div(a.nn, BX_C) * a.ksplit;
+  hipLaunchKernelGGL(bf16_gemm_kernel, dim3((unsigned)grid), dim3(BX_THREADS), BX_LDS_STAGES, stream, a);
   return hipGetLastError();
 }
 
