@@ -16,7 +16,7 @@ SOURCES = [
     "k_greens.hip",
     "k_gemm.hip",
     "k_exchange.hip",
-    "k_exchange_bf16.hip",
+    "k_bf16.hip",
     "k_surrogate.hip",
     "k_fields.hip",
     "k_qr.hip",

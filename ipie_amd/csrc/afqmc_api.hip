@@ -8,6 +8,7 @@
 #include <string.h>
 #include <cstring>
 
+#include <algorithm>
 #include <cmath>
 #include <random>
 #include <vector>
@@ -56,6 +57,8 @@ struct afqmc_ctx {
   uint16_t* lplanes[2] = {nullptr, nullptr};  // per spin: 2 x [X * n_s, kpad] (half-rotated Cholesky)
   uint16_t* xplanes = nullptr;                // 4 x [W, xpad] (shifted fields)
   uint16_t* cplanes = nullptr;                // 2 x [P, xpad] (packed Cholesky)
+  uint16_t* vplanes = nullptr;                // 4 x [W N, kpad] (VHS, Taylor series)
+  uint16_t* tplanes[2] = {nullptr, nullptr};  // 4 x [W, rpad, kpad] each (Taylor terms, transposed)
   bool planes_ready = false;                  // lplanes / cplanes hold the current static tensors
   int xs_used[2] = {1, 1};  // partial-sum count of the last exchange per spin
   int64_t exx_cap = 0;      // capacity (complex entries) of each exchange partial buffer
@@ -77,6 +80,13 @@ struct afqmc_ctx {
     double* resid = nullptr;
     int* walker_of = nullptr;
     double* xbar = nullptr;  // [2, W]: mean, max
+    // physical control variate E_T + 2 <K, dG> + Q_bf16(dG)
+    bool cv_ready = false;   // gT, K, E_T hold the current trial / Hamiltonian
+    double* gT = nullptr;    // [2][nmax, N] complex trial Ghalf per spin
+    double* K = nullptr;     // [2][nmax, N] complex
+    double* et = nullptr;    // [2] complex E_T per spin
+    double* cv = nullptr;    // [W] complex
+    double* gtmp = nullptr;  // [2] complex overlap scratch of the trial GF
     double* e1 = nullptr;    // [W] complex
     float* master = nullptr;
     float* m = nullptr;
@@ -163,6 +173,20 @@ hipError_t dmalloc(void** p, size_t bytes) {
   return hipMalloc(p, bytes);
 }
 
+// Split-BF16 planes of Ghalf and rchol per spin (mixed precision, surrogate).
+int alloc_exchange_planes(afqmc_ctx* c) {
+  const int64_t W = c->d.nwalkers, X = c->d.nchol;
+  for (int s = 0; s < 2; ++s) {
+    if (!c->bx_ok[s] || c->gplanes[s]) continue;
+    const int n = s == 0 ? c->d.nalpha : c->d.nbeta;
+    if (dmalloc((void**)&c->gplanes[s], 2 * (size_t)4 * W * n * c->kpad) != hipSuccess ||
+        dmalloc((void**)&c->lplanes[s], 2 * (size_t)2 * X * n * c->kpad) != hipSuccess)
+      return fail(c, AFQMC_ERR_ALLOC, "bf16 plane allocation failed");
+    c->planes_ready = false;
+  }
+  return AFQMC_OK;
+}
+
 // Workspace is allocated on first use by an entry point that needs it (the
 // Green's function and overlap need none).
 int ensure_ws(afqmc_ctx* c) {
@@ -180,15 +204,16 @@ int ensure_ws(afqmc_ctx* c) {
   ok = ok && dmalloc((void**)&c->exx_a, 16 * (size_t)c->exx_cap) == hipSuccess;
   ok = ok && dmalloc((void**)&c->exx_b, 16 * (size_t)c->exx_cap) == hipSuccess;
   ok = ok && dmalloc((void**)&c->gcomp, 16 * (size_t)W * nmax * N) == hipSuccess;
-  for (int s = 0; s < 2; ++s)
-    if (c->bx_ok[s]) {
-      const int n = s == 0 ? d.nalpha : d.nbeta;
-      ok = ok && dmalloc((void**)&c->gplanes[s], 2 * (size_t)4 * W * n * c->kpad) == hipSuccess;
-      ok = ok && dmalloc((void**)&c->lplanes[s], 2 * (size_t)2 * X * n * c->kpad) == hipSuccess;
-    }
+  if (d.precision == AFQMC_PREC_MIXED) ok = ok && alloc_exchange_planes(c) == AFQMC_OK;
   if (d.precision == AFQMC_PREC_MIXED) {
     ok = ok && dmalloc((void**)&c->xplanes, 2 * (size_t)4 * W * c->xpad) == hipSuccess;
     ok = ok && dmalloc((void**)&c->cplanes, 2 * (size_t)2 * P * c->xpad) == hipSuccess;
+    const size_t tbytes = 2 * (size_t)4 * W * taylor_rows_padded(c->ncol) * c->kpad;
+    ok = ok && dmalloc((void**)&c->vplanes, 2 * (size_t)4 * W * N * c->kpad) == hipSuccess;
+    for (int i = 0; i < 2; ++i) {
+      ok = ok && dmalloc((void**)&c->tplanes[i], tbytes) == hipSuccess;
+      ok = ok && hipMemset(c->tplanes[i], 0, tbytes) == hipSuccess;  // padding stays zero
+    }
   }
   ok = ok && dmalloc((void**)&c->vhs, 16 * (size_t)W * N * N) == hipSuccess;
   for (int i = 0; i < 4; ++i) ok = ok && dmalloc((void**)&c->buf[i], 16 * (size_t)W * N * c->ncol) == hipSuccess;
@@ -224,7 +249,7 @@ int ensure_planes(afqmc_ctx* c) {
   if (c->planes_ready) return AFQMC_OK;
   const int N = c->d.nbasis;
   for (int s = 0; s < 2; ++s)
-    if (c->bx_ok[s]) {
+    if (c->lplanes[s]) {
       const int n = s == 0 ? c->d.nalpha : c->d.nbeta;
       HIP_TRY(c, launch_split_real(s == 0 ? c->s.rchola : c->s.rcholb, (int64_t)c->d.nchol * n, N, c->kpad,
                                    c->lplanes[s], c->stream));
@@ -456,8 +481,36 @@ int scatter(afqmc_ctx* c, int nw, const double* src, double* phia, double* phib,
 }
 
 // Taylor series of exp(VHS) applied to P; returns the buffer holding the result.
+// Split-BF16 Taylor series: Q = P + sum_k out_k with out_k^T = out_{k-1}^T VHS / k.
+int taylor_bf16(afqmc_ctx* c, int nw, const double** result) {
+  const int N = c->d.nbasis, nc = c->ncol, rpad = taylor_rows_padded(nc);
+  double* P = c->buf[0];
+  double* Q = c->buf[3];
+  HIP_TRY(c, launch_split_ghalf(c->vhs, (int64_t)nw * N, N, c->kpad, c->vplanes, c->stream));
+  HIP_TRY(c, launch_split_transpose(P, nw, (int)c->d.nwalkers, N, nc, rpad, c->kpad, c->tplanes[0], c->stream));
+  for (int k = 1; k <= c->d.exp_nmax; ++k) {
+    TaylorBf16Args t{};
+    t.nwalkers = nw;
+    t.wcap = (int)c->d.nwalkers;
+    t.nbasis = N;
+    t.ncol = nc;
+    t.rpad = rpad;
+    t.kpad = c->kpad;
+    t.tin = c->tplanes[(k - 1) & 1];
+    t.tout = k < c->d.exp_nmax ? c->tplanes[k & 1] : nullptr;
+    t.vplanes = c->vplanes;
+    t.init = k == 1 ? P : nullptr;
+    t.q = Q;
+    t.alpha = 1.0 / k;
+    HIP_TRY(c, launch_taylor_bf16(t, c->stream));
+  }
+  *result = Q;
+  return AFQMC_OK;
+}
+
 int taylor(afqmc_ctx* c, int nw, const double** result) {
   PhaseScope ps(c, AFQMC_PHASE_TAYLOR);
+  if (c->vplanes && c->d.exp_nmax > 0) return taylor_bf16(c, nw, result);
   const int N = c->d.nbasis, nc = c->ncol;
   const int64_t NN = (int64_t)N * N, stride = (int64_t)N * nc;
   double* P = c->buf[0];
@@ -489,7 +542,8 @@ int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g, const int* wa
     if (r) return r;
     PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
     HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, walker_of, n));
-    ExchangeBf16Args b;
+    ExchangeBf16Args b{};
+    b.split = 1;
     b.nwalkers = nw;
     b.nchol = c->d.nchol;
     b.nocc = n;
@@ -588,6 +642,68 @@ int64_t inv_mod(int64_t a, int64_t m) {
 
 // Local energy of nw walkers from their Ghalf with the learned exchange control
 // variate (requires the force bias + fields of this Green's function).
+// Trial Ghalf, K = sum_x T0_x L_x and E_T = sum_x tr(T0_x T0_x) per spin, in
+// FP64, once per trial / Hamiltonian (T0_x = L_x G_T^T).
+int sur_prepare_cv(afqmc_ctx* c) {
+  auto& S = c->sur;
+  if (S.cv_ready) return AFQMC_OK;
+  const int N = c->d.nbasis, X = c->d.nchol, nmax = std::max(c->d.nalpha, c->d.nbeta);
+  const int64_t gstride = 2 * (int64_t)nmax * N;
+  int r = do_gf(c, 1, c->s.psia, c->s.psib, S.gT, S.gT + gstride, S.gtmp, nullptr, true);
+  if (r) return r;
+  double* T0 = nullptr;
+  double* part = nullptr;
+  if (dmalloc((void**)&T0, 16 * (size_t)X * nmax * nmax) != hipSuccess ||
+      dmalloc((void**)&part, 16 * (size_t)X) != hipSuccess) {
+    if (T0) (void)hipFree(T0);
+    return fail(c, AFQMC_ERR_ALLOC, "control-variate set-up allocation failed");
+  }
+  const int nspin = two_spins(c) ? 2 : 1;
+  hipError_t e = hipSuccess;
+  for (int s = 0; s < nspin && e == hipSuccess; ++s) {
+    const int n = s == 0 ? c->d.nalpha : c->d.nbeta;
+    e = launch_cv_static(s == 0 ? c->s.rchola : c->s.rcholb, S.gT + s * gstride, X, n, N, T0, S.K + s * gstride,
+                         part, S.et + 2 * s, c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(T0);
+  (void)hipFree(part);
+  if (e != hipSuccess) return fail(c, AFQMC_ERR_HIP, "control-variate set-up: %s", hipGetErrorString(e));
+  S.cv_ready = true;
+  return AFQMC_OK;
+}
+
+// Q_s(w) = sum_x tr(dT_x dT_x), dT_x = L_x (G_s,w - G_T,s)^T, single BF16
+// products, partial sums into the exchange partial buffer of spin s.
+int exchange_q_spin(afqmc_ctx* c, int nw, int spin, const double* g) {
+  const int N = c->d.nbasis;
+  const int n = spin == 0 ? c->d.nalpha : c->d.nbeta;
+  const int64_t gstride = 2 * (int64_t)std::max(c->d.nalpha, c->d.nbeta) * N;
+  PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
+  HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, nullptr, n,
+                                c->sur.gT + spin * gstride));
+  ExchangeBf16Args b{};
+  b.split = 0;
+  b.nwalkers = nw;
+  b.nchol = c->d.nchol;
+  b.nocc = n;
+  b.kpad = c->kpad;
+  b.wb = c->bx_wb[spin];
+  b.xb = c->bx_xb[spin];
+  const int64_t nwb = ceil_div(nw, b.wb), nxb = ceil_div(c->d.nchol, b.xb);
+  int64_t xs = ceil_div(ceil_div(2048, nwb), 8) * 8;
+  if (xs > nxb) xs = nxb;
+  if (xs < 1) xs = 1;
+  b.xsplit = (int)xs;
+  b.gplanes = c->gplanes[spin];
+  b.lplanes = c->lplanes[spin];
+  b.exx_part = spin == 0 ? c->exx_a : c->exx_b;
+  if ((int64_t)b.xsplit * nw > c->exx_cap) return fail(c, AFQMC_ERR_SHAPE, "exchange partial buffer too small");
+  HIP_TRY(c, launch_exchange_bf16(b, c->stream));
+  c->xs_used[spin] = b.xsplit;
+  return AFQMC_OK;
+}
+
 int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb, const double* ovlp,
                         double* energy) {
   auto& S = c->sur;
@@ -603,6 +719,35 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   } else {
     HIP_TRY(c, hipMemsetAsync(S.pred, 0, 16 * (size_t)nw, c->stream));
   }
+  // physical part of the predictor: E_T + 2 <K, dG> + Q_bf16(dG), added to pred
+  r = sur_prepare_cv(c);
+  if (r) return r;
+  r = ensure_planes(c);
+  if (r) return r;
+  const int nspin = two_spins(c) ? 2 : 1;
+  const int N = c->d.nbasis;
+  const int64_t gstride = 2 * (int64_t)std::max(c->d.nalpha, c->d.nbeta) * N;
+  for (int s = 0; s < nspin; ++s) {
+    r = exchange_q_spin(c, nw, s, s == 0 ? ga : gb);
+    if (r) return r;
+  }
+  SurrogateCvArgs cva{};
+  cva.nwalkers = nw;
+  cva.nbasis = N;
+  cva.nspin = nspin;
+  for (int s = 0; s < nspin; ++s) {
+    cva.nocc[s] = s == 0 ? c->d.nalpha : c->d.nbeta;
+    cva.g[s] = s == 0 ? ga : gb;
+    cva.gT[s] = S.gT + s * gstride;
+    cva.K[s] = S.K + s * gstride;
+    cva.part[s] = s == 0 ? c->exx_a : c->exx_b;
+    cva.xs[s] = c->xs_used[s];
+  }
+  cva.et = S.et;
+  cva.cv = S.cv;
+  cva.pred = S.pred;
+  HIP_TRY(c, launch_sur_cv(cva, c->stream));
+  // exact exchange on the sampled subset
   int k = nw;
   int64_t pa = 1, pb = 0;
   if (!warm) {
@@ -632,6 +777,7 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   cb.est = S.est;
   cb.target = S.target;
   cb.resid = S.resid;
+  cb.cv = S.cv;
   HIP_TRY(c, launch_sur_combine(cb, c->stream));
   const int iters = warm ? S.train_iters : S.online_iters;
   if (warm) {
@@ -702,11 +848,12 @@ int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims) {
   snprintf(c->describe_dims, sizeof(c->describe_dims), "N=%d X=%d na=%d nb=%d W=%d", d.nbasis, d.nchol, d.nalpha,
            d.nbeta, d.nwalkers);
   c->kpad = (d.nbasis + 31) / 32 * 32;
+  // split-BF16 exchange geometry (mixed precision, and the surrogate's control variate)
+  for (int s = 0; s < (two ? 2 : 1); ++s) {
+    const int n = s == 0 ? d.nalpha : d.nbeta;
+    c->bx_ok[s] = bf16_exchange_plan(n, &c->bx_wb[s], &c->bx_xb[s]);
+  }
   if (d.precision == AFQMC_PREC_MIXED) {
-    for (int s = 0; s < (two ? 2 : 1); ++s) {
-      const int n = s == 0 ? d.nalpha : d.nbeta;
-      c->bx_ok[s] = bf16_exchange_plan(n, &c->bx_wb[s], &c->bx_xb[s]);
-    }
     c->xpad = (d.nchol + 31) / 32 * 32;
     c->bf16_fb = d.nchol > 0 && c->bx_ok[0] && (!two || c->bx_ok[1]);
     if (c->bf16_fb) c->fb_ksplit = BF16_FB_SPLIT * (two ? 2 : 1);
@@ -732,9 +879,11 @@ void afqmc_destroy(afqmc_ctx* c) {
   else (void)hipDeviceSynchronize();
   void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
                   c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
-                  c->lplanes[1], c->xplanes, c->cplanes, c->gcomp, c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
+                  c->lplanes[1], c->xplanes, c->cplanes, c->vplanes, c->tplanes[0],
+                  c->tplanes[1], c->gcomp, c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
                   c->sur.resid, c->sur.walker_of, c->sur.xbar, c->sur.e1, c->sur.master, c->sur.m,
-                  c->sur.v, c->sur.wbf, c->sur.bias, c->sur.grad, c->sur.loss, c->sur.state};
+                  c->sur.v, c->sur.wbf, c->sur.bias, c->sur.grad, c->sur.loss, c->sur.state, c->sur.gT,
+                  c->sur.K, c->sur.et, c->sur.cv, c->sur.gtmp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
@@ -761,6 +910,7 @@ int afqmc_set_static(afqmc_ctx* c, const afqmc_static* s) {
   c->s = *s;
   c->have_static = true;
   c->planes_ready = false;
+  c->sur.cv_ready = false;
   return AFQMC_OK;
 }
 
@@ -1013,8 +1163,19 @@ int afqmc_surrogate_configure(afqmc_ctx* c, int enable, double fraction, int war
     ok = ok && dmalloc((void**)&S.grad, sizeof(float) * (size_t)groups * SUR_NPARAM) == hipSuccess;
     ok = ok && dmalloc((void**)&S.loss, sizeof(float) * (size_t)groups) == hipSuccess;
     ok = ok && dmalloc((void**)&S.state, sizeof(SurrogateState)) == hipSuccess;
+    const int64_t nmax = std::max(c->d.nalpha, c->d.nbeta);
+    ok = ok && dmalloc((void**)&S.gT, 2 * 16 * (size_t)nmax * c->d.nbasis) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.K, 2 * 16 * (size_t)nmax * c->d.nbasis) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.et, 2 * 16) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.cv, 16 * W) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.gtmp, 2 * 16) == hipSuccess;
     if (!ok) return fail(c, AFQMC_ERR_ALLOC, "surrogate allocation failed");
   }
+  if (!c->bx_ok[0] || (two_spins(c) && !c->bx_ok[1]))
+    return fail(c, AFQMC_ERR_UNSUPPORTED, "the surrogate needs n <= 128 occupied orbitals per spin");
+  r = alloc_exchange_planes(c);
+  if (r) return r;
+  S.cv_ready = false;
   S.enabled = true;
   S.fraction = fraction;
   S.warmup = warmup_steps;

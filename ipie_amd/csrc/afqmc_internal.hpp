@@ -90,13 +90,15 @@ int exchange_xsplit(int nwalkers, int nchol, int nocc);
 struct ExchangeBf16Args {
   int nwalkers, nchol, nocc, kpad;  // kpad = N rounded up to 32
   int wb, xb, xsplit;  // walkers and Cholesky vectors per workgroup tile, Cholesky ranges
+  int split;           // 1: hi*hi + hi*lo + lo*hi products; 0: hi*hi only
+  int lds_bytes;       // set by the launcher
   const uint16_t* gplanes;  // 4 x [W*n, kpad] bf16: re_hi, re_lo, im_hi, im_lo
   const uint16_t* lplanes;  // 2 x [X*n, kpad] bf16: hi, lo
   double* exx_part;         // [xsplit, W] complex
 };
 bool bf16_exchange_plan(int nocc, int* wb, int* xb);
 hipError_t launch_split_ghalf(const double* g, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream,
-                              const int* walker_of = nullptr, int nocc = 1);
+                              const int* walker_of = nullptr, int nocc = 1, const double* gref = nullptr);
 hipError_t launch_split_real(const double* x, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream);
 hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream);
 
@@ -119,6 +121,22 @@ struct Bf16GemmArgs {
   int nbasis;
 };
 hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream);
+
+// One split-BF16 Taylor term per walker: out_k^T = alpha out_{k-1}^T VHS,
+// q = (init or q) + out_k, and (tout) the split planes of out_k^T.
+struct TaylorBf16Args {
+  int nwalkers, wcap, nbasis, ncol, rpad, kpad;
+  const uint16_t* tin;      // [4][wcap][rpad][kpad] planes of out_{k-1}^T
+  uint16_t* tout;           // same layout for out_k^T, or null (last term)
+  const uint16_t* vplanes;  // [4][W N][kpad] planes of VHS
+  const double* init;       // [W][N][ncol] complex (first term) or null
+  double* q;                // [W][N][ncol] complex accumulator
+  double alpha;
+};
+hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream);
+hipError_t launch_split_transpose(const double* phi, int nwalkers, int wcap, int N, int ncol, int rpad, int Kp,
+                                  uint16_t* planes, hipStream_t stream);
+int taylor_rows_padded(int ncol);
 
 // ---------------------------------------------------------------- elementwise
 struct FieldsArgs {
@@ -238,9 +256,25 @@ struct SurrogateCombineArgs {
   const double* part_b;    // [xs_b, k] complex or null
   int xs_b;
   double* est;             // [W] complex
-  double* target;          // [k][2] (exact values of the samples) or null
+  double* target;          // [k][2] exact - cv of the samples (the MLP's target) or null
   double* resid;           // [k] |exact - g|^2 or null
+  const double* cv;        // [W] complex physical part of g, or null
 };
+struct SurrogateCvArgs {
+  int nwalkers, nbasis, nspin;
+  int nocc[2];
+  const double* g[2];      // Ghalf per spin [W, n_s, N] complex
+  const double* gT[2];     // trial Ghalf per spin [n_s, N] complex
+  const double* K[2];      // [n_s, N] complex
+  const double* et;        // [2] complex E_T per spin
+  const double* part[2];   // [xs_s, W] complex partial sums of Q
+  int xs[2];
+  double* cv;              // [W] complex
+  double* pred;            // [W] complex, += cv
+};
+hipError_t launch_cv_static(const double* L, const double* gT, int X, int n, int N, double* T0, double* K,
+                            double* part, double* et, hipStream_t s);
+hipError_t launch_sur_cv(const SurrogateCvArgs& a, hipStream_t s);
 hipError_t launch_sur_features(const SurrogateFeatureArgs& a, hipStream_t s);
 hipError_t launch_sur_stats(const SurrogateStatsArgs& a, hipStream_t s);
 hipError_t launch_sur_forward(const SurrogateForwardArgs& a, hipStream_t s);

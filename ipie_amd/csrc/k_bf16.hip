@@ -1,0 +1,689 @@
+// Split-BF16 kernels of the mixed-precision step (AFQMC_PREC_MIXED) and of the
+// exchange control variate.
+//
+// Every FP64 operand x is split into x_hi = bf16(x), x_lo = bf16(x - x_hi);
+// a product A B is formed as A_hi B_hi + A_hi B_lo + A_lo B_hi with
+// v_mfma_f32_16x16x32_bf16 (FP32 accumulation), which keeps about 16
+// significant bits per product (relative error ~1e-5 per element).
+//   exchange_bf16_kernel  sum_x tr(T_x T_x), T_x = L_x G^T, trace in FP64
+//                         (ipie/estimators/local_energy_sd.py:263-303, 1040-1071);
+//                         also the single-product form (hi * hi only) used by the
+//                         control variate
+//   bf16_gemm_kernel      force bias (ipie/propagation/force_bias.py:79-121) and
+//                         VHS (ipie/propagation/phaseless_generic.py:67-97)
+//   taylor_bf16_kernel    one term of the Taylor series of exp(VHS) phi
+//                         (ipie/propagation/operations.py:58-117)
+//
+// Operand planes are bf16 [rows, Kp] with Kp = K rounded up to 32 and zero
+// padding, so a 32-wide K chunk of a row is one aligned 64-byte segment:
+//   Ghalf: 4 planes (re_hi, re_lo, im_hi, im_lo), rows = (walker, j)
+//   rchol: 2 planes (hi, lo), rows = (x, i)
+// Exchange / GEMM workgroup: 8 waves in a 4 x 2 grid, each wave owns 2 x 5
+// output tiles of 16 x 16, so one workgroup covers 128 rows (wb walkers) by
+// 160 columns (xb Cholesky vectors); two waves share each SIMD, so one wave's
+// LDS and barrier waits are covered by the other's MFMAs.  Double-buffered LDS
+// staging, one barrier per K chunk.  Workgroups that run together on one XCD
+// cover a few walker blocks x 8 Cholesky ranges, so both operand tiles are
+// shared through that XCD's L2.
+#include "afqmc_internal.hpp"
+
+namespace afqmc {
+
+namespace {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+// native 16-byte vector: plain loads and stores, kept in registers (a uint4
+// struct copy lowers to a memcpy that pins the staging arrays to scratch)
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+// 8 waves in a BX_GR x BX_GC grid, each owning BX_MR x BX_MC tiles of 16 x 16
+constexpr int BX_GR = 4, BX_GC = 2, BX_MR = 2, BX_MC = 5;
+constexpr int BX_R = 16 * BX_GR * BX_MR;  // rows of a workgroup tile (walker orbitals)
+constexpr int BX_C = 16 * BX_GC * BX_MC;  // columns (Cholesky-vector orbitals)
+constexpr int BX_THREADS = 64 * BX_GR * BX_GC;
+constexpr int BX_KC = 32;
+constexpr size_t BX_LDS_MAX = 106496;  // LDS of a workgroup: two stages of the 3-product form
+
+// Staging of one K chunk, in 16-byte vectors.  SPLIT (3 products): A planes
+// re_hi, re_lo, im_hi, im_lo and B planes hi, lo.  Otherwise (1 product,
+// hi * hi): A planes re_hi, im_hi (physical planes 0 and 2) and B plane hi.
+template <bool SPLIT>
+struct Bx {
+  static constexpr int NA = SPLIT ? 4 : 2, NB = SPLIT ? 2 : 1;
+  static constexpr int A_VEC = NA * BX_R * 4, B_VEC = NB * BX_C * 4;
+  static constexpr int STAGE_VEC = A_VEC + B_VEC;
+  static constexpr int A_PER = A_VEC / BX_THREADS;
+  static constexpr int B_PER = (B_VEC + BX_THREADS - 1) / BX_THREADS;
+  static constexpr size_t STAGES = 2 * 16 * (size_t)STAGE_VEC;  // double-buffered, bytes
+  static_assert(A_VEC % BX_THREADS == 0, "A staging is split evenly over the threads");
+  static_assert((B_VEC % BX_THREADS) % 64 == 0, "the partial B vector round is wave-uniform");
+  static_assert(STAGES <= BX_LDS_MAX, "staging fits the workgroup LDS");
+  __device__ static int a_plane(int p) { return SPLIT ? p : 2 * p; }
+  __device__ static int b_plane(int p) { return SPLIT ? p : 0; }
+};
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+__device__ __forceinline__ void split_bf16(double x, uint16_t& hi, uint16_t& lo) {
+  hi = f2bf((float)x);
+  lo = f2bf((float)(x - (double)bf2f(hi)));
+}
+
+__device__ __forceinline__ v8bf as_v8bf(v4u u) { return __builtin_bit_cast(v8bf, u); }
+
+__device__ __forceinline__ v4f mfma_bf16(v8bf a, v8bf b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// LDS slot of 16-byte vector `seg` (0..3) of staged row `row`.  Rows are 64
+// bytes, so without a swizzle rows r and r + 4 of one fragment read (16 rows,
+// same segment) fall on the same banks; XOR-ing the segment with bits 2..3 of
+// the row spreads the 16 rows over all 64 banks.
+__device__ __forceinline__ int lds_vec(int row, int seg) { return row * 4 + (seg ^ ((row >> 2) & 3)); }
+
+__device__ __forceinline__ v8bf lds_frag(const uint16_t* base, int row, int seg) {
+  return as_v8bf(reinterpret_cast<const v4u*>(base)[lds_vec(row, seg)]);
+}
+
+// Ghalf [W*n, N] complex -> 4 bf16 planes [rows, Kp]; with walker_of, plane
+// row t*n + j takes Ghalf row walker_of[t]*n + j (sampled walkers, compacted);
+// with gref [n, N], row j of gref is subtracted first (dG = G - G_T).
+__global__ void split_ghalf_kernel(const double* __restrict__ g, int64_t rows, int N, int Kp,
+                                   uint16_t* __restrict__ planes, const int* __restrict__ walker_of, int nocc,
+                                   const double* __restrict__ gref) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * Kp) return;
+  const int64_t r = idx / Kp;
+  const int k = (int)(idx % Kp);
+  uint16_t rh = 0, rl = 0, ih = 0, il = 0;
+  if (k < N) {
+    const int64_t src = walker_of ? (int64_t)walker_of[r / nocc] * nocc + r % nocc : r;
+    cplx v = ld(g, src * N + k);
+    if (gref) v = csub(v, ld(gref, (r % nocc) * N + k));
+    split_bf16(v.re, rh, rl);
+    split_bf16(v.im, ih, il);
+  }
+  const int64_t plane = rows * Kp;
+  planes[idx] = rh;
+  planes[plane + idx] = rl;
+  planes[2 * plane + idx] = ih;
+  planes[3 * plane + idx] = il;
+}
+
+// rchol [rows, N] real -> 2 bf16 planes [rows, Kp].
+__global__ void split_real_kernel(const double* __restrict__ x, int64_t rows, int N, int Kp,
+                                  uint16_t* __restrict__ planes) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * Kp) return;
+  const int64_t r = idx / Kp;
+  const int k = (int)(idx % Kp);
+  uint16_t h = 0, l = 0;
+  if (k < N) split_bf16(x[r * N + k], h, l);
+  planes[idx] = h;
+  planes[rows * Kp + idx] = l;
+}
+
+
+// Source offsets (bf16 units, K chunk 0) of the B vectors thread tid stages
+// for Cholesky block xblk; rows past the end are clamped to the last row.
+template <bool SPLIT>
+__device__ __forceinline__ void bx_set_b(int64_t (&b_src)[Bx<SPLIT>::B_PER], int tid, int xblk, int xb, int n,
+                                         int64_t brows, int64_t bplane, int Kp) {
+#pragma unroll
+  for (int i = 0; i < Bx<SPLIT>::B_PER; ++i) {
+    const int v = (tid + i * BX_THREADS) % Bx<SPLIT>::B_VEC;  // threads past the end load a duplicate, never stored
+    const int p = v / (BX_C * 4), rem = v % (BX_C * 4);
+    int64_t r = (int64_t)xblk * xb * n + rem / 4;
+    if (r > brows - 1) r = brows - 1;
+    b_src[i] = Bx<SPLIT>::b_plane(p) * bplane + r * Kp + (rem % 4) * 8;
+  }
+}
+
+template <bool SPLIT>
+__device__ __forceinline__ void bx_fetch(v4u (&pa)[Bx<SPLIT>::A_PER], v4u (&pb)[Bx<SPLIT>::B_PER], const uint16_t* gp,
+                                         const uint16_t* lp, const int64_t (&a_src)[Bx<SPLIT>::A_PER],
+                                         const int64_t (&b_src)[Bx<SPLIT>::B_PER], int c) {
+  const int64_t k0 = (int64_t)c * BX_KC;
+#pragma unroll
+  for (int i = 0; i < Bx<SPLIT>::A_PER; ++i) pa[i] = *reinterpret_cast<const v4u*>(gp + a_src[i] + k0);
+#pragma unroll
+  for (int i = 0; i < Bx<SPLIT>::B_PER; ++i) pb[i] = *reinterpret_cast<const v4u*>(lp + b_src[i] + k0);
+}
+
+template <bool SPLIT>
+__device__ __forceinline__ void bx_stash(uint4* lds_u4, const v4u (&pa)[Bx<SPLIT>::A_PER],
+                                         const v4u (&pb)[Bx<SPLIT>::B_PER], int tid, int buf) {
+  v4u* base = reinterpret_cast<v4u*>(lds_u4) + buf * Bx<SPLIT>::STAGE_VEC;
+#pragma unroll
+  for (int i = 0; i < Bx<SPLIT>::A_PER; ++i) {
+    const int v = tid + i * BX_THREADS;
+    base[lds_vec(v >> 2, v & 3)] = pa[i];
+  }
+#pragma unroll
+  for (int i = 0; i < Bx<SPLIT>::B_PER; ++i) {
+    const int v = tid + i * BX_THREADS;
+    if (v < Bx<SPLIT>::B_VEC) base[Bx<SPLIT>::A_VEC + lds_vec(v >> 2, v & 3)] = pb[i];
+  }
+}
+
+
+// One K chunk from LDS stage `stage`: the wave's BX_MR x BX_MC tiles,
+// re += Re(A) B and im += Im(A) B, each as hi*hi + hi*lo + lo*hi (SPLIT) or
+// hi*hi.
+template <bool SPLIT>
+__device__ __forceinline__ void bx_chunk(const uint4* stage, int wr, int wc, int l16, int q,
+                                         v4f (&re)[BX_MR][BX_MC], v4f (&im)[BX_MR][BX_MC]) {
+  const uint16_t* As = reinterpret_cast<const uint16_t*>(stage);
+  const uint16_t* Bs = As + 8 * Bx<SPLIT>::A_VEC;
+  v8bf bh[BX_MC], bl[BX_MC];
+#pragma unroll
+  for (int j = 0; j < BX_MC; ++j) {
+    const int col = (wc * BX_MC + j) * 16 + l16;
+    bh[j] = lds_frag(Bs, col, q);
+    if (SPLIT) bl[j] = lds_frag(Bs, BX_C + col, q);
+  }
+#pragma unroll
+  for (int i = 0; i < BX_MR; ++i) {
+    const int row = (wr * BX_MR + i) * 16 + l16;
+    if (SPLIT) {
+      const v8bf arh = lds_frag(As, row, q);
+      const v8bf arl = lds_frag(As, BX_R + row, q);
+      const v8bf aih = lds_frag(As, 2 * BX_R + row, q);
+      const v8bf ail = lds_frag(As, 3 * BX_R + row, q);
+#pragma unroll
+      for (int j = 0; j < BX_MC; ++j) {
+        re[i][j] = mfma_bf16(arh, bh[j], re[i][j]);
+        re[i][j] = mfma_bf16(arh, bl[j], re[i][j]);
+        re[i][j] = mfma_bf16(arl, bh[j], re[i][j]);
+        im[i][j] = mfma_bf16(aih, bh[j], im[i][j]);
+        im[i][j] = mfma_bf16(aih, bl[j], im[i][j]);
+        im[i][j] = mfma_bf16(ail, bh[j], im[i][j]);
+      }
+    } else {
+      const v8bf arh = lds_frag(As, row, q);
+      const v8bf aih = lds_frag(As, BX_R + row, q);
+#pragma unroll
+      for (int j = 0; j < BX_MC; ++j) {
+        re[i][j] = mfma_bf16(arh, bh[j], re[i][j]);
+        im[i][j] = mfma_bf16(aih, bh[j], im[i][j]);
+      }
+    }
+  }
+}
+
+template <bool SPLIT>
+__global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16Args a) {
+  using C = Bx<SPLIT>;
+  extern __shared__ uint4 lds_u4[];
+  __shared__ double red[BX_THREADS / 64];
+  // epilogue staging aliases the stage buffers: S[row][col] complex float, stride n + 1
+  float2* S = reinterpret_cast<float2*>(lds_u4);
+
+  const int n = a.nocc, Kp = a.kpad, wb = a.wb, xb = a.xb;
+  const int rows_valid = wb * n;
+  const int nwb = (int)ceil_div(a.nwalkers, wb);
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int split = lin % a.xsplit, bw = lin / a.xsplit;
+  const int nxb = (int)ceil_div(a.nchol, xb);
+  const int xb_begin = (int)((int64_t)nxb * split / a.xsplit);
+  const int xb_end = (int)((int64_t)nxb * (split + 1) / a.xsplit);
+  if (bw >= nwb) return;
+
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
+  const int wr = wave % BX_GR, wc = wave / BX_GR;
+  const int64_t arows = (int64_t)a.nwalkers * n, brows = (int64_t)a.nchol * n;
+  const int64_t aplane = arows * Kp, bplane = brows * Kp;
+  const int nch = Kp / BX_KC;
+
+  // Source offsets (bf16 units, K chunk 0) of the vectors this thread stages.
+  // Rows past the end of an array are clamped to its last row: padding rows
+  // and columns of the tile only feed outputs that the epilogue never reads.
+  int64_t a_src[C::A_PER], b_src[C::B_PER];
+#pragma unroll
+  for (int i = 0; i < C::A_PER; ++i) {
+    const int v = tid + i * BX_THREADS;
+    const int p = v / (BX_R * 4), rem = v % (BX_R * 4);
+    int64_t r = (int64_t)bw * wb * n + rem / 4;
+    if (r > arows - 1) r = arows - 1;
+    a_src[i] = C::a_plane(p) * aplane + r * Kp + (rem % 4) * 8;
+  }
+  v4u pa[C::A_PER], pb[C::B_PER];
+
+  // trace epilogue: walker wl of the block is summed by threads [wl G, (wl + 1) G)
+  const int G = BX_THREADS / wb;
+  const int my_wl = tid / G, my_idx = tid % G;
+  // vectors staged per epilogue round: as many as the stage buffers hold
+  const int xg = max(1, min(xb, (int)((int64_t)a.lds_bytes / ((int64_t)rows_valid * (n + 1) * sizeof(float2)))));
+  double wacc_r = 0.0, wacc_i = 0.0;
+
+  if (xb_begin < xb_end) {
+    bx_set_b<SPLIT>(b_src, tid, xb_begin, xb, n, brows, bplane, Kp);
+    bx_fetch<SPLIT>(pa, pb, a.gplanes, a.lplanes, a_src, b_src, 0);
+  }
+  for (int xblk = xb_begin; xblk < xb_end; ++xblk) {
+    const int x0 = xblk * xb;
+    bx_stash<SPLIT>(lds_u4, pa, pb, tid, 0);
+    __syncthreads();
+    v4f re[BX_MR][BX_MC], im[BX_MR][BX_MC];
+#pragma unroll
+    for (int i = 0; i < BX_MR; ++i)
+#pragma unroll
+      for (int j = 0; j < BX_MC; ++j) {
+        re[i][j] = zero4f();
+        im[i][j] = zero4f();
+      }
+    for (int c = 0; c < nch; ++c) {
+      // next chunk, or after the last one the first chunk of the next x block
+      // (stashed after the epilogue)
+      const bool more = c + 1 < nch, next_blk = !more && xblk + 1 < xb_end;
+      if (next_blk) bx_set_b<SPLIT>(b_src, tid, xblk + 1, xb, n, brows, bplane, Kp);
+      if (more || next_blk) bx_fetch<SPLIT>(pa, pb, a.gplanes, a.lplanes, a_src, b_src, more ? c + 1 : 0);
+      bx_chunk<SPLIT>(lds_u4 + (c & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im);
+      if (c + 1 < nch) bx_stash<SPLIT>(lds_u4, pa, pb, tid, (c + 1) & 1);
+      __syncthreads();
+    }
+    // epilogue: the block's vectors in groups of xg, staged as S[v][row][col]
+    // (row stride n + 1), then sum_ij T_ij T_ji per walker
+    for (int g0 = 0; g0 < xb; g0 += xg) {
+      const int gn = min(min(xg, xb - g0), a.nchol - (x0 + g0));
+      if (gn <= 0) break;
+#pragma unroll
+      for (int j = 0; j < BX_MC; ++j) {
+        const int colg = (wc * BX_MC + j) * 16 + l16;
+        const int v = colg / n - g0, col = colg % n;
+        if (v >= 0 && v < gn) {
+          float2* Sv = S + (int64_t)v * rows_valid * (n + 1) + col;
+#pragma unroll
+          for (int i = 0; i < BX_MR; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = (wr * BX_MR + i) * 16 + 4 * q + r;
+              if (row < rows_valid) Sv[row * (n + 1)] = make_float2(re[i][j][r], im[i][j][r]);
+            }
+        }
+      }
+      __syncthreads();
+      if (my_wl < wb) {
+        // sum_ij S_ij S_ji = sum_r S_rr^2 + 2 sum_{r<c} S_rc S_cr.  Each unordered
+        // pair {r, c} is visited once from r at distance d = c - r mod n,
+        // d = 1 .. (n-1)/2, plus d = n/2 from r < n/2 when n is even.  Task
+        // (v, r, h) of a thread walks half h of row r's distances.
+        const int ntask = gn * n * 2;
+        for (int task = my_idx; task < ntask; task += G) {
+          const int h = task & 1, vr = task >> 1, v = vr / n, r = vr % n;
+          const int cnt = (n - 1) / 2 + ((n % 2 == 0 && r < n / 2) ? 1 : 0);
+          const int half = (cnt + 1) / 2;
+          const int d0 = h * half, d1 = h ? cnt : half;
+          const float2* Sw = S + ((int64_t)v * rows_valid + my_wl * n) * (n + 1);
+          if (h == 0) {
+            const float2 t = Sw[r * (n + 1) + r];
+            wacc_r += (double)t.x * t.x - (double)t.y * t.y;
+            wacc_i += 2.0 * (double)t.x * t.y;
+          }
+          int c = r + 1 + d0;
+          if (c >= n) c -= n;
+          const float2* p1 = Sw + r * (n + 1) + c;  // S[r][c]
+          const float2* p2 = Sw + c * (n + 1) + r;  // S[c][r]
+          double orr = 0.0, oii = 0.0;
+          for (int d = d0; d < d1; ++d) {
+            const float2 a = *p1, b = *p2;
+            orr += (double)a.x * b.x - (double)a.y * b.y;
+            oii += (double)a.x * b.y + (double)a.y * b.x;
+            ++c;
+            ++p1;
+            p2 += n + 1;
+            if (c == n) {
+              c = 0;
+              p1 -= n;
+              p2 -= n * (n + 1);
+            }
+          }
+          wacc_r += 2.0 * orr;
+          wacc_i += 2.0 * oii;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int wl = 0; wl < wb; ++wl) {
+    const bool mine = my_wl == wl;
+    const double vr = block_sum(mine ? wacc_r : 0.0, red);
+    const double vi = block_sum(mine ? wacc_i : 0.0, red);
+    const int w = bw * wb + wl;
+    if (tid == 0 && w < a.nwalkers) st(a.exx_part, (int64_t)split * a.nwalkers + w, cmake(vr, vi));
+  }
+}
+
+// C = A B^T for split-BF16 operand planes (A: 4 planes, complex; B: 2 planes,
+// real), K chunk range of split z, 128 x 160 tile per workgroup.  Epilogues:
+//   BF16_EPI_PARTIAL  out[z_offset + z][row][col] = scale C     (force bias)
+//   BF16_EPI_VHS      out[w][i][j] = out[w][j][i] = i scale C[w][p],
+//                     (i, j) = (tri_i[p], tri_j[p])              (VHS)
+// Workgroups that run together share the B tile (row blocks vary fastest).
+__global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
+  using C = Bx<true>;
+  extern __shared__ uint4 lds_u4[];
+  const int mb = (int)ceil_div(a.m, BX_R), nb = (int)ceil_div(a.nn, BX_C);
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int rb = lin % mb, cb = (lin / mb) % nb, z = lin / mb / nb;
+  if (z >= a.ksplit) return;
+  const int c0 = (int)((int64_t)a.kchunks * z / a.ksplit), c1 = (int)((int64_t)a.kchunks * (z + 1) / a.ksplit);
+
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
+  const int wr = wave % BX_GR, wc = wave / BX_GR;
+  // staged rows past the end are clamped to the last row (their outputs are not written)
+  int64_t a_src[C::A_PER], b_src[C::B_PER];
+#pragma unroll
+  for (int i = 0; i < C::A_PER; ++i) {
+    const int v = tid + i * BX_THREADS;
+    const int p = v / (BX_R * 4), rem = v % (BX_R * 4);
+    int64_t r = (int64_t)rb * BX_R + rem / 4;
+    if (r > a.m - 1) r = a.m - 1;
+    a_src[i] = p * a.a_plane + r * a.lda + (rem % 4) * 8;
+  }
+#pragma unroll
+  for (int i = 0; i < C::B_PER; ++i) {
+    const int v = (tid + i * BX_THREADS) % C::B_VEC;
+    const int p = v / (BX_C * 4), rem = v % (BX_C * 4);
+    int64_t r = (int64_t)cb * BX_C + rem / 4;
+    if (r > a.nn - 1) r = a.nn - 1;
+    b_src[i] = p * a.b_plane + r * a.ldb + (rem % 4) * 8;
+  }
+  v4f re[BX_MR][BX_MC], im[BX_MR][BX_MC];
+#pragma unroll
+  for (int i = 0; i < BX_MR; ++i)
+#pragma unroll
+    for (int j = 0; j < BX_MC; ++j) {
+      re[i][j] = zero4f();
+      im[i][j] = zero4f();
+    }
+  v4u pa[C::A_PER], pb[C::B_PER];
+  if (c0 < c1) {
+    bx_fetch<true>(pa, pb, a.a, a.b, a_src, b_src, c0);
+    bx_stash<true>(lds_u4, pa, pb, tid, 0);
+    __syncthreads();
+  }
+  for (int c = c0; c < c1; ++c) {
+    const bool more = c + 1 < c1;
+    if (more) bx_fetch<true>(pa, pb, a.a, a.b, a_src, b_src, c + 1);
+    bx_chunk<true>(lds_u4 + ((c - c0) & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im);
+    if (more) bx_stash<true>(lds_u4, pa, pb, tid, (c + 1 - c0) & 1);
+    __syncthreads();
+  }
+  const int64_t NN = (int64_t)a.nbasis * a.nbasis;
+#pragma unroll
+  for (int j = 0; j < BX_MC; ++j) {
+    const int64_t col = (int64_t)cb * BX_C + (wc * BX_MC + j) * 16 + l16;
+    if (col >= a.nn) continue;
+    int ti = 0, tj = 0;
+    if (a.epilogue == BF16_EPI_VHS) {
+      ti = a.tri_i[col];
+      tj = a.tri_j[col];
+    }
+#pragma unroll
+    for (int i = 0; i < BX_MR; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = (int64_t)rb * BX_R + (wr * BX_MR + i) * 16 + 4 * q + r;
+        if (row >= a.m) continue;
+        const double vr = a.scale * re[i][j][r], vi = a.scale * im[i][j][r];
+        if (a.epilogue == BF16_EPI_VHS) {
+          // i scale (re + i im) = scale (-im + i re)
+          const cplx v = cmake(-vi, vr);
+          double* base = a.out + 2 * row * NN;
+          st(base, (int64_t)ti * a.nbasis + tj, v);
+          if (ti != tj) st(base, (int64_t)tj * a.nbasis + ti, v);
+        } else {
+          st(a.out, ((int64_t)(a.z_offset + z) * a.m + row) * a.nn + col, cmake(vr, vi));
+        }
+      }
+  }
+}
+
+// LDS of the exchange: the stages, or more so that the epilogue stages two
+// vectors per barrier pair (capped at the 3-product stages), or one vector.
+template <bool SPLIT>
+size_t bf16_lds_bytes(int wb, int xb, int nocc) {
+  const size_t per_vec = sizeof(float2) * (size_t)(wb * nocc) * (nocc + 1);
+  size_t lds = Bx<SPLIT>::STAGES;
+  const size_t two = per_vec * (size_t)(xb < 2 ? xb : 2);
+  if (two > lds) lds = two < BX_LDS_MAX ? two : BX_LDS_MAX;
+  return lds > per_vec ? lds : per_vec;
+}
+
+// ---------------------------------------------------------------- Taylor series
+// out_k^T = out_{k-1}^T VHS / k for each walker (VHS is symmetric), complex x
+// complex in split-BF16: A = out_{k-1}^T planes [4][W][rpad][kpad] (row = column
+// of the walker matrix, K = basis index), B = VHS planes [4][W N][kpad] (row =
+// output basis index).  re = Ar Br - Ai Bi, im = Ar Bi + Ai Br, each real
+// product as hi*hi + hi*lo + lo*hi (12 MFMAs per tile and K chunk; -Ai comes
+// from sign-flipped fragments).  Workgroup: one walker, 80 rows x 80 output
+// basis indices, 5 waves, wave v owns output column tile v and all 5 row tiles.
+// Epilogue: q = (init or q) + out_k (FP64), and the split planes of out_k^T
+// for the next term.  The out^T planes have a walker stride fixed by the
+// buffer capacity wcap; their padding rows and K entries are never written
+// and must be zero.
+constexpr int TY_WAVES = 5, TY_R = 80, TY_C = 80, TY_THREADS = 64 * TY_WAVES;
+constexpr int TY_A_VEC = 4 * TY_R * 4, TY_B_VEC = 4 * TY_C * 4, TY_STAGE_VEC = TY_A_VEC + TY_B_VEC;
+constexpr int TY_APER = TY_A_VEC / TY_THREADS, TY_BPER = TY_B_VEC / TY_THREADS;
+static_assert(TY_APER == 4 && TY_BPER == 4, "one plane per staged vector and thread");
+constexpr size_t TY_LDS = 2 * 16 * (size_t)TY_STAGE_VEC;
+
+__device__ __forceinline__ v8bf neg_bf16(v8bf x) {
+  return __builtin_bit_cast(v8bf, __builtin_bit_cast(v4u, x) ^ (v4u){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u});
+}
+
+__global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args a) {
+  extern __shared__ uint4 lds_u4[];
+  const int N = a.nbasis, Kp = a.kpad;
+  const int nrb = a.rpad / TY_R, ncb = (int)ceil_div(N, TY_C);
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int cb = lin % ncb, rb = (lin / ncb) % nrb, w = lin / ncb / nrb;
+  if (w >= a.nwalkers) return;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
+  const int64_t tplane = (int64_t)a.wcap * a.rpad * Kp, vplane = (int64_t)a.nwalkers * N * Kp;
+  // staged vector i of this thread: plane i, row tid / 4, 16-byte segment tid % 4
+  const int srow = tid >> 2, sseg = tid & 3;
+  const uint16_t* asrc = a.tin + ((int64_t)w * a.rpad + rb * TY_R + srow) * Kp + sseg * 8;
+  int64_t brow = (int64_t)cb * TY_C + srow;
+  if (brow > N - 1) brow = N - 1;  // padding columns: outputs are not written
+  const uint16_t* bsrc = a.vplanes + ((int64_t)w * N + brow) * Kp + sseg * 8;
+  v4u pa[TY_APER], pb[TY_BPER];
+  auto fetch = [&](int c) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < TY_APER; ++i) pa[i] = *reinterpret_cast<const v4u*>(asrc + i * tplane + c * BX_KC);
+#pragma unroll
+    for (int i = 0; i < TY_BPER; ++i) pb[i] = *reinterpret_cast<const v4u*>(bsrc + i * vplane + c * BX_KC);
+  };
+  auto stash = [&](int buf) __attribute__((always_inline)) {
+    v4u* base = reinterpret_cast<v4u*>(lds_u4) + buf * TY_STAGE_VEC;
+#pragma unroll
+    for (int i = 0; i < TY_APER; ++i) base[lds_vec(i * TY_R + srow, sseg)] = pa[i];
+#pragma unroll
+    for (int i = 0; i < TY_BPER; ++i) base[TY_A_VEC + lds_vec(i * TY_C + srow, sseg)] = pb[i];
+  };
+  v4f re[5], im[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    re[i] = zero4f();
+    im[i] = zero4f();
+  }
+  const int nch = Kp / BX_KC;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int c = 0; c < nch; ++c) {
+    if (c + 1 < nch) fetch(c + 1);
+    const uint16_t* As = reinterpret_cast<const uint16_t*>(lds_u4 + (c & 1) * TY_STAGE_VEC);
+    const uint16_t* Bs = As + 8 * TY_A_VEC;
+    const int bcol = wave * 16 + l16;
+    const v8bf brh = lds_frag(Bs, bcol, q), brl = lds_frag(Bs, TY_C + bcol, q);
+    const v8bf bih = lds_frag(Bs, 2 * TY_C + bcol, q), bil = lds_frag(Bs, 3 * TY_C + bcol, q);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      const int arow = i * 16 + l16;
+      const v8bf arh = lds_frag(As, arow, q), arl = lds_frag(As, TY_R + arow, q);
+      const v8bf aih = lds_frag(As, 2 * TY_R + arow, q), ail = lds_frag(As, 3 * TY_R + arow, q);
+      const v8bf naih = neg_bf16(aih), nail = neg_bf16(ail);
+      re[i] = mfma_bf16(arh, brh, re[i]);
+      re[i] = mfma_bf16(arh, brl, re[i]);
+      re[i] = mfma_bf16(arl, brh, re[i]);
+      re[i] = mfma_bf16(naih, bih, re[i]);
+      re[i] = mfma_bf16(naih, bil, re[i]);
+      re[i] = mfma_bf16(nail, bih, re[i]);
+      im[i] = mfma_bf16(arh, bih, im[i]);
+      im[i] = mfma_bf16(arh, bil, im[i]);
+      im[i] = mfma_bf16(arl, bih, im[i]);
+      im[i] = mfma_bf16(aih, brh, im[i]);
+      im[i] = mfma_bf16(aih, brl, im[i]);
+      im[i] = mfma_bf16(ail, brh, im[i]);
+    }
+    if (c + 1 < nch) stash((c + 1) & 1);
+    __syncthreads();
+  }
+  // epilogue (accumulator row = 4 q + r)
+  const int kcol = cb * TY_C + wave * 16 + l16;
+  if (kcol >= N) return;
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int crow = rb * TY_R + i * 16 + 4 * q + r;
+      if (crow >= a.ncol) continue;
+      const cplx v = cmake(a.alpha * re[i][r], a.alpha * im[i][r]);
+      const int64_t qi = ((int64_t)w * N + kcol) * a.ncol + crow;
+      st(a.q, qi, cadd(a.init ? ld(a.init, qi) : ld(a.q, qi), v));
+      if (a.tout) {
+        uint16_t rh, rl, ih, il;
+        split_bf16(v.re, rh, rl);
+        split_bf16(v.im, ih, il);
+        const int64_t o = ((int64_t)w * a.rpad + crow) * Kp + kcol;
+        a.tout[o] = rh;
+        a.tout[tplane + o] = rl;
+        a.tout[2 * tplane + o] = ih;
+        a.tout[3 * tplane + o] = il;
+      }
+    }
+}
+
+// phi [W][N][ncol] complex -> transposed split planes [4][W][rpad][kpad] at
+// (w, col, k); 32 x 32 tiles through LDS so both sides are coalesced.
+__global__ void __launch_bounds__(256) split_transpose_kernel(const double* __restrict__ phi, int nwalkers, int wcap,
+                                                             int N, int ncol, int rpad, int Kp,
+                                                             uint16_t* __restrict__ planes) {
+  __shared__ double2 tile[32][33];
+  const int ntk = (int)ceil_div(N, 32), ntc = (int)ceil_div(ncol, 32);
+  int b = blockIdx.x;
+  const int tc = b % ntc;
+  b /= ntc;
+  const int tk = b % ntk, w = b / ntk;
+  if (w >= nwalkers) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int yy = ty; yy < 32; yy += 8) {  // rows k, columns col (contiguous)
+    const int k = tk * 32 + yy, col = tc * 32 + tx;
+    double2 v = make_double2(0.0, 0.0);
+    if (k < N && col < ncol) {
+      const cplx c = ld(phi, ((int64_t)w * N + k) * ncol + col);
+      v = make_double2(c.re, c.im);
+    }
+    tile[yy][tx] = v;
+  }
+  __syncthreads();
+  const int64_t plane = (int64_t)wcap * rpad * Kp;
+  for (int yy = ty; yy < 32; yy += 8) {  // rows col, columns k (contiguous)
+    const int col = tc * 32 + yy, k = tk * 32 + tx;
+    if (col >= ncol || k >= N) continue;
+    const double2 v = tile[tx][yy];
+    uint16_t rh, rl, ih, il;
+    split_bf16(v.x, rh, rl);
+    split_bf16(v.y, ih, il);
+    const int64_t o = ((int64_t)w * rpad + col) * Kp + k;
+    planes[o] = rh;
+    planes[plane + o] = rl;
+    planes[2 * plane + o] = ih;
+    planes[3 * plane + o] = il;
+  }
+}
+
+}  // namespace
+
+// Geometry for the split-BF16 exchange: as many walkers (wb) and Cholesky
+// vectors (xb) as fit the 128 x 160 workgroup tile.
+bool bf16_exchange_plan(int nocc, int* wb, int* xb) {
+  if (nocc < 1 || nocc > BX_R || nocc > BX_C) return false;
+  *wb = BX_R / nocc;
+  *xb = BX_C / nocc;
+  return bf16_lds_bytes<true>(*wb, *xb, nocc) <= 160 * 1024;
+}
+
+hipError_t launch_split_ghalf(const double* g, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream,
+                              const int* walker_of, int nocc, const double* gref) {
+  const int64_t total = rows * Kp;
+  hipLaunchKernelGGL(split_ghalf_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream, g, rows, N, Kp,
+                     planes, walker_of, nocc, gref);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_real(const double* x, int64_t rows, int N, int Kp, uint16_t* planes, hipStream_t stream) {
+  const int64_t total = rows * Kp;
+  hipLaunchKernelGGL(split_real_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream, x, rows, N, Kp,
+                     planes);
+  return hipGetLastError();
+}
+
+template <bool SPLIT>
+hipError_t launch_exchange_bf16_t(ExchangeBf16Args a, hipStream_t stream) {
+  a.lds_bytes = (int)bf16_lds_bytes<SPLIT>(a.wb, a.xb, a.nocc);
+  hipError_t e = allow_lds((const void*)exchange_bf16_kernel<SPLIT>, a.lds_bytes);
+  if (e != hipSuccess) return e;
+  const int nwb = (int)ceil_div(a.nwalkers, a.wb);
+  hipLaunchKernelGGL(exchange_bf16_kernel<SPLIT>, dim3(nwb * a.xsplit), dim3(BX_THREADS), a.lds_bytes, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream) {
+  if (a.nocc < 1 || a.wb < 1 || a.xb < 1 || a.wb * a.nocc > BX_R || a.xb * a.nocc > BX_C || a.kpad % BX_KC != 0 ||
+      a.xsplit < 1)
+    return hipErrorInvalidValue;
+  return a.split ? launch_exchange_bf16_t<true>(a, stream) : launch_exchange_bf16_t<false>(a, stream);
+}
+
+hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream) {
+  if (a.m < 1 || a.nn < 1 || a.kchunks < 1 || a.ksplit < 1 || a.lda % BX_KC != 0 || a.ldb % BX_KC != 0 ||
+      (int64_t)a.kchunks * BX_KC > a.lda || (int64_t)a.kchunks * BX_KC > a.ldb)
+    return hipErrorInvalidValue;
+  if (a.epilogue == BF16_EPI_VHS && (int64_t)a.nbasis * (a.nbasis + 1) / 2 != a.nn) return hipErrorInvalidValue;
+  hipError_t e = allow_lds((const void*)bf16_gemm_kernel, Bx<true>::STAGES);
+  if (e != hipSuccess) return e;
+  const int64_t grid = ceil_div(a.m, BX_R) * ceil_div(a.nn, BX_C) * a.ksplit;
+  hipLaunchKernelGGL(bf16_gemm_kernel, dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream) {
+  if (a.nwalkers < 1 || a.wcap < a.nwalkers || a.ncol < 1 || a.rpad % TY_R != 0 || a.rpad < a.ncol ||
+      a.kpad % BX_KC != 0 || a.kpad < a.nbasis)
+    return hipErrorInvalidValue;
+  hipError_t e = allow_lds((const void*)taylor_bf16_kernel, TY_LDS);
+  if (e != hipSuccess) return e;
+  const int64_t grid = (int64_t)a.nwalkers * (a.rpad / TY_R) * ceil_div(a.nbasis, TY_C);
+  hipLaunchKernelGGL(taylor_bf16_kernel, dim3((unsigned)grid), dim3(TY_THREADS), TY_LDS, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_transpose(const double* phi, int nwalkers, int wcap, int N, int ncol, int rpad, int Kp,
+                                  uint16_t* planes, hipStream_t stream) {
+  const int64_t grid = (int64_t)nwalkers * ceil_div(N, 32) * ceil_div(ncol, 32);
+  hipLaunchKernelGGL(split_transpose_kernel, dim3((unsigned)grid), dim3(256), 0, stream, phi, nwalkers, wcap, N, ncol,
+                     rpad, Kp, planes);
+  return hipGetLastError();
+}
+
+int taylor_rows_padded(int ncol) { return (int)ceil_div(ncol, TY_R) * TY_R; }
+
+}  // namespace afqmc

@@ -1,12 +1,20 @@
 // Learned control variate for the exchange energy.
 //
-// Per walker w, cheap exact features f_w (one-body energy, Coulomb energy,
-// mean and max |xbar|, log|overlap|, overlap phase) feed a small MLP
-// g(f) = W3 silu(W2 silu(W1 f + b1) + b2) + b3 (2 hidden layers of H units,
-// BF16 weights for inference, FP32 accumulation) that predicts the walker's
-// exchange sum E_x(w) = sum_x tr(T_x T_x) (complex, both spins).  The exact
-// exchange is evaluated on a subset S of k walkers in which every walker has
-// inclusion probability k / W, and the estimate
+// The exchange sum E_x(w) = sum_x tr(T_x T_x), T_x = L_x G_w^T (complex, both
+// spins), is expanded about the trial's half-rotated Green's function G_T:
+//     E_x(G) = E_T + 2 <K, dG> + Q(dG),   dG = G - G_T,
+//     E_T = sum_x tr(T0_x T0_x),  T0_x = L_x G_T^T,  K = sum_x T0_x L_x,
+// with Q(dG) = sum_x tr(dT_x dT_x), dT_x = L_x dG^T.  E_T and K are computed
+// once in FP64; per step the linear term is an FP64 dot product and Q is
+// evaluated with single BF16 products (1 MFMA per product instead of 3).  The
+// predictor is g(w) = E_T + 2 <K, dG_w> + Q_bf16(w) + m(f_w), where m is a
+// small MLP, m(f) = W3 silu(W2 silu(W1 f + b1) + b2) + b3 (BF16 weights, FP32
+// accumulation), over cheap per-walker features (one-body energy, Coulomb
+// energy, mean and max |xbar|, log|overlap|, overlap phase), trained on the
+// residual E_x - (E_T + 2 <K, dG> + Q_bf16).
+//
+// The exact exchange is evaluated on a subset S of k walkers in which every
+// walker has inclusion probability k / W, and the estimate
 //     E_x_hat(w) = g(w) + [w in S] (W / k) (E_x(w) - g(w))
 // is unbiased for every walker whatever g is (Horvitz-Thompson).  S is the
 // image of [0, k) under w = a^{-1} (t - b) mod W for a random a coprime to W
@@ -284,9 +292,10 @@ __global__ void sur_combine_kernel(SurrogateCombineArgs a) {
     const cplx ex = exact_sum(a, (int)pos);
     const double s = (double)a.nwalkers / (double)a.k;
     est = cadd(g, cscale(csub(ex, g), s));
-    if (a.target) {
-      a.target[2 * pos] = ex.re;
-      a.target[2 * pos + 1] = ex.im;
+    if (a.target) {  // the MLP's target: exact minus the physical part of g
+      const cplx base = a.cv ? ld(a.cv, w) : cmake(0.0, 0.0);
+      a.target[2 * pos] = ex.re - base.re;
+      a.target[2 * pos + 1] = ex.im - base.im;
     }
     if (a.resid) {
       const cplx r = csub(ex, g);
@@ -314,6 +323,99 @@ __global__ void sur_gather_kernel(const double* __restrict__ g, const int* walke
   const int64_t e = idx % per_walker;
   const int64_t src = (int64_t)walker_of[t] * per_walker + e;
   reinterpret_cast<double2*>(out)[idx] = reinterpret_cast<const double2*>(g)[src];
+}
+
+// ---------------------------------------------------------------- control variate
+// T0[x][i][j] = sum_m L[x][i][m] gT[j][m]   (FP64, once per Hamiltonian / trial)
+__global__ void cv_t0_kernel(const double* __restrict__ L, const double* __restrict__ gT, int X, int n, int N,
+                             double* __restrict__ T0) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)X * n * n) return;
+  const int64_t x = idx / ((int64_t)n * n);
+  const int i = (int)((idx / n) % n), j = (int)(idx % n);
+  const double* l = L + (x * n + i) * N;
+  const double* g = gT + 2 * (int64_t)j * N;
+  double re = 0.0, im = 0.0;
+  for (int m = 0; m < N; ++m) {
+    re += l[m] * g[2 * m];
+    im += l[m] * g[2 * m + 1];
+  }
+  T0[2 * idx] = re;
+  T0[2 * idx + 1] = im;
+}
+
+// K[i][m] = sum_x sum_j T0[x][i][j] L[x][j][m]
+__global__ void cv_k_kernel(const double* __restrict__ T0, const double* __restrict__ L, int X, int n, int N,
+                            double* __restrict__ K) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * N) return;
+  const int i = (int)(idx / N), m = (int)(idx % N);
+  double re = 0.0, im = 0.0;
+  for (int64_t x = 0; x < X; ++x) {
+    const double* t = T0 + 2 * (x * n + i) * n;
+    const double* l = L + x * n * N + m;
+    for (int j = 0; j < n; ++j) {
+      const double lv = l[(int64_t)j * N];
+      re += t[2 * j] * lv;
+      im += t[2 * j + 1] * lv;
+    }
+  }
+  K[2 * idx] = re;
+  K[2 * idx + 1] = im;
+}
+
+// part[x] = sum_ij T0[x][i][j] T0[x][j][i]
+__global__ void cv_et_part_kernel(const double* __restrict__ T0, int X, int n, double* __restrict__ part) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= X) return;
+  const double* t = T0 + 2 * (int64_t)x * n * n;
+  double re = 0.0, im = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const cplx a = ld(t, (int64_t)i * n + j), b = ld(t, (int64_t)j * n + i);
+      re += a.re * b.re - a.im * b.im;
+      im += a.re * b.im + a.im * b.re;
+    }
+  part[2 * x] = re;
+  part[2 * x + 1] = im;
+}
+
+// et = sum_x part[x], in order (one thread)
+__global__ void cv_et_sum_kernel(const double* __restrict__ part, int X, double* __restrict__ et) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double re = 0.0, im = 0.0;
+  for (int x = 0; x < X; ++x) {
+    re += part[2 * x];
+    im += part[2 * x + 1];
+  }
+  et[0] = re;
+  et[1] = im;
+}
+
+// cv[w] = sum_s (E_T,s + 2 <K_s, G_s,w - gT_s> + Q_s,w), Q_s,w = sum_z part_s[z][w];
+// pred[w] += cv[w].  One workgroup per walker.
+__global__ void __launch_bounds__(256) sur_cv_kernel(SurrogateCvArgs a) {
+  __shared__ double red[256 / 64];
+  const int w = blockIdx.x;
+  cplx cv = cmake(0.0, 0.0);
+  for (int s = 0; s < a.nspin; ++s) {
+    const int64_t len = (int64_t)a.nocc[s] * a.nbasis;
+    const double* g = a.g[s] + 2 * (int64_t)w * len;
+    double re = 0.0, im = 0.0;
+    for (int64_t e = threadIdx.x; e < len; e += blockDim.x) {
+      const cplx k = ld(a.K[s], e), d = csub(ld(g, e), ld(a.gT[s], e));
+      re += k.re * d.re - k.im * d.im;
+      im += k.re * d.im + k.im * d.re;
+    }
+    re = block_sum(re, red);
+    im = block_sum(im, red);
+    cv = cadd(cv, cadd(ld(a.et, s), cmake(2.0 * re, 2.0 * im)));
+    for (int z = 0; z < a.xs[s]; ++z) cv = cadd(cv, ld(a.part[s], (int64_t)z * a.nwalkers + w));
+  }
+  if (threadIdx.x == 0) {
+    st(a.cv, w, cv);
+    st(a.pred, w, cadd(ld(a.pred, w), cv));
+  }
 }
 
 }  // namespace
@@ -357,6 +459,21 @@ hipError_t launch_sur_gather(const double* g, const int* walker_of, int k, int64
   const int64_t total = (int64_t)k * per_walker;
   hipLaunchKernelGGL(sur_gather_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s, g, walker_of, k,
                      per_walker, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_cv_static(const double* L, const double* gT, int X, int n, int N, double* T0, double* K,
+                            double* part, double* et, hipStream_t s) {
+  const int64_t nt = (int64_t)X * n * n;
+  hipLaunchKernelGGL(cv_t0_kernel, dim3((unsigned)ceil_div(nt, 256)), dim3(256), 0, s, L, gT, X, n, N, T0);
+  hipLaunchKernelGGL(cv_k_kernel, dim3((unsigned)ceil_div((int64_t)n * N, 64)), dim3(64), 0, s, T0, L, X, n, N, K);
+  hipLaunchKernelGGL(cv_et_part_kernel, dim3((unsigned)ceil_div(X, 64)), dim3(64), 0, s, T0, X, n, part);
+  hipLaunchKernelGGL(cv_et_sum_kernel, dim3(1), dim3(64), 0, s, part, X, et);
+  return hipGetLastError();
+}
+
+hipError_t launch_sur_cv(const SurrogateCvArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(sur_cv_kernel, dim3(a.nwalkers), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -1,12 +1,13 @@
-"""Mixed-precision mode: force bias, VHS and exchange run as split-BF16
-products with FP32 accumulation (x = hi + lo with |x - hi - lo| <= 2^-17 |x|,
-three products hi*hi + hi*lo + lo*hi); LU, inverse, weights, Taylor series and
-QR stay FP64.  Tolerances are set by that arithmetic, from a CPU emulation of
-the same split products on these cases:
+"""Mixed-precision mode: force bias, VHS, the Taylor series of exp(VHS) and
+the exchange run as split-BF16 products with FP32 accumulation (x = hi + lo
+with |x - hi - lo| <= 2^-17 |x|, three products hi*hi + hi*lo + lo*hi); LU,
+inverse, weights, the one-body products and QR stay FP64.  Tolerances are set
+by that arithmetic, from a CPU emulation of the same split products on these
+cases:
   exchange (after cancellation)   4.7e-5 (N=200) .. 3.4e-6 (N=64)  -> 1e-4
   force bias, VHS (max-norm rel.) 4e-6 .. 7e-6                     -> 5e-5
-  one propagation step            phi <= 1.7e-6, weight <= 3.2e-5,
-                                  overlap <= 6.6e-6, hybrid energy <= 1.2e-5
+  one propagation step            phi <= 2.2e-6, weight <= 4.7e-5,
+                                  overlap <= 9.1e-6, hybrid energy <= 1.6e-5
                                                                    -> 2e-5 / 3e-4
 The one-body energy stays FP64 and must match to 1e-10."""
 
