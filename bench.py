@@ -254,9 +254,11 @@ def main():
                 "fraction_exact": 0.125,
                 "warmup_steps": SURROGATE_WARMUP,
                 "last_subset": int(ss["last_k"]),
-                "resid2_Ha2": ss["resid2"],
-                "target_var_Ha2": ss["target_var"],
-                "r2": 1.0 - ss["resid2"] / ss["target_var"] if ss["target_var"] > 0 else None,
+                # mean |exact - g|^2 on the last subset (a constant offset in it cancels in the estimate)
+                "mean_sq_residual_Ha2": ss["resid2"],
+                # warm-up variance of exact - (E_T + 2<K,dG> + Q_bf16), the MLP's target
+                "control_variate_residual_var_Ha2": ss["target_var"],
+                "mlp_gain": ss["gain"],
             }
         modes[mode] = rec
         del st
@@ -279,7 +281,10 @@ def main():
     ok = [m for m, r in modes.items() if m == "fp64" or r.get("within_0p1_mHa")] if "fp64" in modes else list(modes)
     head = max(ok, key=lambda m: modes[m]["value"])
     m = modes[head]
-    ex_mode = head if "exchange_tflops" in modes[head] else ("mixed" if "mixed" in modes else head)
+    # the roofline kernel: the exchange of the headline mode, or (the surrogate
+    # mode runs two exchange forms) of the mixed mode, else the FP64 mode
+    ex_mode = head if "exchange_tflops" in modes[head] else next(
+        m for m in ("mixed", "fp64") if m in modes and "exchange_tflops" in modes[m])
     mx = modes[ex_mode]
     peak = PEAK["f64_mfma_tflops"] if ex_mode == "fp64" else PEAK["bf16_mfma_tflops"] / 3.0
     step_fl = step_flops(w.nbasis, w.nchol, w.nalpha, w.nbeta)

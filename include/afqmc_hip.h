@@ -154,11 +154,16 @@ int afqmc_reortho(afqmc_ctx* ctx, int32_t nwalkers, double* phia, double* phib, 
 int afqmc_profile(afqmc_ctx* ctx, int enable);
 int afqmc_profile_read(afqmc_ctx* ctx, double* ms, int64_t* launches);
 /* Learned control variate for the exchange energy of afqmc_propagate's
- * energy_prev output: an MLP (2 x 128 SiLU, BF16 weights, FP32 accumulation)
- * predicts each walker's exchange sum from the one-body and Coulomb energies,
- * mean / max |xbar|, log|overlap| and the overlap phase; the exact exchange is
- * computed for a random subset of round(fraction * W) walkers (equal inclusion
- * probability, redrawn every step) and combined as
+ * energy_prev output.  The predictor of each walker's exchange sum is
+ *   g = E_T + 2 <K, G - G_T> + Q_bf16(G - G_T) + alpha m(f):
+ * the exchange expanded about the trial's Green's function G_T (E_T and K in
+ * FP64, computed once), its quadratic remainder Q in single BF16 products, and
+ * an MLP m (2 x 128 SiLU, BF16 weights, FP32 accumulation) over the one-body
+ * and Coulomb energies, mean / max |xbar|, log|overlap| and the overlap phase,
+ * trained on the residual, with a gain alpha in [0, 1] fitted on earlier
+ * steps' exact samples.  The exact exchange is computed for a random subset of
+ * round(fraction * W) walkers (equal inclusion probability, redrawn every
+ * step) and combined as
  *   E_x = g + [w in S] W / k (E_x_exact - g),
  * which is unbiased per walker.  The first warmup_steps energy evaluations are
  * all exact and train the model (train_iters Adam steps each, learning rate
@@ -166,9 +171,11 @@ int afqmc_profile_read(afqmc_ctx* ctx, double* ms, int64_t* launches);
  * restores the plain exact estimator.  afqmc_local_energy is always exact. */
 int afqmc_surrogate_configure(afqmc_ctx* ctx, int enable, double fraction, int warmup_steps, int train_iters,
                               double lr, uint64_t seed);
-/* out[0..5] = energy evaluations so far, warm-up length, last subset size,
- * last training loss per sample (normalised units), mean |exact - predicted|^2
- * on the last subset (Ha^2), variance of Re(exact) over the warm-up. Synchronises. */
+/* out[0..n-1], n >= 6: energy evaluations so far, warm-up length, last subset
+ * size, last training loss per sample (normalised units), mean
+ * |exact - predicted|^2 on the last subset (Ha^2), variance over the warm-up of
+ * the MLP's target Re(exact - control variate), and (n >= 7) the current gain of
+ * the MLP term.  Synchronises. */
 int afqmc_surrogate_stats(afqmc_ctx* ctx, double* out, int n);
 /* n standard normals (Philox4x32-10, Box-Muller) for stream `counter`. */
 int afqmc_draw_normals(afqmc_ctx* ctx, double* out, int64_t n, uint64_t seed, uint64_t counter);

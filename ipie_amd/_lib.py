@@ -334,11 +334,11 @@ class Context:
                    float(lr), int(seed))
 
     def surrogate_stats(self):
-        out = (_f64 * 6)()
-        st = self.lib.afqmc_surrogate_stats(self.h, out, 6)
+        out = (_f64 * 7)()
+        st = self.lib.afqmc_surrogate_stats(self.h, out, 7)
         if st != AFQMC_OK:
             raise AfqmcError(f"afqmc_surrogate_stats: {self.lib.afqmc_last_error(self.h)}")
-        keys = ["steps", "warmup", "last_k", "train_loss", "resid2", "target_var"]
+        keys = ["steps", "warmup", "last_k", "train_loss", "resid2", "target_var", "gain"]
         return dict(zip(keys, [float(v) for v in out]))
 
     def profile(self, enable=True):

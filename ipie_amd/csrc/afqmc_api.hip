@@ -86,6 +86,7 @@ struct afqmc_ctx {
     double* K = nullptr;     // [2][nmax, N] complex
     double* et = nullptr;    // [2] complex E_T per spin
     double* cv = nullptr;    // [W] complex
+    double* mlp = nullptr;   // [W] complex MLP output
     double* gtmp = nullptr;  // [2] complex overlap scratch of the trial GF
     double* e1 = nullptr;    // [W] complex
     float* master = nullptr;
@@ -746,6 +747,8 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   cva.et = S.et;
   cva.cv = S.cv;
   cva.pred = S.pred;
+  cva.mlp = S.mlp;
+  cva.state = S.state;
   HIP_TRY(c, launch_sur_cv(cva, c->stream));
   // exact exchange on the sampled subset
   int k = nw;
@@ -779,6 +782,8 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   cb.resid = S.resid;
   cb.cv = S.cv;
   HIP_TRY(c, launch_sur_combine(cb, c->stream));
+  const SurrogateGainArgs gain{k, S.walker_of, S.target, S.mlp, S.state, 0.8};
+  HIP_TRY(c, launch_sur_gain(gain, c->stream));
   const int iters = warm ? S.train_iters : S.online_iters;
   if (warm) {
     SurrogateStatsArgs sa{k, S.feat, S.walker_of, S.target, S.state};
@@ -883,7 +888,7 @@ void afqmc_destroy(afqmc_ctx* c) {
                   c->tplanes[1], c->gcomp, c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
                   c->sur.resid, c->sur.walker_of, c->sur.xbar, c->sur.e1, c->sur.master, c->sur.m,
                   c->sur.v, c->sur.wbf, c->sur.bias, c->sur.grad, c->sur.loss, c->sur.state, c->sur.gT,
-                  c->sur.K, c->sur.et, c->sur.cv, c->sur.gtmp};
+                  c->sur.K, c->sur.et, c->sur.cv, c->sur.mlp, c->sur.gtmp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
@@ -1168,6 +1173,7 @@ int afqmc_surrogate_configure(afqmc_ctx* c, int enable, double fraction, int war
     ok = ok && dmalloc((void**)&S.K, 2 * 16 * (size_t)nmax * c->d.nbasis) == hipSuccess;
     ok = ok && dmalloc((void**)&S.et, 2 * 16) == hipSuccess;
     ok = ok && dmalloc((void**)&S.cv, 16 * W) == hipSuccess;
+    ok = ok && dmalloc((void**)&S.mlp, 16 * W) == hipSuccess;
     ok = ok && dmalloc((void**)&S.gtmp, 2 * 16) == hipSuccess;
     if (!ok) return fail(c, AFQMC_ERR_ALLOC, "surrogate allocation failed");
   }
@@ -1214,7 +1220,7 @@ int afqmc_surrogate_stats(afqmc_ctx* c, double* out, int n) {
   out[0] = (double)S.steps;
   out[1] = (double)S.warmup;
   out[2] = (double)S.last_k;
-  out[3] = out[4] = out[5] = 0.0;
+  for (int i = 3; i < n; ++i) out[i] = 0.0;
   if (!S.enabled || S.steps == 0) return AFQMC_OK;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   const int groups = sur_grad_groups(S.last_k);
@@ -1231,6 +1237,7 @@ int afqmc_surrogate_stats(afqmc_ctx* c, double* out, int n) {
   out[4] = rr / S.last_k;                        // mean |exact - prediction|^2 on the sample (Ha^2)
   const double m = st.count > 0 ? st.sum[SUR_FEATURES] / st.count : 0.0;
   out[5] = st.count > 0 ? st.sumsq[SUR_FEATURES] / st.count - m * m : 0.0;  // target variance (Re)
+  if (n > 6) out[6] = st.gain;                   // gain of the MLP term
   return AFQMC_OK;
 }
 

@@ -201,6 +201,9 @@ constexpr int SUR_NPARAM = SUR_HIDDEN * SUR_FEATURES + SUR_HIDDEN * SUR_HIDDEN +
 struct SurrogateState {
   double sum[SUR_FEATURES + 2], sumsq[SUR_FEATURES + 2], count;
   float mean[SUR_FEATURES + 2], inv_std[SUR_FEATURES + 2];
+  // gain of the MLP term, alpha = sum Re(conj(m) r) / sum |m|^2 over the exact
+  // samples of earlier steps (decayed), clamped to [0, 1]
+  double gain_rm, gain_mm, gain;
 };
 
 struct SurrogateFeatureArgs {
@@ -270,11 +273,22 @@ struct SurrogateCvArgs {
   const double* part[2];   // [xs_s, W] complex partial sums of Q
   int xs[2];
   double* cv;              // [W] complex
-  double* pred;            // [W] complex, += cv
+  double* pred;            // [W] complex: MLP output in, cv + gain * MLP out
+  double* mlp;             // [W] complex copy of the MLP output
+  const SurrogateState* state;
+};
+struct SurrogateGainArgs {
+  int k;
+  const int* walker_of;    // [k]
+  const double* target;    // [k][2] exact - cv
+  const double* mlp;       // [W] complex
+  SurrogateState* state;
+  double decay;
 };
 hipError_t launch_cv_static(const double* L, const double* gT, int X, int n, int N, double* T0, double* K,
                             double* part, double* et, hipStream_t s);
 hipError_t launch_sur_cv(const SurrogateCvArgs& a, hipStream_t s);
+hipError_t launch_sur_gain(const SurrogateGainArgs& a, hipStream_t s);
 hipError_t launch_sur_features(const SurrogateFeatureArgs& a, hipStream_t s);
 hipError_t launch_sur_stats(const SurrogateStatsArgs& a, hipStream_t s);
 hipError_t launch_sur_forward(const SurrogateForwardArgs& a, hipStream_t s);

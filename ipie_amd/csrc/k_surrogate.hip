@@ -7,11 +7,14 @@
 // with Q(dG) = sum_x tr(dT_x dT_x), dT_x = L_x dG^T.  E_T and K are computed
 // once in FP64; per step the linear term is an FP64 dot product and Q is
 // evaluated with single BF16 products (1 MFMA per product instead of 3).  The
-// predictor is g(w) = E_T + 2 <K, dG_w> + Q_bf16(w) + m(f_w), where m is a
-// small MLP, m(f) = W3 silu(W2 silu(W1 f + b1) + b2) + b3 (BF16 weights, FP32
+// predictor is g(w) = E_T + 2 <K, dG_w> + Q_bf16(w) + alpha m(f_w), where m is
+// a small MLP, m(f) = W3 silu(W2 silu(W1 f + b1) + b2) + b3 (BF16 weights, FP32
 // accumulation), over cheap per-walker features (one-body energy, Coulomb
 // energy, mean and max |xbar|, log|overlap|, overlap phase), trained on the
-// residual E_x - (E_T + 2 <K, dG> + Q_bf16).
+// residual E_x - (E_T + 2 <K, dG> + Q_bf16).  The gain alpha in [0, 1] is the
+// least-squares coefficient of that residual on m (both centred per step) over
+// the exact samples of earlier steps (exponentially decayed), so an MLP that
+// does not predict the residual is switched off.
 //
 // The exact exchange is evaluated on a subset S of k walkers in which every
 // walker has inclusion probability k / W, and the estimate
@@ -393,7 +396,7 @@ __global__ void cv_et_sum_kernel(const double* __restrict__ part, int X, double*
 }
 
 // cv[w] = sum_s (E_T,s + 2 <K_s, G_s,w - gT_s> + Q_s,w), Q_s,w = sum_z part_s[z][w];
-// pred[w] += cv[w].  One workgroup per walker.
+// pred[w] = cv[w] + gain * m[w] with m the MLP output.  One workgroup per walker.
 __global__ void __launch_bounds__(256) sur_cv_kernel(SurrogateCvArgs a) {
   __shared__ double red[256 / 64];
   const int w = blockIdx.x;
@@ -413,8 +416,47 @@ __global__ void __launch_bounds__(256) sur_cv_kernel(SurrogateCvArgs a) {
     for (int z = 0; z < a.xs[s]; ++z) cv = cadd(cv, ld(a.part[s], (int64_t)z * a.nwalkers + w));
   }
   if (threadIdx.x == 0) {
+    const cplx m = ld(a.pred, w);
     st(a.cv, w, cv);
-    st(a.pred, w, cadd(ld(a.pred, w), cv));
+    st(a.mlp, w, m);
+    st(a.pred, w, cadd(cv, cscale(m, a.state->gain)));
+  }
+}
+
+// Least-squares gain of the MLP term from this step's exact samples, for the
+// next step (so each step's predictor is independent of its own sample).  The
+// residual r and the MLP output m are centred on their sample means first: a
+// constant offset cancels in the estimate, only the within-step variation of
+// r - alpha m matters.
+__global__ void __launch_bounds__(256) sur_gain_kernel(SurrogateGainArgs a) {
+  __shared__ double red[256 / 64];
+  double s_mr = 0.0, s_mi = 0.0, s_rr = 0.0, s_ri = 0.0;
+  for (int t = threadIdx.x; t < a.k; t += blockDim.x) {
+    const cplx m = ld(a.mlp, a.walker_of[t]);
+    s_mr += m.re;
+    s_mi += m.im;
+    s_rr += a.target[2 * t];
+    s_ri += a.target[2 * t + 1];
+  }
+  const double inv_k = 1.0 / a.k;
+  const double m_re = block_sum(s_mr, red) * inv_k, m_im = block_sum(s_mi, red) * inv_k;
+  const double r_re = block_sum(s_rr, red) * inv_k, r_im = block_sum(s_ri, red) * inv_k;
+  double rm = 0.0, mm = 0.0;
+  for (int t = threadIdx.x; t < a.k; t += blockDim.x) {
+    const cplx m = ld(a.mlp, a.walker_of[t]);
+    const double dmr = m.re - m_re, dmi = m.im - m_im;
+    const double drr = a.target[2 * t] - r_re, dri = a.target[2 * t + 1] - r_im;
+    rm += dmr * drr + dmi * dri;
+    mm += dmr * dmr + dmi * dmi;
+  }
+  rm = block_sum(rm, red);
+  mm = block_sum(mm, red);
+  if (threadIdx.x == 0) {
+    SurrogateState* s = a.state;
+    s->gain_rm = a.decay * s->gain_rm + rm;
+    s->gain_mm = a.decay * s->gain_mm + mm;
+    const double g = s->gain_mm > 0.0 ? s->gain_rm / s->gain_mm : 0.0;
+    s->gain = fmin(fmax(g, 0.0), 1.0);
   }
 }
 
@@ -474,6 +516,11 @@ hipError_t launch_cv_static(const double* L, const double* gT, int X, int n, int
 
 hipError_t launch_sur_cv(const SurrogateCvArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(sur_cv_kernel, dim3(a.nwalkers), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_sur_gain(const SurrogateGainArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(sur_gain_kernel, dim3(1), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
