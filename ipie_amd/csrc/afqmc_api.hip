@@ -63,6 +63,8 @@ struct afqmc_ctx {
   int xs_used[2] = {1, 1};  // partial-sum count of the last exchange per spin
   int64_t exx_cap = 0;      // capacity (complex entries) of each exchange partial buffer
   double* gcomp = nullptr;  // [W, nmax, N] complex: Ghalf of sampled walkers (FP64 exchange)
+  double* gf_o = nullptr;     // [W, 2, nmax, nmax] complex: overlap matrices
+  double* gf_oinv = nullptr;  // [W, 2, nmax, nmax] complex: their inverses
   // learned exchange control variate
   struct {
     bool enabled = false;
@@ -189,7 +191,7 @@ int alloc_exchange_planes(afqmc_ctx* c) {
 }
 
 // Workspace is allocated on first use by an entry point that needs it (the
-// Green's function and overlap need none).
+// Green's function and overlap use only their own O / O^-1 buffers, see do_gf).
 int ensure_ws(afqmc_ctx* c) {
   if (c->ws_ready) return AFQMC_OK;
   const afqmc_dims& d = c->d;
@@ -281,8 +283,13 @@ int do_gf(afqmc_ctx* c, int nw, const double* phia, const double* phib, double* 
   a.ovlp = ovlp;
   a.log_shift = log_shift;
   a.flags = c->flags;
+  if (!c->gf_o) {
+    const size_t bytes = gf_work_bytes((int)c->d.nwalkers, c->d.nalpha, c->d.nbeta);
+    if (dmalloc((void**)&c->gf_o, bytes) != hipSuccess || dmalloc((void**)&c->gf_oinv, bytes) != hipSuccess)
+      return fail(c, AFQMC_ERR_ALLOC, "Green's function workspace allocation failed");
+  }
   PhaseScope ps(c, want_ghalf ? AFQMC_PHASE_GF : AFQMC_PHASE_OVERLAP);
-  HIP_TRY(c, launch_gf(a, want_ghalf, c->stream));
+  HIP_TRY(c, launch_gf(a, want_ghalf, c->gf_o, c->gf_oinv, c->stream));
   return AFQMC_OK;
 }
 
@@ -885,7 +892,7 @@ void afqmc_destroy(afqmc_ctx* c) {
   void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
                   c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
                   c->lplanes[1], c->xplanes, c->cplanes, c->vplanes, c->tplanes[0],
-                  c->tplanes[1], c->gcomp, c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
+                  c->tplanes[1], c->gcomp, c->gf_o, c->gf_oinv, c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
                   c->sur.resid, c->sur.walker_of, c->sur.xbar, c->sur.e1, c->sur.master, c->sur.m,
                   c->sur.v, c->sur.wbf, c->sur.bias, c->sur.grad, c->sur.loss, c->sur.state, c->sur.gT,
                   c->sur.K, c->sur.et, c->sur.cv, c->sur.mlp, c->sur.gtmp};

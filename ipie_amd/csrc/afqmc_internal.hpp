@@ -25,7 +25,9 @@ struct GfArgs {
   int* flags;
 };
 size_t gf_lds_bytes(int nalpha, int nbeta);
-hipError_t launch_gf(const GfArgs& a, bool want_ghalf, hipStream_t stream);
+// bytes of each GF work buffer (O, O^{-1}): [W][2][nmax][nmax] complex
+size_t gf_work_bytes(int nwalkers, int nalpha, int nbeta);
+hipError_t launch_gf(const GfArgs& a, bool want_ghalf, double* obuf, double* oinv, hipStream_t stream);
 
 // ---------------------------------------------------------------- GEMMs
 // Rows of a complex operand A (K contiguous) times rows of a real operand B

@@ -1,11 +1,14 @@
-// Batched single-determinant Green's function and overlap (FP64).
-//
-// One workgroup per walker handles both spins in turn:
-//   O_s   = phi_s^T conj(psi_s)                         [n, n]
-//   P O_s = L U      (partial pivoting on |re| + |im|, LAPACK's izamax rule)
-//   log|det O_s| = sum log|U_kk|,  phase = prod U_kk / |U_kk| * (-1)^swaps
-//   Ghalf_s = O_s^{-1} phi_s^T                          [n, N]
-//   ovlp  = phase_a phase_b exp(logdet_a + logdet_b - log_shift)
+// Batched single-determinant Green's function and overlap (FP64), three kernels:
+//   gf_obuild_kernel  O_s = phi_s^T conj(psi_s)           [n, n] per walker and spin
+//                     (v_mfma_f64_16x16x4f64, one wave per 16-row tile of O)
+//   gf_lu_kernel      P O_s = L U (partial pivoting on |re| + |im|, the first
+//                     index on ties, LAPACK's izamax rule); log|det O_s| =
+//                     sum log|U_kk|, phase = prod U_kk / |U_kk| * (-1)^swaps;
+//                     optionally O_s^{-1}; ovlp = phase_a phase_b
+//                     exp(logdet_a + logdet_b - log_shift).  One wave per
+//                     walker, lane = row (factorisation) or column (inverse).
+//   gf_ghalf_kernel   Ghalf_s = O_s^{-1} phi_s^T            [n, N]
+//                     (FP64 MFMA, one wave per 16-row tile of Ghalf)
 // Restates ipie/estimators/greens_function_single_det.py:62-154 and
 // ipie/propagation/overlap.py:29-59.  All reductions are in a fixed order, so
 // the result is bitwise reproducible run to run.
@@ -15,71 +18,84 @@ namespace afqmc {
 
 namespace {
 
-constexpr int GF_THREADS = 256;
-constexpr int GF_MC = 16;  // rows of phi / psi staged per chunk
+constexpr int GF_WAVES = 4;  // waves per workgroup of the MFMA kernels
 
-struct GfSpinOut {
-  double logdet;
-  cplx phase;
-  int singular;
-};
+__device__ __forceinline__ int gf_nspin(const GfArgs& a) { return (a.nbeta > 0 && !a.rhf) ? 2 : 1; }
 
-// Computes O = phi^T conj(psi) into lds_O (row stride ldo) using staged chunks.
-__device__ void build_overlap_matrix(const double* __restrict__ phi, const double* __restrict__ psi, int N,
-                                     int n, cplx* lds_O, int ldo, cplx* stage) {
-  cplx* sphi = stage;             // [GF_MC][n]
-  cplx* spsi = stage + GF_MC * n; // [GF_MC][n]
-  const int tid = threadIdx.x;
-  const int nn = n * n;
-  // each thread owns outputs idx = tid + t * GF_THREADS
-  constexpr int MAXOUT = (64 * 64 + GF_THREADS - 1) / GF_THREADS;  // 16
-  cplx acc[MAXOUT];
+// O[w][s][i][j] = sum_m phi_s[w][m][i] conj(psi_s[m][j]); wave = (w, s, row tile it).
+// MFMA operands: A[i][k = m] = phi[m][i], B[k = m][j] = conj(psi[m][j]).
+template <int NT>
+__global__ void __launch_bounds__(64 * GF_WAVES) gf_obuild_kernel(GfArgs a, double* __restrict__ obuf, int nmax) {
+  const int gw = blockIdx.x * GF_WAVES + (threadIdx.x >> 6);
+  const int nspin = gf_nspin(a);
+  const int task = gw / NT, it = gw % NT;
+  if (task >= a.nwalkers * nspin) return;
+  const int w = task / nspin, s = task % nspin;
+  const int n = s == 0 ? a.nalpha : a.nbeta;
+  if (it * 16 >= n) return;
+  const int N = a.nbasis;
+  const double* phi = (s == 0 ? a.phia : a.phib) + 2 * (int64_t)w * N * n;
+  const double* psi = s == 0 ? a.psia : a.psib;
+  const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int i = it * 16 + r16;
+  v4d re[NT], im[NT];
 #pragma unroll
-  for (int t = 0; t < MAXOUT; ++t) acc[t] = cmake(0.0, 0.0);
-  for (int m0 = 0; m0 < N; m0 += GF_MC) {
-    const int mc = min(GF_MC, N - m0);
-    __syncthreads();
-    for (int e = tid; e < mc * n; e += GF_THREADS) {
-      const int r = e / n, c = e % n;
-      sphi[r * n + c] = ld(phi, (int64_t)(m0 + r) * n + c);
-      spsi[r * n + c] = cconj(ld(psi, (int64_t)(m0 + r) * n + c));
-    }
-    __syncthreads();
+  for (int jt = 0; jt < NT; ++jt) {
+    re[jt] = zero4d();
+    im[jt] = zero4d();
+  }
+  for (int m0 = 0; m0 < N; m0 += 4) {
+    const int m = m0 + kq;
+    const cplx av = (m < N && i < n) ? ld(phi, (int64_t)m * n + i) : cmake(0.0, 0.0);
 #pragma unroll
-    for (int t = 0; t < MAXOUT; ++t) {
-      const int idx = tid + t * GF_THREADS;
-      if (idx < nn) {
-        const int i = idx / n, j = idx % n;
-        cplx a = acc[t];
-        for (int r = 0; r < mc; ++r) cfma(a, sphi[r * n + i], spsi[r * n + j]);
-        acc[t] = a;
-      }
+    for (int jt = 0; jt < NT; ++jt) {
+      const int j = jt * 16 + r16;
+      const cplx bv = (m < N && j < n) ? cconj(ld(psi, (int64_t)m * n + j)) : cmake(0.0, 0.0);
+      re[jt] = mfma64(av.re, bv.re, re[jt]);
+      re[jt] = mfma64(-av.im, bv.im, re[jt]);
+      im[jt] = mfma64(av.re, bv.im, im[jt]);
+      im[jt] = mfma64(av.im, bv.re, im[jt]);
     }
   }
+  double* o = obuf + 2 * ((int64_t)w * 2 + s) * nmax * nmax;
 #pragma unroll
-  for (int t = 0; t < MAXOUT; ++t) {
-    const int idx = tid + t * GF_THREADS;
-    if (idx < nn) lds_O[(idx / n) * ldo + (idx % n)] = acc[t];
-  }
-  __syncthreads();
+  for (int jt = 0; jt < NT; ++jt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = it * 16 + 4 * r + kq, col = jt * 16 + r16;  // accumulator row = 4 r + lane / 16
+      if (row < n && col < n) st(o, (int64_t)row * nmax + col, cmake(re[jt][r], im[jt][r]));
+    }
 }
 
-// In-place LU with partial pivoting of lds_O; piv[k] = row swapped with k.
-__device__ GfSpinOut lu_factor(cplx* O, int ldo, int n, int* piv, double* red) {
-  const int tid = threadIdx.x;
-  __shared__ int s_piv;
-  int swaps = 0;
+// One wave per walker: LU of O_s (in LDS, row stride nmax + 1), log-det and
+// phase, optionally the inverse into oinv, then the overlap of the walker.
+// A single-wave workgroup: __syncthreads() orders the LDS traffic between lanes.
+__global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __restrict__ obuf,
+                                                  double* __restrict__ oinv, int nmax) {
+  extern __shared__ cplx smem[];
+  const int lr = nmax + 1;  // LDS row stride
+  cplx* LU = smem;             // [nmax][lr]
+  cplx* X = smem + nmax * lr;  // [nmax][lr], columns of the inverse
+  __shared__ int piv[64];
+  const int w = blockIdx.x, lane = threadIdx.x;
+  const int nspin = gf_nspin(a);
+  double logdet[2] = {0.0, 0.0};
+  cplx phase[2] = {cmake(1.0, 0.0), cmake(1.0, 0.0)};
   int singular = 0;
-  for (int k = 0; k < n; ++k) {
-    // pivot search by wave 0 (n <= 64 rows, one lane per row)
-    if (tid < 64) {
-      const int i = k + tid;
+  for (int s = 0; s < nspin; ++s) {
+    const int n = s == 0 ? a.nalpha : a.nbeta;
+    const double* o = obuf + 2 * ((int64_t)w * 2 + s) * nmax * nmax;
+    for (int i = 0; i < n; ++i)
+      if (lane < n) LU[i * lr + lane] = ld(o, (int64_t)i * nmax + lane);
+    __syncthreads();
+    int swaps = 0;
+    for (int k = 0; k < n; ++k) {
       double v = -1.0;
-      if (i < n) {
-        const cplx z = O[i * ldo + k];
+      int best = lane;
+      if (lane >= k && lane < n) {
+        const cplx z = LU[lane * lr + k];
         v = fabs(z.re) + fabs(z.im);
       }
-      int best = i;
       for (int off = 1; off < 64; off <<= 1) {
         const double ov = __shfl_xor(v, off, 64);
         const int ob = __shfl_xor(best, off, 64);
@@ -88,146 +104,89 @@ __device__ GfSpinOut lu_factor(cplx* O, int ldo, int n, int* piv, double* red) {
           best = ob;
         }
       }
-      if (tid == 0) s_piv = best;
-    }
-    __syncthreads();
-    const int p = s_piv;
-    if (tid == 0) piv[k] = p;
-    if (p != k) {
-      ++swaps;
-      for (int j = tid; j < n; j += GF_THREADS) {
-        const cplx t = O[k * ldo + j];
-        O[k * ldo + j] = O[p * ldo + j];
-        O[p * ldo + j] = t;
-      }
-    }
-    __syncthreads();
-    const cplx pv = O[k * ldo + k];
-    const bool zero = (pv.re == 0.0 && pv.im == 0.0);
-    if (zero) singular = 1;
-    if (!zero) {
-      const cplx rinv = cdiv(cmake(1.0, 0.0), pv);
-      for (int i = k + 1 + tid; i < n; i += GF_THREADS) O[i * ldo + k] = cmul(O[i * ldo + k], rinv);
-    }
-    __syncthreads();
-    if (!zero) {
-      const int m = n - k - 1;
-      for (int e = tid; e < m * m; e += GF_THREADS) {
-        const int i = k + 1 + e / m, j = k + 1 + e % m;
-        cplx a = O[i * ldo + j];
-        const cplx l = O[i * ldo + k];
-        const cplx u = O[k * ldo + j];
-        a.re -= l.re * u.re - l.im * u.im;
-        a.im -= l.re * u.im + l.im * u.re;
-        O[i * ldo + j] = a;
-      }
-    }
-    __syncthreads();
-  }
-  GfSpinOut out;
-  double logdet = 0.0;
-  cplx phase = cmake((swaps & 1) ? -1.0 : 1.0, 0.0);
-  for (int k = 0; k < n; ++k) {
-    const cplx u = O[k * ldo + k];
-    const double a = cabs(u);
-    logdet += log(a);
-    if (a > 0.0) phase = cmul(phase, cmake(u.re / a, u.im / a));
-    else phase = cmake(0.0, 0.0);
-  }
-  (void)red;
-  out.logdet = logdet;
-  out.phase = phase;
-  out.singular = singular;
-  return out;
-}
-
-// Oinv = O^{-1} from its LU factors (one column of the identity per thread).
-__device__ void lu_inverse(const cplx* LU, int ldo, int n, const int* piv, cplx* Oinv, int ldi) {
-  const int tid = threadIdx.x;
-  // Column j of Oinv solves (L U) x = P e_j; the work column lives in Oinv[:, j].
-  for (int j = tid; j < n; j += GF_THREADS) {
-    for (int i = 0; i < n; ++i) Oinv[i * ldi + j] = cmake(i == j ? 1.0 : 0.0, 0.0);
-    for (int k = 0; k < n; ++k) {
-      const int p = piv[k];
+      const int p = best;  // the same in every lane after the butterfly
+      if (lane == 0) piv[k] = p;
       if (p != k) {
-        const cplx t = Oinv[k * ldi + j];
-        Oinv[k * ldi + j] = Oinv[p * ldi + j];
-        Oinv[p * ldi + j] = t;
+        ++swaps;
+        if (lane < n) {
+          const cplx t = LU[k * lr + lane];
+          LU[k * lr + lane] = LU[p * lr + lane];
+          LU[p * lr + lane] = t;
+        }
       }
-    }
-    for (int i = 0; i < n; ++i) {
-      cplx s = Oinv[i * ldi + j];
-      for (int k = 0; k < i; ++k) {
-        const cplx l = LU[i * ldo + k];
-        const cplx x = Oinv[k * ldi + j];
-        s.re -= l.re * x.re - l.im * x.im;
-        s.im -= l.re * x.im + l.im * x.re;
-      }
-      Oinv[i * ldi + j] = s;
-    }
-    for (int i = n - 1; i >= 0; --i) {
-      cplx s = Oinv[i * ldi + j];
-      for (int k = i + 1; k < n; ++k) {
-        const cplx u = LU[i * ldo + k];
-        const cplx x = Oinv[k * ldi + j];
-        s.re -= u.re * x.re - u.im * x.im;
-        s.im -= u.re * x.im + u.im * x.re;
-      }
-      Oinv[i * ldi + j] = cdiv(s, LU[i * ldo + i]);
-    }
-  }
-  __syncthreads();
-}
-
-// Ghalf[i][m] = sum_j Oinv[i][j] phi[m][j]
-__device__ void build_ghalf(const cplx* Oinv, int ldi, const double* __restrict__ phi, int N, int n,
-                            double* __restrict__ ghalf) {
-  const int tid = threadIdx.x;
-  for (int e = tid; e < n * N; e += GF_THREADS) {
-    const int i = e / N, m = e % N;
-    cplx a = cmake(0.0, 0.0);
-    const double* prow = phi + 2 * (int64_t)m * n;
-    for (int j = 0; j < n; ++j) cfma(a, Oinv[i * ldi + j], ld(prow, j));
-    st(ghalf, (int64_t)i * N + m, a);
-  }
-}
-
-template <bool WANT_GHALF>
-__global__ void __launch_bounds__(GF_THREADS)
-gf_kernel(GfArgs a) {
-  extern __shared__ cplx smem[];
-  const int w = blockIdx.x;
-  const int N = a.nbasis;
-  const int nmax = max(a.nalpha, a.nbeta);
-  const int ld = nmax + 1;
-  cplx* O = smem;                       // [nmax][ld]
-  cplx* Oinv = O + nmax * ld;           // [nmax][ld]
-  cplx* stage = Oinv;                   // [2][GF_MC][nmax], only used before Oinv is formed
-  __shared__ int piv[64];
-  __shared__ double red[GF_THREADS / 64];
-
-  const int nspin = (a.nbeta > 0 && !a.rhf) ? 2 : 1;
-  double logdet[2] = {0.0, 0.0};
-  cplx phase[2] = {cmake(1.0, 0.0), cmake(1.0, 0.0)};
-  int singular = 0;
-  for (int s = 0; s < nspin; ++s) {
-    const int n = s == 0 ? a.nalpha : a.nbeta;
-    const double* phi = (s == 0 ? a.phia : a.phib) + 2 * (int64_t)w * N * n;
-    const double* psi = s == 0 ? a.psia : a.psib;
-    build_overlap_matrix(phi, psi, N, n, O, ld, stage);
-    GfSpinOut r = lu_factor(O, ld, n, piv, red);
-    logdet[s] = r.logdet;
-    phase[s] = r.phase;
-    singular |= r.singular;
-    if (WANT_GHALF) {
       __syncthreads();
-      lu_inverse(O, ld, n, piv, Oinv, ld);
-      double* gh = (s == 0 ? a.ghalfa : a.ghalfb) + 2 * (int64_t)w * n * N;
-      build_ghalf(Oinv, ld, phi, N, n, gh);
+      const cplx pv = LU[k * lr + k];
+      const bool zero = (pv.re == 0.0 && pv.im == 0.0);
+      if (zero) singular = 1;
+      if (!zero && lane > k && lane < n) {
+        const cplx rinv = cdiv(cmake(1.0, 0.0), pv);
+        cplx* row = LU + lane * lr;
+        const cplx l = cmul(row[k], rinv);
+        row[k] = l;
+        const cplx* urow = LU + k * lr;
+        for (int j = k + 1; j < n; ++j) {
+          const cplx u = urow[j];
+          cplx x = row[j];
+          x.re -= l.re * u.re - l.im * u.im;
+          x.im -= l.re * u.im + l.im * u.re;
+          row[j] = x;
+        }
+      }
+      __syncthreads();
+    }
+    double lg = 0.0;
+    cplx ph = cmake((swaps & 1) ? -1.0 : 1.0, 0.0);
+    for (int k = 0; k < n; ++k) {
+      const cplx u = LU[k * lr + k];
+      const double ab = cabs(u);
+      lg += log(ab);
+      if (ab > 0.0) ph = cmul(ph, cmake(u.re / ab, u.im / ab));
+      else ph = cmake(0.0, 0.0);
+    }
+    logdet[s] = lg;
+    phase[s] = ph;
+    if (oinv) {
+      // column j of O^{-1} solves L U x = P e_j (lane j)
+      const int j = lane;
+      if (j < n) {
+        for (int i = 0; i < n; ++i) X[i * lr + j] = cmake(i == j ? 1.0 : 0.0, 0.0);
+        for (int k = 0; k < n; ++k) {
+          const int p = piv[k];
+          if (p != k) {
+            const cplx t = X[k * lr + j];
+            X[k * lr + j] = X[p * lr + j];
+            X[p * lr + j] = t;
+          }
+        }
+        for (int i = 0; i < n; ++i) {
+          cplx sum = X[i * lr + j];
+          for (int k = 0; k < i; ++k) {
+            const cplx l = LU[i * lr + k];
+            const cplx x = X[k * lr + j];
+            sum.re -= l.re * x.re - l.im * x.im;
+            sum.im -= l.re * x.im + l.im * x.re;
+          }
+          X[i * lr + j] = sum;
+        }
+        for (int i = n - 1; i >= 0; --i) {
+          cplx sum = X[i * lr + j];
+          for (int k = i + 1; k < n; ++k) {
+            const cplx u = LU[i * lr + k];
+            const cplx x = X[k * lr + j];
+            sum.re -= u.re * x.re - u.im * x.im;
+            sum.im -= u.re * x.im + u.im * x.re;
+          }
+          X[i * lr + j] = cdiv(sum, LU[i * lr + i]);
+        }
+      }
+      __syncthreads();
+      double* oi = oinv + 2 * ((int64_t)w * 2 + s) * nmax * nmax;
+      for (int i = 0; i < n; ++i)
+        if (lane < n) st(oi, (int64_t)i * nmax + lane, X[i * lr + lane]);
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  if (lane == 0) {
     const double lsh = a.log_shift ? a.log_shift[w] : 0.0;
     cplx ph;
     double lg;
@@ -247,22 +206,91 @@ gf_kernel(GfArgs a) {
   }
 }
 
+// Ghalf_s[w][i][m] = sum_j Oinv[w][s][i][j] phi_s[w][m][j]; wave = (w, s, row tile it).
+// MFMA operands: A[i][k = j] = Oinv[i][j] (held for all K steps), B[k = j][m] = phi[m][j].
+template <int KS>
+__global__ void __launch_bounds__(64 * GF_WAVES) gf_ghalf_kernel(GfArgs a, const double* __restrict__ oinv, int nmax,
+                                                                int ntile) {
+  const int gw = blockIdx.x * GF_WAVES + (threadIdx.x >> 6);
+  const int nspin = gf_nspin(a);
+  const int task = gw / ntile, it = gw % ntile;
+  if (task >= a.nwalkers * nspin) return;
+  const int w = task / nspin, s = task % nspin;
+  const int n = s == 0 ? a.nalpha : a.nbeta;
+  if (it * 16 >= n) return;
+  const int N = a.nbasis;
+  const double* phi = (s == 0 ? a.phia : a.phib) + 2 * (int64_t)w * N * n;
+  const double* oi = oinv + 2 * ((int64_t)w * 2 + s) * nmax * nmax;
+  double* gh = (s == 0 ? a.ghalfa : a.ghalfb) + 2 * (int64_t)w * n * N;
+  const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int i = it * 16 + r16;
+  cplx af[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    const int k = ks * 4 + kq;
+    af[ks] = (i < n && k < n) ? ld(oi, (int64_t)i * nmax + k) : cmake(0.0, 0.0);
+  }
+  const int nmt = (int)ceil_div(N, 16);
+  for (int mt = 0; mt < nmt; ++mt) {
+    const int m = mt * 16 + r16;
+    v4d re = zero4d(), im = zero4d();
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = ks * 4 + kq;
+      const cplx bv = (m < N && k < n) ? ld(phi, (int64_t)m * n + k) : cmake(0.0, 0.0);
+      re = mfma64(af[ks].re, bv.re, re);
+      re = mfma64(-af[ks].im, bv.im, re);
+      im = mfma64(af[ks].re, bv.im, im);
+      im = mfma64(af[ks].im, bv.re, im);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = it * 16 + 4 * r + kq;  // accumulator row = 4 r + lane / 16
+      if (row < n && m < N) st(gh, (int64_t)row * N + m, cmake(re[r], im[r]));
+    }
+  }
+}
+
 }  // namespace
 
 size_t gf_lds_bytes(int nalpha, int nbeta) {
   const int nmax = nalpha > nbeta ? nalpha : nbeta;
-  const size_t inv = (size_t)nmax * (nmax + 1), stage = (size_t)2 * GF_MC * nmax;
-  return sizeof(cplx) * ((size_t)nmax * (nmax + 1) + (inv > stage ? inv : stage));
+  return sizeof(cplx) * 2 * (size_t)nmax * (nmax + 1);
 }
 
-hipError_t launch_gf(const GfArgs& a, bool want_ghalf, hipStream_t stream) {
-  const size_t lds = gf_lds_bytes(a.nalpha, a.nbeta);
-  hipError_t e = allow_lds(want_ghalf ? (const void*)gf_kernel<true> : (const void*)gf_kernel<false>, lds);
+size_t gf_work_bytes(int nwalkers, int nalpha, int nbeta) {
+  const int nmax = nalpha > nbeta ? nalpha : nbeta;
+  return sizeof(cplx) * (size_t)nwalkers * 2 * nmax * nmax;
+}
+
+hipError_t launch_gf(const GfArgs& a, bool want_ghalf, double* obuf, double* oinv, hipStream_t stream) {
+  const int nmax = a.nalpha > a.nbeta ? a.nalpha : a.nbeta;
+  if (nmax < 1 || nmax > 64 || a.nwalkers < 1) return hipErrorInvalidValue;
+  const int nspin = (a.nbeta > 0 && !a.rhf) ? 2 : 1;
+  const int nt = (nmax + 15) / 16;
+  const int64_t waves = (int64_t)a.nwalkers * nspin * nt;
+  const unsigned grid = (unsigned)ceil_div(waves, GF_WAVES);
+  switch (nt) {
+    case 1: hipLaunchKernelGGL(gf_obuild_kernel<1>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, obuf, nmax); break;
+    case 2: hipLaunchKernelGGL(gf_obuild_kernel<2>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, obuf, nmax); break;
+    case 3: hipLaunchKernelGGL(gf_obuild_kernel<3>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, obuf, nmax); break;
+    default: hipLaunchKernelGGL(gf_obuild_kernel<4>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, obuf, nmax); break;
+  }
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (want_ghalf)
-    hipLaunchKernelGGL(gf_kernel<true>, dim3(a.nwalkers), dim3(GF_THREADS), lds, stream, a);
-  else
-    hipLaunchKernelGGL(gf_kernel<false>, dim3(a.nwalkers), dim3(GF_THREADS), lds, stream, a);
+  const size_t lds = gf_lds_bytes(a.nalpha, a.nbeta);
+  e = allow_lds((const void*)gf_lu_kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(gf_lu_kernel, dim3(a.nwalkers), dim3(64), lds, stream, a, obuf, want_ghalf ? oinv : nullptr,
+                     nmax);
+  e = hipGetLastError();
+  if (e != hipSuccess || !want_ghalf) return e;
+  switch (nt) {  // K steps of 4 cover nmax <= 16 nt
+    case 1: hipLaunchKernelGGL(gf_ghalf_kernel<4>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, oinv, nmax, nt); break;
+    case 2: hipLaunchKernelGGL(gf_ghalf_kernel<8>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, oinv, nmax, nt); break;
+    case 3: hipLaunchKernelGGL(gf_ghalf_kernel<12>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, oinv, nmax, nt); break;
+    default: hipLaunchKernelGGL(gf_ghalf_kernel<16>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, oinv, nmax, nt); break;
+  }
   return hipGetLastError();
 }
 
