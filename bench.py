@@ -185,6 +185,23 @@ def cpu_baseline(st, w, sample_walkers, sample_steps):
     }
 
 
+def traffic_per_launch(mode, w):
+    """HBM bytes per launch of the roofline kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same counter on a
+    plain copy of known size), or None when it was not measured for this mode and
+    workload."""
+    path = os.path.join(ROOT, "profiles", "exchange_n400_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        line = line.strip()
+        if line.startswith("{"):
+            rec = json.loads(line)
+            if rec.get("mode") == mode and rec.get("workload") == w.name and rec.get("nwalkers") == w.nwalkers:
+                return rec["hbm_bytes_per_launch"]
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +216,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-walkers", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--calibration-copy-mb", type=int, default=0,
+                    help="only copy a device buffer of this many MiB once and exit (the reference for the "
+                         "FETCH_SIZE / WRITE_SIZE counters behind roofline.traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs one GPU only (asked for --gpus {args.gpus}); the walker-sharded multi-GPU "
@@ -210,6 +230,16 @@ def main():
     import torch
 
     from ipie_amd import synthetic
+
+    if args.calibration_copy_mb > 0:
+        n = args.calibration_copy_mb * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": 8 * n}), flush=True)
+        return
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     w = synthetic.WORKLOADS[args.workload]
@@ -248,7 +278,23 @@ def main():
         if "surrogate" not in mode:
             rec["exchange_tflops"] = fl / ex_avg_s / 1e12
             rec["exchange_flops_per_launch"] = fl
+            rec["roofline_kernel"] = dict(
+                kernel=("exchange_tiled_kernel (FP64" if mode == "fp64" else "exchange_bf16_kernel (split-BF16 x3")
+                + ", one launch per spin)",
+                avg_launch_ms=ex_avg_s * 1e3, flops_per_launch=fl, achieved=fl / ex_avg_s / 1e12,
+                peak=PEAK["f64_mfma_tflops"] if mode == "fp64" else PEAK["bf16_mfma_tflops"] / 3.0,
+                peak_note="FP64 MFMA 78.6 TFLOP/s" if mode == "fp64" else
+                "dense BF16 MFMA 2.5 PFLOP/s / 3 products per algorithmic flop")
         else:
+            # the dominant kernel of this mode: the single-product exchange of the
+            # control variate over all walkers (its own profiling phase)
+            cv_ms, cv_n = prof["exchange_cv"]
+            cv_avg_s = cv_ms / max(cv_n, 1) / 1e3
+            rec["roofline_kernel"] = dict(
+                kernel="exchange_bf16_kernel (single BF16 product, control-variate Q(G - G_T) of every walker, "
+                       "one launch per spin)",
+                avg_launch_ms=cv_avg_s * 1e3, flops_per_launch=fl, achieved=fl / cv_avg_s / 1e12,
+                peak=PEAK["bf16_mfma_tflops"], peak_note="dense BF16 MFMA 2.5 PFLOP/s, one product per flop")
             ss = st["prop"].surrogate_stats()
             rec["surrogate"] = {
                 "fraction_exact": 0.125,
@@ -281,15 +327,11 @@ def main():
     ok = [m for m, r in modes.items() if m == "fp64" or r.get("within_0p1_mHa")] if "fp64" in modes else list(modes)
     head = max(ok, key=lambda m: modes[m]["value"])
     m = modes[head]
-    # the roofline kernel: the exchange of the headline mode, or (the surrogate
-    # mode runs two exchange forms) of the mixed mode, else the FP64 mode
-    ex_mode = head if "exchange_tflops" in modes[head] else next(
-        m for m in ("mixed", "fp64") if m in modes and "exchange_tflops" in modes[m])
-    mx = modes[ex_mode]
-    peak = PEAK["f64_mfma_tflops"] if ex_mode == "fp64" else PEAK["bf16_mfma_tflops"] / 3.0
+    rk = m["roofline_kernel"]
     step_fl = step_flops(w.nbasis, w.nchol, w.nalpha, w.nbeta)
     dtype = {"fp64": "f64",
-             "mixed": "f64 (LU, inverse, Taylor, weights, QR) + bf16x3 force bias, VHS, exchange (fp32 accumulate)",
+             "mixed": "f64 (Green's function, weights, one-body, QR) + bf16x3 force bias, VHS, Taylor, exchange "
+                      "(fp32 accumulate)",
              "mixed+surrogate": "mixed, exchange on 1/8 of walkers + bf16-weight MLP control variate"}[head]
     out = {
         "metric": METRIC,
@@ -317,18 +359,16 @@ def main():
             "parallelism": "single GPU (walkers batched)",
         },
         "roofline": {
-            "kernel": "exchange (fused exchange contraction, one launch per spin), "
-                      + ("FP64" if ex_mode == "fp64" else "split-BF16 x3") + f", measured in mode {ex_mode}",
+            "kernel": rk["kernel"],
             "bound": "mfma",
-            "achieved": mx["exchange_tflops"],
-            "peak": peak,
-            "peak_note": "FP64 MFMA 78.6 TFLOP/s" if ex_mode == "fp64" else
-                         "dense BF16 MFMA 2.5 PFLOP/s / 3 products per algorithmic flop",
+            "achieved": rk["achieved"],
+            "peak": rk["peak"],
+            "peak_note": rk["peak_note"],
             "unit": "TFLOP/s",
-            "frac": mx["exchange_tflops"] / peak,
-            "traffic": None,
-            "avg_launch_ms": mx["exchange_avg_launch_ms"],
-            "flops_per_launch": mx["exchange_flops_per_launch"],
+            "frac": rk["achieved"] / rk["peak"],
+            "traffic": traffic_per_launch(head, w),
+            "avg_launch_ms": rk["avg_launch_ms"],
+            "flops_per_launch": rk["flops_per_launch"],
         },
         "step_tflops_algorithmic": step_fl * m["value"] / 1e12,
         "modes": modes,

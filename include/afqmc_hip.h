@@ -61,12 +61,15 @@ extern "C" {
 #define AFQMC_PHASE_EXCHANGE 8    /* fused exchange contraction (local energy) */
 #define AFQMC_PHASE_ENERGY 9      /* one-body energy and energy assembly */
 #define AFQMC_PHASE_REORTHO 10    /* QR re-orthogonalisation */
-#define AFQMC_NPHASES 11
+#define AFQMC_PHASE_EXCHANGE_CV 11 /* single-product exchange of the control variate (surrogate) */
+#define AFQMC_NPHASES 12
 
 /* precision modes */
 #define AFQMC_PREC_FP64 0   /* every contraction in FP64 */
-#define AFQMC_PREC_MIXED 1  /* exchange as split-BF16 products with FP32 accumulation (FP64
-                               trace sum); LU, inverse, overlaps, weights and QR stay FP64 */
+#define AFQMC_PREC_MIXED 1  /* force bias, VHS, Taylor series and exchange as split-BF16
+                               products (hi*hi + hi*lo + lo*hi) with FP32 accumulation (FP64
+                               trace and Taylor sums); Green's function (O, LU, inverse,
+                               Ghalf), overlaps, weights, one-body products and QR stay FP64 */
 
 typedef struct afqmc_ctx afqmc_ctx;
 

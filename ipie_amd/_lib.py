@@ -94,7 +94,7 @@ SIGNATURES = {
 }
 
 PHASES = ["gf", "one_body", "force_bias", "fields", "vhs", "taylor", "overlap", "weight", "exchange", "energy",
-          "reortho"]
+          "reortho", "exchange_cv"]
 
 _lib = None
 

@@ -687,7 +687,7 @@ int exchange_q_spin(afqmc_ctx* c, int nw, int spin, const double* g) {
   const int N = c->d.nbasis;
   const int n = spin == 0 ? c->d.nalpha : c->d.nbeta;
   const int64_t gstride = 2 * (int64_t)std::max(c->d.nalpha, c->d.nbeta) * N;
-  PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
+  PhaseScope ps(c, AFQMC_PHASE_EXCHANGE_CV);
   HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, nullptr, n,
                                 c->sur.gT + spin * gstride));
   ExchangeBf16Args b{};
