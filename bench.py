@@ -54,7 +54,12 @@ def step_flops(N, X, na, nb):
     return f
 
 
-def exchange_flops_per_launch(N, X, n, W):
+def exchange_flops_per_launch(N, X, n, W, form="t"):
+    """Real flops of one exchange launch (one spin, W walkers) in the form the
+    context uses: 't' forms every T_x = L_x Ghalf^T (ipie's contraction), 'quad'
+    is Y = G Wq (complex G, real Wq of size (nN)^2) plus a dot per walker."""
+    if form == "quad":
+        return float(W) * (4.0 * (n * N) ** 2 + 8.0 * n * N)
     return float(X) * W * (4 * n * n * N + 8 * n * n)
 
 
@@ -262,8 +267,11 @@ def main():
         st = build_device(host, w, mode)
         elapsed, prof, e_steps, finite = run_gpu(st, w, K, warmup)
         ex_ms, ex_n = prof["exchange"]
-        fl = 0.5 * (exchange_flops_per_launch(w.nbasis, w.nchol, w.nalpha, w.nwalkers)
-                    + exchange_flops_per_launch(w.nbasis, w.nchol, w.nbeta, w.nwalkers))
+        form = st["walkers"]._engine.ctx.exchange_form()
+        fl = 0.5 * (exchange_flops_per_launch(w.nbasis, w.nchol, w.nalpha, w.nwalkers, form)
+                    + exchange_flops_per_launch(w.nbasis, w.nchol, w.nbeta, w.nwalkers, form))
+        fl_t = 0.5 * (exchange_flops_per_launch(w.nbasis, w.nchol, w.nalpha, w.nwalkers, "t")
+                      + exchange_flops_per_launch(w.nbasis, w.nchol, w.nbeta, w.nwalkers, "t"))
         ex_avg_s = ex_ms / max(ex_n, 1) / 1e3
         rec = dict(
             value=w.nwalkers * K / elapsed,
@@ -275,12 +283,20 @@ def main():
             finite=finite,
             _e_steps=e_steps,
         )
+        rec["exchange_form"] = form
+        # the same launch in ipie's form (every T_x formed): the work it replaces
+        rec["exchange_flops_reference_form_per_launch"] = fl_t
         if "surrogate" not in mode:
             rec["exchange_tflops"] = fl / ex_avg_s / 1e12
             rec["exchange_flops_per_launch"] = fl
+            if form == "quad":
+                kname = ("Y = G Wq on FP64 MFMA (force_bias_kernel NT mainloop) + quad_reduce_kernel" if mode == "fp64"
+                         else "Y = G Wq split-BF16 x3 (bf16_gemm_kernel, split K) + quad_reduce_kernel")
+            else:
+                kname = ("exchange_tiled_kernel (FP64" if mode == "fp64" else "exchange_bf16_kernel (split-BF16 x3")
+                kname += ", every T_x formed)"
             rec["roofline_kernel"] = dict(
-                kernel=("exchange_tiled_kernel (FP64" if mode == "fp64" else "exchange_bf16_kernel (split-BF16 x3")
-                + ", one launch per spin)",
+                kernel="exchange, one launch per spin: " + kname,
                 avg_launch_ms=ex_avg_s * 1e3, flops_per_launch=fl, achieved=fl / ex_avg_s / 1e12,
                 peak=PEAK["f64_mfma_tflops"] if mode == "fp64" else PEAK["bf16_mfma_tflops"] / 3.0,
                 peak_note="FP64 MFMA 78.6 TFLOP/s" if mode == "fp64" else
@@ -332,7 +348,9 @@ def main():
     dtype = {"fp64": "f64",
              "mixed": "f64 (Green's function, weights, one-body, QR) + bf16x3 force bias, VHS, Taylor, exchange "
                       "(fp32 accumulate)",
-             "mixed+surrogate": "mixed, exchange on 1/8 of walkers + bf16-weight MLP control variate"}[head]
+             "mixed+surrogate": "mixed precision as above; exchange estimated per walker as g + [sampled] 8 (exact - g) "
+                                "with g = E_T + 2<K,dG> (f64) + Q(dG) (bf16 x1) + gain * MLP (bf16 weights), exact "
+                                "bf16x3 exchange on a random 1/8 of walkers"}[head]
     out = {
         "metric": METRIC,
         "value": m["value"],

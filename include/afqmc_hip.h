@@ -109,6 +109,14 @@ int afqmc_sync(afqmc_ctx* ctx);
 int afqmc_get_flags(afqmc_ctx* ctx, int32_t* flags, int clear);
 /* total force-bias caps accumulated on the device (synchronises) */
 int afqmc_get_nfb(afqmc_ctx* ctx, uint64_t* nfb);
+/* How this context evaluates the exchange sum_x tr(T_x T_x) (no counterpart in
+ * ipie; for work accounting): AFQMC_EXCHANGE_T forms every T_x = L_x Ghalf^T
+ * (ipie's contraction); AFQMC_EXCHANGE_QUAD evaluates g^T Wq g with g = vec(Ghalf)
+ * and Wq[(i,m'),(j,m)] = sum_x L_x[i][m] L_x[j][m'] built once (used when
+ * N < X and Wq fits the device budget). */
+#define AFQMC_EXCHANGE_T 0
+#define AFQMC_EXCHANGE_QUAD 1
+int afqmc_exchange_form(const afqmc_ctx* ctx);
 
 /* Ghalf_s = (phi_s^T psi_s^*)^{-1} phi_s^T and ovlp = prod_s det(phi_s^T psi_s^*) e^{-log_shift}.
  * phia [W, N, n_a], phib [W, N, n_b], Ghalfa [W, n_a, N], Ghalfb [W, n_b, N],

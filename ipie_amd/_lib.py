@@ -91,6 +91,7 @@ SIGNATURES = {
     "afqmc_surrogate_configure": (ctypes.c_int, [_dp, ctypes.c_int, _f64, ctypes.c_int, ctypes.c_int, _f64, _u64]),
     "afqmc_surrogate_stats": (ctypes.c_int, [_dp, ctypes.POINTER(_f64), ctypes.c_int]),
     "afqmc_profile_read": (ctypes.c_int, [_dp, ctypes.POINTER(_f64), ctypes.POINTER(_i64)]),
+    "afqmc_exchange_form": (ctypes.c_int, [_dp]),
 }
 
 PHASES = ["gf", "one_body", "force_bias", "fields", "vhs", "taylor", "overlap", "weight", "exchange", "energy",
@@ -332,6 +333,10 @@ class Context:
     def surrogate_configure(self, enable=True, fraction=0.125, warmup=100, train_iters=4, lr=1e-3, seed=11):
         self._call("afqmc_surrogate_configure", int(bool(enable)), float(fraction), int(warmup), int(train_iters),
                    float(lr), int(seed))
+
+    def exchange_form(self):
+        """'quad' (g^T Wq g, Wq built once) or 't' (every T_x = L_x Ghalf^T, ipie's contraction)."""
+        return "quad" if self.lib.afqmc_exchange_form(self.h) == 1 else "t"
 
     def surrogate_stats(self):
         out = (_f64 * 7)()

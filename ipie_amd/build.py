@@ -17,6 +17,7 @@ SOURCES = [
     "k_gemm.hip",
     "k_exchange.hip",
     "k_bf16.hip",
+    "k_quad.hip",
     "k_surrogate.hip",
     "k_fields.hip",
     "k_qr.hip",

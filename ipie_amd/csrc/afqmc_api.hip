@@ -60,6 +60,13 @@ struct afqmc_ctx {
   uint16_t* vplanes = nullptr;                // 4 x [W N, kpad] (VHS, Taylor series)
   uint16_t* tplanes[2] = {nullptr, nullptr};  // 4 x [W, rpad, kpad] each (Taylor terms, transposed)
   bool planes_ready = false;                  // lplanes / cplanes hold the current static tensors
+  // exchange as g^T Wq g (when Wq fits the budget and N < X)
+  bool quad_on = false;
+  bool quad_ready = false;                    // Wq holds the current static tensors
+  int quad_ks = 1;                            // K ranges of the split-BF16 Y = G Wq
+  double* qw64[2] = {nullptr, nullptr};       // FP64 Wq per spin [n_s N, n_s N]
+  uint16_t* qwp[2] = {nullptr, nullptr};      // split-BF16 Wq per spin 2 x [n_s Kp, n_s Kp]
+  double* qy = nullptr;                       // Y = G Wq partials [quad_ks, W, n Np] complex
   int xs_used[2] = {1, 1};  // partial-sum count of the last exchange per spin
   int64_t exx_cap = 0;      // capacity (complex entries) of each exchange partial buffer
   double* gcomp = nullptr;  // [W, nmax, N] complex: Ghalf of sampled walkers (FP64 exchange)
@@ -217,6 +224,16 @@ int ensure_ws(afqmc_ctx* c) {
       ok = ok && dmalloc((void**)&c->tplanes[i], tbytes) == hipSuccess;
       ok = ok && hipMemset(c->tplanes[i], 0, tbytes) == hipSuccess;  // padding stays zero
     }
+  }
+  if (c->quad_on) {
+    const bool mixed = d.precision == AFQMC_PREC_MIXED;
+    const int64_t Np = mixed ? c->kpad : N;
+    for (int s = 0; s < (two_spins(c) ? 2 : 1); ++s) {
+      const int64_t rows = (int64_t)(s == 0 ? d.nalpha : d.nbeta) * Np;
+      if (mixed) ok = ok && dmalloc((void**)&c->qwp[s], 2 * (size_t)2 * rows * rows) == hipSuccess;
+      else ok = ok && dmalloc((void**)&c->qw64[s], 8 * (size_t)rows * rows) == hipSuccess;
+    }
+    ok = ok && dmalloc((void**)&c->qy, 16 * (size_t)c->quad_ks * W * nmax * Np) == hipSuccess;
   }
   ok = ok && dmalloc((void**)&c->vhs, 16 * (size_t)W * N * N) == hipSuccess;
   for (int i = 0; i < 4; ++i) ok = ok && dmalloc((void**)&c->buf[i], 16 * (size_t)W * N * c->ncol) == hipSuccess;
@@ -538,6 +555,70 @@ int taylor(afqmc_ctx* c, int nw, const double** result) {
   return AFQMC_OK;
 }
 
+// Wq per spin from the half-rotated Cholesky vectors, once per static set.
+int ensure_quad(afqmc_ctx* c) {
+  if (c->quad_ready) return AFQMC_OK;
+  const int N = c->d.nbasis;
+  for (int s = 0; s < (two_spins(c) ? 2 : 1); ++s) {
+    const int n = s == 0 ? c->d.nalpha : c->d.nbeta;
+    const double* rchol = s == 0 ? c->s.rchola : c->s.rcholb;
+    if (c->qwp[s]) HIP_TRY(c, launch_quad_w(rchol, c->d.nchol, n, N, c->kpad, nullptr, c->qwp[s], c->stream));
+    else HIP_TRY(c, launch_quad_w(rchol, c->d.nchol, n, N, N, c->qw64[s], nullptr, c->stream));
+  }
+  c->quad_ready = true;
+  return AFQMC_OK;
+}
+
+// Exchange of nw walkers for one spin as g^T Wq g: Y = G Wq (FP64 NT GEMM, or
+// split-BF16 GEMM with split K), then one dot product per walker.
+int exchange_spin_quad(afqmc_ctx* c, int nw, int spin, const double* g, const int* walker_of) {
+  const int N = c->d.nbasis;
+  const int n = spin == 0 ? c->d.nalpha : c->d.nbeta;
+  double* part = spin == 0 ? c->exx_a : c->exx_b;
+  int r = ensure_quad(c);
+  if (r) return r;
+  PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
+  if (c->qwp[spin]) {
+    HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, walker_of, n));
+    const int64_t ldn = (int64_t)n * c->kpad;
+    Bf16GemmArgs b{};
+    b.a = c->gplanes[spin];
+    b.a_plane = (int64_t)nw * ldn;
+    b.lda = ldn;
+    b.m = nw;
+    b.b = c->qwp[spin];
+    b.b_plane = ldn * ldn;
+    b.ldb = ldn;
+    b.nn = ldn;
+    b.kchunks = (int)(ldn / 32);
+    b.ksplit = c->quad_ks;
+    b.epilogue = BF16_EPI_PARTIAL;
+    b.out = c->qy;
+    b.z_offset = 0;
+    b.scale = 1.0;
+    HIP_TRY(c, launch_bf16_gemm(b, c->stream));
+    HIP_TRY(c, launch_quad_reduce(c->qy, c->quad_ks, nw, n, N, c->kpad, g, walker_of, part, c->stream));
+  } else {
+    if (walker_of) {
+      HIP_TRY(c, launch_sur_gather(g, walker_of, nw, (int64_t)n * N, c->gcomp, c->stream));
+      g = c->gcomp;
+    }
+    const int64_t nN = (int64_t)n * N;
+    ForceBiasArgs a;
+    a.nwalkers = nw;
+    a.nchol = (int)nN;
+    a.ksplit = 1;
+    a.nseg = 1;
+    a.seg[0] = NtSegment{g, nN, c->qw64[spin], nN, (int)nN};
+    a.scale = 1.0;
+    a.vbias_part = c->qy;
+    HIP_TRY(c, launch_force_bias(a, c->stream));
+    HIP_TRY(c, launch_quad_reduce(c->qy, 1, nw, n, N, N, g, nullptr, part, c->stream));
+  }
+  c->xs_used[spin] = 1;
+  return AFQMC_OK;
+}
+
 // Exchange partial sums of nw walkers for one spin.  With walker_of, the nw
 // walkers are the sampled ones (walker_of[t], t < nw) of the Ghalf array g.
 int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g, const int* walker_of) {
@@ -545,6 +626,7 @@ int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g, const int* wa
   const int n = spin == 0 ? c->d.nalpha : c->d.nbeta;
   const double* rchol = spin == 0 ? c->s.rchola : c->s.rcholb;
   double* part = spin == 0 ? c->exx_a : c->exx_b;
+  if (c->quad_on) return exchange_spin_quad(c, nw, spin, g, walker_of);
   if (c->d.precision == AFQMC_PREC_MIXED && c->bx_ok[spin]) {
     int r = ensure_planes(c);
     if (r) return r;
@@ -865,6 +947,19 @@ int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims) {
     const int n = s == 0 ? d.nalpha : d.nbeta;
     c->bx_ok[s] = bf16_exchange_plan(n, &c->bx_wb[s], &c->bx_xb[s]);
   }
+  {
+    // exchange as a quadratic form: cheaper than forming every T_x when N < X;
+    // Wq takes (n N)^2 values per spin (FP64, or two BF16 planes of the padded size)
+    const int nmx = d.nalpha > d.nbeta ? d.nalpha : d.nbeta;
+    const double per_spin = d.precision == AFQMC_PREC_MIXED ? 4.0 * std::pow((double)nmx * c->kpad, 2)
+                                                            : 8.0 * std::pow((double)nmx * d.nbasis, 2);
+    c->quad_on = d.nchol > d.nbasis && per_spin <= QUAD_BUDGET_BYTES;
+    if (c->quad_on && d.precision == AFQMC_PREC_MIXED) {
+      const int64_t tiles = ceil_div(d.nwalkers, 128) * ceil_div((int64_t)nmx * c->kpad, 160);
+      int64_t ks = ceil_div(2048, tiles);
+      c->quad_ks = (int)std::min<int64_t>(std::max<int64_t>(ks, 1), 4);
+    }
+  }
   if (d.precision == AFQMC_PREC_MIXED) {
     c->xpad = (d.nchol + 31) / 32 * 32;
     c->bf16_fb = d.nchol > 0 && c->bx_ok[0] && (!two || c->bx_ok[1]);
@@ -892,7 +987,8 @@ void afqmc_destroy(afqmc_ctx* c) {
   void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
                   c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
                   c->lplanes[1], c->xplanes, c->cplanes, c->vplanes, c->tplanes[0],
-                  c->tplanes[1], c->gcomp, c->gf_o, c->gf_oinv, c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
+                  c->tplanes[1], c->gcomp, c->gf_o, c->gf_oinv, c->qw64[0], c->qw64[1], c->qwp[0],
+                  c->qwp[1], c->qy, c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
                   c->sur.resid, c->sur.walker_of, c->sur.xbar, c->sur.e1, c->sur.master, c->sur.m,
                   c->sur.v, c->sur.wbf, c->sur.bias, c->sur.grad, c->sur.loss, c->sur.state, c->sur.gT,
                   c->sur.K, c->sur.et, c->sur.cv, c->sur.mlp, c->sur.gtmp};
@@ -922,11 +1018,14 @@ int afqmc_set_static(afqmc_ctx* c, const afqmc_static* s) {
   c->s = *s;
   c->have_static = true;
   c->planes_ready = false;
+  c->quad_ready = false;
   c->sur.cv_ready = false;
   return AFQMC_OK;
 }
 
 const char* afqmc_last_error(const afqmc_ctx* c) { return c ? c->err : "null context"; }
+
+int afqmc_exchange_form(const afqmc_ctx* c) { return (c && c->quad_on) ? AFQMC_EXCHANGE_QUAD : AFQMC_EXCHANGE_T; }
 
 int afqmc_sync(afqmc_ctx* c) {
   if (!c) return AFQMC_ERR_INVALID_ARG;

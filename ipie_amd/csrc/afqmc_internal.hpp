@@ -124,6 +124,17 @@ struct Bf16GemmArgs {
 };
 hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream);
 
+// ---------------------------------------------------------------- exchange as g^T Wq g
+constexpr double QUAD_BUDGET_BYTES = 24e9;  // largest Wq per spin held on the device
+// Wq[(i,m'),(j,m)] = sum_x L_x[i][m] L_x[j][m'] from rchol [X, n, N]: FP64 [nN][nN]
+// (Np = N) or split-BF16 planes [2][n Np][n Np] (Np = Kp, zero padding); exactly
+// one of w64 / wplanes is non-null.
+hipError_t launch_quad_w(const double* rchol, int X, int n, int N, int Np, double* w64, uint16_t* wplanes,
+                         hipStream_t stream);
+// part[w] = sum_z sum_{i, m < N} g[src][i][m] y[z][w][i Np + m], src = walker_of[w] or w
+hipError_t launch_quad_reduce(const double* y, int ksplit, int nwalkers, int n, int N, int Np, const double* g,
+                              const int* walker_of, double* part, hipStream_t stream);
+
 // One split-BF16 Taylor term per walker: out_k^T = alpha out_{k-1}^T VHS,
 // q = (init or q) + out_k, and (tout) the split planes of out_k^T.
 struct TaylorBf16Args {

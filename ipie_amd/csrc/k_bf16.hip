@@ -44,14 +44,18 @@ constexpr int BX_THREADS = 64 * BX_GR * BX_GC;
 constexpr int BX_KC = 32;
 constexpr size_t BX_LDS_MAX = 106496;  // LDS of a workgroup: two stages of the 3-product form
 
-// Staging of one K chunk, in 16-byte vectors.  SPLIT (3 products): A planes
-// re_hi, re_lo, im_hi, im_lo and B planes hi, lo.  Otherwise (1 product,
-// hi * hi): A planes re_hi, im_hi (physical planes 0 and 2) and B plane hi.
+// Staging, in 16-byte vectors.  SPLIT (3 products): A planes re_hi, re_lo,
+// im_hi, im_lo and B planes hi, lo.  Otherwise (1 product, hi * hi): A planes
+// re_hi, im_hi (physical planes 0 and 2) and B plane hi.  An LDS stage holds
+// KS sub-chunks of 32 K values (the 1-product form uses two, halving the
+// barriers per unit of MFMA work; the 3-product stage already fills the LDS).
 template <bool SPLIT>
 struct Bx {
   static constexpr int NA = SPLIT ? 4 : 2, NB = SPLIT ? 2 : 1;
-  static constexpr int A_VEC = NA * BX_R * 4, B_VEC = NB * BX_C * 4;
-  static constexpr int STAGE_VEC = A_VEC + B_VEC;
+  static constexpr int KS = SPLIT ? 1 : 2;
+  static constexpr int A_VEC = NA * BX_R * 4, B_VEC = NB * BX_C * 4;  // per sub-chunk
+  static constexpr int SUB_VEC = A_VEC + B_VEC;
+  static constexpr int STAGE_VEC = KS * SUB_VEC;
   static constexpr int A_PER = A_VEC / BX_THREADS;
   static constexpr int B_PER = (B_VEC + BX_THREADS - 1) / BX_THREADS;
   static constexpr size_t STAGES = 2 * 16 * (size_t)STAGE_VEC;  // double-buffered, bytes
@@ -144,30 +148,41 @@ __device__ __forceinline__ void bx_set_b(int64_t (&b_src)[Bx<SPLIT>::B_PER], int
   }
 }
 
+// Sub-chunks c32 .. c32 + nsub - 1 (32 K values each) into registers.
 template <bool SPLIT>
-__device__ __forceinline__ void bx_fetch(v4u (&pa)[Bx<SPLIT>::A_PER], v4u (&pb)[Bx<SPLIT>::B_PER], const uint16_t* gp,
+__device__ __forceinline__ void bx_fetch(v4u (&pa)[Bx<SPLIT>::KS][Bx<SPLIT>::A_PER],
+                                         v4u (&pb)[Bx<SPLIT>::KS][Bx<SPLIT>::B_PER], const uint16_t* gp,
                                          const uint16_t* lp, const int64_t (&a_src)[Bx<SPLIT>::A_PER],
-                                         const int64_t (&b_src)[Bx<SPLIT>::B_PER], int c) {
-  const int64_t k0 = (int64_t)c * BX_KC;
+                                         const int64_t (&b_src)[Bx<SPLIT>::B_PER], int c32, int nsub) {
 #pragma unroll
-  for (int i = 0; i < Bx<SPLIT>::A_PER; ++i) pa[i] = *reinterpret_cast<const v4u*>(gp + a_src[i] + k0);
+  for (int sc = 0; sc < Bx<SPLIT>::KS; ++sc) {
+    if (sc >= nsub) break;
+    const int64_t k0 = (int64_t)(c32 + sc) * BX_KC;
 #pragma unroll
-  for (int i = 0; i < Bx<SPLIT>::B_PER; ++i) pb[i] = *reinterpret_cast<const v4u*>(lp + b_src[i] + k0);
+    for (int i = 0; i < Bx<SPLIT>::A_PER; ++i) pa[sc][i] = *reinterpret_cast<const v4u*>(gp + a_src[i] + k0);
+#pragma unroll
+    for (int i = 0; i < Bx<SPLIT>::B_PER; ++i) pb[sc][i] = *reinterpret_cast<const v4u*>(lp + b_src[i] + k0);
+  }
 }
 
 template <bool SPLIT>
-__device__ __forceinline__ void bx_stash(uint4* lds_u4, const v4u (&pa)[Bx<SPLIT>::A_PER],
-                                         const v4u (&pb)[Bx<SPLIT>::B_PER], int tid, int buf) {
-  v4u* base = reinterpret_cast<v4u*>(lds_u4) + buf * Bx<SPLIT>::STAGE_VEC;
+__device__ __forceinline__ void bx_stash(uint4* lds_u4, const v4u (&pa)[Bx<SPLIT>::KS][Bx<SPLIT>::A_PER],
+                                         const v4u (&pb)[Bx<SPLIT>::KS][Bx<SPLIT>::B_PER], int tid, int buf,
+                                         int nsub) {
 #pragma unroll
-  for (int i = 0; i < Bx<SPLIT>::A_PER; ++i) {
-    const int v = tid + i * BX_THREADS;
-    base[lds_vec(v >> 2, v & 3)] = pa[i];
-  }
+  for (int sc = 0; sc < Bx<SPLIT>::KS; ++sc) {
+    if (sc >= nsub) break;
+    v4u* base = reinterpret_cast<v4u*>(lds_u4) + buf * Bx<SPLIT>::STAGE_VEC + sc * Bx<SPLIT>::SUB_VEC;
 #pragma unroll
-  for (int i = 0; i < Bx<SPLIT>::B_PER; ++i) {
-    const int v = tid + i * BX_THREADS;
-    if (v < Bx<SPLIT>::B_VEC) base[Bx<SPLIT>::A_VEC + lds_vec(v >> 2, v & 3)] = pb[i];
+    for (int i = 0; i < Bx<SPLIT>::A_PER; ++i) {
+      const int v = tid + i * BX_THREADS;
+      base[lds_vec(v >> 2, v & 3)] = pa[sc][i];
+    }
+#pragma unroll
+    for (int i = 0; i < Bx<SPLIT>::B_PER; ++i) {
+      const int v = tid + i * BX_THREADS;
+      if (v < Bx<SPLIT>::B_VEC) base[Bx<SPLIT>::A_VEC + lds_vec(v >> 2, v & 3)] = pb[sc][i];
+    }
   }
 }
 
@@ -176,9 +191,9 @@ __device__ __forceinline__ void bx_stash(uint4* lds_u4, const v4u (&pa)[Bx<SPLIT
 // re += Re(A) B and im += Im(A) B, each as hi*hi + hi*lo + lo*hi (SPLIT) or
 // hi*hi.
 template <bool SPLIT>
-__device__ __forceinline__ void bx_chunk(const uint4* stage, int wr, int wc, int l16, int q,
-                                         v4f (&re)[BX_MR][BX_MC], v4f (&im)[BX_MR][BX_MC]) {
-  const uint16_t* As = reinterpret_cast<const uint16_t*>(stage);
+__device__ __forceinline__ void bx_sub(const uint4* sub, int wr, int wc, int l16, int q,
+                                       v4f (&re)[BX_MR][BX_MC], v4f (&im)[BX_MR][BX_MC]) {
+  const uint16_t* As = reinterpret_cast<const uint16_t*>(sub);
   const uint16_t* Bs = As + 8 * Bx<SPLIT>::A_VEC;
   v8bf bh[BX_MC], bl[BX_MC];
 #pragma unroll
@@ -216,6 +231,17 @@ __device__ __forceinline__ void bx_chunk(const uint4* stage, int wr, int wc, int
   }
 }
 
+// The nsub sub-chunks of LDS stage `stage`.
+template <bool SPLIT>
+__device__ __forceinline__ void bx_chunk(const uint4* stage, int wr, int wc, int l16, int q,
+                                         v4f (&re)[BX_MR][BX_MC], v4f (&im)[BX_MR][BX_MC], int nsub) {
+#pragma unroll
+  for (int sc = 0; sc < Bx<SPLIT>::KS; ++sc) {
+    if (sc >= nsub) break;
+    bx_sub<SPLIT>(stage + sc * Bx<SPLIT>::SUB_VEC, wr, wc, l16, q, re, im);
+  }
+}
+
 template <bool SPLIT>
 __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16Args a) {
   using C = Bx<SPLIT>;
@@ -238,7 +264,7 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
   const int wr = wave % BX_GR, wc = wave / BX_GR;
   const int64_t arows = (int64_t)a.nwalkers * n, brows = (int64_t)a.nchol * n;
   const int64_t aplane = arows * Kp, bplane = brows * Kp;
-  const int nch = Kp / BX_KC;
+  const int nch32 = Kp / BX_KC, nch = (int)ceil_div(nch32, C::KS);  // 32-wide K chunks, LDS stages
 
   // Source offsets (bf16 units, K chunk 0) of the vectors this thread stages.
   // Rows past the end of an array are clamped to its last row: padding rows
@@ -252,7 +278,8 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
     if (r > arows - 1) r = arows - 1;
     a_src[i] = C::a_plane(p) * aplane + r * Kp + (rem % 4) * 8;
   }
-  v4u pa[C::A_PER], pb[C::B_PER];
+  v4u pa[C::KS][C::A_PER], pb[C::KS][C::B_PER];
+  const int nsub0 = min(C::KS, nch32);
 
   // trace epilogue: walker wl of the block is summed by threads [wl G, (wl + 1) G)
   const int G = BX_THREADS / wb;
@@ -263,11 +290,11 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
 
   if (xb_begin < xb_end) {
     bx_set_b<SPLIT>(b_src, tid, xb_begin, xb, n, brows, bplane, Kp);
-    bx_fetch<SPLIT>(pa, pb, a.gplanes, a.lplanes, a_src, b_src, 0);
+    bx_fetch<SPLIT>(pa, pb, a.gplanes, a.lplanes, a_src, b_src, 0, nsub0);
   }
   for (int xblk = xb_begin; xblk < xb_end; ++xblk) {
     const int x0 = xblk * xb;
-    bx_stash<SPLIT>(lds_u4, pa, pb, tid, 0);
+    bx_stash<SPLIT>(lds_u4, pa, pb, tid, 0, nsub0);
     __syncthreads();
     v4f re[BX_MR][BX_MC], im[BX_MR][BX_MC];
 #pragma unroll
@@ -281,10 +308,13 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
       // next chunk, or after the last one the first chunk of the next x block
       // (stashed after the epilogue)
       const bool more = c + 1 < nch, next_blk = !more && xblk + 1 < xb_end;
+      const int nsub = min(C::KS, nch32 - c * C::KS);
+      const int nsub_next = more ? min(C::KS, nch32 - (c + 1) * C::KS) : nsub0;
       if (next_blk) bx_set_b<SPLIT>(b_src, tid, xblk + 1, xb, n, brows, bplane, Kp);
-      if (more || next_blk) bx_fetch<SPLIT>(pa, pb, a.gplanes, a.lplanes, a_src, b_src, more ? c + 1 : 0);
-      bx_chunk<SPLIT>(lds_u4 + (c & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im);
-      if (c + 1 < nch) bx_stash<SPLIT>(lds_u4, pa, pb, tid, (c + 1) & 1);
+      if (more || next_blk)
+        bx_fetch<SPLIT>(pa, pb, a.gplanes, a.lplanes, a_src, b_src, more ? (c + 1) * C::KS : 0, nsub_next);
+      bx_chunk<SPLIT>(lds_u4 + (c & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub);
+      if (more) bx_stash<SPLIT>(lds_u4, pa, pb, tid, (c + 1) & 1, nsub_next);
       __syncthreads();
     }
     // epilogue: the block's vectors in groups of xg, staged as S[v][row][col]
@@ -402,17 +432,18 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
       re[i][j] = zero4f();
       im[i][j] = zero4f();
     }
-  v4u pa[C::A_PER], pb[C::B_PER];
+  static_assert(C::KS == 1, "the GEMM stages one 32-wide K chunk at a time");
+  v4u pa[1][C::A_PER], pb[1][C::B_PER];
   if (c0 < c1) {
-    bx_fetch<true>(pa, pb, a.a, a.b, a_src, b_src, c0);
-    bx_stash<true>(lds_u4, pa, pb, tid, 0);
+    bx_fetch<true>(pa, pb, a.a, a.b, a_src, b_src, c0, 1);
+    bx_stash<true>(lds_u4, pa, pb, tid, 0, 1);
     __syncthreads();
   }
   for (int c = c0; c < c1; ++c) {
     const bool more = c + 1 < c1;
-    if (more) bx_fetch<true>(pa, pb, a.a, a.b, a_src, b_src, c + 1);
-    bx_chunk<true>(lds_u4 + ((c - c0) & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im);
-    if (more) bx_stash<true>(lds_u4, pa, pb, tid, (c + 1 - c0) & 1);
+    if (more) bx_fetch<true>(pa, pb, a.a, a.b, a_src, b_src, c + 1, 1);
+    bx_chunk<true>(lds_u4 + ((c - c0) & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im, 1);
+    if (more) bx_stash<true>(lds_u4, pa, pb, tid, (c + 1 - c0) & 1, 1);
     __syncthreads();
   }
   const int64_t NN = (int64_t)a.nbasis * a.nbasis;

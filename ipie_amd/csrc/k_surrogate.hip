@@ -35,7 +35,9 @@ namespace {
 
 constexpr int SG_H = SUR_HIDDEN;
 constexpr int SG_F = SUR_FEATURES;
-constexpr int SG_SAMPLES_PER_WG = 32;
+// samples per gradient workgroup: each walks its samples serially, so few per
+// workgroup and many workgroups (partial gradients are summed in fixed order)
+constexpr int SG_SAMPLES_PER_WG = 4;
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
 __device__ __forceinline__ float dsilu(float z) {

@@ -25,6 +25,7 @@ CASES = {
     "n200_n20": (200, 1000, 150.0, 20, 20, 13, (12.0 / (200 * 150.0)) ** 0.5),
     "n96_n24_n12": (96, 200, 60.0, 24, 12, 5, 0.1),
     "n24_fallback": (24, 72, 12.0, 5, 5, 10, (12.0 / (24 * 12.0)) ** 0.5),
+    "n48_x40": (48, 40, 20.0, 8, 6, 5, 0.1),  # X < N: exchange through every T_x
 }
 
 EXX_TOL = 1e-4

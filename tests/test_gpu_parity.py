@@ -27,6 +27,9 @@ CASES = {
     "n10_k2": (10, 30, 8.0, 4, 2, 5, 0.3),
     "n64_n40": (64, 96, 40.0, 40, 40, 4, 0.05),
     "n200_n20": (200, 1000, 150.0, 20, 20, 6, (12.0 / (200 * 150.0)) ** 0.5),
+    # fewer Cholesky vectors than basis functions: the exchange forms every T_x
+    # instead of the quadratic form g^T Wq g
+    "n48_x40": (48, 40, 20.0, 8, 6, 5, 0.1),
 }
 
 
@@ -99,7 +102,7 @@ def test_energy_estimator_known_answer_on_device():
     assert enumer == pytest.approx(-754.0373585215561, rel=1e-11)
 
 
-@pytest.mark.parametrize("name", ["n24", "n24_norm", "odd", "n10_k2", "n64_n40", "n200_n20"])
+@pytest.mark.parametrize("name", ["n24", "n24_norm", "odd", "n10_k2", "n64_n40", "n200_n20", "n48_x40"])
 def test_propagation_steps_with_resync(name):
     torch = _torch()
     p = make_pair(name)
