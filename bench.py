@@ -306,9 +306,10 @@ def main():
             # control variate over all walkers (its own profiling phase)
             cv_ms, cv_n = prof["exchange_cv"]
             cv_avg_s = cv_ms / max(cv_n, 1) / 1e3
+            cv_kernel = ("Q = dG^T Wq dG, single BF16 product (bf16_gemm_kernel<false>, split K) + quad_reduce_kernel"
+                         if form == "quad" else "exchange_bf16_kernel<false> (single BF16 product, every dT_x formed)")
             rec["roofline_kernel"] = dict(
-                kernel="exchange_bf16_kernel (single BF16 product, control-variate Q(G - G_T) of every walker, "
-                       "one launch per spin)",
+                kernel="control-variate exchange of every walker, one launch per spin: " + cv_kernel,
                 avg_launch_ms=cv_avg_s * 1e3, flops_per_launch=fl, achieved=fl / cv_avg_s / 1e12,
                 peak=PEAK["bf16_mfma_tflops"], peak_note="dense BF16 MFMA 2.5 PFLOP/s, one product per flop")
             ss = st["prop"].surrogate_stats()

@@ -764,14 +764,42 @@ int sur_prepare_cv(afqmc_ctx* c) {
 }
 
 // Q_s(w) = sum_x tr(dT_x dT_x), dT_x = L_x (G_s,w - G_T,s)^T, single BF16
-// products, partial sums into the exchange partial buffer of spin s.
+// products, partial sums into the exchange partial buffer of spin s: as
+// dG^T Wq dG when the split-BF16 Wq planes exist, else through every dT_x.
 int exchange_q_spin(afqmc_ctx* c, int nw, int spin, const double* g) {
   const int N = c->d.nbasis;
   const int n = spin == 0 ? c->d.nalpha : c->d.nbeta;
   const int64_t gstride = 2 * (int64_t)std::max(c->d.nalpha, c->d.nbeta) * N;
+  const double* gT = c->sur.gT + spin * gstride;
+  double* part = spin == 0 ? c->exx_a : c->exx_b;
+  if (c->qwp[spin]) {
+    int r = ensure_quad(c);
+    if (r) return r;
+    PhaseScope ps(c, AFQMC_PHASE_EXCHANGE_CV);
+    HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, nullptr, n, gT));
+    const int64_t ldn = (int64_t)n * c->kpad;
+    Bf16GemmArgs b{};
+    b.a = c->gplanes[spin];
+    b.a_plane = (int64_t)nw * ldn;
+    b.lda = ldn;
+    b.m = nw;
+    b.b = c->qwp[spin];
+    b.b_plane = ldn * ldn;
+    b.ldb = ldn;
+    b.nn = ldn;
+    b.kchunks = (int)(ldn / 32);
+    b.ksplit = c->quad_ks;
+    b.single = 1;
+    b.epilogue = BF16_EPI_PARTIAL;
+    b.out = c->qy;
+    b.scale = 1.0;
+    HIP_TRY(c, launch_bf16_gemm(b, c->stream));
+    HIP_TRY(c, launch_quad_reduce(c->qy, c->quad_ks, nw, n, N, c->kpad, g, nullptr, part, c->stream, gT));
+    c->xs_used[spin] = 1;
+    return AFQMC_OK;
+  }
   PhaseScope ps(c, AFQMC_PHASE_EXCHANGE_CV);
-  HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, nullptr, n,
-                                c->sur.gT + spin * gstride));
+  HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, nullptr, n, gT));
   ExchangeBf16Args b{};
   b.split = 0;
   b.nwalkers = nw;

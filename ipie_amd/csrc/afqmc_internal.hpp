@@ -114,6 +114,7 @@ struct Bf16GemmArgs {
   const uint16_t* b;
   int64_t b_plane, ldb, nn;
   int kchunks, ksplit;
+  int single;        // 1: hi * hi only (A planes 0 and 2, B plane 0), partial epilogue
   int epilogue;
   double* out;       // PARTIAL: [z_offset + ksplit][m][nn] complex; VHS: [m][N][N] complex
   int z_offset;
@@ -131,9 +132,11 @@ constexpr double QUAD_BUDGET_BYTES = 24e9;  // largest Wq per spin held on the d
 // one of w64 / wplanes is non-null.
 hipError_t launch_quad_w(const double* rchol, int X, int n, int N, int Np, double* w64, uint16_t* wplanes,
                          hipStream_t stream);
-// part[w] = sum_z sum_{i, m < N} g[src][i][m] y[z][w][i Np + m], src = walker_of[w] or w
+// part[w] = sum_z sum_{i, m < N} d[i][m] y[z][w][i Np + m], d = g[src] - gref (gref
+// [n, N] or null), src = walker_of[w] or w
 hipError_t launch_quad_reduce(const double* y, int ksplit, int nwalkers, int n, int N, int Np, const double* g,
-                              const int* walker_of, double* part, hipStream_t stream);
+                              const int* walker_of, double* part, hipStream_t stream,
+                              const double* gref = nullptr);
 
 // One split-BF16 Taylor term per walker: out_k^T = alpha out_{k-1}^T VHS,
 // q = (init or q) + out_k, and (tout) the split planes of out_k^T.

@@ -390,13 +390,15 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
 }
 
 // C = A B^T for split-BF16 operand planes (A: 4 planes, complex; B: 2 planes,
-// real), K chunk range of split z, 128 x 160 tile per workgroup.  Epilogues:
+// real), K chunk range of split z, 128 x 160 tile per workgroup; SPLIT = false
+// uses the hi planes only (1 product).  Epilogues:
 //   BF16_EPI_PARTIAL  out[z_offset + z][row][col] = scale C     (force bias)
 //   BF16_EPI_VHS      out[w][i][j] = out[w][j][i] = i scale C[w][p],
 //                     (i, j) = (tri_i[p], tri_j[p])              (VHS)
 // Workgroups that run together share the B tile (row blocks vary fastest).
+template <bool SPLIT>
 __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
-  using C = Bx<true>;
+  using C = Bx<SPLIT>;
   extern __shared__ uint4 lds_u4[];
   const int mb = (int)ceil_div(a.m, BX_R), nb = (int)ceil_div(a.nn, BX_C);
   const int lin = xcd_remap(blockIdx.x, gridDim.x);
@@ -414,7 +416,7 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
     const int p = v / (BX_R * 4), rem = v % (BX_R * 4);
     int64_t r = (int64_t)rb * BX_R + rem / 4;
     if (r > a.m - 1) r = a.m - 1;
-    a_src[i] = p * a.a_plane + r * a.lda + (rem % 4) * 8;
+    a_src[i] = C::a_plane(p) * a.a_plane + r * a.lda + (rem % 4) * 8;
   }
 #pragma unroll
   for (int i = 0; i < C::B_PER; ++i) {
@@ -422,7 +424,7 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
     const int p = v / (BX_C * 4), rem = v % (BX_C * 4);
     int64_t r = (int64_t)cb * BX_C + rem / 4;
     if (r > a.nn - 1) r = a.nn - 1;
-    b_src[i] = p * a.b_plane + r * a.ldb + (rem % 4) * 8;
+    b_src[i] = C::b_plane(p) * a.b_plane + r * a.ldb + (rem % 4) * 8;
   }
   v4f re[BX_MR][BX_MC], im[BX_MR][BX_MC];
 #pragma unroll
@@ -432,18 +434,23 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
       re[i][j] = zero4f();
       im[i][j] = zero4f();
     }
-  static_assert(C::KS == 1, "the GEMM stages one 32-wide K chunk at a time");
-  v4u pa[1][C::A_PER], pb[1][C::B_PER];
-  if (c0 < c1) {
-    bx_fetch<true>(pa, pb, a.a, a.b, a_src, b_src, c0, 1);
-    bx_stash<true>(lds_u4, pa, pb, tid, 0, 1);
+  // LDS stages of up to KS sub-chunks of 32 K values over this split's range [c0, c1)
+  v4u pa[C::KS][C::A_PER], pb[C::KS][C::B_PER];
+  const int nst = (int)ceil_div(c1 - c0, C::KS);
+  if (nst > 0) {
+    const int ns0 = min(C::KS, c1 - c0);
+    bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0, ns0);
+    bx_stash<SPLIT>(lds_u4, pa, pb, tid, 0, ns0);
     __syncthreads();
   }
-  for (int c = c0; c < c1; ++c) {
-    const bool more = c + 1 < c1;
-    if (more) bx_fetch<true>(pa, pb, a.a, a.b, a_src, b_src, c + 1, 1);
-    bx_chunk<true>(lds_u4 + ((c - c0) & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im, 1);
-    if (more) bx_stash<true>(lds_u4, pa, pb, tid, (c + 1 - c0) & 1, 1);
+  for (int t = 0; t < nst; ++t) {
+    const int cs = c0 + t * C::KS;
+    const int nsub = min(C::KS, c1 - cs);
+    const bool more = t + 1 < nst;
+    const int nsub_next = more ? min(C::KS, c1 - (cs + C::KS)) : 0;
+    if (more) bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, cs + C::KS, nsub_next);
+    bx_chunk<SPLIT>(lds_u4 + (t & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub);
+    if (more) bx_stash<SPLIT>(lds_u4, pa, pb, tid, (t + 1) & 1, nsub_next);
     __syncthreads();
   }
   const int64_t NN = (int64_t)a.nbasis * a.nbasis;
@@ -689,10 +696,17 @@ hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream) {
       (int64_t)a.kchunks * BX_KC > a.lda || (int64_t)a.kchunks * BX_KC > a.ldb)
     return hipErrorInvalidValue;
   if (a.epilogue == BF16_EPI_VHS && (int64_t)a.nbasis * (a.nbasis + 1) / 2 != a.nn) return hipErrorInvalidValue;
-  hipError_t e = allow_lds((const void*)bf16_gemm_kernel, Bx<true>::STAGES);
-  if (e != hipSuccess) return e;
   const int64_t grid = ceil_div(a.m, BX_R) * ceil_div(a.nn, BX_C) * a.ksplit;
-  hipLaunchKernelGGL(bf16_gemm_kernel, dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES, stream, a);
+  if (a.single) {
+    if (a.epilogue != BF16_EPI_PARTIAL) return hipErrorInvalidValue;
+    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<false>, Bx<false>::STAGES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bf16_gemm_kernel<false>, dim3((unsigned)grid), dim3(BX_THREADS), Bx<false>::STAGES, stream, a);
+  } else {
+    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true>, Bx<true>::STAGES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bf16_gemm_kernel<true>, dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES, stream, a);
+  }
   return hipGetLastError();
 }
 

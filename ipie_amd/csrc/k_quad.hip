@@ -97,13 +97,14 @@ __global__ void __launch_bounds__(256) quad_w_kernel(const double* __restrict__ 
   }
 }
 
-// E_w = sum_z sum_a g_w[a] Y_z[w][a]; Y index a = i Np + m (m < N used), g
-// from Ghalf [W, n, N] (or its sampled rows: walker_of).  One workgroup per
-// walker, result into part[w] (complex).
+// E_w = sum_z sum_a d_w[a] Y_z[w][a]; Y index a = i Np + m (m < N used), d = g
+// from Ghalf [W, n, N] (or its sampled rows: walker_of), minus gref [n, N] when
+// given (the control variate's dG).  One workgroup per walker, result into
+// part[w] (complex).
 __global__ void __launch_bounds__(256) quad_reduce_kernel(const double* __restrict__ y, int ksplit, int nwalkers,
                                                          int n, int N, int Np, const double* __restrict__ g,
                                                          const int* __restrict__ walker_of,
-                                                         double* __restrict__ part) {
+                                                         double* __restrict__ part, const double* __restrict__ gref) {
   __shared__ double red[256 / 64];
   const int w = blockIdx.x;
   const int64_t ldy = (int64_t)n * Np;
@@ -114,7 +115,9 @@ __global__ void __launch_bounds__(256) quad_reduce_kernel(const double* __restri
     const double* yw = y + 2 * ((int64_t)z * nwalkers + w) * ldy;
     for (int64_t e = threadIdx.x; e < (int64_t)n * N; e += blockDim.x) {
       const int i = (int)(e / N), m = (int)(e % N);
-      const cplx gv = ld(gw, e), yv = ld(yw, (int64_t)i * Np + m);
+      cplx gv = ld(gw, e);
+      if (gref) gv = csub(gv, ld(gref, e));
+      const cplx yv = ld(yw, (int64_t)i * Np + m);
       re += gv.re * yv.re - gv.im * yv.im;
       im += gv.re * yv.im + gv.im * yv.re;
     }
@@ -136,9 +139,9 @@ hipError_t launch_quad_w(const double* rchol, int X, int n, int N, int Np, doubl
 }
 
 hipError_t launch_quad_reduce(const double* y, int ksplit, int nwalkers, int n, int N, int Np, const double* g,
-                              const int* walker_of, double* part, hipStream_t stream) {
+                              const int* walker_of, double* part, hipStream_t stream, const double* gref) {
   hipLaunchKernelGGL(quad_reduce_kernel, dim3(nwalkers), dim3(256), 0, stream, y, ksplit, nwalkers, n, N, Np, g,
-                     walker_of, part);
+                     walker_of, part, gref);
   return hipGetLastError();
 }
 
