@@ -57,9 +57,13 @@ def step_flops(N, X, na, nb):
 def exchange_flops_per_launch(N, X, n, W, form="t"):
     """Real flops of one exchange launch (one spin, W walkers) in the form the
     context uses: 't' forms every T_x = L_x Ghalf^T (ipie's contraction), 'quad'
-    is Y = G Wq (complex G, real Wq of size (nN)^2) plus a dot per walker."""
+    is Y = G Wq (complex G, real Wq of size (nN)^2) plus a dot per walker,
+    'quad_sym' the same quadratic form over the (nN)(nN + 1) / 2 distinct
+    entries of the symmetric Wq (the block-triangular split-BF16 form)."""
     if form == "quad":
         return float(W) * (4.0 * (n * N) ** 2 + 8.0 * n * N)
+    if form == "quad_sym":
+        return float(W) * (2.0 * n * N * (n * N + 1) + 8.0 * n * N)
     return float(X) * W * (4 * n * n * N + 8 * n * n)
 
 
@@ -268,6 +272,8 @@ def main():
         elapsed, prof, e_steps, finite = run_gpu(st, w, K, warmup)
         ex_ms, ex_n = prof["exchange"]
         form = st["walkers"]._engine.ctx.exchange_form()
+        if form == "quad" and mode != "fp64":
+            form = "quad_sym"  # the split-BF16 planes hold the block-triangular Wq
         fl = 0.5 * (exchange_flops_per_launch(w.nbasis, w.nchol, w.nalpha, w.nwalkers, form)
                     + exchange_flops_per_launch(w.nbasis, w.nchol, w.nbeta, w.nwalkers, form))
         fl_t = 0.5 * (exchange_flops_per_launch(w.nbasis, w.nchol, w.nalpha, w.nwalkers, "t")
@@ -290,8 +296,10 @@ def main():
             rec["exchange_tflops"] = fl / ex_avg_s / 1e12
             rec["exchange_flops_per_launch"] = fl
             if form == "quad":
-                kname = ("Y = G Wq on FP64 MFMA (force_bias_kernel NT mainloop) + quad_reduce_kernel" if mode == "fp64"
-                         else "Y = G Wq split-BF16 x3 (bf16_gemm_kernel, split K) + quad_reduce_kernel")
+                kname = "Y = G Wq on FP64 MFMA (force_bias_kernel NT mainloop) + quad_reduce_kernel"
+            elif form == "quad_sym":
+                kname = ("Y = G W' over block-triangular W' split-BF16 x3 (bf16_gemm_kernel<true>, g.Y in the "
+                         "epilogue) + quad_sum_kernel")
             else:
                 kname = ("exchange_tiled_kernel (FP64" if mode == "fp64" else "exchange_bf16_kernel (split-BF16 x3")
                 kname += ", every T_x formed)"
@@ -306,8 +314,9 @@ def main():
             # control variate over all walkers (its own profiling phase)
             cv_ms, cv_n = prof["exchange_cv"]
             cv_avg_s = cv_ms / max(cv_n, 1) / 1e3
-            cv_kernel = ("Q = dG^T Wq dG, single BF16 product (bf16_gemm_kernel<false>, split K) + quad_reduce_kernel"
-                         if form == "quad" else "exchange_bf16_kernel<false> (single BF16 product, every dT_x formed)")
+            cv_kernel = ("Q = dG^T Wq dG over block-triangular W', single BF16 product (bf16_gemm_kernel<false>, "
+                         "dG.Y in the epilogue) + quad_sum_kernel" if form == "quad_sym"
+                         else "exchange_bf16_kernel<false> (single BF16 product, every dT_x formed)")
             rec["roofline_kernel"] = dict(
                 kernel="control-variate exchange of every walker, one launch per spin: " + cv_kernel,
                 avg_launch_ms=cv_avg_s * 1e3, flops_per_launch=fl, achieved=fl / cv_avg_s / 1e12,

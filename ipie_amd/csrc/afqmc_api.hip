@@ -63,10 +63,11 @@ struct afqmc_ctx {
   // exchange as g^T Wq g (when Wq fits the budget and N < X)
   bool quad_on = false;
   bool quad_ready = false;                    // Wq holds the current static tensors
-  int quad_ks = 1;                            // K ranges of the split-BF16 Y = G Wq
   double* qw64[2] = {nullptr, nullptr};       // FP64 Wq per spin [n_s N, n_s N]
   uint16_t* qwp[2] = {nullptr, nullptr};      // split-BF16 Wq per spin 2 x [n_s Kp, n_s Kp]
-  double* qy = nullptr;                       // Y = G Wq partials [quad_ks, W, n Np] complex
+  double* qy = nullptr;                       // FP64: Y = G Wq [W, n N]; BF16: dot partials [slots, W] (complex)
+  int4* qunits[2] = {nullptr, nullptr};       // BF16 column units per spin, one table per row-block count
+  std::vector<int> qunit_off[2], qunit_n[2];  // [mb - 1]: offset and length of the table for mb row blocks
   int xs_used[2] = {1, 1};  // partial-sum count of the last exchange per spin
   int64_t exx_cap = 0;      // capacity (complex entries) of each exchange partial buffer
   double* gcomp = nullptr;  // [W, nmax, N] complex: Ghalf of sampled walkers (FP64 exchange)
@@ -183,6 +184,33 @@ hipError_t dmalloc(void** p, size_t bytes) {
   return hipMalloc(p, bytes);
 }
 
+// Column units of the block-triangular quadratic-form GEMM with mb row blocks:
+// column tile t sums the K chunks [0, min(5 (t + 1), kchunks)), cut into pieces
+// of at most T chunks, T chosen for about 1536 workgroups (at least 32 chunks);
+// heaviest first so that the light units fill the end of the launch.
+std::vector<int4> quad_units(int64_t nb, int kchunks, int64_t mb) {
+  const int per_tile = QUAD_TILE / 32;
+  int64_t total = 0;
+  for (int64_t t = 0; t < nb; ++t) total += std::min<int64_t>(per_tile * (t + 1), kchunks);
+  const int64_t target = std::max<int64_t>(ceil_div(1536, mb), 1);
+  const int64_t T = std::max<int64_t>(ceil_div(total, target), 32);
+  std::vector<int4> u;
+  for (int64_t t = 0; t < nb; ++t) {
+    const int64_t kc = std::min<int64_t>(per_tile * (t + 1), kchunks);
+    const int64_t ks = ceil_div(kc, T);
+    for (int64_t z = 0; z < ks; ++z) {
+      int4 e;
+      e.x = (int)t;
+      e.y = (int)(kc * z / ks);
+      e.z = (int)(kc * (z + 1) / ks);
+      e.w = 0;
+      u.push_back(e);
+    }
+  }
+  std::stable_sort(u.begin(), u.end(), [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
+  return u;
+}
+
 // Split-BF16 planes of Ghalf and rchol per spin (mixed precision, surrogate).
 int alloc_exchange_planes(afqmc_ctx* c) {
   const int64_t W = c->d.nwalkers, X = c->d.nchol;
@@ -233,7 +261,10 @@ int ensure_ws(afqmc_ctx* c) {
       if (mixed) ok = ok && dmalloc((void**)&c->qwp[s], 2 * (size_t)2 * rows * rows) == hipSuccess;
       else ok = ok && dmalloc((void**)&c->qw64[s], 8 * (size_t)rows * rows) == hipSuccess;
     }
-    ok = ok && dmalloc((void**)&c->qy, 16 * (size_t)c->quad_ks * W * nmax * Np) == hipSuccess;
+    // BF16: QUAD_SLOTS dot partials per column unit (at most 1536 / mb + 2 nb units)
+    const size_t ybytes = mixed ? 16 * (size_t)(1536 + 2 * ceil_div(nmax * Np, QUAD_TILE)) * QUAD_SLOTS * W
+                                : 16 * (size_t)W * nmax * Np;
+    ok = ok && dmalloc((void**)&c->qy, ybytes) == hipSuccess;
   }
   ok = ok && dmalloc((void**)&c->vhs, 16 * (size_t)W * N * N) == hipSuccess;
   for (int i = 0; i < 4; ++i) ok = ok && dmalloc((void**)&c->buf[i], 16 * (size_t)W * N * c->ncol) == hipSuccess;
@@ -248,6 +279,20 @@ int ensure_ws(afqmc_ctx* c) {
       tri[P + p] = j;
     }
   HIP_TRY(c, hipMemcpy(c->tri, tri.data(), sizeof(int) * 2 * P, hipMemcpyHostToDevice));
+  for (int s = 0; s < 2; ++s) {
+    if (!c->qwp[s]) continue;
+    const int64_t ldn = (int64_t)(s == 0 ? d.nalpha : d.nbeta) * c->kpad;
+    std::vector<int4> all;
+    for (int64_t mb = 1; mb <= ceil_div(W, 128); ++mb) {
+      std::vector<int4> u = quad_units(ceil_div(ldn, QUAD_TILE), (int)(ldn / 32), mb);
+      c->qunit_off[s].push_back((int)all.size());
+      c->qunit_n[s].push_back((int)u.size());
+      all.insert(all.end(), u.begin(), u.end());
+    }
+    if (dmalloc((void**)&c->qunits[s], sizeof(int4) * all.size()) != hipSuccess)
+      return fail(c, AFQMC_ERR_ALLOC, "workspace allocation failed (%s)", c->describe_dims);
+    HIP_TRY(c, hipMemcpy(c->qunits[s], all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice));
+  }
   HIP_TRY(c, hipMemset(c->wvec, 0, 16 * (size_t)6 * W));
   c->ws_ready = true;
   return AFQMC_OK;
@@ -569,6 +614,47 @@ int ensure_quad(afqmc_ctx* c) {
   return AFQMC_OK;
 }
 
+// Split-BF16 quadratic form of nw walkers for one spin, d^T Wq d with d = G - gref
+// (gref may be null; walker_of selects the walkers): Y = d W' over the tile pairs
+// of the block-triangular planes, the per-tile dot products in the GEMM's
+// epilogue, their sum per walker into part.  single: hi * hi only.
+int quad_bf16(afqmc_ctx* c, int nw, int spin, const double* g, const int* walker_of, const double* gref,
+              bool single, double* part) {
+  const int N = c->d.nbasis;
+  const int n = spin == 0 ? c->d.nalpha : c->d.nbeta;
+  HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, walker_of, n, gref));
+  const int64_t ldn = (int64_t)n * c->kpad;
+  const int64_t mb = ceil_div(nw, 128);
+  if (mb < 1 || mb > (int64_t)c->qunit_n[spin].size()) return fail(c, AFQMC_ERR_INVALID_ARG, "walker count");
+  const int nunits = c->qunit_n[spin][mb - 1];
+  Bf16GemmArgs b{};
+  b.a = c->gplanes[spin];
+  b.a_plane = (int64_t)nw * ldn;
+  b.lda = ldn;
+  b.m = nw;
+  b.b = c->qwp[spin];
+  b.b_plane = ldn * ldn;
+  b.ldb = ldn;
+  b.nn = ldn;
+  b.kchunks = (int)(ldn / 32);
+  b.ksplit = 1;
+  b.single = single ? 1 : 0;
+  b.epilogue = BF16_EPI_QDOT;
+  b.out = c->qy;
+  b.scale = 1.0;
+  b.units = c->qunits[spin] + c->qunit_off[spin][mb - 1];
+  b.nunits = nunits;
+  b.dg = g;
+  b.dref = gref;
+  b.walker_of = walker_of;
+  b.dstride = (int64_t)n * N;
+  b.dn = N;
+  b.dkp = c->kpad;
+  HIP_TRY(c, launch_bf16_gemm(b, c->stream));
+  HIP_TRY(c, launch_quad_sum(c->qy, nunits * QUAD_SLOTS, nw, part, c->stream));
+  return AFQMC_OK;
+}
+
 // Exchange of nw walkers for one spin as g^T Wq g: Y = G Wq (FP64 NT GEMM, or
 // split-BF16 GEMM with split K), then one dot product per walker.
 int exchange_spin_quad(afqmc_ctx* c, int nw, int spin, const double* g, const int* walker_of) {
@@ -579,25 +665,8 @@ int exchange_spin_quad(afqmc_ctx* c, int nw, int spin, const double* g, const in
   if (r) return r;
   PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
   if (c->qwp[spin]) {
-    HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, walker_of, n));
-    const int64_t ldn = (int64_t)n * c->kpad;
-    Bf16GemmArgs b{};
-    b.a = c->gplanes[spin];
-    b.a_plane = (int64_t)nw * ldn;
-    b.lda = ldn;
-    b.m = nw;
-    b.b = c->qwp[spin];
-    b.b_plane = ldn * ldn;
-    b.ldb = ldn;
-    b.nn = ldn;
-    b.kchunks = (int)(ldn / 32);
-    b.ksplit = c->quad_ks;
-    b.epilogue = BF16_EPI_PARTIAL;
-    b.out = c->qy;
-    b.z_offset = 0;
-    b.scale = 1.0;
-    HIP_TRY(c, launch_bf16_gemm(b, c->stream));
-    HIP_TRY(c, launch_quad_reduce(c->qy, c->quad_ks, nw, n, N, c->kpad, g, walker_of, part, c->stream));
+    r = quad_bf16(c, nw, spin, g, walker_of, nullptr, false, part);
+    if (r) return r;
   } else {
     if (walker_of) {
       HIP_TRY(c, launch_sur_gather(g, walker_of, nw, (int64_t)n * N, c->gcomp, c->stream));
@@ -776,25 +845,8 @@ int exchange_q_spin(afqmc_ctx* c, int nw, int spin, const double* g) {
     int r = ensure_quad(c);
     if (r) return r;
     PhaseScope ps(c, AFQMC_PHASE_EXCHANGE_CV);
-    HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, nullptr, n, gT));
-    const int64_t ldn = (int64_t)n * c->kpad;
-    Bf16GemmArgs b{};
-    b.a = c->gplanes[spin];
-    b.a_plane = (int64_t)nw * ldn;
-    b.lda = ldn;
-    b.m = nw;
-    b.b = c->qwp[spin];
-    b.b_plane = ldn * ldn;
-    b.ldb = ldn;
-    b.nn = ldn;
-    b.kchunks = (int)(ldn / 32);
-    b.ksplit = c->quad_ks;
-    b.single = 1;
-    b.epilogue = BF16_EPI_PARTIAL;
-    b.out = c->qy;
-    b.scale = 1.0;
-    HIP_TRY(c, launch_bf16_gemm(b, c->stream));
-    HIP_TRY(c, launch_quad_reduce(c->qy, c->quad_ks, nw, n, N, c->kpad, g, nullptr, part, c->stream, gT));
+    r = quad_bf16(c, nw, spin, g, nullptr, gT, true, part);
+    if (r) return r;
     c->xs_used[spin] = 1;
     return AFQMC_OK;
   }
@@ -982,11 +1034,6 @@ int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims) {
     const double per_spin = d.precision == AFQMC_PREC_MIXED ? 4.0 * std::pow((double)nmx * c->kpad, 2)
                                                             : 8.0 * std::pow((double)nmx * d.nbasis, 2);
     c->quad_on = d.nchol > d.nbasis && per_spin <= QUAD_BUDGET_BYTES;
-    if (c->quad_on && d.precision == AFQMC_PREC_MIXED) {
-      const int64_t tiles = ceil_div(d.nwalkers, 128) * ceil_div((int64_t)nmx * c->kpad, 160);
-      int64_t ks = ceil_div(2048, tiles);
-      c->quad_ks = (int)std::min<int64_t>(std::max<int64_t>(ks, 1), 4);
-    }
   }
   if (d.precision == AFQMC_PREC_MIXED) {
     c->xpad = (d.nchol + 31) / 32 * 32;
@@ -1016,7 +1063,7 @@ void afqmc_destroy(afqmc_ctx* c) {
                   c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
                   c->lplanes[1], c->xplanes, c->cplanes, c->vplanes, c->tplanes[0],
                   c->tplanes[1], c->gcomp, c->gf_o, c->gf_oinv, c->qw64[0], c->qw64[1], c->qwp[0],
-                  c->qwp[1], c->qy, c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
+                  c->qwp[1], c->qy, c->qunits[0], c->qunits[1], c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
                   c->sur.resid, c->sur.walker_of, c->sur.xbar, c->sur.e1, c->sur.master, c->sur.m,
                   c->sur.v, c->sur.wbf, c->sur.bias, c->sur.grad, c->sur.loss, c->sur.state, c->sur.gT,
                   c->sur.K, c->sur.et, c->sur.cv, c->sur.mlp, c->sur.gtmp};

@@ -106,7 +106,7 @@ hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream);
 
 // C = A B^T on split-BF16 planes: A 4 planes [m][lda] (re hi, re lo, im hi,
 // im lo), B 2 planes [nn][ldb] (hi, lo), K = kchunks * 32 (zero padded).
-enum { BF16_EPI_PARTIAL = 0, BF16_EPI_VHS = 1 };
+enum { BF16_EPI_PARTIAL = 0, BF16_EPI_VHS = 1, BF16_EPI_QDOT = 2 };
 constexpr int BF16_FB_SPLIT = 8;  // K ranges per spin of the split-BF16 force bias
 struct Bf16GemmArgs {
   const uint16_t* a;
@@ -122,14 +122,32 @@ struct Bf16GemmArgs {
   const int* tri_i;  // VHS: packed index -> (i, j)
   const int* tri_j;
   int nbasis;
+  // column units {tile, first K chunk, end K chunk, 0} in grid order (grid = row
+  // blocks x nunits, ksplit unused), or null: every tile over K splits of ksplit
+  const int4* units;
+  int nunits;
+  // QDOT: d[row][i][m] = dg[src * dstride + i dn + m] - dref[i dn + m] (dref may be null),
+  // src = walker_of[row] or row; output column col = i dkp + m (m < dn)
+  const double* dg;
+  const double* dref;
+  const int* walker_of;
+  int64_t dstride;
+  int dn, dkp;
 };
 hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream);
 
 // ---------------------------------------------------------------- exchange as g^T Wq g
 constexpr double QUAD_BUDGET_BYTES = 24e9;  // largest Wq per spin held on the device
+// Column tile of the block-triangular split-BF16 Wq (the BF16 GEMM's column tile).
+constexpr int QUAD_TILE = 160;
+// Partial dot products written per column unit by the QDOT epilogue.
+constexpr int QUAD_SLOTS = 2;
 // Wq[(i,m'),(j,m)] = sum_x L_x[i][m] L_x[j][m'] from rchol [X, n, N]: FP64 [nN][nN]
-// (Np = N) or split-BF16 planes [2][n Np][n Np] (Np = Kp, zero padding); exactly
-// one of w64 / wplanes is non-null.
+// (Np = N), or split-BF16 planes [2][n Np][n Np] (Np = Kp, zero padding) of the
+// block-triangular W' with plane row b, column a: W'_ab = 2 Wq_ab when tile(a) <
+// tile(b), Wq_ab when tile(a) = tile(b), 0 otherwise (tile(a) = a / QUAD_TILE),
+// so that g^T Wq g = sum_b g_b sum_{a < QUAD_TILE (tile(b) + 1)} g_a W'_ab.
+// Exactly one of w64 / wplanes is non-null.
 hipError_t launch_quad_w(const double* rchol, int X, int n, int N, int Np, double* w64, uint16_t* wplanes,
                          hipStream_t stream);
 // part[w] = sum_z sum_{i, m < N} d[i][m] y[z][w][i Np + m], d = g[src] - gref (gref
@@ -137,6 +155,8 @@ hipError_t launch_quad_w(const double* rchol, int X, int n, int N, int Np, doubl
 hipError_t launch_quad_reduce(const double* y, int ksplit, int nwalkers, int n, int N, int Np, const double* g,
                               const int* walker_of, double* part, hipStream_t stream,
                               const double* gref = nullptr);
+// out[w] = sum_{s < nslots} part[s][w] (complex, fixed order)
+hipError_t launch_quad_sum(const double* part, int nslots, int nwalkers, double* out, hipStream_t stream);
 
 // One split-BF16 Taylor term per walker: out_k^T = alpha out_{k-1}^T VHS,
 // q = (init or q) + out_k, and (tout) the split planes of out_k^T.

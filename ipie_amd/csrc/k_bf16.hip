@@ -42,6 +42,7 @@ constexpr int BX_R = 16 * BX_GR * BX_MR;  // rows of a workgroup tile (walker or
 constexpr int BX_C = 16 * BX_GC * BX_MC;  // columns (Cholesky-vector orbitals)
 constexpr int BX_THREADS = 64 * BX_GR * BX_GC;
 constexpr int BX_KC = 32;
+static_assert(BX_C == QUAD_TILE && BX_GC == QUAD_SLOTS, "quadratic-form tiles are the GEMM's column tiles");
 constexpr size_t BX_LDS_MAX = 106496;  // LDS of a workgroup: two stages of the 3-product form
 
 // Staging, in 16-byte vectors.  SPLIT (3 products): A planes re_hi, re_lo,
@@ -395,16 +396,32 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
 //   BF16_EPI_PARTIAL  out[z_offset + z][row][col] = scale C     (force bias)
 //   BF16_EPI_VHS      out[w][i][j] = out[w][j][i] = i scale C[w][p],
 //                     (i, j) = (tri_i[p], tri_j[p])              (VHS)
+//   BF16_EPI_QDOT     out[((cb ksplit + z) BX_GC + wc) m + row] = sum over the
+//                     wave's columns col = (i, m') (m' < N) of d[row][i][m'] C[row][col],
+//                     d = g[src(row)] - gref                     (exchange g^T Wq g)
+// With a unit table (block-triangular B), the workgroup's column unit u gives
+// (column tile, K chunk range) = (units[u].x, [units[u].y, units[u].z)) and the
+// QDOT slot is u BX_GC + wc; otherwise column tile cb and split z of ksplit.
 // Workgroups that run together share the B tile (row blocks vary fastest).
 template <bool SPLIT>
 __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
   using C = Bx<SPLIT>;
   extern __shared__ uint4 lds_u4[];
   const int mb = (int)ceil_div(a.m, BX_R), nb = (int)ceil_div(a.nn, BX_C);
+  const int ncu = a.units ? a.nunits : nb;  // column units of the grid
   const int lin = xcd_remap(blockIdx.x, gridDim.x);
-  const int rb = lin % mb, cb = (lin / mb) % nb, z = lin / mb / nb;
-  if (z >= a.ksplit) return;
-  const int c0 = (int)((int64_t)a.kchunks * z / a.ksplit), c1 = (int)((int64_t)a.kchunks * (z + 1) / a.ksplit);
+  const int rb = lin % mb, cu = (lin / mb) % ncu, z = lin / mb / ncu;
+  if (z >= (a.units ? 1 : a.ksplit)) return;
+  int cb = cu, c0, c1;
+  if (a.units) {
+    const int4 e = a.units[cu];
+    cb = e.x;
+    c0 = e.y;
+    c1 = e.z;
+  } else {
+    c0 = (int)((int64_t)a.kchunks * z / a.ksplit);
+    c1 = (int)((int64_t)a.kchunks * (z + 1) / a.ksplit);
+  }
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
   const int wr = wave % BX_GR, wc = wave / BX_GR;
@@ -418,68 +435,113 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
     if (r > a.m - 1) r = a.m - 1;
     a_src[i] = C::a_plane(p) * a.a_plane + r * a.lda + (rem % 4) * 8;
   }
+  {
 #pragma unroll
-  for (int i = 0; i < C::B_PER; ++i) {
-    const int v = (tid + i * BX_THREADS) % C::B_VEC;
-    const int p = v / (BX_C * 4), rem = v % (BX_C * 4);
-    int64_t r = (int64_t)cb * BX_C + rem / 4;
-    if (r > a.nn - 1) r = a.nn - 1;
-    b_src[i] = C::b_plane(p) * a.b_plane + r * a.ldb + (rem % 4) * 8;
-  }
-  v4f re[BX_MR][BX_MC], im[BX_MR][BX_MC];
-#pragma unroll
-  for (int i = 0; i < BX_MR; ++i)
-#pragma unroll
-    for (int j = 0; j < BX_MC; ++j) {
-      re[i][j] = zero4f();
-      im[i][j] = zero4f();
+    for (int i = 0; i < C::B_PER; ++i) {
+      const int v = (tid + i * BX_THREADS) % C::B_VEC;
+      const int p = v / (BX_C * 4), rem = v % (BX_C * 4);
+      int64_t r = (int64_t)cb * BX_C + rem / 4;
+      if (r > a.nn - 1) r = a.nn - 1;
+      b_src[i] = C::b_plane(p) * a.b_plane + r * a.ldb + (rem % 4) * 8;
     }
-  // LDS stages of up to KS sub-chunks of 32 K values over this split's range [c0, c1)
-  v4u pa[C::KS][C::A_PER], pb[C::KS][C::B_PER];
-  const int nst = (int)ceil_div(c1 - c0, C::KS);
-  if (nst > 0) {
-    const int ns0 = min(C::KS, c1 - c0);
-    bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0, ns0);
-    bx_stash<SPLIT>(lds_u4, pa, pb, tid, 0, ns0);
-    __syncthreads();
-  }
-  for (int t = 0; t < nst; ++t) {
-    const int cs = c0 + t * C::KS;
-    const int nsub = min(C::KS, c1 - cs);
-    const bool more = t + 1 < nst;
-    const int nsub_next = more ? min(C::KS, c1 - (cs + C::KS)) : 0;
-    if (more) bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, cs + C::KS, nsub_next);
-    bx_chunk<SPLIT>(lds_u4 + (t & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub);
-    if (more) bx_stash<SPLIT>(lds_u4, pa, pb, tid, (t + 1) & 1, nsub_next);
-    __syncthreads();
-  }
-  const int64_t NN = (int64_t)a.nbasis * a.nbasis;
-#pragma unroll
-  for (int j = 0; j < BX_MC; ++j) {
-    const int64_t col = (int64_t)cb * BX_C + (wc * BX_MC + j) * 16 + l16;
-    if (col >= a.nn) continue;
-    int ti = 0, tj = 0;
-    if (a.epilogue == BF16_EPI_VHS) {
-      ti = a.tri_i[col];
-      tj = a.tri_j[col];
-    }
+    v4f re[BX_MR][BX_MC], im[BX_MR][BX_MC];
 #pragma unroll
     for (int i = 0; i < BX_MR; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = (int64_t)rb * BX_R + (wr * BX_MR + i) * 16 + 4 * q + r;
-        if (row >= a.m) continue;
-        const double vr = a.scale * re[i][j][r], vi = a.scale * im[i][j][r];
-        if (a.epilogue == BF16_EPI_VHS) {
-          // i scale (re + i im) = scale (-im + i re)
-          const cplx v = cmake(-vi, vr);
-          double* base = a.out + 2 * row * NN;
-          st(base, (int64_t)ti * a.nbasis + tj, v);
-          if (ti != tj) st(base, (int64_t)tj * a.nbasis + ti, v);
-        } else {
-          st(a.out, ((int64_t)(a.z_offset + z) * a.m + row) * a.nn + col, cmake(vr, vi));
-        }
+      for (int j = 0; j < BX_MC; ++j) {
+        re[i][j] = zero4f();
+        im[i][j] = zero4f();
       }
+    // LDS stages of up to KS sub-chunks of 32 K values over this split's range [c0, c1)
+    v4u pa[C::KS][C::A_PER], pb[C::KS][C::B_PER];
+    const int nst = (int)ceil_div(c1 - c0, C::KS);
+    if (nst > 0) {
+      const int ns0 = min(C::KS, c1 - c0);
+      bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0, ns0);
+      bx_stash<SPLIT>(lds_u4, pa, pb, tid, 0, ns0);
+      __syncthreads();
+    }
+    for (int t = 0; t < nst; ++t) {
+      const int cs = c0 + t * C::KS;
+      const int nsub = min(C::KS, c1 - cs);
+      const bool more = t + 1 < nst;
+      const int nsub_next = more ? min(C::KS, c1 - (cs + C::KS)) : 0;
+      if (more) bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, cs + C::KS, nsub_next);
+      bx_chunk<SPLIT>(lds_u4 + (t & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub);
+      if (more) bx_stash<SPLIT>(lds_u4, pa, pb, tid, (t + 1) & 1, nsub_next);
+      __syncthreads();
+    }
+    if (a.epilogue == BF16_EPI_QDOT) {
+      // per-row dot with d over this wave's 80 columns, then across the 16 lanes
+      // of a row; lane l16 = 4 i + r keeps the sum of row (i, r)
+      double myr = 0.0, myi = 0.0;
+#pragma unroll
+      for (int i = 0; i < BX_MR; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int64_t row = (int64_t)rb * BX_R + (wr * BX_MR + i) * 16 + 4 * q + r;
+          if (row > a.m - 1) row = a.m - 1;  // clamped rows are computed, never stored
+          const double* drow = a.dg + 2 * (a.walker_of ? (int64_t)a.walker_of[row] : row) * a.dstride;
+          double er = 0.0, ei = 0.0;
+#pragma unroll
+          for (int j = 0; j < BX_MC; ++j) {
+            const int64_t col = (int64_t)cb * BX_C + (wc * BX_MC + j) * 16 + l16;
+            const int io = (int)(col / a.dkp), mm = (int)(col % a.dkp);
+            if (col < a.nn && mm < a.dn) {
+              const int64_t off = (int64_t)io * a.dn + mm;
+              cplx d = ld(drow, off);
+              if (a.dref) d = csub(d, ld(a.dref, off));
+              const double yr = re[i][j][r], yi = im[i][j][r];
+              er += d.re * yr - d.im * yi;
+              ei += d.re * yi + d.im * yr;
+            }
+          }
+#pragma unroll
+          for (int o = 8; o >= 1; o >>= 1) {
+            er += __shfl_xor(er, o, 16);
+            ei += __shfl_xor(ei, o, 16);
+          }
+          if (l16 == 4 * i + r) {
+            myr = er;
+            myi = ei;
+          }
+        }
+      if (l16 < 4 * BX_MR) {
+        const int i = l16 >> 2, r = l16 & 3;
+        const int64_t row = (int64_t)rb * BX_R + (wr * BX_MR + i) * 16 + 4 * q + r;
+        const int64_t slot = (a.units ? (int64_t)cu : (int64_t)cb * a.ksplit + z) * BX_GC + wc;
+        if (row < a.m) st(a.out, slot * a.m + row, cmake(a.scale * myr, a.scale * myi));
+      }
+    } else {
+      const int64_t NN = (int64_t)a.nbasis * a.nbasis;
+#pragma unroll
+      for (int j = 0; j < BX_MC; ++j) {
+        const int64_t col = (int64_t)cb * BX_C + (wc * BX_MC + j) * 16 + l16;
+        if (col >= a.nn) continue;
+        int ti = 0, tj = 0;
+        if (a.epilogue == BF16_EPI_VHS) {
+          ti = a.tri_i[col];
+          tj = a.tri_j[col];
+        }
+#pragma unroll
+        for (int i = 0; i < BX_MR; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int64_t row = (int64_t)rb * BX_R + (wr * BX_MR + i) * 16 + 4 * q + r;
+            if (row >= a.m) continue;
+            const double vr = a.scale * re[i][j][r], vi = a.scale * im[i][j][r];
+            if (a.epilogue == BF16_EPI_VHS) {
+              // i scale (re + i im) = scale (-im + i re)
+              const cplx v = cmake(-vi, vr);
+              double* base = a.out + 2 * row * NN;
+              st(base, (int64_t)ti * a.nbasis + tj, v);
+              if (ti != tj) st(base, (int64_t)tj * a.nbasis + ti, v);
+            } else {
+              st(a.out, ((int64_t)(a.z_offset + z) * a.m + row) * a.nn + col, cmake(vr, vi));
+            }
+          }
+      }
+    }
   }
 }
 
@@ -696,9 +758,13 @@ hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream) {
       (int64_t)a.kchunks * BX_KC > a.lda || (int64_t)a.kchunks * BX_KC > a.ldb)
     return hipErrorInvalidValue;
   if (a.epilogue == BF16_EPI_VHS && (int64_t)a.nbasis * (a.nbasis + 1) / 2 != a.nn) return hipErrorInvalidValue;
-  const int64_t grid = ceil_div(a.m, BX_R) * ceil_div(a.nn, BX_C) * a.ksplit;
+  if (a.epilogue == BF16_EPI_QDOT && (a.dg == nullptr || a.dkp < a.dn || a.dn < 1 || a.dstride < 1))
+    return hipErrorInvalidValue;
+  if (a.units && a.nunits < 1) return hipErrorInvalidValue;
+  const int64_t nb = ceil_div(a.nn, BX_C);
+  const int64_t grid = ceil_div(a.m, BX_R) * (a.units ? (int64_t)a.nunits : nb * a.ksplit);
   if (a.single) {
-    if (a.epilogue != BF16_EPI_PARTIAL) return hipErrorInvalidValue;
+    if (a.epilogue == BF16_EPI_VHS) return hipErrorInvalidValue;
     hipError_t e = allow_lds((const void*)bf16_gemm_kernel<false>, Bx<false>::STAGES);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(bf16_gemm_kernel<false>, dim3((unsigned)grid), dim3(BX_THREADS), Bx<false>::STAGES, stream, a);

@@ -9,9 +9,14 @@
 // is one GEMM Y = G Wq (M = W, N = K = n N) and a dot product per walker, about
 // X / N times fewer flops than forming every T_x (and no n x n trace epilogue).
 // Same value as ipie/estimators/local_energy_sd.py:263-303 (exx_kernel_batch_real_rchol).
+// Wq's symmetry halves the BF16 form: with column tiles of QUAD_TILE, the planes
+// hold W' = Wq on the diagonal tiles, 2 Wq above them and 0 below, and column
+// tile b of Y = G W' sums K over the tiles up to b only (g^T W' g = g^T Wq g).
 //   quad_w_kernel       Wq from rchol, FP64 accumulation, written as FP64 [nN][nN]
-//                       or as split-BF16 planes [2][n Kp][n Kp] (zero padding)
-//   quad_reduce_kernel  E_w = sum_z sum_a g_w[a] Y_z[w][a]
+//                       or as the split-BF16 planes of W' [2][n Kp][n Kp] (zero padding)
+//   quad_reduce_kernel  E_w = sum_z sum_a g_w[a] Y_z[w][a]  (FP64 form)
+//   quad_sum_kernel     E_w = sum of the per-tile dot products written by the
+//                       BF16 GEMM's epilogue
 #include "afqmc_internal.hpp"
 
 namespace afqmc {
@@ -87,7 +92,10 @@ __global__ void __launch_bounds__(256) quad_w_kernel(const double* __restrict__ 
       if (w64) {
         w64[a * ldo + b] = acc[r][c];
       } else {
-        const double v = (mp < N && m < N) ? acc[r][c] : 0.0;
+        // plane row a is the output column, b the K index: W'_{b a}
+        const int ta = (int)(a / QUAD_TILE), tb = (int)(b / QUAD_TILE);
+        const double f = tb < ta ? 2.0 : (tb == ta ? 1.0 : 0.0);
+        const double v = (mp < N && m < N) ? f * acc[r][c] : 0.0;
         const uint16_t hi = qf2bf((float)v);
         const uint16_t lo = qf2bf((float)(v - (double)qbf2f(hi)));
         wplanes[a * ldo + b] = hi;
@@ -127,7 +135,42 @@ __global__ void __launch_bounds__(256) quad_reduce_kernel(const double* __restri
   if (threadIdx.x == 0) st(part, w, cmake(re, im));
 }
 
+// out[w] = sum_s part[s][w]: 16 walkers x 16 slot lanes per workgroup, the
+// slot lanes combined in a fixed order.
+__global__ void __launch_bounds__(256) quad_sum_kernel(const double* __restrict__ part, int nslots, int nwalkers,
+                                                      double* __restrict__ out) {
+  __shared__ double sr[16][17], si[16][17];
+  const int wl = threadIdx.x & 15, sl = threadIdx.x >> 4;
+  const int w = blockIdx.x * 16 + wl;
+  double re = 0.0, im = 0.0;
+  if (w < nwalkers) {
+    for (int s = sl; s < nslots; s += 16) {
+      const cplx v = ld(part, (int64_t)s * nwalkers + w);
+      re += v.re;
+      im += v.im;
+    }
+  }
+  sr[sl][wl] = re;
+  si[sl][wl] = im;
+  __syncthreads();
+  if (sl == 0 && w < nwalkers) {
+    double tr = 0.0, ti = 0.0;
+    for (int k = 0; k < 16; ++k) {
+      tr += sr[k][wl];
+      ti += si[k][wl];
+    }
+    st(out, w, cmake(tr, ti));
+  }
+}
+
 }  // namespace
+
+hipError_t launch_quad_sum(const double* part, int nslots, int nwalkers, double* out, hipStream_t stream) {
+  if (nslots < 1 || nwalkers < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(quad_sum_kernel, dim3((unsigned)ceil_div(nwalkers, 16)), dim3(256), 0, stream, part, nslots,
+                     nwalkers, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_quad_w(const double* rchol, int X, int n, int N, int Np, double* w64, uint16_t* wplanes,
                          hipStream_t stream) {

@@ -26,6 +26,8 @@ CASES = {
     "n96_n24_n12": (96, 200, 60.0, 24, 12, 5, 0.1),
     "n24_fallback": (24, 72, 12.0, 5, 5, 10, (12.0 / (24 * 12.0)) ** 0.5),
     "n48_x40": (48, 40, 20.0, 8, 6, 5, 0.1),  # X < N: exchange through every T_x
+    # three row blocks of walkers in the exchange GEMM, 4 and 3 (partial) column tiles
+    "n64_w300": (64, 96, 40.0, 10, 7, 300, 0.05),
 }
 
 EXX_TOL = 1e-4
