@@ -57,8 +57,11 @@ struct afqmc_ctx {
   uint16_t* lplanes[2] = {nullptr, nullptr};  // per spin: 2 x [X * n_s, kpad] (half-rotated Cholesky)
   uint16_t* xplanes = nullptr;                // 4 x [W, xpad] (shifted fields)
   uint16_t* cplanes = nullptr;                // 2 x [P, xpad] (packed Cholesky)
-  uint16_t* vplanes = nullptr;                // 4 x [W N, kpad] (VHS, Taylor series)
+  uint16_t* vplanes = nullptr;                // VHS split planes in Taylor tiles (W x 80-row blocks x kpad x 4)
   uint16_t* tplanes[2] = {nullptr, nullptr};  // 4 x [W, rpad, kpad] each (Taylor terms, transposed)
+  uint16_t* dplanes = nullptr;                // split-BF16 expH1_s - I per spin, 2 x [N, kpad] each
+  uint16_t* oplanes = nullptr;                // 4 x [W n_s, kpad]: transposed planes of phi_s (one body)
+  bool ptp_ready = false;                     // tplanes[0] holds the planes of P^T from the one-body GEMM
   bool planes_ready = false;                  // lplanes / cplanes hold the current static tensors
   // exchange as g^T Wq g (when Wq fits the budget and N < X)
   bool quad_on = false;
@@ -247,11 +250,15 @@ int ensure_ws(afqmc_ctx* c) {
     ok = ok && dmalloc((void**)&c->xplanes, 2 * (size_t)4 * W * c->xpad) == hipSuccess;
     ok = ok && dmalloc((void**)&c->cplanes, 2 * (size_t)2 * P * c->xpad) == hipSuccess;
     const size_t tbytes = 2 * (size_t)4 * W * taylor_rows_padded(c->ncol) * c->kpad;
-    ok = ok && dmalloc((void**)&c->vplanes, 2 * (size_t)4 * W * N * c->kpad) == hipSuccess;
+    ok = ok && dmalloc((void**)&c->vplanes, 2 * (size_t)4 * W * vhs_tiled_rows(N) * c->kpad) == hipSuccess;
     for (int i = 0; i < 2; ++i) {
       ok = ok && dmalloc((void**)&c->tplanes[i], tbytes) == hipSuccess;
       ok = ok && hipMemset(c->tplanes[i], 0, tbytes) == hipSuccess;  // padding stays zero
     }
+    ok = ok && dmalloc((void**)&c->dplanes, 2 * (size_t)2 * 2 * N * c->kpad) == hipSuccess;
+    const size_t obytes = 2 * (size_t)4 * W * nmax * c->kpad;
+    ok = ok && dmalloc((void**)&c->oplanes, obytes) == hipSuccess;
+    ok = ok && hipMemset(c->oplanes, 0, obytes) == hipSuccess;  // K padding stays zero
   }
   if (c->quad_on) {
     const bool mixed = d.precision == AFQMC_PREC_MIXED;
@@ -322,6 +329,12 @@ int ensure_planes(afqmc_ctx* c) {
   if (c->cplanes) {
     const int64_t P = (int64_t)N * (N + 1) / 2;
     HIP_TRY(c, launch_split_real(c->s.chol_packed, P, c->d.nchol, c->xpad, c->cplanes, c->stream));
+  }
+  if (c->dplanes) {
+    HIP_TRY(c, launch_split_eye(c->s.expH1, N, c->kpad, c->dplanes, c->stream));
+    if (two_spins(c))
+      HIP_TRY(c, launch_split_eye(c->s.expH1 + (int64_t)N * N, N, c->kpad, c->dplanes + 2 * (int64_t)N * c->kpad,
+                                  c->stream));
   }
   c->planes_ready = true;
   return AFQMC_OK;
@@ -458,6 +471,7 @@ int do_vhs(afqmc_ctx* c, int nw) {
     b.ksplit = 1;
     b.epilogue = BF16_EPI_VHS;
     b.out = c->vhs;
+    b.out32 = reinterpret_cast<float*>(c->vhs);  // complex FP32 in the same buffer (only the Taylor planes read it)
     b.scale = sqrt(c->s.dt);
     b.tri_i = c->tri;
     b.tri_j = c->tri + P;
@@ -505,12 +519,75 @@ int nn(afqmc_ctx* c, int nw, const double* A, bool acplx, int64_t a_stride, cons
   return AFQMC_OK;
 }
 
-// phi_s -> P (one body applied when with_expH1, plain copy otherwise)
+// Split-BF16 one body of spin s for nw walkers: out = src + D_s src, D_s =
+// expH1_s - I, one GEMM with walker-column rows r = w n_s + c and the basis
+// index as K.  src / out are addressed as [w][i][off + c] (walker stride, row
+// length, column offset); aplanes are the transposed split planes of src
+// (plane stride a_plane, A row w a_gstride + a_goff + c when a_gstride > 0,
+// else r); tp, when set, receives the transposed planes of out (Taylor layout).
+int one_body_bf16(afqmc_ctx* c, int nw, int s, const uint16_t* aplanes, int64_t a_plane, int a_gstride, int a_goff,
+                  const double* src, int64_t s_ws, int s_ld, int s_off, double* out, int64_t o_ws, int o_ld,
+                  int o_off, uint16_t* tp) {
+  const int N = c->d.nbasis, n = s == 0 ? c->d.nalpha : c->d.nbeta;
+  const int rpad = taylor_rows_padded(c->ncol);
+  Bf16GemmArgs b{};
+  b.a = aplanes;
+  b.a_plane = a_plane;
+  b.lda = c->kpad;
+  b.m = (int64_t)nw * n;
+  b.b = c->dplanes + s * 2 * (int64_t)N * c->kpad;
+  b.b_plane = (int64_t)N * c->kpad;
+  b.ldb = c->kpad;
+  b.nn = N;
+  b.kchunks = c->kpad / 32;
+  b.ksplit = 1;
+  b.epilogue = BF16_EPI_ADD;
+  b.scale = 1.0;
+  b.group = n;
+  b.a_gstride = a_gstride;
+  b.a_goff = a_goff;
+  b.base = src;
+  b.b_ws = s_ws;
+  b.b_ld = s_ld;
+  b.b_off = s_off;
+  b.out = out;
+  b.o_ws = o_ws;
+  b.o_ld = o_ld;
+  b.o_off = o_off;
+  if (tp) {
+    b.tp = tp;
+    b.tp_plane = (int64_t)c->d.nwalkers * rpad * c->kpad;
+    b.tp_rows = rpad;
+    b.tp_off = o_off;
+    b.tp_ld = c->kpad;
+  }
+  HIP_TRY(c, launch_bf16_gemm(b, c->stream));
+  return AFQMC_OK;
+}
+
+// phi_s -> P (one body applied when with_expH1, plain copy otherwise).  In mixed
+// precision the one body is P = phi + (expH1 - I) phi with the product in
+// split-BF16, and the transposed planes of P go to tplanes[0] for the Taylor series.
 int gather(afqmc_ctx* c, int nw, const double* phia, const double* phib, bool with_expH1) {
   PhaseScope ps(c, AFQMC_PHASE_ONE_BODY);
   const int N = c->d.nbasis, na = c->d.nalpha, nb = c->d.nbeta, nc = c->ncol;
   const int64_t NN = (int64_t)N * N;
   double* P = c->buf[0];
+  c->ptp_ready = false;
+  if (with_expH1 && c->dplanes) {
+    int r = ensure_planes(c);
+    if (r) return r;
+    for (int s = 0; s < (two_spins(c) ? 2 : 1); ++s) {
+      const int n = s == 0 ? na : nb;
+      const double* phi = s == 0 ? phia : phib;
+      HIP_TRY(c, launch_split_transpose(phi, nw, nw, N, n, n, c->kpad, c->oplanes, c->stream));
+      r = one_body_bf16(c, nw, s, c->oplanes, (int64_t)nw * n * c->kpad, 0, 0, phi, (int64_t)N * n, n, 0, P,
+                        (int64_t)N * nc, nc, s == 0 ? 0 : na, c->tplanes[0]);
+      if (r) return r;
+    }
+    c->ptp_ready = true;
+    return AFQMC_OK;
+  }
   if (with_expH1) {
     int r = nn(c, nw, c->s.expH1, false, 0, phia, (int64_t)N * na, na, P, (int64_t)N * nc, nc, na, 1.0, nullptr,
                nullptr);
@@ -528,11 +605,29 @@ int gather(afqmc_ctx* c, int nw, const double* phia, const double* phib, bool wi
   return AFQMC_OK;
 }
 
-// src (concatenated) -> phi_s
-int scatter(afqmc_ctx* c, int nw, const double* src, double* phia, double* phib, bool with_expH1) {
+// src (concatenated) -> phi_s; srcT: transposed planes of src in the Taylor
+// layout (mixed precision with the one body), or null.
+int scatter(afqmc_ctx* c, int nw, const double* src, double* phia, double* phib, bool with_expH1,
+            const uint16_t* srcT = nullptr) {
   PhaseScope ps(c, AFQMC_PHASE_ONE_BODY);
   const int N = c->d.nbasis, na = c->d.nalpha, nb = c->d.nbeta, nc = c->ncol;
   const int64_t NN = (int64_t)N * N;
+  if (with_expH1 && c->dplanes) {
+    int r = ensure_planes(c);
+    if (r) return r;
+    const int rpad = taylor_rows_padded(nc);
+    if (!srcT) {
+      HIP_TRY(c, launch_split_transpose(src, nw, (int)c->d.nwalkers, N, nc, rpad, c->kpad, c->tplanes[0], c->stream));
+      srcT = c->tplanes[0];
+    }
+    for (int s = 0; s < (two_spins(c) ? 2 : 1); ++s) {
+      const int n = s == 0 ? na : nb, off = s == 0 ? 0 : na;
+      r = one_body_bf16(c, nw, s, srcT, (int64_t)c->d.nwalkers * rpad * c->kpad, rpad, off, src, (int64_t)N * nc,
+                        nc, off, s == 0 ? phia : phib, (int64_t)N * n, n, 0, nullptr);
+      if (r) return r;
+    }
+    return AFQMC_OK;
+  }
   if (with_expH1) {
     int r = nn(c, nw, c->s.expH1, false, 0, src, (int64_t)N * nc, nc, phia, (int64_t)N * na, na, na, 1.0, nullptr,
                nullptr);
@@ -550,14 +645,17 @@ int scatter(afqmc_ctx* c, int nw, const double* src, double* phia, double* phib,
   return AFQMC_OK;
 }
 
-// Taylor series of exp(VHS) applied to P; returns the buffer holding the result.
+// Taylor series of exp(VHS) applied to P; returns the buffer holding the result
+// and, in mixed precision, the transposed split planes of it (else null).
 // Split-BF16 Taylor series: Q = P + sum_k out_k with out_k^T = out_{k-1}^T VHS / k.
-int taylor_bf16(afqmc_ctx* c, int nw, const double** result) {
+int taylor_bf16(afqmc_ctx* c, int nw, const double** result, const uint16_t** resultT) {
   const int N = c->d.nbasis, nc = c->ncol, rpad = taylor_rows_padded(nc);
   double* P = c->buf[0];
   double* Q = c->buf[3];
-  HIP_TRY(c, launch_split_ghalf(c->vhs, (int64_t)nw * N, N, c->kpad, c->vplanes, c->stream));
-  HIP_TRY(c, launch_split_transpose(P, nw, (int)c->d.nwalkers, N, nc, rpad, c->kpad, c->tplanes[0], c->stream));
+  HIP_TRY(c, launch_split_vhs_tiled(reinterpret_cast<const float*>(c->vhs), nw, N, c->kpad, c->vplanes, c->stream));
+  if (!c->ptp_ready)
+    HIP_TRY(c, launch_split_transpose(P, nw, (int)c->d.nwalkers, N, nc, rpad, c->kpad, c->tplanes[0], c->stream));
+  c->ptp_ready = false;
   for (int k = 1; k <= c->d.exp_nmax; ++k) {
     TaylorBf16Args t{};
     t.nwalkers = nw;
@@ -569,18 +667,25 @@ int taylor_bf16(afqmc_ctx* c, int nw, const double** result) {
     t.tin = c->tplanes[(k - 1) & 1];
     t.tout = k < c->d.exp_nmax ? c->tplanes[k & 1] : nullptr;
     t.vplanes = c->vplanes;
-    t.init = k == 1 ? P : nullptr;
+    t.init = P;
     t.q = Q;
+    t.dacc = reinterpret_cast<float*>(c->buf[1]);  // unused by the split-BF16 path otherwise
+    t.first = k == 1;
+    t.last = k == c->d.exp_nmax;
+    t.split = k <= TAYLOR_SPLIT_TERMS;
     t.alpha = 1.0 / k;
+    t.qout = k == c->d.exp_nmax ? c->tplanes[k & 1] : nullptr;
     HIP_TRY(c, launch_taylor_bf16(t, c->stream));
   }
   *result = Q;
+  if (resultT) *resultT = c->tplanes[c->d.exp_nmax & 1];
   return AFQMC_OK;
 }
 
-int taylor(afqmc_ctx* c, int nw, const double** result) {
+int taylor(afqmc_ctx* c, int nw, const double** result, const uint16_t** resultT = nullptr) {
   PhaseScope ps(c, AFQMC_PHASE_TAYLOR);
-  if (c->vplanes && c->d.exp_nmax > 0) return taylor_bf16(c, nw, result);
+  if (resultT) *resultT = nullptr;
+  if (c->vplanes && c->d.exp_nmax > 0) return taylor_bf16(c, nw, result, resultT);
   const int N = c->d.nbasis, nc = c->ncol;
   const int64_t NN = (int64_t)N * N, stride = (int64_t)N * nc;
   double* P = c->buf[0];
@@ -1061,7 +1166,7 @@ void afqmc_destroy(afqmc_ctx* c) {
   else (void)hipDeviceSynchronize();
   void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
                   c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
-                  c->lplanes[1], c->xplanes, c->cplanes, c->vplanes, c->tplanes[0],
+                  c->lplanes[1], c->xplanes, c->cplanes, c->vplanes, c->tplanes[0], c->dplanes, c->oplanes,
                   c->tplanes[1], c->gcomp, c->gf_o, c->gf_oinv, c->qw64[0], c->qw64[1], c->qwp[0],
                   c->qwp[1], c->qy, c->qunits[0], c->qunits[1], c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
                   c->sur.resid, c->sur.walker_of, c->sur.xbar, c->sur.e1, c->sur.master, c->sur.m,
@@ -1230,9 +1335,10 @@ int afqmc_propagate(afqmc_ctx* c, int32_t nw, double* phia, double* phib, double
                        wv(c, WV_OVLP_OLD));
   if (r) return r;
   const double* res = nullptr;
-  r = taylor(c, nw, &res);
+  const uint16_t* resT = nullptr;
+  r = taylor(c, nw, &res, &resT);
   if (r) return r;
-  r = scatter(c, nw, res, phia, phib, true);
+  r = scatter(c, nw, res, phia, phib, true, resT);
   if (r) return r;
   r = do_gf(c, nw, phia, phib, nullptr, nullptr, wv(c, WV_OVLP_NEW), log_shift, false);
   if (r) return r;

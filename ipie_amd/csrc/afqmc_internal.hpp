@@ -106,7 +106,7 @@ hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream);
 
 // C = A B^T on split-BF16 planes: A 4 planes [m][lda] (re hi, re lo, im hi,
 // im lo), B 2 planes [nn][ldb] (hi, lo), K = kchunks * 32 (zero padded).
-enum { BF16_EPI_PARTIAL = 0, BF16_EPI_VHS = 1, BF16_EPI_QDOT = 2 };
+enum { BF16_EPI_PARTIAL = 0, BF16_EPI_VHS = 1, BF16_EPI_QDOT = 2, BF16_EPI_ADD = 3 };
 constexpr int BF16_FB_SPLIT = 8;  // K ranges per spin of the split-BF16 force bias
 struct Bf16GemmArgs {
   const uint16_t* a;
@@ -116,7 +116,7 @@ struct Bf16GemmArgs {
   int kchunks, ksplit;
   int single;        // 1: hi * hi only (A planes 0 and 2, B plane 0), partial epilogue
   int epilogue;
-  double* out;       // PARTIAL: [z_offset + ksplit][m][nn] complex; VHS: [m][N][N] complex
+  double* out;       // PARTIAL: [z_offset + ksplit][m][nn] complex; VHS: [m][N][N] complex (or out32)
   int z_offset;
   double scale;      // PARTIAL: out = scale C; VHS: out = i scale C
   const int* tri_i;  // VHS: packed index -> (i, j)
@@ -133,6 +133,19 @@ struct Bf16GemmArgs {
   const int* walker_of;
   int64_t dstride;
   int dn, dkp;
+  // Rows as (walker, column): r = w group + c.  With a_gstride > 0 the A row
+  // read for r is w a_gstride + a_goff + c (otherwise r).
+  int group, a_gstride, a_goff;
+  // ADD: out[w o_ws + col o_ld + o_off + c] = base[w b_ws + col b_ld + b_off + c] + scale C[r][col]
+  // (complex), and when tp is set the split planes of that sum,
+  // tp[p tp_plane + (w tp_rows + tp_off + c) tp_ld + col], p = re_hi, re_lo, im_hi, im_lo
+  const double* base;
+  int64_t b_ws, o_ws;
+  int b_ld, b_off, o_ld, o_off;
+  uint16_t* tp;
+  int64_t tp_plane;
+  int tp_rows, tp_off, tp_ld;
+  float* out32;      // VHS: when set, the output as complex FP32 [m][N][N] (the accumulators' precision)
 };
 hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream);
 
@@ -164,14 +177,30 @@ struct TaylorBf16Args {
   int nwalkers, wcap, nbasis, ncol, rpad, kpad;
   const uint16_t* tin;      // [4][wcap][rpad][kpad] planes of out_{k-1}^T
   uint16_t* tout;           // same layout for out_k^T, or null (last term)
-  const uint16_t* vplanes;  // [4][W N][kpad] planes of VHS
-  const double* init;       // [W][N][ncol] complex (first term) or null
-  double* q;                // [W][N][ncol] complex accumulator
+  const uint16_t* vplanes;  // tiled split planes of VHS (launch_split_vhs_tiled)
+  const double* init;       // [W][N][ncol] complex: the series' input (read by the last term)
+  double* q;                // [W][N][ncol] complex: init + sum of the terms (written by the last term)
+  float* dacc;              // [W][N][ncol] complex FP32: sum of the terms so far
+  int first, last;          // this term is the first / the last of the series
+  int split;                // 1: three BF16 products (hi hi + hi lo + lo hi), 0: hi hi only
   double alpha;
+  uint16_t* qout;           // last term: planes of q^T (layout of tin), or null
 };
 hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream);
+// Taylor terms k <= TAYLOR_SPLIT_TERMS use three BF16 products, later ones one.
+// Term k is about |VHS|^k / k! of phi (|VHS|_2 ~ 0.35 at dt = 0.005 for the
+// normalised N = 400 Hamiltonian: terms 1e-1, 2e-2, 1e-3, 1e-4, 6e-6, 4e-7), so
+// one product's 2^-8 relative error on terms 4.. stays below the three-product
+// error (~1e-5 relative) of term 1.
+constexpr int TAYLOR_SPLIT_TERMS = 3;
 hipError_t launch_split_transpose(const double* phi, int nwalkers, int wcap, int N, int ncol, int rpad, int Kp,
                                   uint16_t* planes, hipStream_t stream);
+// VHS [W][N][N] complex FP32 -> the Taylor kernel's tiled split planes (W x
+// vhs_tiled_rows(N) x Kp x 4 bf16 values, zero padding)
+hipError_t launch_split_vhs_tiled(const float* x, int nwalkers, int N, int Kp, uint16_t* planes, hipStream_t stream);
+int64_t vhs_tiled_rows(int N);
+// D = M - I (M real [N][N]) -> 2 bf16 planes [N][Kp] (zero padding)
+hipError_t launch_split_eye(const double* m, int N, int Kp, uint16_t* planes, hipStream_t stream);
 int taylor_rows_padded(int ncol);
 
 // ---------------------------------------------------------------- elementwise

@@ -120,6 +120,19 @@ __global__ void split_ghalf_kernel(const double* __restrict__ g, int64_t rows, i
   planes[3 * plane + idx] = il;
 }
 
+
+// D = m - I (m real [N][N]) -> 2 bf16 planes [N][Kp].
+__global__ void split_eye_kernel(const double* __restrict__ m, int N, int Kp, uint16_t* __restrict__ planes) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)N * Kp) return;
+  const int64_t r = idx / Kp;
+  const int k = (int)(idx % Kp);
+  uint16_t h = 0, l = 0;
+  if (k < N) split_bf16(m[r * N + k] - (r == k ? 1.0 : 0.0), h, l);
+  planes[idx] = h;
+  planes[(int64_t)N * Kp + idx] = l;
+}
+
 // rchol [rows, N] real -> 2 bf16 planes [rows, Kp].
 __global__ void split_real_kernel(const double* __restrict__ x, int64_t rows, int N, int Kp,
                                   uint16_t* __restrict__ planes) {
@@ -190,8 +203,14 @@ __device__ __forceinline__ void bx_stash(uint4* lds_u4, const v4u (&pa)[Bx<SPLIT
 
 // One K chunk from LDS stage `stage`: the wave's BX_MR x BX_MC tiles,
 // re += Re(A) B and im += Im(A) B, each as hi*hi + hi*lo + lo*hi (SPLIT) or
-// hi*hi.
-template <bool SPLIT>
+// hi*hi.  TR swaps the MFMA operands, so the accumulators hold the transposed
+// tiles: element r of tile (i, j) is C[i 16 + l16][j 16 + 4 q + r] instead of
+// C[i 16 + 4 q + r][j 16 + l16].
+__device__ __forceinline__ v4f mfma_ab(v8bf a, v8bf b, v4f c, bool tr) {
+  return tr ? mfma_bf16(b, a, c) : mfma_bf16(a, b, c);
+}
+
+template <bool SPLIT, bool TR = false>
 __device__ __forceinline__ void bx_sub(const uint4* sub, int wr, int wc, int l16, int q,
                                        v4f (&re)[BX_MR][BX_MC], v4f (&im)[BX_MR][BX_MC]) {
   const uint16_t* As = reinterpret_cast<const uint16_t*>(sub);
@@ -213,33 +232,33 @@ __device__ __forceinline__ void bx_sub(const uint4* sub, int wr, int wc, int l16
       const v8bf ail = lds_frag(As, 3 * BX_R + row, q);
 #pragma unroll
       for (int j = 0; j < BX_MC; ++j) {
-        re[i][j] = mfma_bf16(arh, bh[j], re[i][j]);
-        re[i][j] = mfma_bf16(arh, bl[j], re[i][j]);
-        re[i][j] = mfma_bf16(arl, bh[j], re[i][j]);
-        im[i][j] = mfma_bf16(aih, bh[j], im[i][j]);
-        im[i][j] = mfma_bf16(aih, bl[j], im[i][j]);
-        im[i][j] = mfma_bf16(ail, bh[j], im[i][j]);
+        re[i][j] = mfma_ab(arh, bh[j], re[i][j], TR);
+        re[i][j] = mfma_ab(arh, bl[j], re[i][j], TR);
+        re[i][j] = mfma_ab(arl, bh[j], re[i][j], TR);
+        im[i][j] = mfma_ab(aih, bh[j], im[i][j], TR);
+        im[i][j] = mfma_ab(aih, bl[j], im[i][j], TR);
+        im[i][j] = mfma_ab(ail, bh[j], im[i][j], TR);
       }
     } else {
       const v8bf arh = lds_frag(As, row, q);
       const v8bf aih = lds_frag(As, BX_R + row, q);
 #pragma unroll
       for (int j = 0; j < BX_MC; ++j) {
-        re[i][j] = mfma_bf16(arh, bh[j], re[i][j]);
-        im[i][j] = mfma_bf16(aih, bh[j], im[i][j]);
+        re[i][j] = mfma_ab(arh, bh[j], re[i][j], TR);
+        im[i][j] = mfma_ab(aih, bh[j], im[i][j], TR);
       }
     }
   }
 }
 
 // The nsub sub-chunks of LDS stage `stage`.
-template <bool SPLIT>
+template <bool SPLIT, bool TR = false>
 __device__ __forceinline__ void bx_chunk(const uint4* stage, int wr, int wc, int l16, int q,
                                          v4f (&re)[BX_MR][BX_MC], v4f (&im)[BX_MR][BX_MC], int nsub) {
 #pragma unroll
   for (int sc = 0; sc < Bx<SPLIT>::KS; ++sc) {
     if (sc >= nsub) break;
-    bx_sub<SPLIT>(stage + sc * Bx<SPLIT>::SUB_VEC, wr, wc, l16, q, re, im);
+    bx_sub<SPLIT, TR>(stage + sc * Bx<SPLIT>::SUB_VEC, wr, wc, l16, q, re, im);
   }
 }
 
@@ -396,14 +415,17 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
 //   BF16_EPI_PARTIAL  out[z_offset + z][row][col] = scale C     (force bias)
 //   BF16_EPI_VHS      out[w][i][j] = out[w][j][i] = i scale C[w][p],
 //                     (i, j) = (tri_i[p], tri_j[p])              (VHS)
+//   BF16_EPI_ADD      out[w][col][c] = base[w][col][c] + scale C[r][col], r = w group + c,
+//                     optionally with the split planes of the sum, transposed (one body)
 //   BF16_EPI_QDOT     out[((cb ksplit + z) BX_GC + wc) m + row] = sum over the
 //                     wave's columns col = (i, m') (m' < N) of d[row][i][m'] C[row][col],
 //                     d = g[src(row)] - gref                     (exchange g^T Wq g)
 // With a unit table (block-triangular B), the workgroup's column unit u gives
 // (column tile, K chunk range) = (units[u].x, [units[u].y, units[u].z)) and the
 // QDOT slot is u BX_GC + wc; otherwise column tile cb and split z of ksplit.
+// TR (transposed accumulators, see bx_sub) is used with the ADD epilogue only.
 // Workgroups that run together share the B tile (row blocks vary fastest).
-template <bool SPLIT>
+template <bool SPLIT, bool TR>
 __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
   using C = Bx<SPLIT>;
   extern __shared__ uint4 lds_u4[];
@@ -433,6 +455,7 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
     const int p = v / (BX_R * 4), rem = v % (BX_R * 4);
     int64_t r = (int64_t)rb * BX_R + rem / 4;
     if (r > a.m - 1) r = a.m - 1;
+    if (a.a_gstride > 0) r = (r / a.group) * a.a_gstride + a.a_goff + r % a.group;
     a_src[i] = C::a_plane(p) * a.a_plane + r * a.lda + (rem % 4) * 8;
   }
   {
@@ -467,7 +490,7 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
       const bool more = t + 1 < nst;
       const int nsub_next = more ? min(C::KS, c1 - (cs + C::KS)) : 0;
       if (more) bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, cs + C::KS, nsub_next);
-      bx_chunk<SPLIT>(lds_u4 + (t & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub);
+      bx_chunk<SPLIT, TR>(lds_u4 + (t & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub);
       if (more) bx_stash<SPLIT>(lds_u4, pa, pb, tid, (t + 1) & 1, nsub_next);
       __syncthreads();
     }
@@ -512,6 +535,47 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
         const int64_t slot = (a.units ? (int64_t)cu : (int64_t)cb * a.ksplit + z) * BX_GC + wc;
         if (row < a.m) st(a.out, slot * a.m + row, cmake(a.scale * myr, a.scale * myi));
       }
+    } else if (a.epilogue == BF16_EPI_ADD) {
+      // transposed accumulators (TR): lane l16 walks the rows, so the FP64
+      // reads and writes of 16 lanes are consecutive columns c of one walker
+#pragma unroll
+      for (int i = 0; i < BX_MR; ++i) {
+        const int64_t row = (int64_t)rb * BX_R + (wr * BX_MR + i) * 16 + l16;
+        if (row >= a.m) continue;
+        const int64_t w = row / a.group;
+        const int c = (int)(row % a.group);
+#pragma unroll
+        for (int j = 0; j < BX_MC; ++j) {
+          const int64_t col0 = (int64_t)cb * BX_C + (wc * BX_MC + j) * 16 + 4 * q;
+          uint16_t h[4][4];  // [plane][r]: re_hi, re_lo, im_hi, im_lo
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int64_t col = col0 + r;
+            h[0][r] = h[1][r] = h[2][r] = h[3][r] = 0;
+            if (col >= a.nn) continue;
+            const cplx base = ld(a.base, w * a.b_ws + col * a.b_ld + a.b_off + c);
+            const cplx v = cmake(base.re + a.scale * re[i][j][r], base.im + a.scale * im[i][j][r]);
+            st(a.out, w * a.o_ws + col * a.o_ld + a.o_off + c, v);
+            split_bf16(v.re, h[0][r], h[1][r]);
+            split_bf16(v.im, h[2][r], h[3][r]);
+          }
+          if (a.tp && col0 < a.nn) {
+            const int64_t o = (w * a.tp_rows + a.tp_off + c) * a.tp_ld + col0;
+            if (col0 + 3 < a.nn && (o & 3) == 0) {
+#pragma unroll
+              for (int p = 0; p < 4; ++p)
+                *reinterpret_cast<ushort4*>(a.tp + p * a.tp_plane + o) =
+                    make_ushort4(h[p][0], h[p][1], h[p][2], h[p][3]);
+            } else {
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                if (col0 + r < a.nn)
+#pragma unroll
+                  for (int p = 0; p < 4; ++p) a.tp[p * a.tp_plane + o + r] = h[p][r];
+            }
+          }
+        }
+      }
     } else {
       const int64_t NN = (int64_t)a.nbasis * a.nbasis;
 #pragma unroll
@@ -532,10 +596,17 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
             const double vr = a.scale * re[i][j][r], vi = a.scale * im[i][j][r];
             if (a.epilogue == BF16_EPI_VHS) {
               // i scale (re + i im) = scale (-im + i re)
-              const cplx v = cmake(-vi, vr);
-              double* base = a.out + 2 * row * NN;
-              st(base, (int64_t)ti * a.nbasis + tj, v);
-              if (ti != tj) st(base, (int64_t)tj * a.nbasis + ti, v);
+              if (a.out32) {
+                float2* base = reinterpret_cast<float2*>(a.out32) + row * NN;
+                const float2 v = make_float2((float)-vi, (float)vr);
+                base[(int64_t)ti * a.nbasis + tj] = v;
+                if (ti != tj) base[(int64_t)tj * a.nbasis + ti] = v;
+              } else {
+                const cplx v = cmake(-vi, vr);
+                double* base = a.out + 2 * row * NN;
+                st(base, (int64_t)ti * a.nbasis + tj, v);
+                if (ti != tj) st(base, (int64_t)tj * a.nbasis + ti, v);
+              }
             } else {
               st(a.out, ((int64_t)(a.z_offset + z) * a.m + row) * a.nn + col, cmake(vr, vi));
             }
@@ -561,11 +632,13 @@ size_t bf16_lds_bytes(int wb, int xb, int nocc) {
 // complex in split-BF16: A = out_{k-1}^T planes [4][W][rpad][kpad] (row = column
 // of the walker matrix, K = basis index), B = VHS planes [4][W N][kpad] (row =
 // output basis index).  re = Ar Br - Ai Bi, im = Ar Bi + Ai Br, each real
-// product as hi*hi + hi*lo + lo*hi (12 MFMAs per tile and K chunk; -Ai comes
-// from sign-flipped fragments).  Workgroup: one walker, 80 rows x 80 output
+// product as hi*hi + hi*lo + lo*hi (SPLIT: 12 MFMAs per tile and K chunk) or
+// hi*hi (4 MFMAs; the hi planes only); -Ai comes from sign-flipped fragments.  Workgroup: one walker, 80 rows x 80 output
 // basis indices, 5 waves, wave v owns output column tile v and all 5 row tiles.
-// Epilogue: q = (init or q) + out_k (FP64), and the split planes of out_k^T
-// for the next term.  The out^T planes have a walker stride fixed by the
+// Epilogue: the terms are summed in FP32 in dacc (the first term writes it,
+// later ones add to it) and the last term writes q = init + dacc + out_k (FP64)
+// with the split planes of q^T (qout); every term but the last writes the
+// split planes of out_k^T for the next term.  The out^T planes have a walker stride fixed by the
 // buffer capacity wcap; their padding rows and K entries are never written
 // and must be zero.
 constexpr int TY_WAVES = 5, TY_R = 80, TY_C = 80, TY_THREADS = 64 * TY_WAVES;
@@ -573,12 +646,56 @@ constexpr int TY_A_VEC = 4 * TY_R * 4, TY_B_VEC = 4 * TY_C * 4, TY_STAGE_VEC = T
 constexpr int TY_APER = TY_A_VEC / TY_THREADS, TY_BPER = TY_B_VEC / TY_THREADS;
 static_assert(TY_APER == 4 && TY_BPER == 4, "one plane per staged vector and thread");
 constexpr size_t TY_LDS = 2 * 16 * (size_t)TY_STAGE_VEC;
+constexpr int TY_TILE = 4 * TY_C * BX_KC;  // bf16 values of one VHS tile (4 planes x 80 rows x 32 K)
+
+// VHS [W][N][N] complex FP32 -> split planes in Taylor tiles: tile (w, cb, c)
+// at ((w ncb + cb) nch + c) TY_TILE holds planes re_hi, re_lo, im_hi, im_lo of
+// rows cb 80 .. cb 80 + 79 and K = c 32 .. c 32 + 31, each [80][32], so that a
+// workgroup's K chunk is one contiguous 20 KiB block.  Rows >= N and K >= N are zero.
+__global__ void split_vhs_tiled_kernel(const float2* __restrict__ x, int nwalkers, int N, int Kp,
+                                       uint16_t* __restrict__ planes) {
+  const int ncb = (int)ceil_div(N, TY_C), nch = Kp / BX_KC;
+  const int64_t rows = (int64_t)ncb * TY_C;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nwalkers * rows * Kp) return;
+  const int k = (int)(idx % Kp);
+  const int64_t wr = idx / Kp;
+  const int n = (int)(wr % rows);
+  const int64_t w = wr / rows;
+  uint16_t rh = 0, rl = 0, ih = 0, il = 0;
+  if (n < N && k < N) {
+    const float2 v = x[(w * N + n) * N + k];
+    split_bf16(v.x, rh, rl);
+    split_bf16(v.y, ih, il);
+  }
+  const int cb = n / TY_C, rr = n % TY_C, c = k / BX_KC, kk = k % BX_KC;
+  const int64_t o = ((w * ncb + cb) * nch + c) * TY_TILE + rr * BX_KC + kk;
+  constexpr int P = TY_C * BX_KC;
+  planes[o] = rh;
+  planes[o + P] = rl;
+  planes[o + 2 * P] = ih;
+  planes[o + 3 * P] = il;
+}
 
 __device__ __forceinline__ v8bf neg_bf16(v8bf x) {
   return __builtin_bit_cast(v8bf, __builtin_bit_cast(v4u, x) ^ (v4u){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u});
 }
 
+// Staging of one LDS stage, in 16-byte vectors: SPLIT takes one K chunk of the
+// four planes of each operand, the single product two K chunks of the hi planes
+// (re_hi, im_hi: physical planes 0 and 2), so both stages hold 40 KiB and each
+// thread stages 4 + 4 vectors.
+template <bool SPLIT>
+struct Ty {
+  static constexpr int NP = SPLIT ? 4 : 2, KS = SPLIT ? 1 : 2;
+  static constexpr int SUB_A = NP * TY_R * 4, SUB_VEC = SUB_A + NP * TY_C * 4;
+  static_assert(KS * SUB_VEC == TY_STAGE_VEC, "the stage size does not depend on the product count");
+  __device__ static int phys(int pi) { return SPLIT ? pi : 2 * pi; }
+};
+
+template <bool SPLIT>
 __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args a) {
+  using T = Ty<SPLIT>;
   extern __shared__ uint4 lds_u4[];
   const int N = a.nbasis, Kp = a.kpad;
   const int nrb = a.rpad / TY_R, ncb = (int)ceil_div(N, TY_C);
@@ -586,26 +703,34 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
   const int cb = lin % ncb, rb = (lin / ncb) % nrb, w = lin / ncb / nrb;
   if (w >= a.nwalkers) return;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
-  const int64_t tplane = (int64_t)a.wcap * a.rpad * Kp, vplane = (int64_t)a.nwalkers * N * Kp;
-  // staged vector i of this thread: plane i, row tid / 4, 16-byte segment tid % 4
+  const int64_t tplane = (int64_t)a.wcap * a.rpad * Kp;
+  // staged vector i of this thread: sub-chunk i / NP, plane i % NP, row tid / 4, 16-byte segment tid % 4
   const int srow = tid >> 2, sseg = tid & 3;
   const uint16_t* asrc = a.tin + ((int64_t)w * a.rpad + rb * TY_R + srow) * Kp + sseg * 8;
-  int64_t brow = (int64_t)cb * TY_C + srow;
-  if (brow > N - 1) brow = N - 1;  // padding columns: outputs are not written
-  const uint16_t* bsrc = a.vplanes + ((int64_t)w * N + brow) * Kp + sseg * 8;
+  // VHS tiles: the K chunk c of (w, cb) is one contiguous block, thread tid takes 16 bytes at tid 16 of each plane
+  const uint16_t* bsrc = a.vplanes + (int64_t)(w * ncb + cb) * (Kp / BX_KC) * TY_TILE + srow * BX_KC + sseg * 8;
   v4u pa[TY_APER], pb[TY_BPER];
-  auto fetch = [&](int c) __attribute__((always_inline)) {
+  const int nch = Kp / BX_KC, nst = (int)ceil_div(nch, T::KS);
+  auto fetch = [&](int t) __attribute__((always_inline)) {
 #pragma unroll
-    for (int i = 0; i < TY_APER; ++i) pa[i] = *reinterpret_cast<const v4u*>(asrc + i * tplane + c * BX_KC);
-#pragma unroll
-    for (int i = 0; i < TY_BPER; ++i) pb[i] = *reinterpret_cast<const v4u*>(bsrc + i * vplane + c * BX_KC);
+    for (int i = 0; i < TY_APER; ++i) {
+      const int c = t * T::KS + i / T::NP, pl = T::phys(i % T::NP);
+      if (c < nch) {
+        pa[i] = *reinterpret_cast<const v4u*>(asrc + pl * tplane + c * BX_KC);
+        pb[i] = *reinterpret_cast<const v4u*>(bsrc + pl * (TY_C * BX_KC) + c * TY_TILE);
+      }
+    }
   };
-  auto stash = [&](int buf) __attribute__((always_inline)) {
+  auto stash = [&](int t, int buf) __attribute__((always_inline)) {
     v4u* base = reinterpret_cast<v4u*>(lds_u4) + buf * TY_STAGE_VEC;
 #pragma unroll
-    for (int i = 0; i < TY_APER; ++i) base[lds_vec(i * TY_R + srow, sseg)] = pa[i];
-#pragma unroll
-    for (int i = 0; i < TY_BPER; ++i) base[TY_A_VEC + lds_vec(i * TY_C + srow, sseg)] = pb[i];
+    for (int i = 0; i < TY_APER; ++i) {
+      const int sc = i / T::NP, pi = i % T::NP;
+      if (t * T::KS + sc < nch) {
+        base[sc * T::SUB_VEC + lds_vec(pi * TY_R + srow, sseg)] = pa[i];
+        base[sc * T::SUB_VEC + T::SUB_A + lds_vec(pi * TY_C + srow, sseg)] = pb[i];
+      }
+    }
   };
   v4f re[5], im[5];
 #pragma unroll
@@ -613,62 +738,111 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
     re[i] = zero4f();
     im[i] = zero4f();
   }
-  const int nch = Kp / BX_KC;
   fetch(0);
-  stash(0);
+  stash(0, 0);
   __syncthreads();
-  for (int c = 0; c < nch; ++c) {
-    if (c + 1 < nch) fetch(c + 1);
-    const uint16_t* As = reinterpret_cast<const uint16_t*>(lds_u4 + (c & 1) * TY_STAGE_VEC);
-    const uint16_t* Bs = As + 8 * TY_A_VEC;
-    const int bcol = wave * 16 + l16;
-    const v8bf brh = lds_frag(Bs, bcol, q), brl = lds_frag(Bs, TY_C + bcol, q);
-    const v8bf bih = lds_frag(Bs, 2 * TY_C + bcol, q), bil = lds_frag(Bs, 3 * TY_C + bcol, q);
+  for (int t = 0; t < nst; ++t) {
+    if (t + 1 < nst) fetch(t + 1);
 #pragma unroll
-    for (int i = 0; i < 5; ++i) {
-      const int arow = i * 16 + l16;
-      const v8bf arh = lds_frag(As, arow, q), arl = lds_frag(As, TY_R + arow, q);
-      const v8bf aih = lds_frag(As, 2 * TY_R + arow, q), ail = lds_frag(As, 3 * TY_R + arow, q);
-      const v8bf naih = neg_bf16(aih), nail = neg_bf16(ail);
-      re[i] = mfma_bf16(arh, brh, re[i]);
-      re[i] = mfma_bf16(arh, brl, re[i]);
-      re[i] = mfma_bf16(arl, brh, re[i]);
-      re[i] = mfma_bf16(naih, bih, re[i]);
-      re[i] = mfma_bf16(naih, bil, re[i]);
-      re[i] = mfma_bf16(nail, bih, re[i]);
-      im[i] = mfma_bf16(arh, bih, im[i]);
-      im[i] = mfma_bf16(arh, bil, im[i]);
-      im[i] = mfma_bf16(arl, bih, im[i]);
-      im[i] = mfma_bf16(aih, brh, im[i]);
-      im[i] = mfma_bf16(aih, brl, im[i]);
-      im[i] = mfma_bf16(ail, brh, im[i]);
-    }
-    if (c + 1 < nch) stash((c + 1) & 1);
-    __syncthreads();
-  }
-  // epilogue (accumulator row = 4 q + r)
-  const int kcol = cb * TY_C + wave * 16 + l16;
-  if (kcol >= N) return;
+    for (int sc = 0; sc < T::KS; ++sc) {
+      if (t * T::KS + sc >= nch) break;
+      const uint16_t* As = reinterpret_cast<const uint16_t*>(lds_u4 + (t & 1) * TY_STAGE_VEC + sc * T::SUB_VEC);
+      const uint16_t* Bs = As + 8 * T::SUB_A;
+      const int bcol = wave * 16 + l16;
+      // operands swapped in every MFMA: the accumulators hold the transposed tiles
+      if (SPLIT) {
+        const v8bf brh = lds_frag(Bs, bcol, q), brl = lds_frag(Bs, TY_C + bcol, q);
+        const v8bf bih = lds_frag(Bs, 2 * TY_C + bcol, q), bil = lds_frag(Bs, 3 * TY_C + bcol, q);
 #pragma unroll
-  for (int i = 0; i < 5; ++i)
+        for (int i = 0; i < 5; ++i) {
+          const int arow = i * 16 + l16;
+          const v8bf arh = lds_frag(As, arow, q), arl = lds_frag(As, TY_R + arow, q);
+          const v8bf aih = lds_frag(As, 2 * TY_R + arow, q), ail = lds_frag(As, 3 * TY_R + arow, q);
+          const v8bf naih = neg_bf16(aih), nail = neg_bf16(ail);
+          re[i] = mfma_bf16(brh, arh, re[i]);
+          re[i] = mfma_bf16(brl, arh, re[i]);
+          re[i] = mfma_bf16(brh, arl, re[i]);
+          re[i] = mfma_bf16(bih, naih, re[i]);
+          re[i] = mfma_bf16(bil, naih, re[i]);
+          re[i] = mfma_bf16(bih, nail, re[i]);
+          im[i] = mfma_bf16(bih, arh, im[i]);
+          im[i] = mfma_bf16(bil, arh, im[i]);
+          im[i] = mfma_bf16(bih, arl, im[i]);
+          im[i] = mfma_bf16(brh, aih, im[i]);
+          im[i] = mfma_bf16(brl, aih, im[i]);
+          im[i] = mfma_bf16(brh, ail, im[i]);
+        }
+      } else {
+        const v8bf brh = lds_frag(Bs, bcol, q), bih = lds_frag(Bs, TY_C + bcol, q);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int crow = rb * TY_R + i * 16 + 4 * q + r;
-      if (crow >= a.ncol) continue;
-      const cplx v = cmake(a.alpha * re[i][r], a.alpha * im[i][r]);
-      const int64_t qi = ((int64_t)w * N + kcol) * a.ncol + crow;
-      st(a.q, qi, cadd(a.init ? ld(a.init, qi) : ld(a.q, qi), v));
-      if (a.tout) {
-        uint16_t rh, rl, ih, il;
-        split_bf16(v.re, rh, rl);
-        split_bf16(v.im, ih, il);
-        const int64_t o = ((int64_t)w * a.rpad + crow) * Kp + kcol;
-        a.tout[o] = rh;
-        a.tout[tplane + o] = rl;
-        a.tout[2 * tplane + o] = ih;
-        a.tout[3 * tplane + o] = il;
+        for (int i = 0; i < 5; ++i) {
+          const int arow = i * 16 + l16;
+          const v8bf arh = lds_frag(As, arow, q), aih = lds_frag(As, TY_R + arow, q);
+          re[i] = mfma_bf16(brh, arh, re[i]);
+          re[i] = mfma_bf16(bih, neg_bf16(aih), re[i]);
+          im[i] = mfma_bf16(bih, arh, im[i]);
+          im[i] = mfma_bf16(brh, aih, im[i]);
+        }
       }
     }
+    if (t + 1 < nst) stash(t + 1, (t + 1) & 1);
+    __syncthreads();
+  }
+  // epilogue: element r of tile i is row crow = i 16 + l16 (a column of the
+  // walker matrix) and basis index kcol = wave 16 + 4 q + r, so 16 lanes touch
+  // 16 consecutive columns of q / dacc and each lane 4 consecutive K of a plane
+  const int kc0 = cb * TY_C + wave * 16 + 4 * q;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    const int crow = rb * TY_R + i * 16 + l16;
+    if (crow >= a.ncol || kc0 >= N) continue;
+    uint16_t ht[4][4], hq[4][4];  // [plane][r] of out_k and of q
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int kcol = kc0 + r;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) ht[p][r] = hq[p][r] = 0;
+      if (kcol >= N) continue;
+      const cplx v = cmake(a.alpha * re[i][r], a.alpha * im[i][r]);
+      const int64_t qi = ((int64_t)w * N + kcol) * a.ncol + crow;
+      if (a.last) {
+        cplx qv = cadd(ld(a.init, qi), v);
+        if (!a.first) {
+          const float2 d = reinterpret_cast<const float2*>(a.dacc)[qi];
+          qv.re += (double)d.x;
+          qv.im += (double)d.y;
+        }
+        st(a.q, qi, qv);
+        split_bf16(qv.re, hq[0][r], hq[1][r]);
+        split_bf16(qv.im, hq[2][r], hq[3][r]);
+      } else {
+        float2 d = a.first ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(a.dacc)[qi];
+        d.x = (float)((double)d.x + v.re);
+        d.y = (float)((double)d.y + v.im);
+        reinterpret_cast<float2*>(a.dacc)[qi] = d;
+      }
+      split_bf16(v.re, ht[0][r], ht[1][r]);
+      split_bf16(v.im, ht[2][r], ht[3][r]);
+    }
+    const int64_t o = ((int64_t)w * a.rpad + crow) * Kp + kc0;  // a multiple of 4: Kp % 32 == 0, kc0 % 4 == 0
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+      uint16_t* dst = pass == 0 ? a.tout : a.qout;
+      if (!dst) continue;
+      const uint16_t(&h)[4][4] = pass == 0 ? ht : hq;
+      if (kc0 + 3 < N) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          *reinterpret_cast<ushort4*>(dst + p * tplane + o) = make_ushort4(h[p][0], h[p][1], h[p][2], h[p][3]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (kc0 + r < N)
+#pragma unroll
+            for (int p = 0; p < 4; ++p) dst[p * tplane + o + r] = h[p][r];
+      }
+    }
+  }
 }
 
 // phi [W][N][ncol] complex -> transposed split planes [4][W][rpad][kpad] at
@@ -761,29 +935,43 @@ hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream) {
   if (a.epilogue == BF16_EPI_QDOT && (a.dg == nullptr || a.dkp < a.dn || a.dn < 1 || a.dstride < 1))
     return hipErrorInvalidValue;
   if (a.units && a.nunits < 1) return hipErrorInvalidValue;
+  if ((a.a_gstride > 0 || a.epilogue == BF16_EPI_ADD) && a.group < 1) return hipErrorInvalidValue;
+  if (a.epilogue == BF16_EPI_ADD && (a.base == nullptr || a.out == nullptr || a.single)) return hipErrorInvalidValue;
   const int64_t nb = ceil_div(a.nn, BX_C);
   const int64_t grid = ceil_div(a.m, BX_R) * (a.units ? (int64_t)a.nunits : nb * a.ksplit);
   if (a.single) {
     if (a.epilogue == BF16_EPI_VHS) return hipErrorInvalidValue;
-    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<false>, Bx<false>::STAGES);
+    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<false, false>, Bx<false>::STAGES);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(bf16_gemm_kernel<false>, dim3((unsigned)grid), dim3(BX_THREADS), Bx<false>::STAGES, stream, a);
+    hipLaunchKernelGGL((bf16_gemm_kernel<false, false>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<false>::STAGES,
+                       stream, a);
+  } else if (a.epilogue == BF16_EPI_ADD) {
+    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true, true>, Bx<true>::STAGES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((bf16_gemm_kernel<true, true>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES,
+                       stream, a);
   } else {
-    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true>, Bx<true>::STAGES);
+    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true, false>, Bx<true>::STAGES);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(bf16_gemm_kernel<true>, dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES, stream, a);
+    hipLaunchKernelGGL((bf16_gemm_kernel<true, false>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES,
+                       stream, a);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream) {
   if (a.nwalkers < 1 || a.wcap < a.nwalkers || a.ncol < 1 || a.rpad % TY_R != 0 || a.rpad < a.ncol ||
-      a.kpad % BX_KC != 0 || a.kpad < a.nbasis)
+      a.kpad % BX_KC != 0 || a.kpad < a.nbasis || a.init == nullptr || (a.last && a.q == nullptr) ||
+      (!(a.first && a.last) && a.dacc == nullptr))
     return hipErrorInvalidValue;
-  hipError_t e = allow_lds((const void*)taylor_bf16_kernel, TY_LDS);
+  const void* kern = a.split ? (const void*)taylor_bf16_kernel<true> : (const void*)taylor_bf16_kernel<false>;
+  hipError_t e = allow_lds(kern, TY_LDS);
   if (e != hipSuccess) return e;
   const int64_t grid = (int64_t)a.nwalkers * (a.rpad / TY_R) * ceil_div(a.nbasis, TY_C);
-  hipLaunchKernelGGL(taylor_bf16_kernel, dim3((unsigned)grid), dim3(TY_THREADS), TY_LDS, stream, a);
+  if (a.split)
+    hipLaunchKernelGGL(taylor_bf16_kernel<true>, dim3((unsigned)grid), dim3(TY_THREADS), TY_LDS, stream, a);
+  else
+    hipLaunchKernelGGL(taylor_bf16_kernel<false>, dim3((unsigned)grid), dim3(TY_THREADS), TY_LDS, stream, a);
   return hipGetLastError();
 }
 
@@ -792,6 +980,22 @@ hipError_t launch_split_transpose(const double* phi, int nwalkers, int wcap, int
   const int64_t grid = (int64_t)nwalkers * ceil_div(N, 32) * ceil_div(ncol, 32);
   hipLaunchKernelGGL(split_transpose_kernel, dim3((unsigned)grid), dim3(256), 0, stream, phi, nwalkers, wcap, N, ncol,
                      rpad, Kp, planes);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_vhs_tiled(const float* x, int nwalkers, int N, int Kp, uint16_t* planes, hipStream_t stream) {
+  if (Kp % BX_KC != 0 || Kp < N) return hipErrorInvalidValue;
+  const int64_t total = (int64_t)nwalkers * ceil_div(N, TY_C) * TY_C * Kp;
+  hipLaunchKernelGGL(split_vhs_tiled_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream,
+                     reinterpret_cast<const float2*>(x), nwalkers, N, Kp, planes);
+  return hipGetLastError();
+}
+
+int64_t vhs_tiled_rows(int N) { return ceil_div(N, TY_C) * TY_C; }
+
+hipError_t launch_split_eye(const double* m, int N, int Kp, uint16_t* planes, hipStream_t stream) {
+  const int64_t total = (int64_t)N * Kp;
+  hipLaunchKernelGGL(split_eye_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream, m, N, Kp, planes);
   return hipGetLastError();
 }
 

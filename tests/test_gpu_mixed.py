@@ -1,9 +1,11 @@
 """Mixed-precision mode: force bias, VHS, the Taylor series of exp(VHS) and
 the exchange run as split-BF16 products with FP32 accumulation (x = hi + lo
-with |x - hi - lo| <= 2^-17 |x|, three products hi*hi + hi*lo + lo*hi); LU,
-inverse, weights, the one-body products and QR stay FP64.  Tolerances are set
-by that arithmetic, from a CPU emulation of the same split products on these
-cases:
+with |x - hi - lo| <= 2^-17 |x|, three products hi*hi + hi*lo + lo*hi); the
+one-body propagator is applied as phi + (expH1 - I) phi with that product in
+split-BF16 and the sum in FP64; Taylor terms beyond the third use one BF16
+product (they are < 1e-3 of phi); LU, inverse, weights and QR stay FP64.
+Tolerances are set by that arithmetic, from a CPU emulation of the same split
+products on these cases:
   exchange (after cancellation)   4.7e-5 (N=200) .. 3.4e-6 (N=64)  -> 1e-4
   force bias, VHS (max-norm rel.) 4e-6 .. 7e-6                     -> 5e-5
   one propagation step            phi <= 2.2e-6, weight <= 4.7e-5,
@@ -61,6 +63,20 @@ def test_mixed_local_energy(name):
 FB_TOL = 5e-5
 PHI_TOL = 2e-5
 WEIGHT_TOL = 3e-4
+# one body as phi + (expH1 - I) phi with the product in split-BF16: the error is
+# ~1e-5 of |expH1 - I| |phi|, and |expH1 - I| ~ dt |H1| / 2 is small
+ONE_BODY_TOL = 5e-6
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_mixed_one_body(name):
+    p = make(name, "mixed")
+    p.prop.propagate_walkers_one_body(p.walkers)
+    p.orc.propagator.propagate_walkers_one_body(p.orc.walkers)
+    for attr in ("phia", "phib"):
+        err = rel_err(getattr(p.walkers, attr).cpu().numpy(), numpy.asarray(getattr(p.orc.walkers, attr)))
+        print(f"{attr} rel err {err:.2e}")
+        assert err < ONE_BODY_TOL, attr
 
 
 @pytest.mark.parametrize("name", list(CASES))
