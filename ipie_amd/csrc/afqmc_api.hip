@@ -41,7 +41,6 @@ struct afqmc_ctx {
   double* exx_b = nullptr;       // [xs_b, W] complex
   double* vhs = nullptr;         // [W, N, N] complex
   double* buf[4] = {nullptr, nullptr, nullptr, nullptr};  // P, Ta, Tb, Q: [W, N, ncol] complex
-  double* qr_work = nullptr;     // [W, N, nmax] complex
   int* tri = nullptr;            // [2, P] packed-index rows / cols
   int* flags = nullptr;
   unsigned long long* nfb = nullptr;
@@ -275,7 +274,6 @@ int ensure_ws(afqmc_ctx* c) {
   }
   ok = ok && dmalloc((void**)&c->vhs, 16 * (size_t)W * N * N) == hipSuccess;
   for (int i = 0; i < 4; ++i) ok = ok && dmalloc((void**)&c->buf[i], 16 * (size_t)W * N * c->ncol) == hipSuccess;
-  ok = ok && dmalloc((void**)&c->qr_work, 16 * (size_t)W * N * nmax) == hipSuccess;
   ok = ok && dmalloc((void**)&c->tri, sizeof(int) * 2 * (size_t)P) == hipSuccess;
   if (!ok) return fail(c, AFQMC_ERR_ALLOC, "workspace allocation failed (%s)", c->describe_dims);
   std::vector<int> tri(2 * P);
@@ -1165,7 +1163,7 @@ void afqmc_destroy(afqmc_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   else (void)hipDeviceSynchronize();
   void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
-                  c->buf[2], c->buf[3], c->qr_work, c->tri, c->flags, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
+                  c->buf[2], c->buf[3], c->tri, c->flags, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
                   c->lplanes[1], c->xplanes, c->cplanes, c->vplanes, c->tplanes[0], c->dplanes, c->oplanes,
                   c->tplanes[1], c->gcomp, c->gf_o, c->gf_oinv, c->qw64[0], c->qw64[1], c->qwp[0],
                   c->qwp[1], c->qy, c->qunits[0], c->qunits[1], c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
@@ -1375,7 +1373,6 @@ int afqmc_reortho(afqmc_ctx* c, int32_t nw, double* phia, double* phib, double* 
   a.nbeta = c->d.nbeta;
   a.phia = phia;
   a.phib = phib;
-  a.work = c->qr_work;
   a.ovlp = ovlp;
   a.detR = detR;
   a.log_detR = log_detR;

@@ -370,7 +370,6 @@ struct ReorthoArgs {
   int nwalkers, nbasis, nalpha, nbeta;
   double* phia;
   double* phib;
-  double* work;          // [W, N, max(na, nb)] complex scratch
   double* ovlp;          // [W] complex, divided by detR
   double* detR;          // [W] real out
   double* log_detR;      // [W] real, += log detR
