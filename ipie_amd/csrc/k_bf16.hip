@@ -60,9 +60,11 @@ struct Bx {
   static constexpr int A_PER = A_VEC / BX_THREADS;
   static constexpr int B_PER = (B_VEC + BX_THREADS - 1) / BX_THREADS;
   static constexpr size_t STAGES = 2 * 16 * (size_t)STAGE_VEC;  // double-buffered, bytes
+  static constexpr size_t STAGES3 = 3 * 16 * (size_t)STAGE_VEC;  // triple-buffered (bf16_gemm_kernel)
   static_assert(A_VEC % BX_THREADS == 0, "A staging is split evenly over the threads");
   static_assert((B_VEC % BX_THREADS) % 64 == 0, "the partial B vector round is wave-uniform");
   static_assert(STAGES <= BX_LDS_MAX, "staging fits the workgroup LDS");
+  static_assert(STAGES3 <= 163840, "three stages fit the 160 KiB of LDS");
   __device__ static int a_plane(int p) { return SPLIT ? p : 2 * p; }
   __device__ static int b_plane(int p) { return SPLIT ? p : 0; }
 };
@@ -414,7 +416,8 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
 // uses the hi planes only (1 product).  Epilogues:
 //   BF16_EPI_PARTIAL  out[z_offset + z][row][col] = scale C     (force bias)
 //   BF16_EPI_VHS      out[w][i][j] = out[w][j][i] = i scale C[w][p],
-//                     (i, j) = (tri_i[p], tri_j[p])              (VHS)
+//                     (i, j) = (tri_i[p], tri_j[p])              (VHS); with
+//                     out32, complex FP32 and the upper triangle i <= j only
 //   BF16_EPI_ADD      out[w][col][c] = base[w][col][c] + scale C[r][col], r = w group + c,
 //                     optionally with the split planes of the sum, transposed (one body)
 //   BF16_EPI_QDOT     out[((cb ksplit + z) BX_GC + wc) m + row] = sum over the
@@ -475,24 +478,30 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
         re[i][j] = zero4f();
         im[i][j] = zero4f();
       }
-    // LDS stages of up to KS sub-chunks of 32 K values over this split's range [c0, c1)
+    // LDS stages of up to KS sub-chunks of 32 K values over this split's range
+    // [c0, c1), three buffers: stage t + 2 is staged while t is computed, from
+    // registers loaded during stage t - 1 (two stages of latency cover)
     v4u pa[C::KS][C::A_PER], pb[C::KS][C::B_PER];
     const int nst = (int)ceil_div(c1 - c0, C::KS);
+    auto nsub_of = [&](int t) { return min(C::KS, c1 - (c0 + t * C::KS)); };
     if (nst > 0) {
-      const int ns0 = min(C::KS, c1 - c0);
-      bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0, ns0);
-      bx_stash<SPLIT>(lds_u4, pa, pb, tid, 0, ns0);
-      __syncthreads();
+      bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0, nsub_of(0));
+      bx_stash<SPLIT>(lds_u4, pa, pb, tid, 0, nsub_of(0));
     }
+    if (nst > 1) {
+      bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0 + C::KS, nsub_of(1));
+      bx_stash<SPLIT>(lds_u4, pa, pb, tid, 1, nsub_of(1));
+    }
+    if (nst > 2) bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0 + 2 * C::KS, nsub_of(2));
+    __syncthreads();
+    int buf = 0;
     for (int t = 0; t < nst; ++t) {
-      const int cs = c0 + t * C::KS;
-      const int nsub = min(C::KS, c1 - cs);
-      const bool more = t + 1 < nst;
-      const int nsub_next = more ? min(C::KS, c1 - (cs + C::KS)) : 0;
-      if (more) bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, cs + C::KS, nsub_next);
-      bx_chunk<SPLIT, TR>(lds_u4 + (t & 1) * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub);
-      if (more) bx_stash<SPLIT>(lds_u4, pa, pb, tid, (t + 1) & 1, nsub_next);
+      bx_chunk<SPLIT, TR>(lds_u4 + buf * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub_of(t));
+      const int nbuf = buf == 0 ? 2 : buf - 1;  // (t + 2) % 3: the buffer computed at t - 1
+      if (t + 2 < nst) bx_stash<SPLIT>(lds_u4, pa, pb, tid, nbuf, nsub_of(t + 2));
+      if (t + 3 < nst) bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0 + (t + 3) * C::KS, nsub_of(t + 3));
       __syncthreads();
+      buf = buf == 2 ? 0 : buf + 1;
     }
     if (a.epilogue == BF16_EPI_QDOT) {
       // per-row dot with d over this wave's 80 columns, then across the 16 lanes
@@ -597,10 +606,9 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
             if (a.epilogue == BF16_EPI_VHS) {
               // i scale (re + i im) = scale (-im + i re)
               if (a.out32) {
-                float2* base = reinterpret_cast<float2*>(a.out32) + row * NN;
-                const float2 v = make_float2((float)-vi, (float)vr);
-                base[(int64_t)ti * a.nbasis + tj] = v;
-                if (ti != tj) base[(int64_t)tj * a.nbasis + ti] = v;
+                // upper triangle only (ti <= tj); the reader mirrors it
+                reinterpret_cast<float2*>(a.out32)[row * NN + (int64_t)ti * a.nbasis + tj] =
+                    make_float2((float)-vi, (float)vr);
               } else {
                 const cplx v = cmake(-vi, vr);
                 double* base = a.out + 2 * row * NN;
@@ -648,33 +656,53 @@ static_assert(TY_APER == 4 && TY_BPER == 4, "one plane per staged vector and thr
 constexpr size_t TY_LDS = 2 * 16 * (size_t)TY_STAGE_VEC;
 constexpr int TY_TILE = 4 * TY_C * BX_KC;  // bf16 values of one VHS tile (4 planes x 80 rows x 32 K)
 
-// VHS [W][N][N] complex FP32 -> split planes in Taylor tiles: tile (w, cb, c)
-// at ((w ncb + cb) nch + c) TY_TILE holds planes re_hi, re_lo, im_hi, im_lo of
-// rows cb 80 .. cb 80 + 79 and K = c 32 .. c 32 + 31, each [80][32], so that a
-// workgroup's K chunk is one contiguous 20 KiB block.  Rows >= N and K >= N are zero.
-__global__ void split_vhs_tiled_kernel(const float2* __restrict__ x, int nwalkers, int N, int Kp,
-                                       uint16_t* __restrict__ planes) {
+// VHS [W][N][N] complex FP32, upper triangle (n <= k) only, -> split planes in
+// Taylor tiles: tile (w, cb, c) at ((w ncb + cb) nch + c) TY_TILE holds planes
+// re_hi, re_lo, im_hi, im_lo of rows cb 80 .. cb 80 + 79 and K = c 32 .. c 32 + 31,
+// each [80][32], so that a workgroup's K chunk is one contiguous 20 KiB block.
+// Rows >= N and K >= N are zero.  One workgroup per (walker, 32 x 32 block of
+// (n, k)): blocks below the diagonal are read from their mirror above it and
+// transposed through LDS, so reads and writes are coalesced.
+__global__ void __launch_bounds__(256) split_vhs_tiled_kernel(const float2* __restrict__ x, int nwalkers, int N,
+                                                             int Kp, uint16_t* __restrict__ planes) {
+  __shared__ float2 blk[32][33];
   const int ncb = (int)ceil_div(N, TY_C), nch = Kp / BX_KC;
-  const int64_t rows = (int64_t)ncb * TY_C;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)nwalkers * rows * Kp) return;
-  const int k = (int)(idx % Kp);
-  const int64_t wr = idx / Kp;
-  const int n = (int)(wr % rows);
-  const int64_t w = wr / rows;
-  uint16_t rh = 0, rl = 0, ih = 0, il = 0;
-  if (n < N && k < N) {
-    const float2 v = x[(w * N + n) * N + k];
-    split_bf16(v.x, rh, rl);
-    split_bf16(v.y, ih, il);
+  const int rows = ncb * TY_C, nbn = (int)ceil_div(rows, 32), nbk = Kp / 32;
+  int b = blockIdx.x;
+  const int bk = b % nbk;
+  b /= nbk;
+  const int bn = b % nbn, w = b / nbn;
+  if (w >= nwalkers) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const float2* xw = x + (int64_t)w * N * N;
+  // stage the block that holds the values: (bn, bk) itself when bn <= bk, else its mirror (bk, bn)
+  const int r0 = 32 * min(bn, bk), c0 = 32 * max(bn, bk);
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int r = r0 + yy, c = c0 + tx;
+    blk[yy][tx] = (r < N && c < N) ? xw[(int64_t)r * N + c] : make_float2(0.0f, 0.0f);
   }
-  const int cb = n / TY_C, rr = n % TY_C, c = k / BX_KC, kk = k % BX_KC;
-  const int64_t o = ((w * ncb + cb) * nch + c) * TY_TILE + rr * BX_KC + kk;
+  __syncthreads();
   constexpr int P = TY_C * BX_KC;
-  planes[o] = rh;
-  planes[o + P] = rl;
-  planes[o + 2 * P] = ih;
-  planes[o + 3 * P] = il;
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int n = 32 * bn + yy, k = 32 * bk + tx;
+    if (n >= rows) continue;
+    // value (n, k) = VHS[min][max]: in the staged block at (n - r0, k - c0) or its transpose
+    float2 v;
+    if (bn < bk) v = blk[yy][tx];
+    else if (bn > bk) v = blk[tx][yy];
+    else v = yy <= tx ? blk[yy][tx] : blk[tx][yy];
+    uint16_t rh = 0, rl = 0, ih = 0, il = 0;
+    if (n < N && k < N) {
+      split_bf16(v.x, rh, rl);
+      split_bf16(v.y, ih, il);
+    }
+    const int cb = n / TY_C, rr = n % TY_C;
+    const int64_t o = (((int64_t)w * ncb + cb) * nch + bk) * TY_TILE + rr * BX_KC + tx;
+    planes[o] = rh;
+    planes[o + P] = rl;
+    planes[o + 2 * P] = ih;
+    planes[o + 3 * P] = il;
+  }
 }
 
 __device__ __forceinline__ v8bf neg_bf16(v8bf x) {
@@ -941,19 +969,19 @@ hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream) {
   const int64_t grid = ceil_div(a.m, BX_R) * (a.units ? (int64_t)a.nunits : nb * a.ksplit);
   if (a.single) {
     if (a.epilogue == BF16_EPI_VHS) return hipErrorInvalidValue;
-    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<false, false>, Bx<false>::STAGES);
+    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<false, false>, Bx<false>::STAGES3);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((bf16_gemm_kernel<false, false>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<false>::STAGES,
+    hipLaunchKernelGGL((bf16_gemm_kernel<false, false>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<false>::STAGES3,
                        stream, a);
   } else if (a.epilogue == BF16_EPI_ADD) {
-    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true, true>, Bx<true>::STAGES);
+    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true, true>, Bx<true>::STAGES3);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((bf16_gemm_kernel<true, true>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES,
+    hipLaunchKernelGGL((bf16_gemm_kernel<true, true>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES3,
                        stream, a);
   } else {
-    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true, false>, Bx<true>::STAGES);
+    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true, false>, Bx<true>::STAGES3);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((bf16_gemm_kernel<true, false>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES,
+    hipLaunchKernelGGL((bf16_gemm_kernel<true, false>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES3,
                        stream, a);
   }
   return hipGetLastError();
@@ -984,9 +1012,10 @@ hipError_t launch_split_transpose(const double* phi, int nwalkers, int wcap, int
 }
 
 hipError_t launch_split_vhs_tiled(const float* x, int nwalkers, int N, int Kp, uint16_t* planes, hipStream_t stream) {
+  const int64_t rows = ceil_div(N, TY_C) * TY_C;
   if (Kp % BX_KC != 0 || Kp < N) return hipErrorInvalidValue;
-  const int64_t total = (int64_t)nwalkers * ceil_div(N, TY_C) * TY_C * Kp;
-  hipLaunchKernelGGL(split_vhs_tiled_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream,
+  const int64_t grid = (int64_t)nwalkers * ceil_div(rows, 32) * (Kp / 32);
+  hipLaunchKernelGGL(split_vhs_tiled_kernel, dim3((unsigned)grid), dim3(256), 0, stream,
                      reinterpret_cast<const float2*>(x), nwalkers, N, Kp, planes);
   return hipGetLastError();
 }
