@@ -62,7 +62,8 @@ extern "C" {
 #define AFQMC_PHASE_ENERGY 9      /* one-body energy and energy assembly */
 #define AFQMC_PHASE_REORTHO 10    /* QR re-orthogonalisation */
 #define AFQMC_PHASE_EXCHANGE_CV 11 /* single-product exchange of the control variate (surrogate) */
-#define AFQMC_NPHASES 12
+#define AFQMC_PHASE_SPLIT 12       /* Ghalf / dG into split-BF16 planes for the quadratic-form exchange */
+#define AFQMC_NPHASES 13
 
 /* precision modes */
 #define AFQMC_PREC_FP64 0   /* every contraction in FP64 */

@@ -95,7 +95,7 @@ SIGNATURES = {
 }
 
 PHASES = ["gf", "one_body", "force_bias", "fields", "vhs", "taylor", "overlap", "weight", "exchange", "energy",
-          "reortho", "exchange_cv"]
+          "reortho", "exchange_cv", "split"]
 
 _lib = None
 

@@ -718,14 +718,19 @@ int ensure_quad(afqmc_ctx* c) {
 }
 
 // Split-BF16 quadratic form of nw walkers for one spin, d^T Wq d with d = G - gref
-// (gref may be null; walker_of selects the walkers): Y = d W' over the tile pairs
-// of the block-triangular planes, the per-tile dot products in the GEMM's
-// epilogue, their sum per walker into part.  single: hi * hi only.
+// (gref may be null; walker_of selects the walkers): Y = d W' over the column
+// units of the block-triangular planes, the per-tile dot products in the GEMM's
+// epilogue, their sum per walker into part.  single: hi * hi only.  The split
+// of d into planes is timed as AFQMC_PHASE_SPLIT, the rest as `phase`.
 int quad_bf16(afqmc_ctx* c, int nw, int spin, const double* g, const int* walker_of, const double* gref,
-              bool single, double* part) {
+              bool single, double* part, int phase) {
   const int N = c->d.nbasis;
   const int n = spin == 0 ? c->d.nalpha : c->d.nbeta;
-  HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, walker_of, n, gref));
+  {
+    PhaseScope ps(c, AFQMC_PHASE_SPLIT);
+    HIP_TRY(c, launch_split_ghalf(g, (int64_t)nw * n, N, c->kpad, c->gplanes[spin], c->stream, walker_of, n, gref));
+  }
+  PhaseScope ps(c, phase);
   const int64_t ldn = (int64_t)n * c->kpad;
   const int64_t mb = ceil_div(nw, 128);
   if (mb < 1 || mb > (int64_t)c->qunit_n[spin].size()) return fail(c, AFQMC_ERR_INVALID_ARG, "walker count");
@@ -766,11 +771,11 @@ int exchange_spin_quad(afqmc_ctx* c, int nw, int spin, const double* g, const in
   double* part = spin == 0 ? c->exx_a : c->exx_b;
   int r = ensure_quad(c);
   if (r) return r;
-  PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
   if (c->qwp[spin]) {
-    r = quad_bf16(c, nw, spin, g, walker_of, nullptr, false, part);
+    r = quad_bf16(c, nw, spin, g, walker_of, nullptr, false, part, AFQMC_PHASE_EXCHANGE);
     if (r) return r;
   } else {
+    PhaseScope ps(c, AFQMC_PHASE_EXCHANGE);
     if (walker_of) {
       HIP_TRY(c, launch_sur_gather(g, walker_of, nw, (int64_t)n * N, c->gcomp, c->stream));
       g = c->gcomp;
@@ -947,8 +952,7 @@ int exchange_q_spin(afqmc_ctx* c, int nw, int spin, const double* g) {
   if (c->qwp[spin]) {
     int r = ensure_quad(c);
     if (r) return r;
-    PhaseScope ps(c, AFQMC_PHASE_EXCHANGE_CV);
-    r = quad_bf16(c, nw, spin, g, nullptr, gT, true, part);
+    r = quad_bf16(c, nw, spin, g, nullptr, gT, true, part, AFQMC_PHASE_EXCHANGE_CV);
     if (r) return r;
     c->xs_used[spin] = 1;
     return AFQMC_OK;
