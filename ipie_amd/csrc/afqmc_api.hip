@@ -187,9 +187,12 @@ hipError_t dmalloc(void** p, size_t bytes) {
 }
 
 // Column units of the block-triangular quadratic-form GEMM with mb row blocks:
-// column tile t sums the K chunks [0, min(5 (t + 1), kchunks)), cut into pieces
-// of at most T chunks, T chosen for about 1536 workgroups (at least 32 chunks);
-// heaviest first so that the light units fill the end of the launch.
+// column tile t sums the K chunks [0, min(5 (t + 1), kchunks)), cut at the
+// multiples of T chunks (T chosen for about 1536 workgroups, at least 32), and
+// the units are listed by K block, then tile.  The grid takes the row blocks
+// fastest, so the workgroups that run together on one XCD (consecutive logical
+// indices) are a few tiles of the same K block: they read the same d rows and
+// the same K range, which L2 then serves once.
 std::vector<int4> quad_units(int64_t nb, int kchunks, int64_t mb) {
   const int per_tile = QUAD_TILE / 32;
   int64_t total = 0;
@@ -197,19 +200,17 @@ std::vector<int4> quad_units(int64_t nb, int kchunks, int64_t mb) {
   const int64_t target = std::max<int64_t>(ceil_div(1536, mb), 1);
   const int64_t T = std::max<int64_t>(ceil_div(total, target), 32);
   std::vector<int4> u;
-  for (int64_t t = 0; t < nb; ++t) {
-    const int64_t kc = std::min<int64_t>(per_tile * (t + 1), kchunks);
-    const int64_t ks = ceil_div(kc, T);
-    for (int64_t z = 0; z < ks; ++z) {
+  for (int64_t z = 0; z * T < kchunks; ++z)
+    for (int64_t t = 0; t < nb; ++t) {
+      const int64_t kc = std::min<int64_t>(per_tile * (t + 1), kchunks);
+      if (z * T >= kc) continue;
       int4 e;
       e.x = (int)t;
-      e.y = (int)(kc * z / ks);
-      e.z = (int)(kc * (z + 1) / ks);
+      e.y = (int)(z * T);
+      e.z = (int)std::min<int64_t>((z + 1) * T, kc);
       e.w = 0;
       u.push_back(e);
     }
-  }
-  std::stable_sort(u.begin(), u.end(), [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
   return u;
 }
 
