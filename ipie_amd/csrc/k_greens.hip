@@ -1,12 +1,12 @@
-// Batched single-determinant Green's function and overlap (FP64), three kernels:
+// Batched single-determinant Green's function and overlap (FP64), four kernels:
 //   gf_obuild_kernel  O_s = phi_s^T conj(psi_s)           [n, n] per walker and spin
 //                     (v_mfma_f64_16x16x4f64, one wave per 16-row tile of O)
 //   gf_lu_kernel      P O_s = L U (partial pivoting on |re| + |im|, the first
 //                     index on ties, LAPACK's izamax rule); log|det O_s| =
 //                     sum log|U_kk|, phase = prod U_kk / |U_kk| * (-1)^swaps;
-//                     optionally O_s^{-1}; ovlp = phase_a phase_b
-//                     exp(logdet_a + logdet_b - log_shift).  One wave per
-//                     walker, lane = row (factorisation) or column (inverse).
+//                     optionally O_s^{-1}.  One wave per walker and spin,
+//                     lane = row (factorisation) or column (inverse).
+//   gf_combine_kernel ovlp = phase_a phase_b exp(logdet_a + logdet_b - log_shift)
 //   gf_ghalf_kernel   Ghalf_s = O_s^{-1} phi_s^T            [n, N]
 //                     (FP64 MFMA, one wave per 16-row tile of Ghalf)
 // Restates ipie/estimators/greens_function_single_det.py:62-154 and
@@ -67,22 +67,41 @@ __global__ void __launch_bounds__(64 * GF_WAVES) gf_obuild_kernel(GfArgs a, doub
     }
 }
 
+// sum_{k0 <= k < k1} row[k] col[k ld] (complex), four independent partial sums.
+__device__ __forceinline__ cplx lu_dot(const cplx* row, const cplx* col, int ld, int k0, int k1) {
+  cplx s4[4] = {cmake(0.0, 0.0), cmake(0.0, 0.0), cmake(0.0, 0.0), cmake(0.0, 0.0)};
+  int k = k0;
+  for (; k + 4 <= k1; k += 4) {
+    cplx r[4], x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      r[u] = row[k + u];
+      x[u] = col[(k + u) * ld];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) cfma(s4[u], r[u], x[u]);
+  }
+  for (; k < k1; ++k) cfma(s4[0], row[k], col[k * ld]);
+  return cadd(cadd(s4[0], s4[1]), cadd(s4[2], s4[3]));
+}
+
 // One wave per walker: LU of O_s (in LDS, row stride nmax + 1), log-det and
 // phase, optionally the inverse into oinv, then the overlap of the walker.
 // A single-wave workgroup: __syncthreads() orders the LDS traffic between lanes.
 __global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __restrict__ obuf,
-                                                  double* __restrict__ oinv, int nmax) {
+                                                  double* __restrict__ oinv, int nmax, double* __restrict__ res) {
   extern __shared__ cplx smem[];
   const int lr = nmax + 1;  // LDS row stride
   cplx* LU = smem;             // [nmax][lr]
   cplx* X = smem + nmax * lr;  // [nmax][lr], columns of the inverse
   __shared__ int piv[64];
-  const int w = blockIdx.x, lane = threadIdx.x;
+  const int lane = threadIdx.x;
   const int nspin = gf_nspin(a);
+  const int w = blockIdx.x / nspin, s = blockIdx.x % nspin;
   double logdet[2] = {0.0, 0.0};
   cplx phase[2] = {cmake(1.0, 0.0), cmake(1.0, 0.0)};
   int singular = 0;
-  for (int s = 0; s < nspin; ++s) {
+  {
     const int n = s == 0 ? a.nalpha : a.nbeta;
     const double* o = obuf + 2 * ((int64_t)w * 2 + s) * nmax * nmax;
     for (int i = 0; i < n; ++i)
@@ -158,25 +177,13 @@ __global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __res
             X[p * lr + j] = t;
           }
         }
+        // substitutions: dot products with four independent partial sums, so the
+        // LDS loads of four terms are in flight together
         for (int i = 0; i < n; ++i) {
-          cplx sum = X[i * lr + j];
-          for (int k = 0; k < i; ++k) {
-            const cplx l = LU[i * lr + k];
-            const cplx x = X[k * lr + j];
-            sum.re -= l.re * x.re - l.im * x.im;
-            sum.im -= l.re * x.im + l.im * x.re;
-          }
-          X[i * lr + j] = sum;
+          X[i * lr + j] = csub(X[i * lr + j], lu_dot(LU + i * lr, X + j, lr, 0, i));
         }
         for (int i = n - 1; i >= 0; --i) {
-          cplx sum = X[i * lr + j];
-          for (int k = i + 1; k < n; ++k) {
-            const cplx u = LU[i * lr + k];
-            const cplx x = X[k * lr + j];
-            sum.re -= u.re * x.re - u.im * x.im;
-            sum.im -= u.re * x.im + u.im * x.re;
-          }
-          X[i * lr + j] = cdiv(sum, LU[i * lr + i]);
+          X[i * lr + j] = cdiv(csub(X[i * lr + j], lu_dot(LU + i * lr, X + j, lr, i + 1, n)), LU[i * lr + i]);
         }
       }
       __syncthreads();
@@ -187,23 +194,39 @@ __global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __res
     __syncthreads();
   }
   if (lane == 0) {
-    const double lsh = a.log_shift ? a.log_shift[w] : 0.0;
-    cplx ph;
-    double lg;
-    if (a.nbeta > 0 && !a.rhf) {
-      ph = cmul(phase[0], phase[1]);
-      lg = logdet[0] + logdet[1];
-    } else if (a.nbeta > 0 && a.rhf) {
-      ph = cmul(phase[0], phase[0]);
-      lg = logdet[0] + logdet[0];
-    } else {
-      ph = phase[0];
-      lg = logdet[0];
-    }
-    const double e = exp(lg - lsh);
-    st(a.ovlp, w, cmake(ph.re * e, ph.im * e));
-    if (singular && a.flags) atomicOr(a.flags, AFQMC_FLAG_SINGULAR);
+    double* r = res + 4 * ((int64_t)w * 2 + s);
+    r[0] = logdet[s];
+    r[1] = phase[s].re;
+    r[2] = phase[s].im;
+    r[3] = singular ? 1.0 : 0.0;
   }
+}
+
+// ovlp[w] = phase_a phase_b exp(logdet_a + logdet_b - log_shift[w]) from the
+// per-spin results of gf_lu_kernel (RHF: the alpha factors squared).
+__global__ void __launch_bounds__(256) gf_combine_kernel(GfArgs a, const double* __restrict__ res) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= a.nwalkers) return;
+  const double* r = res + 8 * (int64_t)w;
+  const cplx pa = cmake(r[1], r[2]);
+  cplx ph;
+  double lg;
+  bool singular = r[3] != 0.0;
+  if (a.nbeta > 0 && !a.rhf) {
+    ph = cmul(pa, cmake(r[5], r[6]));
+    lg = r[0] + r[4];
+    singular = singular || r[7] != 0.0;
+  } else if (a.nbeta > 0 && a.rhf) {
+    ph = cmul(pa, pa);
+    lg = r[0] + r[0];
+  } else {
+    ph = pa;
+    lg = r[0];
+  }
+  const double lsh = a.log_shift ? a.log_shift[w] : 0.0;
+  const double e = exp(lg - lsh);
+  st(a.ovlp, w, cmake(ph.re * e, ph.im * e));
+  if (singular && a.flags) atomicOr(a.flags, AFQMC_FLAG_SINGULAR);
 }
 
 // Ghalf_s[w][i][m] = sum_j Oinv[w][s][i][j] phi_s[w][m][j]; wave = (w, s, row tile it).
@@ -258,9 +281,11 @@ size_t gf_lds_bytes(int nalpha, int nbeta) {
   return sizeof(cplx) * 2 * (size_t)nmax * (nmax + 1);
 }
 
+// O matrices [W, 2, nmax, nmax] complex, then the per-(walker, spin) LU results
+// [W, 2, 4] (log|det|, phase re, im, singular)
 size_t gf_work_bytes(int nwalkers, int nalpha, int nbeta) {
   const int nmax = nalpha > nbeta ? nalpha : nbeta;
-  return sizeof(cplx) * (size_t)nwalkers * 2 * nmax * nmax;
+  return sizeof(cplx) * (size_t)nwalkers * 2 * nmax * nmax + sizeof(double) * 8 * (size_t)nwalkers;
 }
 
 hipError_t launch_gf(const GfArgs& a, bool want_ghalf, double* obuf, double* oinv, hipStream_t stream) {
@@ -281,8 +306,12 @@ hipError_t launch_gf(const GfArgs& a, bool want_ghalf, double* obuf, double* oin
   const size_t lds = gf_lds_bytes(a.nalpha, a.nbeta);
   e = allow_lds((const void*)gf_lu_kernel, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(gf_lu_kernel, dim3(a.nwalkers), dim3(64), lds, stream, a, obuf, want_ghalf ? oinv : nullptr,
-                     nmax);
+  double* res = obuf + 2 * (int64_t)a.nwalkers * 2 * nmax * nmax;
+  hipLaunchKernelGGL(gf_lu_kernel, dim3((unsigned)(a.nwalkers * nspin)), dim3(64), lds, stream, a, obuf,
+                     want_ghalf ? oinv : nullptr, nmax, res);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(gf_combine_kernel, dim3((unsigned)ceil_div(a.nwalkers, 256)), dim3(256), 0, stream, a, res);
   e = hipGetLastError();
   if (e != hipSuccess || !want_ghalf) return e;
   switch (nt) {  // K steps of 4 cover nmax <= 16 nt
