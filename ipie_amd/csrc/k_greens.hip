@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __res
   extern __shared__ cplx smem[];
   const int lr = nmax + 1;  // LDS row stride
   cplx* LU = smem;             // [nmax][lr]
-  cplx* X = smem + nmax * lr;  // [nmax][lr], columns of the inverse
+  cplx* X = smem + nmax * lr;  // [nmax][lr], columns of the inverse (LDS only when oinv is set)
   __shared__ int piv[64];
   const int lane = threadIdx.x;
   const int nspin = gf_nspin(a);
@@ -303,7 +303,8 @@ hipError_t launch_gf(const GfArgs& a, bool want_ghalf, double* obuf, double* oin
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const size_t lds = gf_lds_bytes(a.nalpha, a.nbeta);
+  // the inverse's columns need a second matrix of LDS; the overlap alone does not
+  const size_t lds = gf_lds_bytes(a.nalpha, a.nbeta) / (want_ghalf ? 1 : 2);
   e = allow_lds((const void*)gf_lu_kernel, lds);
   if (e != hipSuccess) return e;
   double* res = obuf + 2 * (int64_t)a.nwalkers * 2 * nmax * nmax;
