@@ -392,6 +392,7 @@ int do_force_bias_bf16(afqmc_ctx* c, int nw, const double* ga, const double* gb)
     b.ksplit = BF16_FB_SPLIT;
     b.epilogue = BF16_EPI_PARTIAL;
     b.out = c->vbias_part;
+    b.out32 = reinterpret_cast<float*>(c->vbias_part);  // read back by fields_kernel as FP32 (part32)
     b.z_offset = s * BF16_FB_SPLIT;
     b.scale = (c->d.rhf && c->d.nbeta > 0) ? 2.0 : 1.0;
     HIP_TRY(c, launch_bf16_gemm(b, c->stream));
@@ -428,6 +429,7 @@ int do_fields(afqmc_ctx* c, int nw, bool want_fields, const double* xi, uint64_t
   a.nchol = c->d.nchol;
   a.ksplit = c->fb_ksplit;
   a.vbias_part = c->vbias_part;
+  a.part32 = c->bf16_fb ? 1 : 0;
   a.mf_shift = c->s.mf_shift;
   a.xi = xi;
   a.seed = seed;
@@ -672,6 +674,7 @@ int taylor_bf16(afqmc_ctx* c, int nw, const double** result, const uint16_t** re
     t.first = k == 1;
     t.last = k == c->d.exp_nmax;
     t.split = k <= TAYLOR_SPLIT_TERMS;
+    t.tout_hi_only = k + 1 > TAYLOR_SPLIT_TERMS;
     t.alpha = 1.0 / k;
     t.qout = k == c->d.exp_nmax ? c->tplanes[k & 1] : nullptr;
     HIP_TRY(c, launch_taylor_bf16(t, c->stream));

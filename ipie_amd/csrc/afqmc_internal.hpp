@@ -183,6 +183,7 @@ struct TaylorBf16Args {
   float* dacc;              // [W][N][ncol] complex FP32: sum of the terms so far
   int first, last;          // this term is the first / the last of the series
   int split;                // 1: three BF16 products (hi hi + hi lo + lo hi), 0: hi hi only
+  int tout_hi_only;         // 1: write the hi planes of out_k^T only (the next term uses one product)
   double alpha;
   uint16_t* qout;           // last term: planes of q^T (layout of tin), or null
 };
@@ -206,7 +207,8 @@ int taylor_rows_padded(int ncol);
 // ---------------------------------------------------------------- elementwise
 struct FieldsArgs {
   int nwalkers, nchol, ksplit;
-  const double* vbias_part;   // [ksplit, W, X] complex
+  const double* vbias_part;   // [ksplit, W, X] complex (complex FP32 when part32)
+  int part32;
   const double* mf_shift;     // [X] complex
   const double* xi;           // [W, X] real, or null -> Philox draw
   uint64_t seed, counter;

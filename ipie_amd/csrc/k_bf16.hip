@@ -414,7 +414,8 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
 // C = A B^T for split-BF16 operand planes (A: 4 planes, complex; B: 2 planes,
 // real), K chunk range of split z, 128 x 160 tile per workgroup; SPLIT = false
 // uses the hi planes only (1 product).  Epilogues:
-//   BF16_EPI_PARTIAL  out[z_offset + z][row][col] = scale C     (force bias)
+//   BF16_EPI_PARTIAL  out[z_offset + z][row][col] = scale C     (force bias; complex FP32
+//                     into out32 when set)
 //   BF16_EPI_VHS      out[w][i][j] = out[w][j][i] = i scale C[w][p],
 //                     (i, j) = (tri_i[p], tri_j[p])              (VHS); with
 //                     out32, complex FP32 and the upper triangle i <= j only
@@ -615,6 +616,9 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
                 st(base, (int64_t)ti * a.nbasis + tj, v);
                 if (ti != tj) st(base, (int64_t)tj * a.nbasis + ti, v);
               }
+            } else if (a.out32) {
+              reinterpret_cast<float2*>(a.out32)[((int64_t)(a.z_offset + z) * a.m + row) * a.nn + col] =
+                  make_float2((float)vr, (float)vi);
             } else {
               st(a.out, ((int64_t)(a.z_offset + z) * a.m + row) * a.nn + col, cmake(vr, vi));
             }
@@ -858,16 +862,18 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
       uint16_t* dst = pass == 0 ? a.tout : a.qout;
       if (!dst) continue;
       const uint16_t(&h)[4][4] = pass == 0 ? ht : hq;
+      // a next term with one product reads the hi planes (0 and 2) only
+      const int pstep = (pass == 0 && a.tout_hi_only) ? 2 : 1;
       if (kc0 + 3 < N) {
 #pragma unroll
-        for (int p = 0; p < 4; ++p)
+        for (int p = 0; p < 4; p += pstep)
           *reinterpret_cast<ushort4*>(dst + p * tplane + o) = make_ushort4(h[p][0], h[p][1], h[p][2], h[p][3]);
       } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           if (kc0 + r < N)
 #pragma unroll
-            for (int p = 0; p < 4; ++p) dst[p * tplane + o + r] = h[p][r];
+            for (int p = 0; p < 4; p += pstep) dst[p * tplane + o + r] = h[p][r];
       }
     }
   }

@@ -59,8 +59,18 @@ __global__ void __launch_bounds__(FLD_THREADS) fields_kernel(FieldsArgs a) {
   unsigned long long ncap = 0;
   for (int x = threadIdx.x; x < X; x += FLD_THREADS) {
     const int64_t idx = (int64_t)w * X + x;
-    cplx vb = ld(a.vbias_part, idx);
-    for (int z = 1; z < a.ksplit; ++z) vb = cadd(vb, ld(a.vbias_part, z * plane + idx));
+    cplx vb;
+    if (a.part32) {  // split-BF16 partials, FP32 (their accumulation precision), summed in FP64
+      const float2* p32 = reinterpret_cast<const float2*>(a.vbias_part);
+      vb = cmake(0.0, 0.0);
+      for (int z = 0; z < a.ksplit; ++z) {
+        const float2 v = p32[z * plane + idx];
+        vb = cadd(vb, cmake((double)v.x, (double)v.y));
+      }
+    } else {
+      vb = ld(a.vbias_part, idx);
+      for (int z = 1; z < a.ksplit; ++z) vb = cadd(vb, ld(a.vbias_part, z * plane + idx));
+    }
     if (a.vbias_out) st(a.vbias_out, idx, vb);
     ec_r += vb.re * vb.re - vb.im * vb.im;
     ec_i += 2.0 * vb.re * vb.im;
