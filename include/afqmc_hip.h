@@ -119,6 +119,32 @@ int afqmc_get_nfb(afqmc_ctx* ctx, uint64_t* nfb);
 #define AFQMC_EXCHANGE_QUAD 1
 int afqmc_exchange_form(const afqmc_ctx* ctx);
 
+/* Population control of the context's walkers on its stream, one rank
+ * (ipie PopController.pop_control, ipie/walkers/pop_controller.py:78-126, with
+ * pair_branch :304-402, comb :199-301, stochastic_reconfiguration :405-450 for
+ * comm.size == 1).  afqmc_pop_plan rescales weight [nw] to target_weight
+ * (total_weight <- sum |w|; nothing else happens when that is below 1e-8, the
+ * reference's ValueError, which the caller raises) and writes the method's
+ * source map src [nw] (slot i takes the buffer of walker src[i]) and the
+ * weights afterwards new_weight [nw] (may alias weight); npairs [1] counts the
+ * branched pairs.  uniforms: pair_branch >= nw / 2 of them, consumed one per
+ * branched pair in order; comb and reconfiguration one; null for rescale only.
+ * nw <= 4096.  All arrays are device memory; nothing synchronises. */
+#define AFQMC_POP_PAIR_BRANCH 0
+#define AFQMC_POP_COMB 1
+#define AFQMC_POP_RECONFIGURATION 2
+#define AFQMC_POP_RESCALE 3
+int afqmc_pop_plan(afqmc_ctx* ctx, int32_t method, int32_t nw, const double* weight, double target_weight,
+                   double min_weight, double max_weight, const double* uniforms, int32_t* src, double* new_weight,
+                   double* total_weight, int32_t* npairs);
+/* out row i <- in row src[i] for nw rows of bytes_per_walker bytes (a walker
+ * buffer field; ipie get_buffer / set_buffer, pop_controller.py:128-196).
+ * in == out is allowed when no source row is also a destination (pair_branch,
+ * comb); reconfiguration maps need a separate output.  nw <= 65535; a row
+ * whose src[i] lies outside [0, nw) is left untouched. */
+int afqmc_pop_gather(afqmc_ctx* ctx, int32_t nw, const int32_t* src, const void* in, void* out,
+                     int64_t bytes_per_walker);
+
 /* Ghalf_s = (phi_s^T psi_s^*)^{-1} phi_s^T and ovlp = prod_s det(phi_s^T psi_s^*) e^{-log_shift}.
  * phia [W, N, n_a], phib [W, N, n_b], Ghalfa [W, n_a, N], Ghalfb [W, n_b, N],
  * ovlp [W] complex; log_shift [W] real or NULL. */

@@ -92,7 +92,11 @@ SIGNATURES = {
     "afqmc_surrogate_stats": (ctypes.c_int, [_dp, ctypes.POINTER(_f64), ctypes.c_int]),
     "afqmc_profile_read": (ctypes.c_int, [_dp, ctypes.POINTER(_f64), ctypes.POINTER(_i64)]),
     "afqmc_exchange_form": (ctypes.c_int, [_dp]),
+    "afqmc_pop_plan": (ctypes.c_int, [_dp, _i32, _i32, _dp, _f64, _f64, _f64, _dp, _dp, _dp, _dp, _dp]),
+    "afqmc_pop_gather": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _i64]),
 }
+
+POP_METHODS = {"pair_branch": 0, "comb": 1, "stochastic_reconfiguration": 2, "rescale": 3}
 
 PHASES = ["gf", "one_body", "force_bias", "fields", "vhs", "taylor", "overlap", "weight", "exchange", "energy",
           "reortho", "exchange_cv", "split"]
@@ -360,6 +364,34 @@ class Context:
         if st != AFQMC_OK:
             raise AfqmcError(f"afqmc_profile_read: {self.lib.afqmc_last_error(self.h)}")
         return {PHASES[i]: (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+    def pop_plan(self, method, weight, target_weight, min_weight, max_weight, uniforms, src, new_weight,
+                 total_weight, npairs):
+        """Population-control decisions for the walkers (afqmc_pop_plan): rescaling
+        to target_weight, then src (int32 [W]: slot i takes walker src[i]) and
+        new_weight (float64 [W]); total_weight (float64 [1]) and npairs (int32
+        [1]) on the device."""
+        nw = weight.shape[0]
+        dev = self.device
+        _check_tensor("weight", weight, torch.float64, (nw,), dev)
+        _check_tensor("src", src, torch.int32, (nw,), dev)
+        _check_tensor("new_weight", new_weight, torch.float64, (nw,), dev)
+        _check_tensor("total_weight", total_weight, torch.float64, (1,), dev)
+        _check_tensor("npairs", npairs, torch.int32, (1,), dev)
+        if uniforms is not None:
+            _check_tensor("uniforms", uniforms, torch.float64, (uniforms.shape[0],), dev)
+        self._call("afqmc_pop_plan", POP_METHODS[method], nw, ptr(weight), float(target_weight), float(min_weight),
+                   float(max_weight), ptr(uniforms), ptr(src), ptr(new_weight), ptr(total_weight), ptr(npairs))
+
+    def pop_gather(self, src, inp, out):
+        """out[i] = inp[src[i]] along the walker (first) axis (afqmc_pop_gather)."""
+        nw = src.shape[0]
+        if inp.shape != out.shape or inp.dtype != out.dtype or inp.shape[0] != nw:
+            raise ValueError("pop_gather: input and output must have the same shape, walkers first")
+        for name, t in (("src", src), ("input", inp), ("output", out)):
+            if not t.is_contiguous() or t.device != self.device:
+                raise ValueError(f"{name} must be contiguous on {self.device}")
+        self._call("afqmc_pop_gather", nw, ptr(src), ptr(inp), ptr(out), inp.element_size() * inp[0].numel())
 
     def sync(self):
         st = self.lib.afqmc_sync(self.h)

@@ -21,6 +21,7 @@ SOURCES = [
     "k_surrogate.hip",
     "k_fields.hip",
     "k_qr.hip",
+    "k_pop.hip",
 ]
 
 HIPCC_FLAGS = [

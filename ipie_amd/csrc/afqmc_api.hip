@@ -1238,6 +1238,46 @@ int afqmc_get_nfb(afqmc_ctx* c, uint64_t* nfb) {
   return AFQMC_OK;
 }
 
+static_assert(AFQMC_POP_PAIR_BRANCH == POP_PAIR_BRANCH && AFQMC_POP_COMB == POP_COMB &&
+                  AFQMC_POP_RECONFIGURATION == POP_RECONFIGURATION && AFQMC_POP_RESCALE == POP_RESCALE,
+              "population control method codes");
+
+int afqmc_pop_plan(afqmc_ctx* c, int32_t method, int32_t nw, const double* weight, double target_weight,
+                   double min_weight, double max_weight, const double* uniforms, int32_t* src, double* new_weight,
+                   double* total_weight, int32_t* npairs) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (nw < 1 || nw > POP_MAX) return fail(c, AFQMC_ERR_SHAPE, "population control of %d walkers (1..%d)", nw, POP_MAX);
+  if (method < AFQMC_POP_PAIR_BRANCH || method > AFQMC_POP_RESCALE)
+    return fail(c, AFQMC_ERR_INVALID_ARG, "unknown population control method %d", method);
+  if (!weight || !src || !new_weight || !total_weight || !npairs || (!uniforms && method != AFQMC_POP_RESCALE))
+    return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  if (!(target_weight > 0.0)) return fail(c, AFQMC_ERR_INVALID_ARG, "target weight must be positive");
+  PopPlanArgs a;
+  a.nwalkers = nw;
+  a.method = method;
+  a.weight = weight;
+  a.target = target_weight;
+  a.min_weight = min_weight;
+  a.max_weight = max_weight;
+  a.uniforms = uniforms;
+  a.src = src;
+  a.new_weight = new_weight;
+  a.total = total_weight;
+  a.npairs = npairs;
+  HIP_TRY(c, launch_pop_plan(a, c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_pop_gather(afqmc_ctx* c, int32_t nw, const int32_t* src, const void* in, void* out,
+                     int64_t bytes_per_walker) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (nw < 1 || nw > 65535 || bytes_per_walker < 1)
+    return fail(c, AFQMC_ERR_SHAPE, "%d walker rows of %lld bytes (1..65535 rows)", nw, (long long)bytes_per_walker);
+  if (!src || !in || !out) return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  HIP_TRY(c, launch_pop_gather(nw, src, in, out, bytes_per_walker, c->stream));
+  return AFQMC_OK;
+}
+
 int afqmc_greens_function(afqmc_ctx* c, int32_t nw, const double* phia, const double* phib, double* ga,
                           double* gb, double* ovlp, const double* log_shift) {
   int r = check_ready(c, nw);

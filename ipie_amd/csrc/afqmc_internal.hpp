@@ -380,6 +380,24 @@ struct ReorthoArgs {
 };
 hipError_t launch_reortho(const ReorthoArgs& a, hipStream_t stream);
 
+// ---------------------------------------------------------------- population control
+constexpr int POP_MAX = 4096;  // walkers per device handled by one plan workgroup
+enum { POP_PAIR_BRANCH = 0, POP_COMB = 1, POP_RECONFIGURATION = 2, POP_RESCALE = 3 };
+struct PopPlanArgs {
+  int nwalkers, method;
+  const double* weight;     // [W] current weights
+  double target;            // target total weight (the walker count)
+  double min_weight, max_weight;
+  const double* uniforms;   // pair_branch: >= W/2 uniforms; comb, SR: 1 (null for POP_RESCALE)
+  int* src;                 // [W] out: slot i takes walker src[i]'s buffer
+  double* new_weight;       // [W] out: weights after the control
+  double* total;            // [1] out: sum |w| before rescaling
+  int* npairs;              // [1] out: branched pairs (pair_branch)
+};
+hipError_t launch_pop_plan(const PopPlanArgs& a, hipStream_t stream);
+hipError_t launch_pop_gather(int nw, const int* src, const void* in, void* out, int64_t row_bytes,
+                             hipStream_t stream);
+
 // Raise a kernel's dynamic-LDS limit when it needs more than the 64 KiB default
 // (gfx950 has 160 KiB per CU).
 inline hipError_t allow_lds(const void* kernel, size_t bytes) {
