@@ -145,6 +145,27 @@ int afqmc_pop_plan(afqmc_ctx* ctx, int32_t method, int32_t nw, const double* wei
 int afqmc_pop_gather(afqmc_ctx* ctx, int32_t nw, const int32_t* src, const void* in, void* out,
                      int64_t bytes_per_walker);
 
+/* Step graphs (no counterpart in the reference, whose steps are host-driven
+ * CuPy calls; for callers that replay a block of steps with one launch).
+ * afqmc_graph_begin opens a hipStreamBeginCapture on the context's stream
+ * (which must not be the null stream); every entry point called on this
+ * context until afqmc_graph_end is recorded, not run.  While recording, the
+ * field-stream counter and eshift are read from device memory, each recorded
+ * step advances the counter by one, and the control variate's subset draws
+ * and Adam steps also live in device memory, so a replay continues the
+ * trajectory.  afqmc_graph_end instantiates the graph (replacing the previous
+ * one) and returns the number of recorded steps.  afqmc_graph_launch sets the
+ * counter of the first replayed step and eshift, then replays the graph
+ * `replays` times on the context's stream: replay r, recorded step j uses
+ * counter + r * steps + j.  Results equal the eager calls with those counters.
+ * Calls that allocate workspace, synchronise or read back (afqmc_sync,
+ * afqmc_get_flags, afqmc_get_nfb, afqmc_profile_read, the surrogate's
+ * configure / stats and warm-up steps, afqmc_set_static) fail while a capture
+ * is open: run the recorded sequence once eagerly first. */
+int afqmc_graph_begin(afqmc_ctx* ctx);
+int afqmc_graph_end(afqmc_ctx* ctx, int32_t* steps);
+int afqmc_graph_launch(afqmc_ctx* ctx, int32_t replays, uint64_t counter, double eshift);
+
 /* Ghalf_s = (phi_s^T psi_s^*)^{-1} phi_s^T and ovlp = prod_s det(phi_s^T psi_s^*) e^{-log_shift}.
  * phia [W, N, n_a], phib [W, N, n_b], Ghalfa [W, n_a, N], Ghalfb [W, n_b, N],
  * ovlp [W] complex; log_shift [W] real or NULL. */

@@ -94,6 +94,9 @@ SIGNATURES = {
     "afqmc_exchange_form": (ctypes.c_int, [_dp]),
     "afqmc_pop_plan": (ctypes.c_int, [_dp, _i32, _i32, _dp, _f64, _f64, _f64, _dp, _dp, _dp, _dp, _dp]),
     "afqmc_pop_gather": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _i64]),
+    "afqmc_graph_begin": (ctypes.c_int, [_dp]),
+    "afqmc_graph_end": (ctypes.c_int, [_dp, ctypes.POINTER(_i32)]),
+    "afqmc_graph_launch": (ctypes.c_int, [_dp, _i32, _u64, _f64]),
 }
 
 POP_METHODS = {"pair_branch": 0, "comb": 1, "stochastic_reconfiguration": 2, "rescale": 3}
@@ -315,6 +318,22 @@ class Context:
             self._chk_vec("detR_shift", detR_shift, nw, torch.float64)
         self._call("afqmc_reortho", nw, ptr(phia), ptr(phib), ptr(ovlp), ptr(detR), ptr(log_detR),
                    ptr(detR_shift))
+
+    def graph_begin(self):
+        """Start recording this context's calls into a step graph (afqmc_graph_begin);
+        torch's current stream must not be the null stream."""
+        self._call("afqmc_graph_begin")
+
+    def graph_end(self):
+        """Close the recording and instantiate it; returns the recorded step count."""
+        n = _i32(0)
+        self._call("afqmc_graph_end", ctypes.byref(n))
+        return int(n.value)
+
+    def graph_launch(self, replays, counter, eshift):
+        """Replay the step graph `replays` times; the first replayed step draws the
+        field stream `counter`."""
+        self._call("afqmc_graph_launch", int(replays), int(counter), float(eshift))
 
     def draw_normals(self, out, seed, counter):
         _check_tensor("out", out, torch.float64, None, self.device)

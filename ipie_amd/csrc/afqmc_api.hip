@@ -82,9 +82,9 @@ struct afqmc_ctx {
     int warmup = 100, train_iters = 4, online_iters = 1;
     float lr = 1e-3f;
     int64_t steps = 0;
-    int adam_step = 0;
     int last_k = 0;
-    std::mt19937_64 rng;
+    uint64_t seed = 0;       // Philox key of the subset draws
+    std::mt19937_64 rng;     // initial MLP weights
     float* feat = nullptr;
     double* pred = nullptr;
     double* est = nullptr;
@@ -110,6 +110,13 @@ struct afqmc_ctx {
     float* loss = nullptr;
     SurrogateState* state = nullptr;
   } sur;
+  // step graphs: values that change per step live in dstate; while a capture is
+  // open the kernels read them from there
+  StepState* dstate = nullptr;
+  bool capturing = false;
+  int capture_steps = 0, capture_sur = 0;  // field draws / control-variate energies recorded
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0, graph_sur = 0;
   // per-phase HIP-event timing (afqmc_profile / afqmc_profile_read)
   bool prof = false;
   std::vector<hipEvent_t> ev_free;
@@ -153,7 +160,7 @@ struct PhaseScope {
   int phase;
   hipEvent_t start = nullptr;
   PhaseScope(afqmc_ctx* ctx, int ph) : c(ctx), phase(ph) {
-    if (c->prof) {
+    if (c->prof && !c->capturing) {
       start = take_event(c);
       if (start) (void)hipEventRecord(start, c->stream);
     }
@@ -181,9 +188,18 @@ bool two_spins(const afqmc_ctx* c) { return c->d.nbeta > 0 && !c->d.rhf; }
 double* wv(afqmc_ctx* c, int i) { return c->wvec + 2 * (int64_t)i * c->d.nwalkers; }
 enum { WV_CMF = 0, WV_CFB = 1, WV_ECOUL = 2, WV_OVLP_OLD = 3, WV_OVLP_NEW = 4 };
 
+// set while this thread holds an open step-graph capture: workspace cannot be
+// allocated then (run the captured calls once eagerly first)
+thread_local bool t_capturing = false;
+
 hipError_t dmalloc(void** p, size_t bytes) {
+  if (t_capturing) return hipErrorStreamCaptureUnsupported;
   if (bytes == 0) bytes = 256;
   return hipMalloc(p, bytes);
+}
+
+int refuse_in_capture(afqmc_ctx* c, const char* what) {
+  return fail(c, AFQMC_ERR_INVALID_ARG, "%s is not allowed while a step graph is being captured", what);
 }
 
 // Column units of the block-triangular quadratic-form GEMM with mb row blocks:
@@ -434,6 +450,7 @@ int do_fields(afqmc_ctx* c, int nw, bool want_fields, const double* xi, uint64_t
   a.xi = xi;
   a.seed = seed;
   a.counter = counter;
+  a.state = c->capturing ? c->dstate : nullptr;
   a.sqrt_dt = sqrt(c->s.dt);
   a.fbbound = 1.0;
   a.xi_out = xi_out;
@@ -887,30 +904,6 @@ int do_energy(afqmc_ctx* c, int nw, const double* ga, const double* gb, double* 
   return AFQMC_OK;
 }
 
-int64_t gcd64(int64_t a, int64_t b) {
-  while (b) {
-    const int64_t t = a % b;
-    a = b;
-    b = t;
-  }
-  return a;
-}
-
-int64_t inv_mod(int64_t a, int64_t m) {
-  int64_t g = m, x = 0, x1 = 1, a1 = a % m;
-  while (a1) {
-    const int64_t q = g / a1;
-    int64_t t = g - q * a1;
-    g = a1;
-    a1 = t;
-    t = x - q * x1;
-    x = x1;
-    x1 = t;
-  }
-  x %= m;
-  return x < 0 ? x + m : x;
-}
-
 // Local energy of nw walkers from their Ghalf with the learned exchange control
 // variate (requires the force bias + fields of this Green's function).
 // Trial Ghalf, K = sum_x T0_x L_x and E_T = sum_x tr(T0_x T0_x) per spin, in
@@ -989,6 +982,8 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
                         double* energy) {
   auto& S = c->sur;
   const bool warm = S.steps < S.warmup;
+  if (c->capturing && (warm || S.steps == 0))
+    return fail(c, AFQMC_ERR_INVALID_ARG, "the control variate's warm-up steps run eagerly, before a capture");
   // one-body energy for the features (the energy output is rewritten below)
   int r = do_energy(c, nw, ga, gb, energy, S.pred, S.e1);
   if (r) return r;
@@ -1030,28 +1025,22 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   cva.mlp = S.mlp;
   cva.state = S.state;
   HIP_TRY(c, launch_sur_cv(cva, c->stream));
-  // exact exchange on the sampled subset
+  // exact exchange on the sampled subset (permutation drawn on the device)
   int k = nw;
-  int64_t pa = 1, pb = 0;
   if (!warm) {
     k = (int)std::lround(S.fraction * nw);
     if (k < 1) k = 1;
     if (k > nw) k = nw;
-    std::uniform_int_distribution<int64_t> da(1, nw > 1 ? nw - 1 : 1), db(0, nw - 1);
-    do {
-      pa = da(S.rng);
-    } while (gcd64(pa, nw) != 1);
-    pb = db(S.rng);
   }
   S.last_k = k;
-  HIP_TRY(c, launch_sur_select(nw, k, inv_mod(pa, nw), pb, S.walker_of, c->stream));
+  HIP_TRY(c, launch_sur_plan(c->dstate, S.seed, nw, warm ? 1 : 0, c->stream));
+  HIP_TRY(c, launch_sur_select(nw, k, c->dstate, S.walker_of, c->stream));
   r = do_exchange(c, k, ga, gb, S.walker_of);
   if (r) return r;
   SurrogateCombineArgs cb;
   cb.nwalkers = nw;
   cb.k = k;
-  cb.perm_a = pa;
-  cb.perm_b = pb;
+  cb.perm = c->dstate;
   cb.pred = S.pred;
   cb.part_a = c->exx_a;
   cb.xs_a = c->xs_used[0];
@@ -1072,10 +1061,12 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   for (int it = 0; it < iters; ++it) {
     SurrogateTrainArgs ta{k, S.feat, S.walker_of, S.target, S.master, S.state, S.grad, S.loss};
     HIP_TRY(c, launch_sur_grad(ta, c->stream));
-    SurrogateAdamArgs ad{sur_grad_groups(k), ++S.adam_step, S.lr, S.grad, S.master, S.m, S.v, S.wbf, S.bias};
+    SurrogateAdamArgs ad{sur_grad_groups(k), it, c->dstate, S.lr, S.grad, S.master, S.m, S.v, S.wbf, S.bias};
     HIP_TRY(c, launch_sur_adam(ad, c->stream));
   }
+  if (iters > 0) HIP_TRY(c, launch_step_advance(c->dstate, 0, iters, c->stream));
   S.steps += 1;
+  if (c->capturing) c->capture_sur += 1;
   return do_energy(c, nw, ga, gb, energy, S.est, nullptr);
 }
 
@@ -1086,6 +1077,10 @@ int do_two_body_core(afqmc_ctx* c, int nw, const double* ga, const double* gb, c
   if (r) return r;
   r = do_fields(c, nw, true, xi, seed, counter, xi_out, vbias_out);
   if (r) return r;
+  if (c->capturing) {  // the next replayed step draws the next field stream
+    HIP_TRY(c, launch_step_advance(c->dstate, 1, 0, c->stream));
+    c->capture_steps += 1;
+  }
   if (energy_too) {
     if (c->sur.enabled && ovlp) {
       r = do_energy_surrogate(c, nw, ga, gb, ovlp, energy);
@@ -1152,12 +1147,14 @@ int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims) {
     if (c->bf16_fb) c->fb_ksplit = BF16_FB_SPLIT * (two ? 2 : 1);
   }
   if (dmalloc((void**)&c->flags, sizeof(int)) != hipSuccess ||
-      dmalloc((void**)&c->nfb, sizeof(unsigned long long)) != hipSuccess) {
+      dmalloc((void**)&c->nfb, sizeof(unsigned long long)) != hipSuccess ||
+      dmalloc((void**)&c->dstate, sizeof(StepState)) != hipSuccess) {
     afqmc_destroy(c);
     return AFQMC_ERR_ALLOC;
   }
   if (hipMemset(c->flags, 0, sizeof(int)) != hipSuccess ||
-      hipMemset(c->nfb, 0, sizeof(unsigned long long)) != hipSuccess) {
+      hipMemset(c->nfb, 0, sizeof(unsigned long long)) != hipSuccess ||
+      hipMemset(c->dstate, 0, sizeof(StepState)) != hipSuccess) {
     afqmc_destroy(c);
     return AFQMC_ERR_HIP;
   }
@@ -1177,9 +1174,10 @@ void afqmc_destroy(afqmc_ctx* c) {
                   c->qwp[1], c->qy, c->qunits[0], c->qunits[1], c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
                   c->sur.resid, c->sur.walker_of, c->sur.xbar, c->sur.e1, c->sur.master, c->sur.m,
                   c->sur.v, c->sur.wbf, c->sur.bias, c->sur.grad, c->sur.loss, c->sur.state, c->sur.gT,
-                  c->sur.K, c->sur.et, c->sur.cv, c->sur.mlp, c->sur.gtmp};
+                  c->sur.K, c->sur.et, c->sur.cv, c->sur.mlp, c->sur.gtmp, c->dstate};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
   for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
   for (const PendingEvent& pe : c->ev_pending) {
     (void)hipEventDestroy(pe.start);
@@ -1196,6 +1194,7 @@ int afqmc_set_stream(afqmc_ctx* c, void* stream) {
 
 int afqmc_set_static(afqmc_ctx* c, const afqmc_static* s) {
   if (!c || !s) return AFQMC_ERR_INVALID_ARG;
+  if (c->capturing) return refuse_in_capture(c, "afqmc_set_static");
   const bool two = c->d.nbeta > 0 && !c->d.rhf;
   // The trial orbitals are always required; the Hamiltonian tensors only by the
   // entry points that use them (checked there).
@@ -1215,12 +1214,14 @@ int afqmc_exchange_form(const afqmc_ctx* c) { return (c && c->quad_on) ? AFQMC_E
 
 int afqmc_sync(afqmc_ctx* c) {
   if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (c->capturing) return refuse_in_capture(c, "afqmc_sync");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return AFQMC_OK;
 }
 
 int afqmc_get_flags(afqmc_ctx* c, int32_t* flags, int clear) {
   if (!c || !flags) return AFQMC_ERR_INVALID_ARG;
+  if (c->capturing) return refuse_in_capture(c, "afqmc_get_flags");
   int h = 0;
   HIP_TRY(c, hipMemcpyAsync(&h, c->flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -1231,6 +1232,7 @@ int afqmc_get_flags(afqmc_ctx* c, int32_t* flags, int clear) {
 
 int afqmc_get_nfb(afqmc_ctx* c, uint64_t* nfb) {
   if (!c || !nfb) return AFQMC_ERR_INVALID_ARG;
+  if (c->capturing) return refuse_in_capture(c, "afqmc_get_nfb");
   unsigned long long h = 0;
   HIP_TRY(c, hipMemcpyAsync(&h, c->nfb, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -1354,6 +1356,7 @@ int afqmc_update_weight(afqmc_ctx* c, int32_t nw, const double* ovlp, const doub
   a.cmf = cmf;
   a.dt = c->s.dt;
   a.eshift = eshift;
+  a.state = c->capturing ? c->dstate : nullptr;
   a.ebound = sqrt(2.0 / c->s.dt);
   a.weight = weight;
   a.hybrid_energy = hybrid_energy;
@@ -1431,6 +1434,61 @@ int afqmc_reortho(afqmc_ctx* c, int32_t nw, double* phia, double* phib, double* 
   return AFQMC_OK;
 }
 
+int afqmc_graph_begin(afqmc_ctx* c) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (c->capturing) return fail(c, AFQMC_ERR_INVALID_ARG, "a step-graph capture is already open");
+  if (!c->stream)
+    return fail(c, AFQMC_ERR_INVALID_ARG, "a step graph is captured on a stream other than the null stream");
+  if (t_capturing) return fail(c, AFQMC_ERR_INVALID_ARG, "this thread already captures another context");
+  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  t_capturing = true;
+  c->capture_steps = 0;
+  c->capture_sur = 0;
+  return AFQMC_OK;
+}
+
+int afqmc_graph_end(afqmc_ctx* c, int32_t* steps) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (!c->capturing) return fail(c, AFQMC_ERR_INVALID_ARG, "no step-graph capture is open");
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->capturing = false;
+  t_capturing = false;
+  // the recorded control-variate energies did not run
+  c->sur.steps -= c->capture_sur;
+  if (e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(c, AFQMC_ERR_HIP, "step-graph capture failed: %s (run the captured calls once before capturing, so "
+                "that every workspace exists)", hipGetErrorString(e));
+  }
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->graph = nullptr;
+    return fail(c, AFQMC_ERR_HIP, "step-graph instantiation failed: %s", hipGetErrorString(e));
+  }
+  c->graph_steps = c->capture_steps;
+  c->graph_sur = c->capture_sur;
+  if (steps) *steps = c->graph_steps;
+  return AFQMC_OK;
+}
+
+int afqmc_graph_launch(afqmc_ctx* c, int32_t replays, uint64_t counter, double eshift) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (c->capturing) return refuse_in_capture(c, "afqmc_graph_launch");
+  if (!c->graph) return fail(c, AFQMC_ERR_INVALID_ARG, "no step graph has been captured");
+  if (replays < 0) return fail(c, AFQMC_ERR_INVALID_ARG, "replays must be >= 0");
+  HIP_TRY(c, launch_step_set(c->dstate, counter, eshift, c->stream));
+  for (int i = 0; i < replays; ++i) HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
+  c->sur.steps += (int64_t)replays * c->graph_sur;
+  return AFQMC_OK;
+}
+
 int afqmc_draw_normals(afqmc_ctx* c, double* out, int64_t n, uint64_t seed, uint64_t counter) {
   if (!c || !out || n < 0) return AFQMC_ERR_INVALID_ARG;
   HIP_TRY(c, launch_draw_normals(out, n, seed, counter, c->stream));
@@ -1449,6 +1507,7 @@ int afqmc_profile(afqmc_ctx* c, int enable) {
 
 int afqmc_profile_read(afqmc_ctx* c, double* ms, int64_t* launches) {
   if (!c || !ms || !launches) return AFQMC_ERR_INVALID_ARG;
+  if (c->capturing) return refuse_in_capture(c, "afqmc_profile_read");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   for (const PendingEvent& pe : c->ev_pending) {
     float t = 0.0f;
@@ -1469,6 +1528,7 @@ int afqmc_profile_read(afqmc_ctx* c, double* ms, int64_t* launches) {
 int afqmc_surrogate_configure(afqmc_ctx* c, int enable, double fraction, int warmup_steps, int train_iters,
                               double lr, uint64_t seed) {
   if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (c->capturing) return refuse_in_capture(c, "afqmc_surrogate_configure");
   if (!enable) {
     c->sur.enabled = false;
     return AFQMC_OK;
@@ -1520,7 +1580,7 @@ int afqmc_surrogate_configure(afqmc_ctx* c, int enable, double fraction, int war
   S.train_iters = train_iters;
   S.lr = (float)lr;
   S.steps = 0;
-  S.adam_step = 0;
+  S.seed = seed;
   S.rng.seed(seed);
   // weights ~ N(0, 1/fan_in), biases 0 (host RNG, then upload)
   std::vector<float> P(SUR_NPARAM, 0.0f);
@@ -1542,12 +1602,14 @@ int afqmc_surrogate_configure(afqmc_ctx* c, int enable, double fraction, int war
   HIP_TRY(c, hipMemsetAsync(S.m, 0, sizeof(float) * SUR_NPARAM, c->stream));
   HIP_TRY(c, hipMemsetAsync(S.v, 0, sizeof(float) * SUR_NPARAM, c->stream));
   HIP_TRY(c, hipMemsetAsync(S.state, 0, sizeof(SurrogateState), c->stream));
+  HIP_TRY(c, hipMemsetAsync(c->dstate, 0, sizeof(StepState), c->stream));  // subset draws, Adam steps
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return AFQMC_OK;
 }
 
 int afqmc_surrogate_stats(afqmc_ctx* c, double* out, int n) {
   if (!c || !out || n < 6) return AFQMC_ERR_INVALID_ARG;
+  if (c->capturing) return refuse_in_capture(c, "afqmc_surrogate_stats");
   auto& S = c->sur;
   out[0] = (double)S.steps;
   out[1] = (double)S.warmup;

@@ -11,6 +11,37 @@ enum : int {
   AFQMC_FLAG_CHOLESKY = 4,    // a Gram matrix was not positive definite in reortho
 };
 
+// Values that change from step to step, kept in device memory so that a
+// captured step graph reads them when it is replayed (the arguments of a
+// captured kernel are fixed at capture).  Kernels read them only when handed
+// a non-null pointer; eager calls pass their values as arguments instead.
+struct StepState {
+  unsigned long long counter;   // field-stream counter of the next step
+  double eshift;
+  unsigned long long sur_draws; // control-variate subset draws so far
+  long long adam_t;             // Adam steps taken by the control variate's MLP
+  long long perm_a, perm_b, inv_a;  // current subset: sample position of walker w is (a w + b) mod W
+};
+// state->counter = counter, state->eshift = eshift
+hipError_t launch_step_set(StepState* state, unsigned long long counter, double eshift, hipStream_t stream);
+// state->counter += dcounter, state->adam_t += dadam
+hipError_t launch_step_advance(StepState* state, long long dcounter, long long dadam, hipStream_t stream);
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 4 x 32 random bits per
+// (counter, key).
+__device__ __forceinline__ uint4 philox4x32(uint4 ctr, uint2 key) {
+  constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(M0, ctr.x), lo0 = M0 * ctr.x;
+    const uint32_t hi1 = __umulhi(M1, ctr.z), lo1 = M1 * ctr.z;
+    ctr = make_uint4(hi1 ^ ctr.y ^ key.x, lo1, hi0 ^ ctr.w ^ key.y, lo0);
+    key.x += W0;
+    key.y += W1;
+  }
+  return ctr;
+}
+
 // ---------------------------------------------------------------- GF / overlap
 struct GfArgs {
   int nwalkers, nbasis, nalpha, nbeta, rhf;
@@ -212,6 +243,7 @@ struct FieldsArgs {
   const double* mf_shift;     // [X] complex
   const double* xi;           // [W, X] real, or null -> Philox draw
   uint64_t seed, counter;
+  const StepState* state;     // non-null: the counter comes from state->counter
   double sqrt_dt, fbbound;
   double* xi_out;             // optional: the fields used, [W, X]
   double* xshifted;           // [W, X] complex
@@ -233,6 +265,7 @@ struct WeightArgs {
   const double* cfb;
   const double* cmf;
   double dt, eshift, ebound;
+  const StepState* state;  // non-null: eshift comes from state->eshift
   double* weight;          // [W] real
   double* hybrid_energy;   // [W] complex (in: previous, out: new)
   double* walker_ovlp;     // [W] complex (out: ovlp_new)
@@ -308,7 +341,8 @@ struct SurrogateTrainArgs {
   float* loss;             // [groups]
 };
 struct SurrogateAdamArgs {
-  int ngroups, step;
+  int ngroups, it;         // Adam step number: state->adam_t + it + 1
+  const StepState* state;
   float lr;
   const float* grad;
   float* master;
@@ -319,7 +353,7 @@ struct SurrogateAdamArgs {
 };
 struct SurrogateCombineArgs {
   int nwalkers, k;
-  int64_t perm_a, perm_b;  // sample position of walker w: (a w + b) mod W
+  const StepState* perm;   // sample position of walker w: (perm_a w + perm_b) mod W
   const double* pred;      // [W] complex
   const double* part_a;    // [xs_a, k] complex
   int xs_a;
@@ -363,7 +397,11 @@ int sur_grad_groups(int count);
 hipError_t launch_sur_grad(const SurrogateTrainArgs& a, hipStream_t s);
 hipError_t launch_sur_adam(const SurrogateAdamArgs& a, hipStream_t s);
 hipError_t launch_sur_combine(const SurrogateCombineArgs& a, hipStream_t s);
-hipError_t launch_sur_select(int nwalkers, int k, int64_t inv_a, int64_t b, int* walker_of, hipStream_t s);
+hipError_t launch_sur_select(int nwalkers, int k, const StepState* perm, int* walker_of, hipStream_t s);
+// draws the step's subset permutation into state (perm_a coprime with W, perm_b
+// uniform in [0, W)) from Philox(seed, state->sur_draws), then sur_draws += 1;
+// identity != 0 sets the identity (a = 1, b = 0) without a draw
+hipError_t launch_sur_plan(StepState* state, uint64_t seed, int nwalkers, int identity, hipStream_t s);
 hipError_t launch_sur_gather(const double* g, const int* walker_of, int k, int64_t per_walker, double* out,
                              hipStream_t s);
 

@@ -15,21 +15,7 @@ namespace {
 
 constexpr int FLD_THREADS = 256;
 
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), key = seed, counter = (index, stream).
-__device__ __forceinline__ uint4 philox4x32(uint4 ctr, uint2 key) {
-  constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const uint32_t hi0 = __umulhi(M0, ctr.x), lo0 = M0 * ctr.x;
-    const uint32_t hi1 = __umulhi(M1, ctr.z), lo1 = M1 * ctr.z;
-    ctr = make_uint4(hi1 ^ ctr.y ^ key.x, lo1, hi0 ^ ctr.w ^ key.y, lo0);
-    key.x += W0;
-    key.y += W1;
-  }
-  return ctr;
-}
-
-// Two standard normals for pair index p of stream `counter`.
+// Two standard normals (Philox key = seed, counter = (pair index, stream)) for pair index p of stream `counter`.
 __device__ __forceinline__ double2 normal_pair(uint64_t seed, uint64_t counter, uint64_t p) {
   const uint4 c = make_uint4((uint32_t)p, (uint32_t)(p >> 32), (uint32_t)counter, (uint32_t)(counter >> 32));
   const uint4 r = philox4x32(c, make_uint2((uint32_t)seed, (uint32_t)(seed >> 32)));
@@ -57,6 +43,7 @@ __global__ void __launch_bounds__(FLD_THREADS) fields_kernel(FieldsArgs a) {
   double cmf_r = 0, cmf_i = 0, cfb_r = 0, cfb_i = 0, ec_r = 0, ec_i = 0;
   double xb_sum = 0.0, xb_max = 0.0;
   unsigned long long ncap = 0;
+  const uint64_t counter = a.state ? a.state->counter : a.counter;
   for (int x = threadIdx.x; x < X; x += FLD_THREADS) {
     const int64_t idx = (int64_t)w * X + x;
     cplx vb;
@@ -86,7 +73,7 @@ __global__ void __launch_bounds__(FLD_THREADS) fields_kernel(FieldsArgs a) {
     xb_sum += axc;
     xb_max = fmax(xb_max, axc);
     if (!a.want_fields) continue;
-    const double xi = a.xi ? a.xi[idx] : draw_normal(a.seed, a.counter, (uint64_t)idx);
+    const double xi = a.xi ? a.xi[idx] : draw_normal(a.seed, counter, (uint64_t)idx);
     if (a.xi_out) a.xi_out[idx] = xi;
     const cplx xs = cmake(xi - xb.re, -xb.im);
     st(a.xshifted, idx, xs);
@@ -134,13 +121,14 @@ __global__ void update_weight_kernel(WeightArgs a) {
   const cplx ratio = cdiv(o_new, o_old);
   const cplx lr = clog(ratio);
   const cplx cfb = ld(a.cfb, w), cmf = ld(a.cmf, w);
+  const double eshift = a.state ? a.state->eshift : a.eshift;
   cplx eh = cmake(-(lr.re + cfb.re + cmf.re) / a.dt, -(lr.im + cfb.im + cmf.im) / a.dt);
-  if (fabs(a.eshift) >= 1e-10) {
-    const double emax = a.eshift + a.ebound, emin = a.eshift - a.ebound;
+  if (fabs(eshift) >= 1e-10) {
+    const double emax = eshift + a.ebound, emin = eshift - a.ebound;
     eh.re = fmin(fmax(eh.re, emin), emax);
   }
   const cplx prev = ld(a.hybrid_energy, w);
-  const cplx arg = cmake(-a.dt * (0.5 * (eh.re + prev.re) - a.eshift), -a.dt * 0.5 * (eh.im + prev.im));
+  const cplx arg = cmake(-a.dt * (0.5 * (eh.re + prev.re) - eshift), -a.dt * 0.5 * (eh.im + prev.im));
   const double magn = exp(arg.re);  // |exp(arg)|
   const double dtheta = -a.dt * eh.im - cfb.im;
   double cosf = cos(dtheta);
@@ -192,6 +180,20 @@ __global__ void __launch_bounds__(FLD_THREADS) energy_kernel(EnergyArgs a) {
   }
 }
 
+__global__ void step_set_kernel(StepState* s, unsigned long long counter, double eshift) {
+  if (threadIdx.x == 0) {
+    s->counter = counter;
+    s->eshift = eshift;
+  }
+}
+
+__global__ void step_advance_kernel(StepState* s, long long dcounter, long long dadam) {
+  if (threadIdx.x == 0) {
+    s->counter += (unsigned long long)dcounter;
+    s->adam_t += dadam;
+  }
+}
+
 __global__ void normals_kernel(double* out, int64_t n, uint64_t seed, uint64_t counter) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (2 * p >= n) return;
@@ -214,6 +216,16 @@ hipError_t launch_update_weight(const WeightArgs& a, hipStream_t stream) {
 
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t stream) {
   hipLaunchKernelGGL(energy_kernel, dim3(a.nwalkers), dim3(FLD_THREADS), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_set(StepState* state, unsigned long long counter, double eshift, hipStream_t stream) {
+  hipLaunchKernelGGL(step_set_kernel, dim3(1), dim3(64), 0, stream, state, counter, eshift);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_advance(StepState* state, long long dcounter, long long dadam, hipStream_t stream) {
+  hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(64), 0, stream, state, dcounter, dadam);
   return hipGetLastError();
 }
 

@@ -277,8 +277,9 @@ __global__ void sur_adam_kernel(SurrogateAdamArgs a) {
   const float v = b2 * a.v[p] + (1.0f - b2) * g * g;
   a.m[p] = m;
   a.v[p] = v;
-  const float mh = m / (1.0f - powf(b1, (float)a.step));
-  const float vh = v / (1.0f - powf(b2, (float)a.step));
+  const float step = (float)(a.state->adam_t + a.it + 1);
+  const float mh = m / (1.0f - powf(b1, step));
+  const float vh = v / (1.0f - powf(b2, step));
   const float w = a.master[p] - a.lr * mh / (sqrtf(vh) + eps);
   a.master[p] = w;
   constexpr int NW = SG_H * SG_F + SG_H * SG_H + 2 * SG_H;  // weights, then biases
@@ -292,7 +293,7 @@ __global__ void sur_combine_kernel(SurrogateCombineArgs a) {
   if (w >= a.nwalkers) return;
   const cplx g = ld(a.pred, w);
   cplx est = g;
-  const int64_t pos = ((int64_t)a.perm_a * w + a.perm_b) % a.nwalkers;
+  const int64_t pos = ((int64_t)a.perm->perm_a * w + a.perm->perm_b) % a.nwalkers;
   if (pos < a.k) {
     const cplx ex = exact_sum(a, (int)pos);
     const double s = (double)a.nwalkers / (double)a.k;
@@ -311,12 +312,64 @@ __global__ void sur_combine_kernel(SurrogateCombineArgs a) {
 }
 
 // walker index of sample t: w = a^{-1} (t - b) mod W
-__global__ void sur_select_kernel(int nwalkers, int k, int64_t inv_a, int64_t b, int* walker_of) {
+__global__ void sur_select_kernel(int nwalkers, int k, const StepState* perm, int* walker_of) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= k) return;
-  int64_t v = ((int64_t)t - b) % nwalkers;
+  int64_t v = ((int64_t)t - perm->perm_b) % nwalkers;
   if (v < 0) v += nwalkers;
-  walker_of[t] = (int)((inv_a * v) % nwalkers);
+  walker_of[t] = (int)((perm->inv_a * v) % nwalkers);
+}
+
+__device__ int64_t gcd_dev(int64_t a, int64_t b) {
+  while (b) {
+    const int64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// a^{-1} mod m for gcd(a, m) = 1 (extended Euclid)
+__device__ int64_t inv_mod_dev(int64_t a, int64_t m) {
+  int64_t g = m, x = 0, x1 = 1, a1 = a % m;
+  while (a1) {
+    const int64_t q = g / a1;
+    int64_t t = g - q * a1;
+    g = a1;
+    a1 = t;
+    t = x - q * x1;
+    x = x1;
+    x1 = t;
+  }
+  x %= m;
+  return x < 0 ? x + m : x;
+}
+
+// The step's subset permutation.  The inclusion probability k / W of every
+// walker needs only perm_b uniform on [0, W); perm_a (coprime with W) varies
+// which walkers are sampled together.  Up to 64 Philox draws for perm_a, then
+// perm_a = 1.
+__global__ void sur_plan_kernel(StepState* s, uint64_t seed, int nwalkers, int identity) {
+  if (threadIdx.x != 0) return;
+  int64_t pa = 1, pb = 0;
+  if (!identity && nwalkers > 1) {
+    const unsigned long long d = s->sur_draws;
+    const uint2 key = make_uint2((uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint4 rb = philox4x32(make_uint4(0u, 0x5EEDu, (uint32_t)d, (uint32_t)(d >> 32)), key);
+    pb = (int64_t)((((uint64_t)rb.x << 32) | rb.y) % (uint64_t)nwalkers);
+    for (uint32_t j = 1; j <= 64; ++j) {
+      const uint4 r = philox4x32(make_uint4(j, 0x5EEDu, (uint32_t)d, (uint32_t)(d >> 32)), key);
+      const int64_t cand = 1 + (int64_t)((((uint64_t)r.x << 32) | r.y) % (uint64_t)(nwalkers - 1));
+      if (gcd_dev(cand, nwalkers) == 1) {
+        pa = cand;
+        break;
+      }
+    }
+    s->sur_draws = d + 1;
+  }
+  s->perm_a = pa;
+  s->perm_b = pb;
+  s->inv_a = nwalkers > 1 ? inv_mod_dev(pa, nwalkers) : 0;
 }
 
 // Ghalf rows of the sampled walkers, compacted: out[t] = g[walker_of[t]] ([n, N] complex each).
@@ -493,9 +546,13 @@ hipError_t launch_sur_combine(const SurrogateCombineArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(sur_combine_kernel, dim3((unsigned)ceil_div(a.nwalkers, 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
-hipError_t launch_sur_select(int nwalkers, int k, int64_t inv_a, int64_t b, int* walker_of, hipStream_t s) {
-  hipLaunchKernelGGL(sur_select_kernel, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, s, nwalkers, k, inv_a, b,
+hipError_t launch_sur_select(int nwalkers, int k, const StepState* perm, int* walker_of, hipStream_t s) {
+  hipLaunchKernelGGL(sur_select_kernel, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, s, nwalkers, k, perm,
                      walker_of);
+  return hipGetLastError();
+}
+hipError_t launch_sur_plan(StepState* state, uint64_t seed, int nwalkers, int identity, hipStream_t s) {
+  hipLaunchKernelGGL(sur_plan_kernel, dim3(1), dim3(64), 0, s, state, seed, nwalkers, identity);
   return hipGetLastError();
 }
 hipError_t launch_sur_gather(const double* g, const int* walker_of, int k, int64_t per_walker, double* out,

@@ -155,6 +155,55 @@ class PhaselessGeneric:
             torch.cuda.synchronize(walkers.device)
             self.timer.tgemm += time.time() - t0
 
+    # ------------------------------------------------------------ step graphs
+    def capture_steps(self, walkers, hamiltonian, trial, nsteps, reortho_every=0, energy_prev=None):
+        """Record `nsteps` propagate_walkers calls into one hipGraph (and
+        walkers.reortho() after every `reortho_every`-th of them); nothing runs.
+        The same calls must have run eagerly before (workspace, the control
+        variate's warm-up).  Fields come from the device generator: a caller
+        that injects them (xi_source) cannot capture.  Returns the number of
+        recorded steps."""
+        if self.xi_source is not None:
+            raise RuntimeError("a step graph draws its fields on the device; unset xi_source to capture")
+        eng = self._eng(walkers)
+        dev = walkers.device
+        if getattr(self, "_graph_stream", None) is None:
+            self._graph_stream = torch.cuda.Stream(device=dev)
+        s = self._graph_stream
+        s.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(s):
+            eng.ctx.graph_begin()
+            try:
+                for j in range(nsteps):
+                    eng.ctx.propagate(
+                        walkers.phia, walkers.phib, walkers.Ghalfa, walkers.Ghalfb, walkers.weight, walkers.ovlp,
+                        walkers.hybrid_energy, 0.0, xi=None, seed=self.seed, counter=0,
+                        log_shift=walkers.log_shift, energy_prev=energy_prev,
+                    )
+                    if reortho_every and (j + 1) % reortho_every == 0:
+                        walkers.reortho()
+            except BaseException:
+                try:
+                    eng.ctx.graph_end()
+                except Exception:
+                    pass
+                raise
+            steps = eng.ctx.graph_end()
+        self._graph_steps = steps
+        return steps
+
+    def replay_steps(self, walkers, eshift, replays=1):
+        """Replay the captured steps `replays` times with the given eshift; the
+        fields continue the propagator's counter, as eager steps would."""
+        eng = self._eng(walkers)
+        dev = walkers.device
+        s = self._graph_stream
+        s.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(s):
+            eng.ctx.graph_launch(replays, self.counter + 1, float(numpy.real(eshift)))
+        torch.cuda.current_stream(dev).wait_stream(s)
+        self.counter += replays * self._graph_steps
+
     def propagate_walkers_one_body(self, walkers):
         """phi_s <- expH1[s] phi_s (phaseless_base.py:274-280)."""
         t0 = time.time()
