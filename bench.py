@@ -106,9 +106,58 @@ def build_device(host, w, mode):
     return dict(host, walkers=walkers, prop=prop)
 
 
-def run_gpu(st, w, steps, warmup):
-    """Warm-up, then `steps` timed steps; returns (seconds, per-phase profile,
-    per-step weighted energies (numpy, complex), finite flag)."""
+def run_graph(st, w, steps, one_step, counter, energy, eshift):
+    """`steps` steps replayed from one step graph: a block of REORTHO_EVERY
+    propagate calls (each followed by the device sums of the weighted local
+    energy of the incoming walkers) and a reortho, recorded once and launched
+    steps / REORTHO_EVERY times.  Returns (seconds, per-step weighted energies)."""
+    import torch
+
+    walkers, prop = st["walkers"], st["prop"]
+    ctx = walkers._engine.ctx
+    dev = walkers.device
+    while counter[0] % REORTHO_EVERY:  # blocks start right after a reortho
+        one_step()
+    nb = max(steps // REORTHO_EVERY, 1)
+    acc = torch.zeros(REORTHO_EVERY, 3, dtype=torch.complex128, device=dev)
+    series = torch.zeros(nb, REORTHO_EVERY, 3, dtype=torch.complex128, device=dev)
+    wprev = torch.empty_like(walkers.weight)
+    s = torch.cuda.Stream(device=dev)
+    s.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(s):
+        ctx.graph_begin()
+        for j in range(REORTHO_EVERY):
+            wprev.copy_(walkers.weight)
+            ctx.propagate(walkers.phia, walkers.phib, walkers.Ghalfa, walkers.Ghalfb, walkers.weight, walkers.ovlp,
+                          walkers.hybrid_energy, eshift, seed=prop.seed, counter=0, log_shift=walkers.log_shift,
+                          energy_prev=energy)
+            ctx.walker_sums(wprev, acc[j], values=energy)
+        walkers.reortho()
+        nrec = ctx.graph_end()
+    assert nrec == REORTHO_EVERY, nrec
+    torch.cuda.synchronize()
+    log(f"[bench] step graph recorded ({nrec} steps + reortho); {nb} launches")
+    t0 = time.perf_counter()
+    with torch.cuda.stream(s):
+        for r in range(nb):
+            ctx.graph_launch(1, prop.counter + 1 + r * REORTHO_EVERY, eshift)
+            series[r].copy_(acc)
+            acc.zero_()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prop.counter += nb * REORTHO_EVERY
+    counter[0] += nb * REORTHO_EVERY
+    ser = series.cpu().numpy().reshape(-1, 3)
+    e_steps = ser[:, 2] / ser[:, 0].real
+    log(f"[bench] graph: {nb * REORTHO_EVERY} steps in {elapsed:.3f}s; block E={numpy.mean(e_steps.real):.6f}")
+    return elapsed, e_steps, nb * REORTHO_EVERY
+
+
+def run_gpu(st, w, steps, warmup, graph=True):
+    """Warm-up, then `steps` eager steps timed with per-phase HIP events, then
+    (graph=True) the same number of steps replayed from a step graph.  Returns
+    (eager seconds, per-phase profile, eager per-step weighted energies, finite
+    flag, graph result (seconds, energies, steps) or None)."""
     import torch
 
     walkers, prop, ham, trial = st["walkers"], st["prop"], st["ham"], st["trial"]
@@ -148,10 +197,13 @@ def run_gpu(st, w, steps, warmup):
     prof = ctx.profile_read()
     ctx.profile(False)
     e_steps = (numer / denom).cpu().numpy()
+    log(f"[bench] eager: {steps} steps in {elapsed:.3f}s; block E={numpy.mean(e_steps.real):.6f}")
+    g = run_graph(st, w, steps, one_step, counter, energy, eshift) if graph else None
     wts = walkers.weight.cpu().numpy()
     finite = bool(numpy.all(numpy.isfinite(e_steps)) and numpy.all(numpy.isfinite(wts)))
-    log(f"[bench] timed {steps} steps in {elapsed:.3f}s; block E={numpy.mean(e_steps.real):.6f} finite={finite}")
-    return elapsed, prof, e_steps, finite
+    if g is not None:
+        finite = finite and bool(numpy.all(numpy.isfinite(g[1])))
+    return elapsed, prof, e_steps, finite, g
 
 
 def cpu_baseline(st, w, sample_walkers, sample_steps):
@@ -222,6 +274,9 @@ def main():
     ap.add_argument("--modes", default="fp64,mixed,mixed+surrogate",
                     help="comma list of fp64, mixed, mixed+surrogate; the headline value is the fastest mode "
                     "whose block energy is within 0.1 mHa of the FP64 mode on the same trajectory")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eager steps only (default: the value is the step-graph replay, the per-phase "
+                         "profile and the roofline kernel time come from the eager pass before it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-walkers", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -269,7 +324,7 @@ def main():
     modes = {}
     for mode in mode_list:
         st = build_device(host, w, mode)
-        elapsed, prof, e_steps, finite = run_gpu(st, w, K, warmup)
+        elapsed, prof, e_steps, finite, g = run_gpu(st, w, K, warmup, graph=not args.no_graph)
         ex_ms, ex_n = prof["exchange"]
         form = st["walkers"]._engine.ctx.exchange_form()
         if form == "quad" and mode != "fp64":
@@ -282,6 +337,7 @@ def main():
         rec = dict(
             value=w.nwalkers * K / elapsed,
             ms_per_step=1e3 * elapsed / K,
+            timed="eager",
             exchange_avg_launch_ms=ex_avg_s * 1e3,
             exchange_launches=ex_n,
             phases_ms_per_step={k: round(v[0] / K, 3) for k, v in prof.items()},
@@ -289,6 +345,12 @@ def main():
             finite=finite,
             _e_steps=e_steps,
         )
+        if g is not None:
+            g_s, g_e, g_k = g
+            rec["eager_value"] = rec["value"]
+            rec["eager_ms_per_step"] = rec["ms_per_step"]
+            rec.update(value=w.nwalkers * g_k / g_s, ms_per_step=1e3 * g_s / g_k, timed="step graph",
+                       graph_steps=g_k, block_energy=float(numpy.mean(g_e.real)), _e_steps=g_e)
         rec["exchange_form"] = form
         # the same launch in ipie's form (every T_x formed): the work it replaces
         rec["exchange_flops_reference_form_per_launch"] = fl_t
@@ -397,7 +459,12 @@ def main():
             "traffic": traffic_per_launch(head, w),
             "avg_launch_ms": rk["avg_launch_ms"],
             "flops_per_launch": rk["flops_per_launch"],
+            "timing": "HIP events on the context's stream around each launch, over the eager pass of the same "
+                      "number of steps that precedes the step-graph replay (events cannot bracket kernels "
+                      "inside a replayed graph)" if m.get("timed") == "step graph" else
+                      "HIP events on the context's stream around each launch, over the timed steps",
         },
+        "timed": m.get("timed"),
         "step_tflops_algorithmic": step_fl * m["value"] / 1e12,
         "modes": modes,
         "bias_measured": "fp64" in modes,

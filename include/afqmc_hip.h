@@ -145,6 +145,19 @@ int afqmc_pop_plan(afqmc_ctx* ctx, int32_t method, int32_t nw, const double* wei
 int afqmc_pop_gather(afqmc_ctx* ctx, int32_t nw, const int32_t* src, const void* in, void* out,
                      int64_t bytes_per_walker);
 
+/* The per-step walker sums of ipie's WalkerAccumulator.update
+ * (ipie/walkers/base_walkers.py:43-50): acc [3] complex += (sum weight,
+ * sum unscaled_weight, sum weight * values[i * value_stride]) over nw walkers;
+ * unscaled_weight or values may be NULL (their entry is left alone).  values is
+ * complex; value_stride counts complex entries (1 for hybrid_energy [W], 3 for
+ * column 0 of an energy [W, 3]).  Sums run in a fixed order on the device; no
+ * synchronisation. */
+int afqmc_walker_sums(afqmc_ctx* ctx, int32_t nw, const double* weight, const double* unscaled_weight,
+                      const double* values, int64_t value_stride, double* acc);
+/* weight <- clip(weight, -bound, bound) in place, NaN kept: the driver's clip
+ * after each step (ipie/qmc/afqmc.py:584-588, bound = 0.1 x the total weight). */
+int afqmc_clip_weights(afqmc_ctx* ctx, int32_t nw, double* weight, double bound);
+
 /* Step graphs (no counterpart in the reference, whose steps are host-driven
  * CuPy calls; for callers that replay a block of steps with one launch).
  * afqmc_graph_begin opens a hipStreamBeginCapture on the context's stream

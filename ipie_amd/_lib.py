@@ -94,6 +94,8 @@ SIGNATURES = {
     "afqmc_exchange_form": (ctypes.c_int, [_dp]),
     "afqmc_pop_plan": (ctypes.c_int, [_dp, _i32, _i32, _dp, _f64, _f64, _f64, _dp, _dp, _dp, _dp, _dp]),
     "afqmc_pop_gather": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _i64]),
+    "afqmc_walker_sums": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _i64, _dp]),
+    "afqmc_clip_weights": (ctypes.c_int, [_dp, _i32, _dp, _f64]),
     "afqmc_graph_begin": (ctypes.c_int, [_dp]),
     "afqmc_graph_end": (ctypes.c_int, [_dp, ctypes.POINTER(_i32)]),
     "afqmc_graph_launch": (ctypes.c_int, [_dp, _i32, _u64, _f64]),
@@ -318,6 +320,31 @@ class Context:
             self._chk_vec("detR_shift", detR_shift, nw, torch.float64)
         self._call("afqmc_reortho", nw, ptr(phia), ptr(phib), ptr(ovlp), ptr(detR), ptr(log_detR),
                    ptr(detR_shift))
+
+    def walker_sums(self, weight, acc, unscaled_weight=None, values=None):
+        """acc (complex128 [3]) += (sum weight, sum unscaled_weight, sum weight * values)
+        (afqmc_walker_sums).  values: complex128 [W] or a [W, k] view's column 0."""
+        nw = weight.shape[0]
+        self._chk_vec("weight", weight, nw, torch.float64)
+        _check_tensor("acc", acc, torch.complex128, (3,), self.device)
+        if unscaled_weight is not None:
+            self._chk_vec("unscaled_weight", unscaled_weight, nw, torch.float64)
+        stride = 1
+        if values is not None:
+            if values.dtype != torch.complex128 or values.device != self.device or values.shape[0] != nw:
+                raise ValueError("values must be complex128 on the context's device, walkers first")
+            if values.dim() == 2:
+                if not values.is_contiguous():
+                    raise ValueError("values must be contiguous")
+                stride = values.shape[1]
+            elif values.dim() != 1 or not values.is_contiguous():
+                raise ValueError("values must be [W] or [W, k] contiguous")
+        self._call("afqmc_walker_sums", nw, ptr(weight), ptr(unscaled_weight), ptr(values), stride, ptr(acc))
+
+    def clip_weights(self, weight, bound):
+        nw = weight.shape[0]
+        self._chk_vec("weight", weight, nw, torch.float64)
+        self._call("afqmc_clip_weights", nw, ptr(weight), float(bound))
 
     def graph_begin(self):
         """Start recording this context's calls into a step graph (afqmc_graph_begin);

@@ -1270,6 +1270,24 @@ int afqmc_pop_plan(afqmc_ctx* c, int32_t method, int32_t nw, const double* weigh
   return AFQMC_OK;
 }
 
+int afqmc_walker_sums(afqmc_ctx* c, int32_t nw, const double* weight, const double* unscaled_weight,
+                      const double* values, int64_t value_stride, double* acc) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (nw < 1) return fail(c, AFQMC_ERR_SHAPE, "nwalkers=%d", nw);
+  if (!weight || !acc) return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  if (value_stride < 1) return fail(c, AFQMC_ERR_INVALID_ARG, "value stride must be >= 1");
+  HIP_TRY(c, launch_walker_sums(nw, weight, unscaled_weight, values, value_stride, acc, c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_clip_weights(afqmc_ctx* c, int32_t nw, double* weight, double bound) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (nw < 1) return fail(c, AFQMC_ERR_SHAPE, "nwalkers=%d", nw);
+  if (!weight) return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  HIP_TRY(c, launch_clip_weights(nw, weight, bound, c->stream));
+  return AFQMC_OK;
+}
+
 int afqmc_pop_gather(afqmc_ctx* c, int32_t nw, const int32_t* src, const void* in, void* out,
                      int64_t bytes_per_walker) {
   if (!c) return AFQMC_ERR_INVALID_ARG;

@@ -433,6 +433,12 @@ struct PopPlanArgs {
   int* npairs;              // [1] out: branched pairs (pair_branch)
 };
 hipError_t launch_pop_plan(const PopPlanArgs& a, hipStream_t stream);
+// acc[0] += sum w, acc[1] += sum uw (uw non-null), acc[2] += sum w v[i * vstride]
+// (v non-null); acc is [3] complex, sums in a fixed order (one workgroup)
+hipError_t launch_walker_sums(int nw, const double* weight, const double* unscaled, const double* values,
+                              int64_t vstride, double* acc, hipStream_t stream);
+// weight <- min(max(weight, -bound), bound), NaN kept
+hipError_t launch_clip_weights(int nw, double* weight, double bound, hipStream_t stream);
 hipError_t launch_pop_gather(int nw, const int* src, const void* in, void* out, int64_t row_bytes,
                              hipStream_t stream);
 

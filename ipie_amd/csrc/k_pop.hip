@@ -16,6 +16,10 @@
 //                                     (i + zeta) / W of the total
 //                        rescale      the rescaling only
 //   pop_gather_kernel  out[i] = in[src[i]] for walker rows of any size
+//   walker_sums_kernel the per-step sums of ipie's WalkerAccumulator.update
+//                      (base_walkers.py:32-50): sum w, sum unscaled w,
+//                      sum w E, added to a device accumulator
+//   clip_weights_kernel the driver's weight clip (afqmc.py:584-588)
 //
 // Restates ipie/walkers/pop_controller.py:78-126 (rescale), 304-402
 // (pair_branch), 199-301 (comb), 405-450 (stochastic_reconfiguration) for
@@ -207,7 +211,68 @@ __global__ void __launch_bounds__(256) pop_gather_kernel(int nw, const int* __re
   }
 }
 
+constexpr int SUM_THREADS = 256;
+
+// One workgroup: thread t sums walkers t, t + 256, ... in order, then a fixed
+// tree over the threads, so the result does not depend on scheduling.
+__global__ void __launch_bounds__(SUM_THREADS) walker_sums_kernel(int nw, const double* weight, const double* unscaled,
+                                                                  const double* values, int64_t vstride, double* acc) {
+  __shared__ double red[5][SUM_THREADS];
+  const int t = threadIdx.x;
+  double sw = 0.0, su = 0.0, sr = 0.0, si = 0.0;
+  for (int i = t; i < nw; i += SUM_THREADS) {
+    const double w = weight[i];
+    sw += w;
+    if (unscaled) su += unscaled[i];
+    if (values) {
+      sr += w * values[2 * (int64_t)i * vstride];
+      si += w * values[2 * (int64_t)i * vstride + 1];
+    }
+  }
+  red[0][t] = sw;
+  red[1][t] = su;
+  red[2][t] = sr;
+  red[3][t] = si;
+  __syncthreads();
+  for (int h = SUM_THREADS / 2; h > 0; h >>= 1) {
+    if (t < h)
+      for (int q = 0; q < 4; ++q) red[q][t] += red[q][t + h];
+    __syncthreads();
+  }
+  if (t == 0) {
+    acc[0] += red[0][0];
+    if (unscaled) acc[2] += red[1][0];
+    if (values) {
+      acc[4] += red[2][0];
+      acc[5] += red[3][0];
+    }
+  }
+}
+
+__global__ void clip_weights_kernel(int nw, double* weight, double bound) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nw) return;
+  double w = weight[i];
+  w = w < -bound ? -bound : w;  // comparisons are false for NaN: NaN is kept, as numpy.clip does
+  w = w > bound ? bound : w;
+  weight[i] = w;
+}
+
 }  // namespace
+
+hipError_t launch_walker_sums(int nw, const double* weight, const double* unscaled, const double* values,
+                              int64_t vstride, double* acc, hipStream_t stream) {
+  if (nw < 1 || !weight || !acc || vstride < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(walker_sums_kernel, dim3(1), dim3(SUM_THREADS), 0, stream, nw, weight, unscaled, values, vstride,
+                     acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_clip_weights(int nw, double* weight, double bound, hipStream_t stream) {
+  if (nw < 1 || !weight) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(clip_weights_kernel, dim3((unsigned)ceil_div(nw, 256)), dim3(256), 0, stream, nw, weight, bound);
+  return hipGetLastError();
+}
 
 hipError_t launch_pop_plan(const PopPlanArgs& a, hipStream_t stream) {
   if (a.nwalkers < 1 || a.nwalkers > POP_MAX || !(a.target > 0.0) || !a.weight || !a.src || !a.new_weight ||

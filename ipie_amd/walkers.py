@@ -75,3 +75,78 @@ class UHFWalkers:
     def cast_to_cupy(self, verbose=False):
         """The population already lives on the device."""
         return None
+
+
+class WalkerAccumulator:
+    """Per-block walker sums (ipie/walkers/base_walkers.py:32-83) kept on the
+    device: ``update`` adds (sum weight, sum unscaled_weight, sum weight *
+    hybrid_energy) with one kernel and no host synchronisation (the reference
+    reads three sums back every step); ``buffer`` copies them to the host when
+    it is read (print_block, once per block)."""
+
+    def __init__(self, names, nsteps):
+        self.names = names
+        self.size = len(names)
+        self._data_index = {k: i for i, k in enumerate(self.names)}
+        self.nsteps_per_block = nsteps
+        self._eshift = 0.0
+        self._dev = None
+        self._host = numpy.zeros((self.size,), dtype=numpy.complex128)
+
+    def update(self, walkers):
+        eng = walkers._engine
+        if eng is None:
+            raise RuntimeError("WalkerAccumulator.update needs walkers built against a trial (walkers.build)")
+        if self._dev is None:
+            self._dev = torch.zeros(3, dtype=torch.complex128, device=walkers.weight.device)
+            self._dev.copy_(torch.from_numpy(self._host[:3].copy()))
+        eng.ctx.walker_sums(walkers.weight, self._dev, walkers.unscaled_weight, walkers.hybrid_energy)
+
+    @property
+    def buffer(self):
+        if self._dev is not None:
+            self._host[:3] = self._dev.cpu().numpy()
+        return self._host
+
+    def zero(self):
+        self._host.fill(0.0j)
+        if self._dev is not None:
+            self._dev.zero_()
+
+    def get_index(self, name):
+        index = self._data_index.get(name, None)
+        if index is None:
+            raise RuntimeError(f"Unknown walker property {name}")
+        return index
+
+    @property
+    def eshift(self):
+        return self._eshift.real
+
+    @eshift.setter
+    def eshift(self, value):
+        self._eshift = value
+
+    def post_reduce_hook(self, vals, block):
+        """Block averages from the reduced sums (base_walkers.py:70-80)."""
+        assert len(vals) == len(self.names)
+        factor = 1 if block == 0 else self.nsteps_per_block
+        nume = self.get_index("HybridEnergy")
+        deno = self.get_index("Weight")
+        vals[nume] = vals[nume] / vals[deno]
+        vals[deno] = vals[deno] / factor
+        ix = self.get_index("WeightFactor")
+        vals[ix] = vals[ix] / factor
+
+    def to_text(self, vals):
+        """Fixed-width text of the real parts (ipie/utils/io.py:197-198 format)."""
+        return " ".join(f"{f: .16e}" for f in numpy.asarray(vals).real)
+
+
+def clip_weights(walkers, wbound):
+    """The driver's clip of the weights to [-wbound, wbound] after each step
+    (ipie/qmc/afqmc.py:584-588), in place on the device."""
+    eng = walkers._engine
+    if eng is None:
+        raise RuntimeError("clip_weights needs walkers built against a trial (walkers.build)")
+    eng.ctx.clip_weights(walkers.weight, wbound)

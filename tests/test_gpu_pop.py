@@ -208,3 +208,42 @@ def test_pop_controller_drop_in(method, freq):
         assert nxt == u[npairs]
     else:  # one draw (comb, reconfiguration call) or none (rescale-only call)
         assert nxt == follow[0]
+
+
+def test_walker_sums_and_clip():
+    torch = _torch()
+    from ipie_amd.walkers import WalkerAccumulator, clip_weights
+
+    nw = 64
+    p = _pair(nw)
+    wk = p.walkers
+    rs = numpy.random.RandomState(2)
+    w = numpy.exp(rs.normal(size=nw))
+    uw = numpy.exp(rs.normal(size=nw))
+    eh = rs.normal(size=nw) + 1j * rs.normal(size=nw)
+    wk.weight.copy_(torch.from_numpy(w))
+    wk.unscaled_weight.copy_(torch.from_numpy(uw))
+    wk.hybrid_energy.copy_(torch.from_numpy(eh))
+    acc = WalkerAccumulator(["Weight", "WeightFactor", "HybridEnergy"], 5)
+    acc.update(wk)
+    acc.update(wk)
+    want = 2 * numpy.array([w.sum(), uw.sum(), (w * eh).sum()])
+    assert numpy.allclose(acc.buffer, want, rtol=1e-13, atol=0)
+    acc.zero()
+    assert numpy.array_equal(acc.buffer, numpy.zeros(3))
+    # column 0 of an energy [W, 3] (stride 3); unscaled weights left out
+    e = torch.from_numpy(rs.normal(size=(nw, 3)) + 1j * rs.normal(size=(nw, 3))).to(wk.weight.device)
+    out = torch.zeros(3, dtype=torch.complex128, device=wk.weight.device)
+    wk._engine.ctx.walker_sums(wk.weight, out, values=e)
+    got = out.cpu().numpy()
+    assert got[1] == 0
+    assert numpy.allclose(got[[0, 2]], [w.sum(), (w * e.cpu().numpy()[:, 0]).sum()], rtol=1e-13, atol=0)
+    # clip, NaN kept as numpy.clip keeps it
+    w2 = rs.normal(size=nw) * 3.0
+    w2[5] = numpy.nan
+    wk.weight.copy_(torch.from_numpy(w2))
+    clip_weights(wk, 1.5)
+    got = wk.weight.cpu().numpy()
+    want = numpy.clip(w2, -1.5, 1.5)
+    assert numpy.array_equal(numpy.isnan(got), numpy.isnan(want))
+    assert numpy.array_equal(got[~numpy.isnan(got)], want[~numpy.isnan(want)])
