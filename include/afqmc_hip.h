@@ -179,6 +179,26 @@ int afqmc_graph_begin(afqmc_ctx* ctx);
 int afqmc_graph_end(afqmc_ctx* ctx, int32_t* steps);
 int afqmc_graph_launch(afqmc_ctx* ctx, int32_t replays, uint64_t counter, double eshift);
 
+/* Multi-system step graphs (no counterpart in the reference, which runs one
+ * system per process): independent contexts (systems of different sizes, one
+ * device) recorded into one graph, each on its own stream, so their kernels
+ * overlap.  afqmc_batch_begin starts the capture on the first context's stream
+ * and forks every other context's stream from it (each context needs its own
+ * non-null stream, set with afqmc_set_stream before begin and kept until end);
+ * the calls made on each context until afqmc_batch_end are recorded as for
+ * afqmc_graph_begin; afqmc_batch_end joins the streams and instantiates.
+ * afqmc_batch_steps returns the steps recorded per context ([n]).
+ * afqmc_batch_launch sets each context's counter and eshift ([n] each) and
+ * replays the graph on the first context's stream.  Errors are reported in
+ * afqmc_last_error of the first context. */
+typedef struct afqmc_batch afqmc_batch;
+int afqmc_batch_create(afqmc_batch** out, afqmc_ctx* const* ctxs, int32_t n);
+void afqmc_batch_destroy(afqmc_batch* batch);
+int afqmc_batch_begin(afqmc_batch* batch);
+int afqmc_batch_end(afqmc_batch* batch);
+int afqmc_batch_steps(afqmc_batch* batch, int32_t* steps);
+int afqmc_batch_launch(afqmc_batch* batch, int32_t replays, const uint64_t* counters, const double* eshifts);
+
 /* Ghalf_s = (phi_s^T psi_s^*)^{-1} phi_s^T and ovlp = prod_s det(phi_s^T psi_s^*) e^{-log_shift}.
  * phia [W, N, n_a], phib [W, N, n_b], Ghalfa [W, n_a, N], Ghalfb [W, n_b, N],
  * ovlp [W] complex; log_shift [W] real or NULL. */

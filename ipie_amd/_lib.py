@@ -99,6 +99,12 @@ SIGNATURES = {
     "afqmc_graph_begin": (ctypes.c_int, [_dp]),
     "afqmc_graph_end": (ctypes.c_int, [_dp, ctypes.POINTER(_i32)]),
     "afqmc_graph_launch": (ctypes.c_int, [_dp, _i32, _u64, _f64]),
+    "afqmc_batch_create": (ctypes.c_int, [ctypes.POINTER(_dp), ctypes.POINTER(_dp), _i32]),
+    "afqmc_batch_destroy": (None, [_dp]),
+    "afqmc_batch_begin": (ctypes.c_int, [_dp]),
+    "afqmc_batch_end": (ctypes.c_int, [_dp]),
+    "afqmc_batch_steps": (ctypes.c_int, [_dp, ctypes.POINTER(_i32)]),
+    "afqmc_batch_launch": (ctypes.c_int, [_dp, _i32, ctypes.POINTER(_u64), ctypes.POINTER(_f64)]),
 }
 
 POP_METHODS = {"pair_branch": 0, "comb": 1, "stochastic_reconfiguration": 2, "rescale": 3}
@@ -345,6 +351,10 @@ class Context:
         nw = weight.shape[0]
         self._chk_vec("weight", weight, nw, torch.float64)
         self._call("afqmc_clip_weights", nw, ptr(weight), float(bound))
+
+    def bind_stream(self, stream):
+        """Set the context's stream (a torch.cuda.Stream) without making a call."""
+        self.lib.afqmc_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream))
 
     def graph_begin(self):
         """Start recording this context's calls into a step graph (afqmc_graph_begin);

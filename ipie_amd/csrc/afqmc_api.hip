@@ -1507,6 +1507,139 @@ int afqmc_graph_launch(afqmc_ctx* c, int32_t replays, uint64_t counter, double e
   return AFQMC_OK;
 }
 
+// ---------------------------------------------------------------- multi-system step graphs
+struct afqmc_batch {
+  std::vector<afqmc_ctx*> ctx;
+  std::vector<hipEvent_t> ev;  // [0]: fork from the first stream; [i]: join of context i
+  hipGraphExec_t graph = nullptr;
+  std::vector<int> steps, sur;
+  bool capturing = false;
+};
+
+int afqmc_batch_create(afqmc_batch** out, afqmc_ctx* const* ctxs, int32_t n) {
+  if (!out || !ctxs || n < 1) return AFQMC_ERR_INVALID_ARG;
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i]) return AFQMC_ERR_INVALID_ARG;
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) return fail(ctxs[0], AFQMC_ERR_INVALID_ARG, "context %d appears twice", i);
+    if (ctxs[i]->device != ctxs[0]->device)
+      return fail(ctxs[0], AFQMC_ERR_INVALID_ARG, "the contexts of a batch share one device");
+  }
+  afqmc_batch* b = new afqmc_batch();
+  b->ctx.assign(ctxs, ctxs + n);
+  b->ev.assign(n, nullptr);
+  b->steps.assign(n, 0);
+  b->sur.assign(n, 0);
+  for (int i = 0; i < n; ++i)
+    if (hipEventCreateWithFlags(&b->ev[i], hipEventDisableTiming) != hipSuccess) {
+      afqmc_batch_destroy(b);
+      return fail(ctxs[0], AFQMC_ERR_HIP, "event creation failed");
+    }
+  *out = b;
+  return AFQMC_OK;
+}
+
+void afqmc_batch_destroy(afqmc_batch* b) {
+  if (!b) return;
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  for (hipEvent_t e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete b;
+}
+
+int afqmc_batch_begin(afqmc_batch* b) {
+  if (!b) return AFQMC_ERR_INVALID_ARG;
+  afqmc_ctx* c0 = b->ctx[0];
+  const int n = (int)b->ctx.size();
+  if (b->capturing || t_capturing) return fail(c0, AFQMC_ERR_INVALID_ARG, "a capture is already open");
+  for (int i = 0; i < n; ++i) {
+    if (!b->ctx[i]->stream)
+      return fail(c0, AFQMC_ERR_INVALID_ARG, "context %d has the null stream: each context needs its own stream", i);
+    if (b->ctx[i]->capturing) return fail(c0, AFQMC_ERR_INVALID_ARG, "context %d is capturing", i);
+    for (int j = 0; j < i; ++j)
+      if (b->ctx[j]->stream == b->ctx[i]->stream)
+        return fail(c0, AFQMC_ERR_INVALID_ARG, "contexts %d and %d share a stream", j, i);
+  }
+  HIP_TRY(c0, hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipEventRecord(b->ev[0], c0->stream);
+  for (int i = 1; i < n && e == hipSuccess; ++i) e = hipStreamWaitEvent(b->ctx[i]->stream, b->ev[0], 0);
+  if (e != hipSuccess) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c0->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    return fail(c0, AFQMC_ERR_HIP, "forking the capture failed: %s", hipGetErrorString(e));
+  }
+  for (afqmc_ctx* c : b->ctx) {
+    c->capturing = true;
+    c->capture_steps = 0;
+    c->capture_sur = 0;
+  }
+  b->capturing = true;
+  t_capturing = true;
+  return AFQMC_OK;
+}
+
+int afqmc_batch_end(afqmc_batch* b) {
+  if (!b) return AFQMC_ERR_INVALID_ARG;
+  afqmc_ctx* c0 = b->ctx[0];
+  if (!b->capturing) return fail(c0, AFQMC_ERR_INVALID_ARG, "no capture is open");
+  const int n = (int)b->ctx.size();
+  hipError_t e = hipSuccess;
+  for (int i = 1; i < n && e == hipSuccess; ++i) {
+    e = hipEventRecord(b->ev[i], b->ctx[i]->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c0->stream, b->ev[i], 0);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(c0->stream, &g);
+  if (e == hipSuccess) e = e2;
+  b->capturing = false;
+  t_capturing = false;
+  for (int i = 0; i < n; ++i) {
+    afqmc_ctx* c = b->ctx[i];
+    c->capturing = false;
+    c->sur.steps -= c->capture_sur;  // recorded, not run
+    b->steps[i] = c->capture_steps;
+    b->sur[i] = c->capture_sur;
+  }
+  if (e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(c0, AFQMC_ERR_HIP, "multi-system capture failed: %s (run the recorded calls once before capturing)",
+                hipGetErrorString(e));
+  }
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    b->graph = nullptr;
+    return fail(c0, AFQMC_ERR_HIP, "multi-system graph instantiation failed: %s", hipGetErrorString(e));
+  }
+  return AFQMC_OK;
+}
+
+int afqmc_batch_steps(afqmc_batch* b, int32_t* steps) {
+  if (!b || !steps) return AFQMC_ERR_INVALID_ARG;
+  for (size_t i = 0; i < b->ctx.size(); ++i) steps[i] = b->steps[i];
+  return AFQMC_OK;
+}
+
+int afqmc_batch_launch(afqmc_batch* b, int32_t replays, const uint64_t* counters, const double* eshifts) {
+  if (!b || !counters || !eshifts) return AFQMC_ERR_INVALID_ARG;
+  afqmc_ctx* c0 = b->ctx[0];
+  if (b->capturing || t_capturing) return refuse_in_capture(c0, "afqmc_batch_launch");
+  if (!b->graph) return fail(c0, AFQMC_ERR_INVALID_ARG, "no multi-system graph has been captured");
+  if (replays < 0) return fail(c0, AFQMC_ERR_INVALID_ARG, "replays must be >= 0");
+  // every context's step values are set on the launch stream, ahead of the graph
+  for (size_t i = 0; i < b->ctx.size(); ++i)
+    HIP_TRY(c0, launch_step_set(b->ctx[i]->dstate, counters[i], eshifts[i], c0->stream));
+  for (int r = 0; r < replays; ++r) HIP_TRY(c0, hipGraphLaunch(b->graph, c0->stream));
+  for (size_t i = 0; i < b->ctx.size(); ++i) b->ctx[i]->sur.steps += (int64_t)replays * b->sur[i];
+  return AFQMC_OK;
+}
+
 int afqmc_draw_normals(afqmc_ctx* c, double* out, int64_t n, uint64_t seed, uint64_t counter) {
   if (!c || !out || n < 0) return AFQMC_ERR_INVALID_ARG;
   HIP_TRY(c, launch_draw_normals(out, n, seed, counter, c->stream));
