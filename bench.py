@@ -106,6 +106,101 @@ def build_device(host, w, mode):
     return dict(host, walkers=walkers, prop=prop)
 
 
+def run_multi(args):
+    """The uneven multi-system batch: 16 independent systems (N 64-256, X = 5 N,
+    N / 10 occupied per spin, 64-512 walkers, mixed precision, local energy
+    every step, QR every 5 steps), stepped (a) eagerly one system after
+    another on one stream, (b) from one step graph per system, replayed one
+    system after another, (c) from one multi-system graph with every system on
+    its own stream.  Prints one JSON line; the value is (c)."""
+    import torch
+
+    from ipie_amd import synthetic
+    from ipie_amd.batch import MultiSystemBatch
+
+    ws = synthetic.multi_system_workloads()
+    K = max(args.steps // REORTHO_EVERY, 1) * REORTHO_EVERY
+    systems = []
+    for w in ws:
+        host = build_host(w, math.sqrt(12.0 / (w.nbasis * w.decay)))
+        st = build_device(host, w, "mixed")
+        st["energy"] = torch.empty(w.nwalkers, 3, dtype=torch.complex128, device=st["walkers"].device)
+        st["eshift"] = float(host["trial"].energy.real)
+        st["w"] = w
+        systems.append(st)
+    total_w = sum(w.nwalkers for w in ws)
+    log(f"[bench] multi: {len(ws)} systems, N {min(w.nbasis for w in ws)}-{max(w.nbasis for w in ws)}, "
+        f"{total_w} walkers")
+
+    def eager_block(st):
+        for _ in range(REORTHO_EVERY):
+            st["prop"].propagate_walkers(st["walkers"], st["ham"], st["trial"], st["eshift"],
+                                         energy_prev=st["energy"])
+        st["walkers"].reortho()
+
+    for st in systems:  # warm-up: workspace, then one block
+        eager_block(st)
+    torch.cuda.synchronize()
+    # (a) eager, one system after another
+    t0 = time.perf_counter()
+    for _ in range(K // REORTHO_EVERY):
+        for st in systems:
+            eager_block(st)
+    torch.cuda.synchronize()
+    t_eager = time.perf_counter() - t0
+    log(f"[bench] multi eager sequential: {K} steps in {t_eager:.3f}s")
+    # (b) one graph per system, replayed one system after another
+    for st in systems:
+        st["prop"].capture_steps(st["walkers"], st["ham"], st["trial"], REORTHO_EVERY, reortho_every=REORTHO_EVERY,
+                                 energy_prev=st["energy"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K // REORTHO_EVERY):
+        for st in systems:
+            st["prop"].replay_steps(st["walkers"], st["eshift"], 1)
+    torch.cuda.synchronize()
+    t_graphs = time.perf_counter() - t0
+    log(f"[bench] multi per-system graphs: {K} steps in {t_graphs:.3f}s")
+    # (c) one graph for all systems, each on its own stream
+    batch = MultiSystemBatch([(st["prop"], st["walkers"], st["ham"], st["trial"]) for st in systems])
+    batch.capture(REORTHO_EVERY, reortho_every=REORTHO_EVERY, energies=[st["energy"] for st in systems])
+    torch.cuda.synchronize()
+    shifts = [st["eshift"] for st in systems]
+    t0 = time.perf_counter()
+    batch.replay(shifts, replays=K // REORTHO_EVERY)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    log(f"[bench] multi batched graph: {K} steps in {t_batch:.3f}s")
+    finite = all(bool(torch.isfinite(st["walkers"].weight).all()) for st in systems)
+    rate = lambda t: total_w * K / t  # noqa: E731
+    out = {
+        "metric": "aggregate walker-steps/s, 16 uneven systems in one step graph",
+        "value": rate(t_batch),
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": REORTHO_EVERY,
+        "ms_per_step": 1e3 * t_batch / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (Green's function, weights, one-body, QR) + bf16x3 force bias, VHS, Taylor, exchange",
+        "data": "synthetic",
+        "config": {
+            "workload": "multi16: 16 systems, N uniform [64, 256], X = 5N, N/10 occupied per spin, walkers uniform "
+                        "[64, 512], D = 0.75 N, seed 7",
+            "systems": [dict(N=w.nbasis, X=w.nchol, n=w.nalpha, W=w.nwalkers) for w in ws],
+            "total_walkers": total_w,
+            "parallelism": "single GPU, one stream per system inside one graph",
+        },
+        "sequential_eager_value": rate(t_eager),
+        "sequential_graphs_value": rate(t_graphs),
+        "speedup_vs_sequential_eager": t_eager / t_batch,
+        "finite": finite,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_graph(st, w, steps, one_step, counter, energy, eshift):
     """`steps` steps replayed from one step graph: a block of REORTHO_EVERY
     propagate calls (each followed by the device sums of the weighted local
@@ -306,6 +401,9 @@ def main():
         return
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload == "multi16":
+        run_multi(args)
+        return
     w = synthetic.WORKLOADS[args.workload]
     if args.nwalkers:
         w = dataclasses.replace(w, nwalkers=args.nwalkers)

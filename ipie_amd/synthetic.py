@@ -44,6 +44,21 @@ WORKLOADS = {
 }
 
 
+def multi_system_workloads(count=16, seed=7):
+    """The uneven multi-system batch: `count` systems drawn from one seed with
+    N uniform in [64, 256], X = 5 N, N / 10 occupied orbitals per spin
+    (rounded), walkers uniform in [64, 512]; decay D = 0.75 N (as the other
+    workloads' D / N); each system's generator seed is seed + 1 + its index."""
+    rs = numpy.random.RandomState(seed)
+    out = []
+    for i in range(count):
+        n = int(rs.randint(64, 257))
+        w = int(rs.randint(64, 513))
+        nocc = max(1, int(round(n / 10)))
+        out.append(Workload(f"sys{i}", n, 5 * n, 0.75 * n, nocc, nocc, w, seed=seed + 1 + i))
+    return out
+
+
 def make_hamiltonian(nbasis, nchol, decay, seed=7, dtype=numpy.float64):
     """Return (h1e [2,N,N], chol [N*N, X], ecore) for the synthetic generator."""
     rs = numpy.random.RandomState(seed)
