@@ -106,6 +106,42 @@ def build_device(host, w, mode):
     return dict(host, walkers=walkers, prop=prop)
 
 
+def run_setup_timing(w, scale):
+    """Set-up from the raw arrays until the propagator is built: on the host
+    (packing, h1e_mod, half rotation, mean-field shift, expH1 in numpy / scipy,
+    then upload) against the device set-up kernels (upload of chol, then
+    afqmc_setup_*).  Prints one JSON line."""
+    import torch
+
+    from ipie_amd import synthetic
+    from ipie_amd.hamiltonian import GenericRealChol
+    from ipie_amd.propagator import PhaselessGeneric
+    from ipie_amd.trial import SingleDet
+    from ipie_amd.walkers import UHFWalkers
+
+    h1e, chol, ecore = synthetic.make_hamiltonian(w.nbasis, w.nchol, w.decay, seed=w.seed)
+    chol *= scale
+    psi = synthetic.make_trial(h1e, w.nalpha, w.nbeta)
+    times = {}
+    for mode in ("host", "device", "host", "device"):  # second round timed (first one warms up)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ham = GenericRealChol(h1e, chol, ecore=ecore, host_setup=(mode == "host"))
+        trial = SingleDet(psi, (w.nalpha, w.nbeta), w.nbasis)
+        walkers = UHFWalkers(psi, w.nalpha, w.nbeta, w.nbasis, 8)
+        prop = PhaselessGeneric(w.dt, precision="fp64")
+        prop.build(ham, trial, walkers, device_setup=(mode == "device"))
+        torch.cuda.synchronize()
+        times[mode] = time.perf_counter() - t0
+        log(f"[bench] set-up on the {mode}: {times[mode]:.2f}s")
+        del ham, trial, walkers, prop
+        torch.cuda.empty_cache()
+    print(json.dumps({"metric": "set-up seconds from raw arrays to a built propagator", "workload": w.name,
+                      "N": w.nbasis, "X": w.nchol, "host_s": times["host"], "device_s": times["device"],
+                      "speedup": times["host"] / times["device"],
+                      "host_threads": int(os.environ.get("OMP_NUM_THREADS", "0") or 0)}), flush=True)
+
+
 def run_multi(args):
     """The uneven multi-system batch: 16 independent systems (N 64-256, X = 5 N,
     N / 10 occupied per spin, 64-512 walkers, mixed precision, local energy
@@ -372,6 +408,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager steps only (default: the value is the step-graph replay, the per-phase "
                          "profile and the roofline kernel time come from the eager pass before it)")
+    ap.add_argument("--setup-timing", action="store_true",
+                    help="only time the set-up of the workload on the host and on the device, then exit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-walkers", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -408,6 +446,9 @@ def main():
     if args.nwalkers:
         w = dataclasses.replace(w, nwalkers=args.nwalkers)
     scale = 1.0 if args.cholesky_scale == "as-specified" else math.sqrt(12.0 / (w.nbasis * w.decay))
+    if args.setup_timing:
+        run_setup_timing(w, scale)
+        return
     host = build_host(w, scale)
     K = args.steps
     mode_list = [m for m in args.modes.split(",") if m]

@@ -145,6 +145,33 @@ int afqmc_pop_plan(afqmc_ctx* ctx, int32_t method, int32_t nw, const double* wei
 int afqmc_pop_gather(afqmc_ctx* ctx, int32_t nw, const int32_t* src, const void* in, void* out,
                      int64_t bytes_per_walker);
 
+/* Set-up producers on the device (once per Hamiltonian and trial; ipie runs
+ * them on the host), for a real Hamiltonian and a real trial.  Sizes come
+ * from the context's dims (N = nbasis, X = nchol, n_s = nalpha / nbeta);
+ * every array is device memory; each call returns when its results are
+ * complete.  chol [N*N, X] f64 is ipie's GenericRealChol.chol.
+ *   afqmc_setup_pack: chol_packed [N(N+1)/2, X] in numpy.triu_indices order
+ *     (ipie/hamiltonians/generic.py:64-72, utils/pack_numba.py:36-45).
+ *   afqmc_setup_h1e_mod: h1e_mod[s] = h1e[s] - 1/2 sum_{k,x} L[i,k,x] L[j,k,x],
+ *     h1e and h1e_mod [2, N, N] (generic.py:31-43).
+ *   afqmc_setup_half_rotate: rchol [X, n_s N] f64 and rh1 [n_s, N] complex for
+ *     spin s from psi [N, n_s] f64 (row stride ld_psi) and h1 [N, N]
+ *     (trial_wavefunction/half_rotate.py:11-116).
+ *   afqmc_setup_mf_shift: mf [X] complex = i L^T Re(g) - L^T Im(g) for the
+ *     charge density g [N*N] complex, the sum of both spins' G
+ *     (propagation/phaseless_base.py:153-172).
+ *   afqmc_setup_expH1: expH1 [2, N, N] f64 = exp(-dt/2 (h1e_mod[s] -
+ *     i sum_x L_x mf_x)) for a purely imaginary mf (its real part is not read),
+ *     by scaling and squaring a degree-18 Taylor polynomial in place of
+ *     scipy.linalg.expm (phaseless_base.py:27-74). */
+int afqmc_setup_pack(afqmc_ctx* ctx, const double* chol, double* chol_packed);
+int afqmc_setup_h1e_mod(afqmc_ctx* ctx, const double* chol, const double* h1e, double* h1e_mod);
+int afqmc_setup_half_rotate(afqmc_ctx* ctx, int32_t spin, const double* psi, int32_t ld_psi, const double* chol,
+                            const double* h1, double* rchol, double* rh1);
+int afqmc_setup_mf_shift(afqmc_ctx* ctx, const double* chol, const double* gcharge, double* mf);
+int afqmc_setup_expH1(afqmc_ctx* ctx, const double* chol, const double* h1e_mod, const double* mf, double dt,
+                      double* expH1);
+
 /* The per-step walker sums of ipie's WalkerAccumulator.update
  * (ipie/walkers/base_walkers.py:43-50): acc [3] complex += (sum weight,
  * sum unscaled_weight, sum weight * values[i * value_stride]) over nw walkers;

@@ -94,6 +94,11 @@ SIGNATURES = {
     "afqmc_exchange_form": (ctypes.c_int, [_dp]),
     "afqmc_pop_plan": (ctypes.c_int, [_dp, _i32, _i32, _dp, _f64, _f64, _f64, _dp, _dp, _dp, _dp, _dp]),
     "afqmc_pop_gather": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _i64]),
+    "afqmc_setup_pack": (ctypes.c_int, [_dp, _dp, _dp]),
+    "afqmc_setup_h1e_mod": (ctypes.c_int, [_dp, _dp, _dp, _dp]),
+    "afqmc_setup_half_rotate": (ctypes.c_int, [_dp, _i32, _dp, _i32, _dp, _dp, _dp, _dp]),
+    "afqmc_setup_mf_shift": (ctypes.c_int, [_dp, _dp, _dp, _dp]),
+    "afqmc_setup_expH1": (ctypes.c_int, [_dp, _dp, _dp, _dp, _f64, _dp]),
     "afqmc_walker_sums": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _i64, _dp]),
     "afqmc_clip_weights": (ctypes.c_int, [_dp, _i32, _dp, _f64]),
     "afqmc_graph_begin": (ctypes.c_int, [_dp]),
@@ -326,6 +331,50 @@ class Context:
             self._chk_vec("detR_shift", detR_shift, nw, torch.float64)
         self._call("afqmc_reortho", nw, ptr(phia), ptr(phib), ptr(ovlp), ptr(detR), ptr(log_detR),
                    ptr(detR_shift))
+
+    # ------------------------------------------------------------ set-up producers
+    def setup_pack(self, chol, chol_packed):
+        d, dev = self.dims, self.device
+        N, X = d.nbasis, d.nchol
+        _check_tensor("chol", chol, torch.float64, (N * N, X), dev)
+        _check_tensor("chol_packed", chol_packed, torch.float64, (N * (N + 1) // 2, X), dev)
+        self._call("afqmc_setup_pack", ptr(chol), ptr(chol_packed))
+
+    def setup_h1e_mod(self, chol, h1e, h1e_mod):
+        d, dev = self.dims, self.device
+        N, X = d.nbasis, d.nchol
+        _check_tensor("chol", chol, torch.float64, (N * N, X), dev)
+        _check_tensor("h1e", h1e, torch.float64, (2, N, N), dev)
+        _check_tensor("h1e_mod", h1e_mod, torch.float64, (2, N, N), dev)
+        self._call("afqmc_setup_h1e_mod", ptr(chol), ptr(h1e), ptr(h1e_mod))
+
+    def setup_half_rotate(self, spin, psi, chol, h1, rchol, rh1):
+        d, dev = self.dims, self.device
+        N, X = d.nbasis, d.nchol
+        n = d.nalpha if spin == 0 else d.nbeta
+        _check_tensor("psi", psi, torch.float64, (N, n), dev)
+        _check_tensor("chol", chol, torch.float64, (N * N, X), dev)
+        _check_tensor("h1", h1, torch.float64, (N, N), dev)
+        _check_tensor("rchol", rchol, torch.float64, (X, n * N), dev)
+        _check_tensor("rh1", rh1, torch.complex128, (n, N), dev)
+        self._call("afqmc_setup_half_rotate", int(spin), ptr(psi), n, ptr(chol), ptr(h1), ptr(rchol), ptr(rh1))
+
+    def setup_mf_shift(self, chol, gcharge, mf):
+        d, dev = self.dims, self.device
+        N, X = d.nbasis, d.nchol
+        _check_tensor("chol", chol, torch.float64, (N * N, X), dev)
+        _check_tensor("gcharge", gcharge, torch.complex128, (N * N,), dev)
+        _check_tensor("mf", mf, torch.complex128, (X,), dev)
+        self._call("afqmc_setup_mf_shift", ptr(chol), ptr(gcharge), ptr(mf))
+
+    def setup_expH1(self, chol, h1e_mod, mf, dt, expH1):
+        d, dev = self.dims, self.device
+        N, X = d.nbasis, d.nchol
+        _check_tensor("chol", chol, torch.float64, (N * N, X), dev)
+        _check_tensor("h1e_mod", h1e_mod, torch.float64, (2, N, N), dev)
+        _check_tensor("mf", mf, torch.complex128, (X,), dev)
+        _check_tensor("expH1", expH1, torch.float64, (2, N, N), dev)
+        self._call("afqmc_setup_expH1", ptr(chol), ptr(h1e_mod), ptr(mf), float(dt), ptr(expH1))
 
     def walker_sums(self, weight, acc, unscaled_weight=None, values=None):
         """acc (complex128 [3]) += (sum weight, sum unscaled_weight, sum weight * values)

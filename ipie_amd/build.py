@@ -21,7 +21,7 @@ SOURCES = [
     "k_surrogate.hip",
     "k_fields.hip",
     "k_qr.hip",
-    "k_pop.hip",
+    "k_pop.hip", "k_setup.hip",
 ]
 
 HIPCC_FLAGS = [

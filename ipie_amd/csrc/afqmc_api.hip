@@ -1270,6 +1270,251 @@ int afqmc_pop_plan(afqmc_ctx* c, int32_t method, int32_t nw, const double* weigh
   return AFQMC_OK;
 }
 
+// ---------------------------------------------------------------- set-up producers
+namespace {
+
+// split-K slices for >= ~1024 workgroups, K >= 256 per slice, planes <= 256 MB
+int setup_splits(int64_t m, int64_t n, int64_t k) {
+  const int64_t tiles = ceil_div(m, 64) * ceil_div(n, 64);
+  int64_t nz = ceil_div(1024, tiles);
+  nz = std::min<int64_t>(nz, std::max<int64_t>(k / 256, 1));
+  nz = std::min<int64_t>(nz, std::max<int64_t>((int64_t)(256ll << 20) / (8 * m * n), 1));
+  return (int)std::max<int64_t>(std::min<int64_t>(nz, 4096), 1);
+}
+
+// one set-up product on the context's stream; split-K planes are allocated
+// for the call and freed after it
+int setup_gemm(afqmc_ctx* c, SetupGemmArgs g) {
+  const int nz = setup_splits(g.m, g.n, g.k);
+  const int64_t pd = setup_gemm_part_doubles(g.m, g.n, nz);
+  double* part = nullptr;
+  if (pd > 0 && dmalloc((void**)&part, 8 * (size_t)pd) != hipSuccess)
+    return fail(c, AFQMC_ERR_ALLOC, "set-up product: %lld split-K values", (long long)pd);
+  g.part = part;
+  hipError_t e = launch_setup_gemm(g, nz, c->stream);
+  if (part) {
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(part);
+  }
+  if (e != hipSuccess) return fail(c, AFQMC_ERR_HIP, "set-up product: %s", hipGetErrorString(e));
+  return AFQMC_OK;
+}
+
+SetupGemmArgs setup_args(int64_t m, int64_t n, int64_t k) {
+  SetupGemmArgs g{};
+  g.m = m;
+  g.n = n;
+  g.k = k;
+  g.alpha = 1.0;
+  g.beta = 0.0;
+  g.store = SETUP_STORE_STRIDED;
+  return g;
+}
+
+int setup_ready(afqmc_ctx* c) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (c->capturing) return refuse_in_capture(c, "a set-up producer");
+  if (c->d.nchol < 1) return fail(c, AFQMC_ERR_SHAPE, "the set-up producers need nchol >= 1");
+  c->err[0] = 0;
+  return AFQMC_OK;
+}
+
+}  // namespace
+
+int afqmc_setup_pack(afqmc_ctx* c, const double* chol, double* chol_packed) {
+  int r = setup_ready(c);
+  if (r) return r;
+  if (!chol || !chol_packed) return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  HIP_TRY(c, launch_setup_pack(c->d.nbasis, c->d.nchol, chol, chol_packed, c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_setup_h1e_mod(afqmc_ctx* c, const double* chol, const double* h1e, double* h1e_mod) {
+  int r = setup_ready(c);
+  if (r) return r;
+  if (!chol || !h1e || !h1e_mod) return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  const int64_t N = c->d.nbasis, X = c->d.nchol, NN = N * N;
+  double* v = nullptr;
+  if (dmalloc((void**)&v, 8 * (size_t)NN) != hipSuccess) return fail(c, AFQMC_ERR_ALLOC, "set-up scratch");
+  // v = -1/2 Lv Lv^T with Lv = chol viewed [N, N X]
+  SetupGemmArgs g = setup_args(N, N, N * X);
+  g.a = chol;
+  g.sar = N * X;
+  g.sac = 1;
+  g.b = chol;
+  g.sbr = 1;
+  g.sbc = N * X;
+  g.c = v;
+  g.scr = N;
+  g.scc = 1;
+  g.alpha = -0.5;
+  r = setup_gemm(c, g);
+  hipError_t e = hipSuccess;
+  for (int s = 0; s < 2 && r == AFQMC_OK && e == hipSuccess; ++s)
+    e = launch_setup_scale_shift(NN, h1e + s * NN, v, 1.0, h1e_mod + s * NN, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(v);
+  if (r) return r;
+  if (e != hipSuccess) return fail(c, AFQMC_ERR_HIP, "h1e_mod: %s", hipGetErrorString(e));
+  return AFQMC_OK;
+}
+
+int afqmc_setup_half_rotate(afqmc_ctx* c, int32_t spin, const double* psi, int32_t ld_psi, const double* chol,
+                            const double* h1, double* rchol, double* rh1) {
+  int r = setup_ready(c);
+  if (r) return r;
+  if (spin != 0 && spin != 1) return fail(c, AFQMC_ERR_INVALID_ARG, "spin must be 0 or 1");
+  const int64_t N = c->d.nbasis, X = c->d.nchol;
+  const int64_t n = spin == 0 ? c->d.nalpha : c->d.nbeta;
+  if (n < 1) return AFQMC_OK;
+  if (ld_psi < n) return fail(c, AFQMC_ERR_SHAPE, "ld_psi=%d < %lld occupied orbitals", ld_psi, (long long)n);
+  if (!psi || !chol || !h1 || !rchol || !rh1) return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  // rchol[x, (i, m)] = sum_k psi[k, i] L[k, m, x]: C [n, N X] = psi^T Lv, stored as [X, n N]
+  SetupGemmArgs g = setup_args(n, N * X, N);
+  g.a = psi;
+  g.sar = 1;
+  g.sac = ld_psi;
+  g.b = chol;
+  g.sbr = N * X;
+  g.sbc = 1;
+  g.c = rchol;
+  g.store = SETUP_STORE_HALF_ROTATED;
+  g.hr_x = X;
+  g.hr_nN = n * N;
+  g.hr_N = N;
+  r = setup_gemm(c, g);
+  if (r) return r;
+  // rH1[i, q] = sum_p psi[p, i] H1[p, q], complex with a zero imaginary part
+  HIP_TRY(c, launch_setup_fill(rh1, 2 * n * N, 0.0, c->stream));
+  SetupGemmArgs h = setup_args(n, N, N);
+  h.a = psi;
+  h.sar = 1;
+  h.sac = ld_psi;
+  h.b = h1;
+  h.sbr = N;
+  h.sbc = 1;
+  h.c = rh1;
+  h.scr = 2 * N;
+  h.scc = 2;
+  return setup_gemm(c, h);
+}
+
+int afqmc_setup_mf_shift(afqmc_ctx* c, const double* chol, const double* gcharge, double* mf) {
+  int r = setup_ready(c);
+  if (r) return r;
+  if (!chol || !gcharge || !mf) return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  const int64_t N = c->d.nbasis, X = c->d.nchol;
+  // mf = i L^T Re(g) - L^T Im(g): imaginary part from Re(g), real part from -Im(g)
+  for (int part = 0; part < 2; ++part) {
+    SetupGemmArgs g = setup_args(X, 1, N * N);
+    g.a = chol;
+    g.sar = 1;
+    g.sac = X;
+    g.b = gcharge + (part == 0 ? 0 : 1);
+    g.sbr = 2;
+    g.sbc = 1;
+    g.c = mf + (part == 0 ? 1 : 0);
+    g.scr = 2;
+    g.scc = 1;
+    g.alpha = part == 0 ? 1.0 : -1.0;
+    r = setup_gemm(c, g);
+    if (r) return r;
+  }
+  return AFQMC_OK;
+}
+
+int afqmc_setup_expH1(afqmc_ctx* c, const double* chol, const double* h1e_mod, const double* mf, double dt,
+                      double* expH1) {
+  int r = setup_ready(c);
+  if (r) return r;
+  if (!chol || !h1e_mod || !mf || !expH1) return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  if (!(dt > 0.0)) return fail(c, AFQMC_ERR_INVALID_ARG, "dt must be positive");
+  const int N = c->d.nbasis;
+  const int64_t X = c->d.nchol, NN = (int64_t)N * N;
+  double* buf = nullptr;  // v, A, P, T, norm
+  if (dmalloc((void**)&buf, 8 * (size_t)(4 * NN + 1)) != hipSuccess)
+    return fail(c, AFQMC_ERR_ALLOC, "set-up scratch");
+  double *v = buf, *A = buf + NN, *P = buf + 2 * NN, *T = buf + 3 * NN, *nrm = buf + 4 * NN;
+  auto done = [&](int code) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    return code;
+  };
+  auto hip_fail = [&](hipError_t e) { return done(fail(c, AFQMC_ERR_HIP, "expH1: %s", hipGetErrorString(e))); };
+  // v = sum_x L_x Im(mf_x): H1' = h1e_mod - i sum_x L_x mf_x = h1e_mod + v for a purely imaginary mf
+  SetupGemmArgs g = setup_args(NN, 1, X);
+  g.a = chol;
+  g.sar = X;
+  g.sac = 1;
+  g.b = mf + 1;
+  g.sbr = 2;
+  g.sbc = 1;
+  g.c = v;
+  g.scr = 1;
+  g.scc = 1;
+  r = setup_gemm(c, g);
+  if (r) return done(r);
+  constexpr int DEG = 18;  // Taylor degree on ||A||_1 <= 1/2: remainder below 1e-22
+  for (int s = 0; s < 2; ++s) {
+    hipError_t e = launch_setup_axpby(NN, -0.5 * dt, h1e_mod + s * NN, -0.5 * dt, v, A, c->stream);
+    if (e == hipSuccess) e = launch_setup_norm1(N, A, nrm, c->stream);
+    double hn = 0.0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&hn, nrm, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e);
+    if (!std::isfinite(hn)) return done(fail(c, AFQMC_ERR_INVALID_ARG, "expH1: non-finite one-body operator"));
+    int sq = 0;
+    while (hn > 0.5 && sq < 64) {
+      hn *= 0.5;
+      ++sq;
+    }
+    const double sc = std::ldexp(1.0, -sq);
+    // A <- A / 2^sq; P = I + A / DEG
+    e = launch_setup_axpby(NN, sc, A, 0.0, A, A, c->stream);
+    if (e == hipSuccess) e = launch_setup_axpby(NN, 1.0 / DEG, A, 0.0, A, P, c->stream);
+    if (e == hipSuccess) e = launch_setup_add_identity(N, P, c->stream);
+    if (e != hipSuccess) return hip_fail(e);
+    // Horner: P <- I + A P / k for k = DEG - 1 .. 1
+    for (int k = DEG - 1; k >= 1; --k) {
+      SetupGemmArgs h = setup_args(N, N, N);
+      h.a = A;
+      h.sar = N;
+      h.sac = 1;
+      h.b = P;
+      h.sbr = N;
+      h.sbc = 1;
+      h.c = T;
+      h.scr = N;
+      h.scc = 1;
+      h.alpha = 1.0 / k;
+      r = setup_gemm(c, h);
+      if (r) return done(r);
+      e = launch_setup_add_identity(N, T, c->stream);
+      if (e != hipSuccess) return hip_fail(e);
+      std::swap(P, T);
+    }
+    // squaring: exp(A 2^sq) = exp(A)^(2^sq)
+    for (int q = 0; q < sq; ++q) {
+      SetupGemmArgs h = setup_args(N, N, N);
+      h.a = P;
+      h.sar = N;
+      h.sac = 1;
+      h.b = P;
+      h.sbr = N;
+      h.sbc = 1;
+      h.c = T;
+      h.scr = N;
+      h.scc = 1;
+      r = setup_gemm(c, h);
+      if (r) return done(r);
+      std::swap(P, T);
+    }
+    e = hipMemcpyAsync(expH1 + s * NN, P, 8 * (size_t)NN, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) return hip_fail(e);
+  }
+  return done(AFQMC_OK);
+}
+
 int afqmc_walker_sums(afqmc_ctx* c, int32_t nw, const double* weight, const double* unscaled_weight,
                       const double* values, int64_t value_stride, double* acc) {
   if (!c) return AFQMC_ERR_INVALID_ARG;

@@ -433,6 +433,33 @@ struct PopPlanArgs {
   int* npairs;              // [1] out: branched pairs (pair_branch)
 };
 hipError_t launch_pop_plan(const PopPlanArgs& a, hipStream_t stream);
+// ---------------------------------------------------------------- set-up producers
+enum { SETUP_STORE_STRIDED = 0, SETUP_STORE_HALF_ROTATED = 1 };
+struct SetupGemmArgs {
+  int64_t m, n, k;          // C [m, n] = alpha sum_k A[i, k] B[k, j] + beta C
+  const double* a;
+  int64_t sar, sac;         // A[i, k] = a[i sar + k sac]
+  const double* b;
+  int64_t sbr, sbc;         // B[k, j] = b[k sbr + j sbc]
+  double* c;
+  int64_t scr, scc;         // C[i, j] = c[i scr + j scc] (SETUP_STORE_STRIDED)
+  double alpha, beta;
+  int64_t kslice;           // set by the launcher
+  double* part;             // split-K planes [slices, m, n] (launcher's nz > 1)
+  int store;
+  int64_t hr_x, hr_nN, hr_N;  // SETUP_STORE_HALF_ROTATED: column j = m X + x -> c[x nN + i N + m]
+};
+int64_t setup_gemm_part_doubles(int64_t m, int64_t n, int nz);
+hipError_t launch_setup_gemm(SetupGemmArgs g, int nz, hipStream_t stream);
+hipError_t launch_setup_pack(int N, int X, const double* chol, double* packed, hipStream_t stream);
+hipError_t launch_setup_fill(double* out, int64_t n, double v, hipStream_t stream);
+hipError_t launch_setup_scale_shift(int64_t n, const double* h, const double* v, double scale, double* out,
+                                    hipStream_t stream);
+hipError_t launch_setup_axpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* out,
+                              hipStream_t stream);
+hipError_t launch_setup_add_identity(int N, double* a, hipStream_t stream);
+hipError_t launch_setup_norm1(int N, const double* a, double* out, hipStream_t stream);
+
 // acc[0] += sum w, acc[1] += sum uw (uw non-null), acc[2] += sum w v[i * vstride]
 // (v non-null); acc is [3] complex, sums in a fixed order (one workgroup)
 hipError_t launch_walker_sums(int nw, const double* weight, const double* unscaled, const double* values,

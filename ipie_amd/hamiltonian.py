@@ -12,7 +12,9 @@ import torch
 
 
 class GenericRealChol:
-    def __init__(self, h1e, chol, ecore=0.0, verbose=False):
+    def __init__(self, h1e, chol, ecore=0.0, verbose=False, host_setup=True):
+        """host_setup=False leaves chol_packed and h1e_mod to the device set-up
+        (PhaselessGeneric.build(..., device_setup=True))."""
         h1e = numpy.asarray(h1e)
         assert h1e.shape[0] == 2, "h1e must hold both spin components"
         chol = numpy.asarray(chol)
@@ -27,14 +29,17 @@ class GenericRealChol:
         self.sym_idx = numpy.triu_indices(self.nbasis)
         self.sym_idx_i = self.sym_idx[0].copy()
         self.sym_idx_j = self.sym_idx[1].copy()
-        chol3 = chol.reshape(self.nbasis, self.nbasis, self.nchol)
-        self.chol_packed = numpy.ascontiguousarray(chol3[self.sym_idx_i, self.sym_idx_j, :])
         self.chunked = False
-        # one-body term from re-ordering the two-body operator
-        chol_view = chol.reshape(self.nbasis, self.nbasis * self.nchol)
-        v0 = 0.5 * chol_view.dot(chol_view.T)
-        self.h1e_mod = numpy.array([h1e[0] - v0, h1e[1] - v0])
         self._dev = {}
+        self.chol_packed = None
+        self.h1e_mod = None
+        if host_setup:
+            chol3 = chol.reshape(self.nbasis, self.nbasis, self.nchol)
+            self.chol_packed = numpy.ascontiguousarray(chol3[self.sym_idx_i, self.sym_idx_j, :])
+            # one-body term from re-ordering the two-body operator
+            chol_view = chol.reshape(self.nbasis, self.nbasis * self.nchol)
+            v0 = 0.5 * chol_view.dot(chol_view.T)
+            self.h1e_mod = numpy.array([h1e[0] - v0, h1e[1] - v0])
         if verbose:
             print(f"# GenericRealChol: N={self.nbasis} X={self.nchol}")
 
@@ -42,6 +47,9 @@ class GenericRealChol:
         """chol_packed as a device tensor (uploaded once per device)."""
         key = str(device)
         if key not in self._dev:
+            if self.chol_packed is None:
+                raise RuntimeError("chol_packed was left to the device set-up: build the propagator with "
+                                   "device_setup=True")
             self._dev[key] = {
                 "chol_packed": torch.from_numpy(self.chol_packed).to(device),
             }

@@ -87,17 +87,24 @@ class PhaselessGeneric:
         self._engine = None
 
     # ------------------------------------------------------------------ set-up
-    def build(self, hamiltonian, trial=None, walkers=None, mpi_handler=None, verbose=False):
-        """Mean-field shift, one-body propagator and device context (phaseless_base.py:256-272)."""
+    def build(self, hamiltonian, trial=None, walkers=None, mpi_handler=None, verbose=False, device_setup=False):
+        """Mean-field shift, one-body propagator and device context (phaseless_base.py:256-272).
+
+        device_setup=True builds every static tensor (packed Cholesky vectors,
+        h1e_mod, half rotation, mean-field shift, expH1) with the gfx950
+        set-up kernels from hamiltonian.chol instead of on the host."""
         if walkers is None:
             raise ValueError("the MI355X propagator needs the walkers at build time (device context)")
-        if not trial.half_rotated:
-            trial.half_rotate(hamiltonian)
-        self.mf_shift = construct_mean_field_shift(hamiltonian, trial)
-        self.expH1 = construct_one_body_propagator(hamiltonian, self.mf_shift, self.dt)
-        eng = _engine.Engine(trial, walkers, nchol=hamiltonian.nchol, exp_nmax=self.exp_nmax,
-                             precision=self.precision)
-        eng.set_hamiltonian(hamiltonian, self.expH1, self.mf_shift, self.dt)
+        if device_setup:
+            eng = self._setup_on_device(hamiltonian, trial, walkers)
+        else:
+            if not trial.half_rotated:
+                trial.half_rotate(hamiltonian)
+            self.mf_shift = construct_mean_field_shift(hamiltonian, trial)
+            self.expH1 = construct_one_body_propagator(hamiltonian, self.mf_shift, self.dt)
+            eng = _engine.Engine(trial, walkers, nchol=hamiltonian.nchol, exp_nmax=self.exp_nmax,
+                                 precision=self.precision)
+            eng.set_hamiltonian(hamiltonian, self.expH1, self.mf_shift, self.dt)
         if self.surrogate:
             opts = dict(fraction=0.125, warmup=100, train_iters=4, lr=1e-3, seed=self.seed)
             opts.update(self.surrogate if isinstance(self.surrogate, dict) else {})
@@ -105,6 +112,54 @@ class PhaselessGeneric:
         walkers._engine = eng
         self._engine = eng
         self.mpi_handler = mpi_handler
+
+    def _setup_on_device(self, ham, trial, walkers):
+        """Static tensors from the set-up kernels (generic.py:31-43, 64-72;
+        half_rotate.py:11-116; phaseless_base.py:27-74, 153-172), for a real
+        Hamiltonian and a real trial; the host keeps small copies of mf_shift,
+        expH1 and h1e_mod."""
+        psi = numpy.asarray(trial.psi)
+        if numpy.iscomplexobj(psi) and numpy.abs(psi.imag).max(initial=0.0) > 0.0:
+            raise NotImplementedError("device set-up needs a real trial")
+        gcharge = numpy.asarray(trial.G[0] + trial.G[1]).ravel()
+        if numpy.iscomplexobj(gcharge) and numpy.abs(gcharge.imag).max(initial=0.0) > 0.0:
+            raise NotImplementedError("device set-up needs a real trial density")
+        eng = _engine.Engine(trial, walkers, nchol=ham.nchol, exp_nmax=self.exp_nmax, precision=self.precision)
+        ctx, dev = eng.ctx, walkers.device
+        N, X = ham.nbasis, ham.nchol
+        f64, c128 = torch.float64, torch.complex128
+        chol = torch.from_numpy(numpy.ascontiguousarray(ham.chol, dtype=numpy.float64)).to(dev)
+        h1e = torch.from_numpy(numpy.ascontiguousarray(ham.H1, dtype=numpy.float64)).to(dev)
+        packed = torch.empty(N * (N + 1) // 2, X, dtype=f64, device=dev)
+        ctx.setup_pack(chol, packed)
+        h1e_mod = torch.empty(2, N, N, dtype=f64, device=dev)
+        ctx.setup_h1e_mod(chol, h1e, h1e_mod)
+        rot = {}
+        for s, (orbs, n) in enumerate(((trial.psi0a, trial.nalpha), (trial.psi0b, trial.nbeta))):
+            r = torch.zeros(X, n * N, dtype=f64, device=dev)
+            rh = torch.zeros(n, N, dtype=c128, device=dev)
+            if n > 0:
+                p = torch.from_numpy(numpy.ascontiguousarray(numpy.real(orbs), dtype=numpy.float64)).to(dev)
+                ctx.setup_half_rotate(s, p, chol, h1e[s].contiguous(), r, rh)
+            rot[s] = (r, rh)
+        mf = torch.empty(X, dtype=c128, device=dev)
+        ctx.setup_mf_shift(chol, torch.from_numpy(gcharge.astype(numpy.complex128)).to(dev), mf)
+        expH1 = torch.empty(2, N, N, dtype=f64, device=dev)
+        ctx.setup_expH1(chol, h1e_mod, mf, self.dt, expH1)
+        del chol
+        key = str(dev)
+        ham._dev[key] = {"chol_packed": packed}
+        td = trial.device_tensors(dev)
+        td.update(rchola=rot[0][0], rcholb=rot[1][0], rH1a=rot[0][1], rH1b=rot[1][1])
+        trial.half_rotated = True
+        trial._rchola = trial._rcholb = trial._rH1a = trial._rH1b = None  # on the device; fetched when read
+        eng._set_static_partial(chol_packed=packed, rchola=rot[0][0], rcholb=rot[1][0], rH1a=rot[0][1],
+                                rH1b=rot[1][1], expH1=expH1, mf_shift=mf, dt=float(self.dt), ecore=float(ham.ecore))
+        self.mf_shift = mf.cpu().numpy()
+        self.expH1 = expH1.cpu().numpy()
+        if getattr(ham, "h1e_mod", None) is None:
+            ham.h1e_mod = h1e_mod.cpu().numpy()
+        return eng
 
     def cast_to_cupy(self, verbose=False):
         return None

@@ -73,8 +73,18 @@ class SingleDet:
         self.half_rotated = True
         self._dev = {}
 
+    def _host_rotated(self):
+        """Host copies of rchol / rH1 after a device half rotation."""
+        if self._rchola is None and self._dev:
+            d = next(iter(self._dev.values()))
+            self._rchola = d["rchola"].cpu().numpy()
+            self._rcholb = d["rcholb"].cpu().numpy()
+            self._rH1a = d["rH1a"].cpu().numpy()
+            self._rH1b = d["rH1b"].cpu().numpy()
+
     def calculate_energy(self, system, hamiltonian):
         """Trial energy (ipie/trial_wavefunction/single_det.py:65-86)."""
+        self._host_rotated()
         Ga, Gb = self.Ghalf
         self.e1b = numpy.sum(Ga * self._rH1a) + numpy.sum(Gb * self._rH1b) + hamiltonian.ecore
         Xa = self._rchola.dot(Ga.ravel()) if self.nalpha else 0.0
