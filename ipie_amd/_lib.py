@@ -376,9 +376,9 @@ class Context:
         _check_tensor("expH1", expH1, torch.float64, (2, N, N), dev)
         self._call("afqmc_setup_expH1", ptr(chol), ptr(h1e_mod), ptr(mf), float(dt), ptr(expH1))
 
-    def walker_sums(self, weight, acc, unscaled_weight=None, values=None):
+    def walker_sums(self, weight, acc, unscaled_weight=None, values=None, column=0):
         """acc (complex128 [3]) += (sum weight, sum unscaled_weight, sum weight * values)
-        (afqmc_walker_sums).  values: complex128 [W] or a [W, k] view's column 0."""
+        (afqmc_walker_sums).  values: complex128 [W], or [W, k] read at `column`."""
         nw = weight.shape[0]
         self._chk_vec("weight", weight, nw, torch.float64)
         _check_tensor("acc", acc, torch.complex128, (3,), self.device)
@@ -392,9 +392,12 @@ class Context:
                 if not values.is_contiguous():
                     raise ValueError("values must be contiguous")
                 stride = values.shape[1]
-            elif values.dim() != 1 or not values.is_contiguous():
+                if not 0 <= column < stride:
+                    raise ValueError(f"column {column} outside [0, {stride})")
+            elif values.dim() != 1 or not values.is_contiguous() or column != 0:
                 raise ValueError("values must be [W] or [W, k] contiguous")
-        self._call("afqmc_walker_sums", nw, ptr(weight), ptr(unscaled_weight), ptr(values), stride, ptr(acc))
+        vptr = None if values is None else ctypes.c_void_p(values.data_ptr() + 16 * column)
+        self._call("afqmc_walker_sums", nw, ptr(weight), ptr(unscaled_weight), vptr, stride, ptr(acc))
 
     def clip_weights(self, weight, bound):
         nw = weight.shape[0]

@@ -48,13 +48,19 @@ class EnergyEstimator:
         return self.names.index(name)
 
     def compute_estimator(self, system=None, walkers=None, hamiltonian=None, trial=None):
+        """Weighted sums on the device (afqmc_walker_sums, fixed order), one
+        read-back for the block."""
         trial.calc_greens_function(walkers)
         energy = local_energy(system, hamiltonian, walkers, trial)
-        w = walkers.weight
-        self._data["ENumer"] = complex(torch.sum(w * energy[:, 0].real).item())
-        self._data["EDenom"] = complex(torch.sum(w).item())
-        self._data["E1Body"] = complex(torch.sum(w * energy[:, 1].real).item())
-        self._data["E2Body"] = complex(torch.sum(w * energy[:, 2].real).item())
+        ctx = _engine.engine_for(trial, walkers).ctx
+        acc = torch.zeros(3, 3, dtype=torch.complex128, device=walkers.device)
+        for k in range(3):
+            ctx.walker_sums(walkers.weight, acc[k], values=energy, column=k)
+        s = acc.cpu().numpy()
+        self._data["ENumer"] = complex(s[0, 2].real)
+        self._data["EDenom"] = complex(s[0, 0].real)
+        self._data["E1Body"] = complex(s[1, 2].real)
+        self._data["E2Body"] = complex(s[2, 2].real)
         return self.data
 
     def post_reduce_hook(self, data):

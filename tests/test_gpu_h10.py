@@ -58,3 +58,27 @@ def test_h10_steps_match_oracle():
         for attr in ("phia", "phib", "weight", "ovlp"):
             err = rel_err(getattr(walkers, attr).cpu().numpy(), getattr(orc.walkers, attr))
             assert err < 1e-10, (step, attr, err)
+
+
+def test_h10_energy_estimator_block0():
+    """The EnergyEstimator drop-in (weighted sums on the device) gives ipie's
+    block-0 row: ENumer / EDenom etc. after post_reduce_hook."""
+    import torch
+
+    from ipie_amd import io
+    from ipie_amd.estimators import EnergyEstimator
+
+    print("[case] h10 EnergyEstimator", flush=True)
+    ref = json.load(open(REF))
+    W = 25
+    system, ham, trial, walkers, prop = io.load_generic(H5, nwalkers=W)
+    rs = numpy.random.RandomState(1)
+    w = rs.uniform(0.5, 1.5, size=W)
+    walkers.weight.copy_(torch.from_numpy(w))
+    est = EnergyEstimator(system, ham, trial)
+    data = numpy.array(est.compute_estimator(system, walkers, ham, trial))
+    assert abs(data[est.get_index("EDenom")] - w.sum()) < 1e-12 * w.sum()
+    est.post_reduce_hook(data)
+    assert abs(data[est.get_index("ETotal")].real - ref["ETotal"]) < 1e-10
+    assert abs(data[est.get_index("E1Body")].real - ref["E1Body"]) < 1e-10
+    assert abs(data[est.get_index("E2Body")].real - ref["E2Body"]) < 1e-10
