@@ -69,6 +69,7 @@ class PopController:
         self.target_weight = self.ntot_walkers
         self.total_weight = self.ntot_walkers
         self.timer = PopControllerTimer()
+        self.last_npairs = 0  # walkers branched by the last pair_branch call
         self._bufs = {}
         if verbose:
             print(f"# Using {self.method} population control algorithm.")
@@ -128,9 +129,11 @@ class PopController:
         ctx.pop_plan(method, walkers.weight, self.target_weight, self.min_weight, self.max_weight, u_dev, b["src"],
                      b["new_weight"], b["total"], b["npairs"])
         total = float(b["total"].item())
+        self.last_npairs = 0
         if state is not None:
             numpy.random.set_state(state)
             npairs = int(b["npairs"].item())
+            self.last_npairs = npairs
             if npairs:
                 numpy.random.rand(npairs)
         if total < 1e-8:
