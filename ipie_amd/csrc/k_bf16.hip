@@ -87,14 +87,27 @@ __device__ __forceinline__ v4f mfma_bf16(v8bf a, v8bf b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// LDS slot of 16-byte vector `seg` (0..3) of staged row `row`.  Rows are 64
-// bytes, so without a swizzle rows r and r + 4 of one fragment read (16 rows,
-// same segment) fall on the same banks; XOR-ing the segment with bits 2..3 of
-// the row spreads the 16 rows over all 64 banks.
-__device__ __forceinline__ int lds_vec(int row, int seg) { return row * 4 + (seg ^ ((row >> 2) & 3)); }
+// LDS slot of 16-byte vector `seg` (0..3) of staged row `row`.  A fragment
+// read takes one segment of 16 consecutive rows (64-byte rows: 4 slots), and a
+// 16-byte LDS read is served a group of lanes at a time.  The segment is
+// XOR-ed with ((row >> 1) ^ (row >> 3)) & 3 so that every aligned group of 8
+// rows covers 8 distinct slots modulo 8 (32 banks of 4 bytes) and every
+// aligned group of 16 rows 16 distinct slots modulo 16 (64 banks): no bank
+// conflicts whichever the group size.  (XOR with bits 2..3 of the row alone
+// leaves 2-way conflicts in 8-lane groups, which the bank-conflict counter
+// showed on every GEMM.)
+__device__ __forceinline__ int lds_swz(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
+__device__ __forceinline__ int lds_vec(int row, int seg) { return row * 4 + (seg ^ lds_swz(row)); }
 
 __device__ __forceinline__ v8bf lds_frag(const uint16_t* base, int row, int seg) {
   return as_v8bf(reinterpret_cast<const v4u*>(base)[lds_vec(row, seg)]);
+}
+
+// The Taylor kernel's staging keeps the XOR with bits 2..3 of the row: with
+// its 80 x 80 tiles and 5 waves it measured faster than the swizzle above.
+__device__ __forceinline__ int lds_vec_ty(int row, int seg) { return row * 4 + (seg ^ ((row >> 2) & 3)); }
+__device__ __forceinline__ v8bf lds_frag_ty(const uint16_t* base, int row, int seg) {
+  return as_v8bf(reinterpret_cast<const v4u*>(base)[lds_vec_ty(row, seg)]);
 }
 
 // Ghalf [W*n, N] complex -> 4 bf16 planes [rows, Kp]; with walker_of, plane
@@ -202,6 +215,98 @@ __device__ __forceinline__ void bx_stash(uint4* lds_u4, const v4u (&pa)[Bx<SPLIT
   }
 }
 
+
+// Direct global -> LDS staging (global_load_lds_dwordx4, no VGPR round
+// trip).  Slot s of a sub-chunk region holds staged vector (row s / 4,
+// segment (s % 4) ^ ((row / 4) % 4)): the XOR swizzle of lds_vec moves from
+// the LDS address to the source address, so the 64 lanes of one load fill 64
+// consecutive slots, as the instruction requires.  Each wave stages an equal
+// share of the A and B vectors with the same instruction count (the last
+// instruction of a share may have idle lanes), so a wait on vmcnt can name
+// the loads of a later stage that may stay in flight.
+template <bool SPLIT>
+struct Dl {
+  static constexpr int WAVES = BX_THREADS / 64;
+  static constexpr int A_WAVE = Bx<SPLIT>::A_VEC / WAVES, B_WAVE = Bx<SPLIT>::B_VEC / WAVES;
+  static constexpr int A_INS = (A_WAVE + 63) / 64, B_INS = (B_WAVE + 63) / 64;
+  static constexpr int INS_SUB = A_INS + B_INS;  // load instructions per sub-chunk
+  static_assert(Bx<SPLIT>::A_VEC % WAVES == 0 && Bx<SPLIT>::B_VEC % WAVES == 0, "equal shares per wave");
+  static_assert(A_WAVE % 64 == 0, "A shares are whole instructions");
+  static_assert(Bx<SPLIT>::KS * INS_SUB <= 63, "vmcnt counts the loads of one stage");
+};
+
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// source offsets (in bf16 elements, K offset 0) of this lane's slots
+template <bool SPLIT>
+__device__ __forceinline__ void dl_sources(int64_t (&a_src)[Dl<SPLIT>::A_INS], int64_t (&b_src)[Dl<SPLIT>::B_INS],
+                                           const Bf16GemmArgs& a, int rb, int cb, int wave, int lane) {
+  using C = Bx<SPLIT>;
+#pragma unroll
+  for (int j = 0; j < Dl<SPLIT>::A_INS; ++j) {
+    const int s = wave * Dl<SPLIT>::A_WAVE + j * 64 + lane;
+    const int row = s >> 2, seg = (s & 3) ^ lds_swz(row);
+    const int p = row / BX_R;
+    int64_t r = (int64_t)rb * BX_R + row % BX_R;
+    if (r > a.m - 1) r = a.m - 1;
+    if (a.a_gstride > 0) r = (r / a.group) * a.a_gstride + a.a_goff + r % a.group;
+    a_src[j] = C::a_plane(p) * a.a_plane + r * a.lda + seg * 8;
+  }
+#pragma unroll
+  for (int j = 0; j < Dl<SPLIT>::B_INS; ++j) {
+    int s = wave * Dl<SPLIT>::B_WAVE + j * 64 + lane;
+    if (j * 64 + lane >= Dl<SPLIT>::B_WAVE) s = wave * Dl<SPLIT>::B_WAVE;  // idle lane: any valid address
+    const int row = s >> 2, seg = (s & 3) ^ lds_swz(row);
+    const int p = row / BX_C;
+    int64_t r = (int64_t)cb * BX_C + row % BX_C;
+    if (r > a.nn - 1) r = a.nn - 1;
+    b_src[j] = C::b_plane(p) * a.b_plane + r * a.ldb + seg * 8;
+  }
+}
+
+// issue the loads of sub-chunks c32 .. c32 + nsub - 1 into LDS stage `buf`
+template <bool SPLIT>
+__device__ __forceinline__ void dl_issue(uint4* lds_u4, int buf, int nsub, const uint16_t* ga, const uint16_t* gb,
+                                         const int64_t (&a_src)[Dl<SPLIT>::A_INS],
+                                         const int64_t (&b_src)[Dl<SPLIT>::B_INS], int c32, int wave, int lane) {
+  using C = Bx<SPLIT>;
+  using D = Dl<SPLIT>;
+#pragma unroll
+  for (int sc = 0; sc < C::KS; ++sc) {
+    if (sc >= nsub) break;
+    const int64_t k0 = (int64_t)(c32 + sc) * BX_KC;
+    uint4* base = lds_u4 + buf * C::STAGE_VEC + sc * C::SUB_VEC;
+#pragma unroll
+    for (int j = 0; j < D::A_INS; ++j)
+      __builtin_amdgcn_global_load_lds(
+          (const void*)(ga + a_src[j] + k0),
+          (__attribute__((address_space(3))) void*)(base + wave * D::A_WAVE + j * 64), 16, 0, 0);
+#pragma unroll
+    for (int j = 0; j < D::B_INS; ++j)
+      if (j * 64 + lane < D::B_WAVE)
+        __builtin_amdgcn_global_load_lds(
+            (const void*)(gb + b_src[j] + k0),
+            (__attribute__((address_space(3))) void*)(base + C::A_VEC + wave * D::B_WAVE + j * 64), 16, 0, 0);
+  }
+}
+
+// workgroup barrier for the direct-load pipeline: this wave's LDS reads are
+// complete and its loads of the next stage landed (dl_wait); unlike
+// __syncthreads it does not wait for the loads of the stage after, which stay
+// in flight across it
+__device__ __forceinline__ void dl_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// wait until at most `pend` sub-chunks of direct loads are outstanding
+template <bool SPLIT>
+__device__ __forceinline__ void dl_wait(int pend) {
+  constexpr int I = Dl<SPLIT>::INS_SUB;
+  if (pend >= 2) wait_vm<2 * I>();
+  else if (pend == 1) wait_vm<I>();
+  else wait_vm<0>();
+}
 
 // One K chunk from LDS stage `stage`: the wave's BX_MR x BX_MC tiles,
 // re += Re(A) B and im += Im(A) B, each as hi*hi + hi*lo + lo*hi (SPLIT) or
@@ -429,7 +534,7 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
 // QDOT slot is u BX_GC + wc; otherwise column tile cb and split z of ksplit.
 // TR (transposed accumulators, see bx_sub) is used with the ADD epilogue only.
 // Workgroups that run together share the B tile (row blocks vary fastest).
-template <bool SPLIT, bool TR>
+template <bool SPLIT, bool TR, bool DL>
 __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
   using C = Bx<SPLIT>;
   extern __shared__ uint4 lds_u4[];
@@ -482,9 +587,32 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
     // LDS stages of up to KS sub-chunks of 32 K values over this split's range
     // [c0, c1), three buffers: stage t + 2 is staged while t is computed, from
     // registers loaded during stage t - 1 (two stages of latency cover)
-    v4u pa[C::KS][C::A_PER], pb[C::KS][C::B_PER];
     const int nst = (int)ceil_div(c1 - c0, C::KS);
     auto nsub_of = [&](int t) { return min(C::KS, c1 - (c0 + t * C::KS)); };
+    if constexpr (DL) {
+      // direct loads: stage t + 2 is issued before stage t is computed; the
+      // wait after it leaves stage t + 2 in flight and lands stage t + 1
+      int64_t da[Dl<SPLIT>::A_INS], db[Dl<SPLIT>::B_INS];
+      dl_sources<SPLIT>(da, db, a, rb, cb, wave, lane);
+      if (nst > 0) dl_issue<SPLIT>(lds_u4, 0, nsub_of(0), a.a, a.b, da, db, c0, wave, lane);
+      if (nst > 1) dl_issue<SPLIT>(lds_u4, 1, nsub_of(1), a.a, a.b, da, db, c0 + C::KS, wave, lane);
+      dl_wait<SPLIT>(nst > 1 ? nsub_of(1) : 0);
+      dl_barrier();
+      int buf = 0;
+      for (int t = 0; t < nst; ++t) {
+        int pend = 0;
+        if (t + 2 < nst) {
+          dl_issue<SPLIT>(lds_u4, buf == 0 ? 2 : buf - 1, nsub_of(t + 2), a.a, a.b, da, db, c0 + (t + 2) * C::KS,
+                          wave, lane);
+          pend = nsub_of(t + 2);
+        }
+        bx_chunk<SPLIT, TR>(lds_u4 + buf * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub_of(t));
+        dl_wait<SPLIT>(pend);
+        dl_barrier();
+        buf = buf == 2 ? 0 : buf + 1;
+      }
+    } else {
+    v4u pa[C::KS][C::A_PER], pb[C::KS][C::B_PER];
     if (nst > 0) {
       bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0, nsub_of(0));
       bx_stash<SPLIT>(lds_u4, pa, pb, tid, 0, nsub_of(0));
@@ -503,6 +631,7 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
       if (t + 3 < nst) bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0 + (t + 3) * C::KS, nsub_of(t + 3));
       __syncthreads();
       buf = buf == 2 ? 0 : buf + 1;
+    }
     }
     if (a.epilogue == BF16_EPI_QDOT) {
       // per-row dot with d over this wave's 80 columns, then across the 16 lanes
@@ -759,8 +888,8 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
     for (int i = 0; i < TY_APER; ++i) {
       const int sc = i / T::NP, pi = i % T::NP;
       if (t * T::KS + sc < nch) {
-        base[sc * T::SUB_VEC + lds_vec(pi * TY_R + srow, sseg)] = pa[i];
-        base[sc * T::SUB_VEC + T::SUB_A + lds_vec(pi * TY_C + srow, sseg)] = pb[i];
+        base[sc * T::SUB_VEC + lds_vec_ty(pi * TY_R + srow, sseg)] = pa[i];
+        base[sc * T::SUB_VEC + T::SUB_A + lds_vec_ty(pi * TY_C + srow, sseg)] = pb[i];
       }
     }
   };
@@ -783,13 +912,13 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
       const int bcol = wave * 16 + l16;
       // operands swapped in every MFMA: the accumulators hold the transposed tiles
       if (SPLIT) {
-        const v8bf brh = lds_frag(Bs, bcol, q), brl = lds_frag(Bs, TY_C + bcol, q);
-        const v8bf bih = lds_frag(Bs, 2 * TY_C + bcol, q), bil = lds_frag(Bs, 3 * TY_C + bcol, q);
+        const v8bf brh = lds_frag_ty(Bs, bcol, q), brl = lds_frag_ty(Bs, TY_C + bcol, q);
+        const v8bf bih = lds_frag_ty(Bs, 2 * TY_C + bcol, q), bil = lds_frag_ty(Bs, 3 * TY_C + bcol, q);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
           const int arow = i * 16 + l16;
-          const v8bf arh = lds_frag(As, arow, q), arl = lds_frag(As, TY_R + arow, q);
-          const v8bf aih = lds_frag(As, 2 * TY_R + arow, q), ail = lds_frag(As, 3 * TY_R + arow, q);
+          const v8bf arh = lds_frag_ty(As, arow, q), arl = lds_frag_ty(As, TY_R + arow, q);
+          const v8bf aih = lds_frag_ty(As, 2 * TY_R + arow, q), ail = lds_frag_ty(As, 3 * TY_R + arow, q);
           const v8bf naih = neg_bf16(aih), nail = neg_bf16(ail);
           re[i] = mfma_bf16(brh, arh, re[i]);
           re[i] = mfma_bf16(brl, arh, re[i]);
@@ -805,11 +934,11 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
           im[i] = mfma_bf16(brh, ail, im[i]);
         }
       } else {
-        const v8bf brh = lds_frag(Bs, bcol, q), bih = lds_frag(Bs, TY_C + bcol, q);
+        const v8bf brh = lds_frag_ty(Bs, bcol, q), bih = lds_frag_ty(Bs, TY_C + bcol, q);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
           const int arow = i * 16 + l16;
-          const v8bf arh = lds_frag(As, arow, q), aih = lds_frag(As, TY_R + arow, q);
+          const v8bf arh = lds_frag_ty(As, arow, q), aih = lds_frag_ty(As, TY_R + arow, q);
           re[i] = mfma_bf16(brh, arh, re[i]);
           re[i] = mfma_bf16(bih, neg_bf16(aih), re[i]);
           im[i] = mfma_bf16(bih, arh, im[i]);
@@ -961,6 +1090,11 @@ hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream) {
   return a.split ? launch_exchange_bf16_t<true>(a, stream) : launch_exchange_bf16_t<false>(a, stream);
 }
 
+#ifndef AFQMC_BF16_DL
+#define AFQMC_BF16_DL 1
+#endif
+constexpr bool kDl = AFQMC_BF16_DL != 0;
+
 hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream) {
   if (a.m < 1 || a.nn < 1 || a.kchunks < 1 || a.ksplit < 1 || a.lda % BX_KC != 0 || a.ldb % BX_KC != 0 ||
       (int64_t)a.kchunks * BX_KC > a.lda || (int64_t)a.kchunks * BX_KC > a.ldb)
@@ -975,19 +1109,19 @@ hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream) {
   const int64_t grid = ceil_div(a.m, BX_R) * (a.units ? (int64_t)a.nunits : nb * a.ksplit);
   if (a.single) {
     if (a.epilogue == BF16_EPI_VHS) return hipErrorInvalidValue;
-    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<false, false>, Bx<false>::STAGES3);
+    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<false, false, kDl>, Bx<false>::STAGES3);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((bf16_gemm_kernel<false, false>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<false>::STAGES3,
+    hipLaunchKernelGGL((bf16_gemm_kernel<false, false, kDl>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<false>::STAGES3,
                        stream, a);
   } else if (a.epilogue == BF16_EPI_ADD) {
-    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true, true>, Bx<true>::STAGES3);
+    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true, true, kDl>, Bx<true>::STAGES3);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((bf16_gemm_kernel<true, true>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES3,
+    hipLaunchKernelGGL((bf16_gemm_kernel<true, true, kDl>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES3,
                        stream, a);
   } else {
-    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true, false>, Bx<true>::STAGES3);
+    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true, false, kDl>, Bx<true>::STAGES3);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((bf16_gemm_kernel<true, false>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES3,
+    hipLaunchKernelGGL((bf16_gemm_kernel<true, false, kDl>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES3,
                        stream, a);
   }
   return hipGetLastError();
