@@ -499,8 +499,8 @@ def main():
             if form == "quad":
                 kname = "Y = G Wq on FP64 MFMA (force_bias_kernel NT mainloop) + quad_reduce_kernel"
             elif form == "quad_sym":
-                kname = ("Y = G W' over block-triangular W' split-BF16 x3 (bf16_gemm_kernel<true, false>, g.Y in "
-                         "the epilogue) + quad_sum_kernel")
+                kname = ("Y = G W' over block-triangular W' split-BF16 x3 (bf16_gemm_kernel<true, false, true>, g.Y "
+                         "in the epilogue) + quad_sum_kernel")
             else:
                 kname = ("exchange_tiled_kernel (FP64" if mode == "fp64" else "exchange_bf16_kernel (split-BF16 x3")
                 kname += ", every T_x formed)"
@@ -516,7 +516,7 @@ def main():
             cv_ms, cv_n = prof["exchange_cv"]
             cv_avg_s = cv_ms / max(cv_n, 1) / 1e3
             cv_kernel = ("Q = dG^T Wq dG over block-triangular W', single BF16 product (bf16_gemm_kernel<false, "
-                         "false>, dG.Y in the epilogue) + quad_sum_kernel" if form == "quad_sym"
+                         "false, true>, dG.Y in the epilogue) + quad_sum_kernel" if form == "quad_sym"
                          else "exchange_bf16_kernel<false> (single BF16 product, every dT_x formed)")
             rec["roofline_kernel"] = dict(
                 kernel="control-variate exchange of every walker, one launch per spin: " + cv_kernel,
