@@ -4,8 +4,8 @@ Covers what the integral and wavefunction files ipie reads use: superblock
 versions 0 and 1, groups as symbol tables (v1 B-trees of symbol nodes with a
 local heap) and as link messages, object headers version 1 and 2, datasets
 with contiguous or compact layout (chunked storage without filters too), and
-element types IEEE float, integer and compound (read as numpy structured
-types; complex pairs as written by h5py become complex128).  It follows the
+element types IEEE float, integer, fixed-length string and compound (read as
+numpy structured types; complex pairs as written by h5py become complex128).  It follows the
 public HDF5 File Format Specification (version 3.0); everything it does not
 cover raises NotImplementedError.  Nothing in a file is executed.
 
@@ -221,6 +221,8 @@ class H5File:
             return numpy.dtype(f"{order}{'i' if bits & 0x08 else 'u'}{size}")
         if cls == 1:
             return numpy.dtype(f"{order}f{size}")
+        if cls == 3:  # fixed-length string
+            return numpy.dtype(f"S{size}")
         if cls == 6:
             ver = b[p] >> 4
             nmem = bits & 0xFFFF
@@ -258,6 +260,8 @@ class H5File:
             return dt, 8 + 4
         if cls == 1:
             return dt, 8 + 12
+        if cls == 3:
+            return dt, 8
         raise NotImplementedError("nested compound members")
 
     def _read_dataset(self, addr):

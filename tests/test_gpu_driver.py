@@ -24,11 +24,12 @@ H5 = os.path.join(HERE, "golden", "h10_afqmc.h5")
 TOL = 1e-9
 
 
-def test_driver_loop_matches_oracle():
+def test_driver_loop_matches_oracle(tmp_path):
     import torch
 
     from ipie_amd import io
-    from ipie_amd.estimators import EnergyEstimator
+    from ipie_amd.estimators import EnergyEstimator, EstimatorHandler
+    from ipie_amd.h5read import H5File
     from ipie_amd.pop_controller import PopController
     from ipie_amd.walkers import WalkerAccumulator, clip_weights
 
@@ -42,6 +43,10 @@ def test_driver_loop_matches_oracle():
     pcontrol = PopController(W, block)
     acc = WalkerAccumulator(["Weight", "WeightFactor", "HybridEnergy"], block)
     est = EnergyEstimator(system, ham, trial)
+    h5_path = os.path.join(str(tmp_path), "estimates.0.h5")
+    handler = EstimatorHandler(None, system, ham, trial, walker_state=acc, filename=h5_path)
+    handler.initialize(None, print_header=False)
+    blocks = []
     o_acc = numpy.zeros(3, dtype=numpy.complex128)
     o_total = float(W)
     eshift = float(trial.energy.real)
@@ -90,5 +95,17 @@ def test_driver_loop_matches_oracle():
                                numpy.sum(ow.weight * e[:, 1].real), numpy.sum(ow.weight * e[:, 2].real)])
             got = data[[est.get_index(k) for k in ("ENumer", "EDenom", "E1Body", "E2Body")]].real
             assert numpy.allclose(got, ref, rtol=TOL, atol=0), (step, got, ref)
+            # the handler's block row and file (ipie's layout)
+            handler.compute_estimators(system, ham, trial, walkers)
+            handler.print_block(None, step // block, acc, print_line=False)
+            blocks.append((o_acc.copy(), ref))
             acc.zero()
             o_acc[:] = 0.0
+    with H5File(h5_path) as f:
+        rows = f["block_size_1/data/000000000"][: int(f["block_size_1/max_block/000000000"]) + 1]
+    assert rows.shape == (nsteps // block, 8)
+    for b, (wsum, ref) in enumerate(blocks):
+        # blocks are numbered from 1 here, so the walker sums are divided by the block length
+        assert rows[b, 0].real == pytest.approx(wsum[0].real / block, rel=TOL)
+        assert rows[b, 3 + 2].real == pytest.approx(ref[0] / ref[1], rel=TOL)  # ETotal
+        assert rows[b, 3 + 3].real == pytest.approx(ref[2] / ref[1], rel=TOL)  # E1Body
