@@ -104,8 +104,10 @@ __device__ __forceinline__ v8bf lds_frag(const uint16_t* base, int row, int seg)
 }
 
 // The Taylor kernel's staging keeps the XOR with bits 2..3 of the row: with
-// its 80 x 80 tiles and 5 waves it measured faster than the swizzle above.
-__device__ __forceinline__ int lds_vec_ty(int row, int seg) { return row * 4 + (seg ^ ((row >> 2) & 3)); }
+// its 80 x 80 tiles and 5 waves it measured 12% faster than the swizzle above
+// (with register or direct staging alike).
+__device__ __forceinline__ int lds_swz_ty(int row) { return (row >> 2) & 3; }
+__device__ __forceinline__ int lds_vec_ty(int row, int seg) { return row * 4 + (seg ^ lds_swz_ty(row)); }
 __device__ __forceinline__ v8bf lds_frag_ty(const uint16_t* base, int row, int seg) {
   return as_v8bf(reinterpret_cast<const v4u*>(base)[lds_vec_ty(row, seg)]);
 }
@@ -854,6 +856,13 @@ struct Ty {
   __device__ static int phys(int pi) { return SPLIT ? pi : 2 * pi; }
 };
 
+// Direct global -> LDS staging of the Taylor operands (1 = default; 0 restores
+// the register staging, 11% slower on the N=400 workload).
+#ifndef AFQMC_TY_DL
+#define AFQMC_TY_DL 1
+#endif
+constexpr bool kTyDl = AFQMC_TY_DL != 0;
+
 template <bool SPLIT>
 __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args a) {
   using T = Ty<SPLIT>;
@@ -893,17 +902,59 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
       }
     }
   };
+  // direct global -> LDS staging: slot s of a plane region holds (row s / 4,
+  // segment (s % 4) ^ ((row / 4) % 4)), so each wave's loads fill 64
+  // consecutive slots; every wave issues NP loads per operand and sub-chunk
+  const uint16_t* dla[T::NP];
+  const uint16_t* dlb[T::NP];
+#pragma unroll
+  for (int j = 0; j < T::NP; ++j) {
+    const int sl = wave * T::NP * 64 + j * 64 + lane;
+    const int row = sl >> 2, seg = (sl & 3) ^ lds_swz_ty(row);
+    const int pi = row / TY_R, r = row % TY_R;
+    dla[j] = a.tin + T::phys(pi) * tplane + ((int64_t)w * a.rpad + rb * TY_R + r) * Kp + seg * 8;
+    dlb[j] = a.vplanes + (int64_t)(w * ncb + cb) * (Kp / BX_KC) * TY_TILE + T::phys(pi) * (TY_C * BX_KC) + r * BX_KC +
+             seg * 8;
+  }
+  auto dl_issue_ty = [&](int t, int buf) __attribute__((always_inline)) {
+    uint4* base = lds_u4 + buf * TY_STAGE_VEC;
+#pragma unroll
+    for (int sc = 0; sc < T::KS; ++sc) {
+      const int c = t * T::KS + sc;
+      if (c >= nch) break;
+#pragma unroll
+      for (int j = 0; j < T::NP; ++j) {
+        __builtin_amdgcn_global_load_lds(
+            (const void*)(dla[j] + c * BX_KC),
+            (__attribute__((address_space(3))) void*)(base + sc * T::SUB_VEC + wave * T::NP * 64 + j * 64), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(
+            (const void*)(dlb[j] + (int64_t)c * TY_TILE),
+            (__attribute__((address_space(3))) void*)(base + sc * T::SUB_VEC + T::SUB_A + wave * T::NP * 64 + j * 64),
+            16, 0, 0);
+      }
+    }
+  };
   v4f re[5], im[5];
 #pragma unroll
   for (int i = 0; i < 5; ++i) {
     re[i] = zero4f();
     im[i] = zero4f();
   }
-  fetch(0);
-  stash(0, 0);
-  __syncthreads();
+  if constexpr (kTyDl) {
+    dl_issue_ty(0, 0);
+    wait_vm<0>();
+    dl_barrier();
+  } else {
+    fetch(0);
+    stash(0, 0);
+    __syncthreads();
+  }
   for (int t = 0; t < nst; ++t) {
-    if (t + 1 < nst) fetch(t + 1);
+    if constexpr (kTyDl) {
+      if (t + 1 < nst) dl_issue_ty(t + 1, (t + 1) & 1);
+    } else {
+      if (t + 1 < nst) fetch(t + 1);
+    }
 #pragma unroll
     for (int sc = 0; sc < T::KS; ++sc) {
       if (t * T::KS + sc >= nch) break;
@@ -946,8 +997,13 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
         }
       }
     }
-    if (t + 1 < nst) stash(t + 1, (t + 1) & 1);
-    __syncthreads();
+    if constexpr (kTyDl) {
+      wait_vm<0>();
+      dl_barrier();
+    } else {
+      if (t + 1 < nst) stash(t + 1, (t + 1) & 1);
+      __syncthreads();
+    }
   }
   // epilogue: element r of tile i is row crow = i 16 + l16 (a column of the
   // walker matrix) and basis index kcol = wave 16 + 4 q + r, so 16 lanes touch
@@ -1090,6 +1146,8 @@ hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream) {
   return a.split ? launch_exchange_bf16_t<true>(a, stream) : launch_exchange_bf16_t<false>(a, stream);
 }
 
+// Direct global -> LDS staging of the GEMM operands (1 = default; 0 restores
+// the register staging, 1-3% slower).
 #ifndef AFQMC_BF16_DL
 #define AFQMC_BF16_DL 1
 #endif
