@@ -785,10 +785,6 @@ size_t bf16_lds_bytes(int wb, int xb, int nocc) {
 // buffer capacity wcap; their padding rows and K entries are never written
 // and must be zero.
 constexpr int TY_WAVES = 5, TY_R = 80, TY_C = 80, TY_THREADS = 64 * TY_WAVES;
-constexpr int TY_A_VEC = 4 * TY_R * 4, TY_B_VEC = 4 * TY_C * 4, TY_STAGE_VEC = TY_A_VEC + TY_B_VEC;
-constexpr int TY_APER = TY_A_VEC / TY_THREADS, TY_BPER = TY_B_VEC / TY_THREADS;
-static_assert(TY_APER == 4 && TY_BPER == 4, "one plane per staged vector and thread");
-constexpr size_t TY_LDS = 2 * 16 * (size_t)TY_STAGE_VEC;
 constexpr int TY_TILE = 4 * TY_C * BX_KC;  // bf16 values of one VHS tile (4 planes x 80 rows x 32 K)
 
 // VHS [W][N][N] complex FP32, upper triangle (n <= k) only, -> split planes in
@@ -844,24 +840,20 @@ __device__ __forceinline__ v8bf neg_bf16(v8bf x) {
   return __builtin_bit_cast(v8bf, __builtin_bit_cast(v4u, x) ^ (v4u){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u});
 }
 
-// Staging of one LDS stage, in 16-byte vectors: SPLIT takes one K chunk of the
-// four planes of each operand, the single product two K chunks of the hi planes
-// (re_hi, im_hi: physical planes 0 and 2), so both stages hold 40 KiB and each
-// thread stages 4 + 4 vectors.
+// Staging of one LDS stage, in 16-byte vectors: one K chunk of the four planes
+// of each operand (SPLIT) or of the hi planes (re_hi, im_hi: physical planes 0
+// and 2) for the single product.  The single product's 20 KiB stages let four
+// workgroups share a CU (two stages each); two K chunks per stage (fewer
+// barriers, two workgroups per CU) measured 2.5% slower.
 template <bool SPLIT>
 struct Ty {
-  static constexpr int NP = SPLIT ? 4 : 2, KS = SPLIT ? 1 : 2;
+  static constexpr int NP = SPLIT ? 4 : 2, KS = 1;
   static constexpr int SUB_A = NP * TY_R * 4, SUB_VEC = SUB_A + NP * TY_C * 4;
-  static_assert(KS * SUB_VEC == TY_STAGE_VEC, "the stage size does not depend on the product count");
+  static constexpr int STAGE_VEC = KS * SUB_VEC;
+  static constexpr size_t LDS = 2 * 16 * (size_t)STAGE_VEC;  // two stages
+  static_assert(SUB_A % (64 * TY_WAVES) == 0, "each wave stages whole load instructions");
   __device__ static int phys(int pi) { return SPLIT ? pi : 2 * pi; }
 };
-
-// Direct global -> LDS staging of the Taylor operands (1 = default; 0 restores
-// the register staging, 11% slower on the N=400 workload).
-#ifndef AFQMC_TY_DL
-#define AFQMC_TY_DL 1
-#endif
-constexpr bool kTyDl = AFQMC_TY_DL != 0;
 
 template <bool SPLIT>
 __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args a) {
@@ -874,37 +866,11 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
   if (w >= a.nwalkers) return;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
   const int64_t tplane = (int64_t)a.wcap * a.rpad * Kp;
-  // staged vector i of this thread: sub-chunk i / NP, plane i % NP, row tid / 4, 16-byte segment tid % 4
-  const int srow = tid >> 2, sseg = tid & 3;
-  const uint16_t* asrc = a.tin + ((int64_t)w * a.rpad + rb * TY_R + srow) * Kp + sseg * 8;
-  // VHS tiles: the K chunk c of (w, cb) is one contiguous block, thread tid takes 16 bytes at tid 16 of each plane
-  const uint16_t* bsrc = a.vplanes + (int64_t)(w * ncb + cb) * (Kp / BX_KC) * TY_TILE + srow * BX_KC + sseg * 8;
-  v4u pa[TY_APER], pb[TY_BPER];
   const int nch = Kp / BX_KC, nst = (int)ceil_div(nch, T::KS);
-  auto fetch = [&](int t) __attribute__((always_inline)) {
-#pragma unroll
-    for (int i = 0; i < TY_APER; ++i) {
-      const int c = t * T::KS + i / T::NP, pl = T::phys(i % T::NP);
-      if (c < nch) {
-        pa[i] = *reinterpret_cast<const v4u*>(asrc + pl * tplane + c * BX_KC);
-        pb[i] = *reinterpret_cast<const v4u*>(bsrc + pl * (TY_C * BX_KC) + c * TY_TILE);
-      }
-    }
-  };
-  auto stash = [&](int t, int buf) __attribute__((always_inline)) {
-    v4u* base = reinterpret_cast<v4u*>(lds_u4) + buf * TY_STAGE_VEC;
-#pragma unroll
-    for (int i = 0; i < TY_APER; ++i) {
-      const int sc = i / T::NP, pi = i % T::NP;
-      if (t * T::KS + sc < nch) {
-        base[sc * T::SUB_VEC + lds_vec_ty(pi * TY_R + srow, sseg)] = pa[i];
-        base[sc * T::SUB_VEC + T::SUB_A + lds_vec_ty(pi * TY_C + srow, sseg)] = pb[i];
-      }
-    }
-  };
   // direct global -> LDS staging: slot s of a plane region holds (row s / 4,
   // segment (s % 4) ^ ((row / 4) % 4)), so each wave's loads fill 64
-  // consecutive slots; every wave issues NP loads per operand and sub-chunk
+  // consecutive slots; every wave issues NP loads per operand and sub-chunk.
+  // The VHS tiles hold K chunk c of (w, cb) as one contiguous block.
   const uint16_t* dla[T::NP];
   const uint16_t* dlb[T::NP];
 #pragma unroll
@@ -917,7 +883,7 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
              seg * 8;
   }
   auto dl_issue_ty = [&](int t, int buf) __attribute__((always_inline)) {
-    uint4* base = lds_u4 + buf * TY_STAGE_VEC;
+    uint4* base = lds_u4 + buf * T::STAGE_VEC;
 #pragma unroll
     for (int sc = 0; sc < T::KS; ++sc) {
       const int c = t * T::KS + sc;
@@ -940,25 +906,15 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
     re[i] = zero4f();
     im[i] = zero4f();
   }
-  if constexpr (kTyDl) {
-    dl_issue_ty(0, 0);
-    wait_vm<0>();
-    dl_barrier();
-  } else {
-    fetch(0);
-    stash(0, 0);
-    __syncthreads();
-  }
+  dl_issue_ty(0, 0);
+  wait_vm<0>();
+  dl_barrier();
   for (int t = 0; t < nst; ++t) {
-    if constexpr (kTyDl) {
-      if (t + 1 < nst) dl_issue_ty(t + 1, (t + 1) & 1);
-    } else {
-      if (t + 1 < nst) fetch(t + 1);
-    }
+    if (t + 1 < nst) dl_issue_ty(t + 1, (t + 1) & 1);
 #pragma unroll
     for (int sc = 0; sc < T::KS; ++sc) {
       if (t * T::KS + sc >= nch) break;
-      const uint16_t* As = reinterpret_cast<const uint16_t*>(lds_u4 + (t & 1) * TY_STAGE_VEC + sc * T::SUB_VEC);
+      const uint16_t* As = reinterpret_cast<const uint16_t*>(lds_u4 + (t & 1) * T::STAGE_VEC + sc * T::SUB_VEC);
       const uint16_t* Bs = As + 8 * T::SUB_A;
       const int bcol = wave * 16 + l16;
       // operands swapped in every MFMA: the accumulators hold the transposed tiles
@@ -997,13 +953,8 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
         }
       }
     }
-    if constexpr (kTyDl) {
-      wait_vm<0>();
-      dl_barrier();
-    } else {
-      if (t + 1 < nst) stash(t + 1, (t + 1) & 1);
-      __syncthreads();
-    }
+    wait_vm<0>();
+    dl_barrier();
   }
   // epilogue: element r of tile i is row crow = i 16 + l16 (a column of the
   // walker matrix) and basis index kcol = wave 16 + 4 q + r, so 16 lanes touch
@@ -1191,13 +1142,14 @@ hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream) {
       (!(a.first && a.last) && a.dacc == nullptr))
     return hipErrorInvalidValue;
   const void* kern = a.split ? (const void*)taylor_bf16_kernel<true> : (const void*)taylor_bf16_kernel<false>;
-  hipError_t e = allow_lds(kern, TY_LDS);
+  const size_t lds = a.split ? Ty<true>::LDS : Ty<false>::LDS;
+  hipError_t e = allow_lds(kern, lds);
   if (e != hipSuccess) return e;
   const int64_t grid = (int64_t)a.nwalkers * (a.rpad / TY_R) * ceil_div(a.nbasis, TY_C);
   if (a.split)
-    hipLaunchKernelGGL(taylor_bf16_kernel<true>, dim3((unsigned)grid), dim3(TY_THREADS), TY_LDS, stream, a);
+    hipLaunchKernelGGL(taylor_bf16_kernel<true>, dim3((unsigned)grid), dim3(TY_THREADS), lds, stream, a);
   else
-    hipLaunchKernelGGL(taylor_bf16_kernel<false>, dim3((unsigned)grid), dim3(TY_THREADS), TY_LDS, stream, a);
+    hipLaunchKernelGGL(taylor_bf16_kernel<false>, dim3((unsigned)grid), dim3(TY_THREADS), lds, stream, a);
   return hipGetLastError();
 }
 
