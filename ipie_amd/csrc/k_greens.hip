@@ -44,17 +44,31 @@ __global__ void __launch_bounds__(64 * GF_WAVES) gf_obuild_kernel(GfArgs a, doub
     re[jt] = zero4d();
     im[jt] = zero4d();
   }
-  for (int m0 = 0; m0 < N; m0 += 4) {
-    const int m = m0 + kq;
-    const cplx av = (m < N && i < n) ? ld(phi, (int64_t)m * n + i) : cmake(0.0, 0.0);
+  // four K steps per iteration, their loads issued before the first MFMA (the
+  // same products in the same order as one step at a time)
+  constexpr int KU = 4;
+  for (int m0 = 0; m0 < N; m0 += 4 * KU) {
+    cplx av[KU], bv[KU][NT];
 #pragma unroll
-    for (int jt = 0; jt < NT; ++jt) {
-      const int j = jt * 16 + r16;
-      const cplx bv = (m < N && j < n) ? cconj(ld(psi, (int64_t)m * n + j)) : cmake(0.0, 0.0);
-      re[jt] = mfma64(av.re, bv.re, re[jt]);
-      re[jt] = mfma64(-av.im, bv.im, re[jt]);
-      im[jt] = mfma64(av.re, bv.im, im[jt]);
-      im[jt] = mfma64(av.im, bv.re, im[jt]);
+    for (int u = 0; u < KU; ++u) {
+      const int m = m0 + 4 * u + kq;
+      av[u] = (m < N && i < n) ? ld(phi, (int64_t)m * n + i) : cmake(0.0, 0.0);
+#pragma unroll
+      for (int jt = 0; jt < NT; ++jt) {
+        const int j = jt * 16 + r16;
+        bv[u][jt] = (m < N && j < n) ? cconj(ld(psi, (int64_t)m * n + j)) : cmake(0.0, 0.0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      if (m0 + 4 * u >= N) break;  // wave-uniform: a step past the end adds nothing
+#pragma unroll
+      for (int jt = 0; jt < NT; ++jt) {
+        re[jt] = mfma64(av[u].re, bv[u][jt].re, re[jt]);
+        re[jt] = mfma64(-av[u].im, bv[u][jt].im, re[jt]);
+        im[jt] = mfma64(av[u].re, bv[u][jt].im, im[jt]);
+        im[jt] = mfma64(av[u].im, bv[u][jt].re, im[jt]);
+      }
     }
   }
   double* o = obuf + 2 * ((int64_t)w * 2 + s) * nmax * nmax;
@@ -98,8 +112,8 @@ __global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __res
   const int lane = threadIdx.x;
   const int nspin = gf_nspin(a);
   const int w = blockIdx.x / nspin, s = blockIdx.x % nspin;
-  double logdet[2] = {0.0, 0.0};
-  cplx phase[2] = {cmake(1.0, 0.0), cmake(1.0, 0.0)};
+  double logdet = 0.0;
+  cplx phase = cmake(1.0, 0.0);
   int singular = 0;
   {
     const int n = s == 0 ? a.nalpha : a.nbeta;
@@ -143,7 +157,23 @@ __global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __res
         const cplx l = cmul(row[k], rinv);
         row[k] = l;
         const cplx* urow = LU + k * lr;
-        for (int j = k + 1; j < n; ++j) {
+        int j = k + 1;
+        // four columns at a time, loads before stores (row k is not this lane's row)
+        for (; j + 4 <= n; j += 4) {
+          cplx u[4], x[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            u[q] = urow[j + q];
+            x[q] = row[j + q];
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            x[q].re -= l.re * u[q].re - l.im * u[q].im;
+            x[q].im -= l.re * u[q].im + l.im * u[q].re;
+            row[j + q] = x[q];
+          }
+        }
+        for (; j < n; ++j) {
           const cplx u = urow[j];
           cplx x = row[j];
           x.re -= l.re * u.re - l.im * u.im;
@@ -153,17 +183,24 @@ __global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __res
       }
       __syncthreads();
     }
+    // lane k takes log|U_kk| and U_kk / |U_kk|; a butterfly (every lane adds
+    // and multiplies the same pairs, so all lanes agree) sums and multiplies them
     double lg = 0.0;
-    cplx ph = cmake((swaps & 1) ? -1.0 : 1.0, 0.0);
-    for (int k = 0; k < n; ++k) {
-      const cplx u = LU[k * lr + k];
+    cplx ph = cmake(1.0, 0.0);
+    if (lane < n) {
+      const cplx u = LU[lane * lr + lane];
       const double ab = cabs(u);
-      lg += log(ab);
-      if (ab > 0.0) ph = cmul(ph, cmake(u.re / ab, u.im / ab));
-      else ph = cmake(0.0, 0.0);
+      lg = log(ab);
+      ph = ab > 0.0 ? cmake(u.re / ab, u.im / ab) : cmake(0.0, 0.0);
     }
-    logdet[s] = lg;
-    phase[s] = ph;
+    for (int off = 1; off < 64; off <<= 1) {
+      lg += __shfl_xor(lg, off, 64);
+      const cplx o = cmake(__shfl_xor(ph.re, off, 64), __shfl_xor(ph.im, off, 64));
+      ph = cmul(ph, o);
+    }
+    if (swaps & 1) ph = cmake(-ph.re, -ph.im);
+    logdet = lg;
+    phase = ph;
     if (oinv) {
       // column j of O^{-1} solves L U x = P e_j (lane j)
       const int j = lane;
@@ -195,9 +232,9 @@ __global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __res
   }
   if (lane == 0) {
     double* r = res + 4 * ((int64_t)w * 2 + s);
-    r[0] = logdet[s];
-    r[1] = phase[s].re;
-    r[2] = phase[s].im;
+    r[0] = logdet;
+    r[1] = phase.re;
+    r[2] = phase.im;
     r[3] = singular ? 1.0 : 0.0;
   }
 }
@@ -253,23 +290,38 @@ __global__ void __launch_bounds__(64 * GF_WAVES) gf_ghalf_kernel(GfArgs a, const
     const int k = ks * 4 + kq;
     af[ks] = (i < n && k < n) ? ld(oi, (int64_t)i * nmax + k) : cmake(0.0, 0.0);
   }
+  // two column tiles per iteration: the loads of both are issued before the
+  // MFMAs and stores of either
+  constexpr int MU = 2;
   const int nmt = (int)ceil_div(N, 16);
-  for (int mt = 0; mt < nmt; ++mt) {
-    const int m = mt * 16 + r16;
-    v4d re = zero4d(), im = zero4d();
+  for (int mt0 = 0; mt0 < nmt; mt0 += MU) {
+    cplx bv[MU][KS];
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const int k = ks * 4 + kq;
-      const cplx bv = (m < N && k < n) ? ld(phi, (int64_t)m * n + k) : cmake(0.0, 0.0);
-      re = mfma64(af[ks].re, bv.re, re);
-      re = mfma64(-af[ks].im, bv.im, re);
-      im = mfma64(af[ks].re, bv.im, im);
-      im = mfma64(af[ks].im, bv.re, im);
+    for (int u = 0; u < MU; ++u) {
+      const int m = (mt0 + u) * 16 + r16;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int k = ks * 4 + kq;
+        bv[u][ks] = (m < N && k < n) ? ld(phi, (int64_t)m * n + k) : cmake(0.0, 0.0);
+      }
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = it * 16 + 4 * r + kq;  // accumulator row = 4 r + lane / 16
-      if (row < n && m < N) st(gh, (int64_t)row * N + m, cmake(re[r], im[r]));
+    for (int u = 0; u < MU; ++u) {
+      if (mt0 + u >= nmt) break;  // wave-uniform
+      const int m = (mt0 + u) * 16 + r16;
+      v4d re = zero4d(), im = zero4d();
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        re = mfma64(af[ks].re, bv[u][ks].re, re);
+        re = mfma64(-af[ks].im, bv[u][ks].im, re);
+        im = mfma64(af[ks].re, bv[u][ks].im, im);
+        im = mfma64(af[ks].im, bv[u][ks].re, im);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = it * 16 + 4 * r + kq;  // accumulator row = 4 r + lane / 16
+        if (row < n && m < N) st(gh, (int64_t)row * N + m, cmake(re[r], im[r]));
+      }
     }
   }
 }
