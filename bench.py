@@ -88,7 +88,7 @@ def build_host(w, scale):
     return dict(h1e=h1e, chol=chol, ecore=ecore, psi=psi, system=system, ham=ham, trial=trial)
 
 
-def build_device(host, w, mode):
+def build_device(host, w, mode, tuner=None):
     import torch
 
     from ipie_amd.propagator import PhaselessGeneric
@@ -99,10 +99,11 @@ def build_device(host, w, mode):
     t0 = time.time()
     walkers = UHFWalkers(host["psi"], w.nalpha, w.nbeta, w.nbasis, w.nwalkers)
     walkers.build(host["trial"])
-    prop = PhaselessGeneric(w.dt, seed=w.seed, precision=precision, surrogate=surrogate)
+    prop = PhaselessGeneric(w.dt, seed=w.seed, precision=precision, surrogate=surrogate, autotune=tuner)
     prop.build(host["ham"], host["trial"], walkers)
     torch.cuda.synchronize()
-    log(f"[bench] {mode}: device context + upload in {time.time() - t0:.1f}s")
+    log(f"[bench] {mode}: device context + upload in {time.time() - t0:.1f}s"
+        + (f", launch shapes {prop.tuning}" if prop.tuning is not None else ""))
     return dict(host, walkers=walkers, prop=prop)
 
 
@@ -142,6 +143,21 @@ def run_setup_timing(w, scale):
                       "host_threads": int(os.environ.get("OMP_NUM_THREADS", "0") or 0)}), flush=True)
 
 
+def shape_key(ctx, surrogate):
+    from ipie_amd.autotune import shape_key as key
+
+    return key(ctx, surrogate)
+
+
+def make_tuner(args):
+    """The launch-shape autotuner (in memory: every run tunes afresh), or None."""
+    if args.no_autotune:
+        return None
+    from ipie_amd.autotune import Autotuner
+
+    return Autotuner(cache_path=None)
+
+
 def run_multi(args):
     """The uneven multi-system batch: 16 independent systems (N 64-256, X = 5 N,
     N / 10 occupied per spin, 64-512 walkers, mixed precision, local energy
@@ -156,10 +172,11 @@ def run_multi(args):
 
     ws = synthetic.multi_system_workloads()
     K = max(args.steps // REORTHO_EVERY, 1) * REORTHO_EVERY
+    tuner = make_tuner(args)
     systems = []
     for w in ws:
         host = build_host(w, math.sqrt(12.0 / (w.nbasis * w.decay)))
-        st = build_device(host, w, "mixed")
+        st = build_device(host, w, "mixed", tuner)
         st["energy"] = torch.empty(w.nwalkers, 3, dtype=torch.complex128, device=st["walkers"].device)
         st["eshift"] = float(host["trial"].energy.real)
         st["w"] = w
@@ -233,6 +250,7 @@ def run_multi(args):
         "sequential_graphs_value": rate(t_graphs),
         "speedup_vs_sequential_eager": t_eager / t_batch,
         "finite": finite,
+        "launch_shapes": [st["prop"].tuning for st in systems] if tuner is not None else "heuristic",
     }
     print(json.dumps(out), flush=True)
 
@@ -411,6 +429,8 @@ def main():
     ap.add_argument("--setup-timing", action="store_true",
                     help="only time the set-up of the workload on the host and on the device, then exit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the heuristic launch shapes (default: tune them per context at build time)")
     ap.add_argument("--cpu-walkers", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
@@ -461,8 +481,9 @@ def main():
     if any("surrogate" in m for m in mode_list):
         warmup = max(warmup, SURROGATE_WARMUP + 5)
     modes = {}
+    tuner = make_tuner(args)
     for mode in mode_list:
-        st = build_device(host, w, mode)
+        st = build_device(host, w, mode, tuner)
         elapsed, prof, e_steps, finite, g = run_gpu(st, w, K, warmup, graph=not args.no_graph)
         ex_ms, ex_n = prof["exchange"]
         form = st["walkers"]._engine.ctx.exchange_form()
@@ -483,7 +504,12 @@ def main():
             block_energy=float(numpy.mean(e_steps.real)),
             finite=finite,
             _e_steps=e_steps,
+            launch_shapes=st["prop"].tuning if tuner is not None else "heuristic",
         )
+        if tuner is not None:
+            rep = tuner.reports.get(shape_key(st["walkers"]._engine.ctx, "surrogate" in mode))
+            if rep:
+                rec["autotune_ms"] = rep
         if g is not None:
             g_s, g_e, g_k = g
             rec["eager_value"] = rec["value"]

@@ -272,6 +272,26 @@ int afqmc_reortho(afqmc_ctx* ctx, int32_t nwalkers, double* phia, double* phib, 
  * (arrays of AFQMC_NPHASES). */
 int afqmc_profile(afqmc_ctx* ctx, int enable);
 int afqmc_profile_read(afqmc_ctx* ctx, double* ms, int64_t* launches);
+/* Launch shapes, for the autotuner (ipie_amd/autotune.py).  No counterpart in
+ * ipie, whose GPU kernels take CuPy's / numba's fixed shapes.  Knobs:
+ *   AFQMC_TUNE_FB_SPLIT  K ranges of the force-bias contraction (FP64: in
+ *                        total; mixed precision: per spin)
+ *   AFQMC_TUNE_QUAD_WGS  workgroups the column units of the split-BF16
+ *                        quadratic-form exchange aim for (sets the K-block
+ *                        length of a unit; mixed precision, quadratic form)
+ *   AFQMC_TUNE_EXX_SPLIT Cholesky-range split of the T-form exchange
+ *                        (0 restores the heuristic)
+ * afqmc_tune_get returns the current value and the admissible range [lo, hi];
+ * hi = 0 when the knob does not apply to this context.  afqmc_tune_set
+ * changes only launch shapes and summation grouping (results agree to
+ * rounding); it is refused during graph capture, may allocate a larger
+ * partial buffer, and keeps the buffers it replaces until afqmc_destroy, so
+ * a graph captured before stays valid (with the old shapes). */
+#define AFQMC_TUNE_FB_SPLIT 1
+#define AFQMC_TUNE_QUAD_WGS 2
+#define AFQMC_TUNE_EXX_SPLIT 3
+int afqmc_tune_get(afqmc_ctx* ctx, int32_t knob, int32_t* value, int32_t* lo, int32_t* hi);
+int afqmc_tune_set(afqmc_ctx* ctx, int32_t knob, int32_t value);
 /* Learned control variate for the exchange energy of afqmc_propagate's
  * energy_prev output.  The predictor of each walker's exchange sum is
  *   g = E_T + 2 <K, G - G_T> + Q_bf16(G - G_T) + alpha m(f):

@@ -91,6 +91,8 @@ SIGNATURES = {
     "afqmc_surrogate_configure": (ctypes.c_int, [_dp, ctypes.c_int, _f64, ctypes.c_int, ctypes.c_int, _f64, _u64]),
     "afqmc_surrogate_stats": (ctypes.c_int, [_dp, ctypes.POINTER(_f64), ctypes.c_int]),
     "afqmc_profile_read": (ctypes.c_int, [_dp, ctypes.POINTER(_f64), ctypes.POINTER(_i64)]),
+    "afqmc_tune_get": (ctypes.c_int, [_dp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "afqmc_tune_set": (ctypes.c_int, [_dp, _i32, _i32]),
     "afqmc_exchange_form": (ctypes.c_int, [_dp]),
     "afqmc_pop_plan": (ctypes.c_int, [_dp, _i32, _i32, _dp, _f64, _f64, _f64, _dp, _dp, _dp, _dp, _dp]),
     "afqmc_pop_gather": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _i64]),
@@ -113,6 +115,9 @@ SIGNATURES = {
 }
 
 POP_METHODS = {"pair_branch": 0, "comb": 1, "stochastic_reconfiguration": 2, "rescale": 3}
+
+# launch-shape knobs of afqmc_tune_get / afqmc_tune_set
+TUNE_KNOBS = {"fb_split": 1, "quad_wgs": 2, "exx_split": 3}
 
 PHASES = ["gf", "one_body", "force_bias", "fields", "vhs", "taylor", "overlap", "weight", "exchange", "energy",
           "reortho", "exchange_cv", "split"]
@@ -457,6 +462,21 @@ class Context:
             raise AfqmcError(f"afqmc_surrogate_stats: {self.lib.afqmc_last_error(self.h)}")
         keys = ["steps", "warmup", "last_k", "train_loss", "resid2", "target_var", "gain"]
         return dict(zip(keys, [float(v) for v in out]))
+
+    def tune_get(self, knob):
+        """(value, lo, hi) of a launch-shape knob; hi == 0 when it does not apply."""
+        v, lo, hi = _i32(), _i32(), _i32()
+        st = self.lib.afqmc_tune_get(self.h, TUNE_KNOBS[knob], ctypes.byref(v), ctypes.byref(lo), ctypes.byref(hi))
+        if st != AFQMC_OK:
+            raise AfqmcError(f"afqmc_tune_get: {STATUS_NAMES.get(st, st)}: {self.lib.afqmc_last_error(self.h)}")
+        return v.value, lo.value, hi.value
+
+    def tune_set(self, knob, value):
+        with torch.cuda.device(self.device):
+            st = self.lib.afqmc_tune_set(self.h, TUNE_KNOBS[knob], int(value))
+        if st != AFQMC_OK:
+            msg = self.lib.afqmc_last_error(self.h)
+            raise AfqmcError(f"afqmc_tune_set: {STATUS_NAMES.get(st, st)}: {msg.decode() if msg else ''}")
 
     def profile(self, enable=True):
         st = self.lib.afqmc_profile(self.h, int(bool(enable)))

@@ -1,0 +1,192 @@
+"""Autotuner of a device context's launch shapes.
+
+Three launch shapes of the step depend on the system's dimensions:
+
+  fb_split   K ranges of the force-bias contraction (split-K; per spin in
+             mixed precision, in total in FP64)
+  quad_wgs   workgroups the column units of the split-BF16 quadratic-form
+             exchange aim for, which sets the K-block length of a unit
+  exx_split  Cholesky-range split of the T-form exchange (0: heuristic)
+
+A context starts from heuristics (``afqmc_create``).  ``Autotuner.tune`` times
+every admissible candidate of each knob that applies on the context's own
+shapes, through entry points that change no walker state
+(``afqmc_force_bias`` for fb_split, ``afqmc_local_energy`` for the exchange
+knobs), and keeps the fastest unless the heuristic is within ``min_gain`` of
+it.  The choices are cached per device and shape: a multi-system batch whose
+systems share a shape tunes that shape once, and a JSON cache file carries the
+choices to the next run.  The knobs change launch shapes and the grouping of
+partial sums only, so results agree with the untuned context to rounding.
+
+A context that runs the learned control variate computes its exchange inside
+``afqmc_propagate`` (the single-product quadratic form of every walker and the
+exact form of a sampled subset), a path no state-free entry point exposes:
+for it only fb_split is tuned and the exchange shapes keep their heuristic
+(its cache entries are keyed apart).  ``PhaselessGeneric(autotune=...)``
+tunes at build time.
+
+No counterpart in ipie, whose GPU kernels take CuPy's and numba's shapes.
+"""
+
+import json
+import os
+import time
+
+import torch
+
+KNOBS = ("fb_split", "quad_wgs", "exx_split")
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "ipie_amd", "autotune.json")
+
+_FB = (1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 24, 32)
+_QUAD = (512, 768, 1024, 1536, 2048, 3072, 4096, 6144)
+_EXX = (1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 192, 256)
+
+
+def shape_key(ctx, surrogate=False):
+    """Cache key: the device name, every dimension the launch shapes depend on,
+    and whether the context runs the learned control variate."""
+    d = ctx.dims
+    name = torch.cuda.get_device_name(ctx.device) if torch.cuda.is_available() else "cpu"
+    return (f"{name}|N={d.nbasis}|X={d.nchol}|na={d.nalpha}|nb={d.nbeta}|W={d.nwalkers}|rhf={d.rhf}"
+            f"|{ctx.precision}" + ("|surrogate" if surrogate else ""))
+
+
+def candidates(knob, value, lo, hi):
+    """Values of `knob` to time: a fixed ladder within [lo, hi], plus the current value."""
+    if hi < 1:
+        return []
+    ladder = {"fb_split": _FB, "quad_wgs": _QUAD, "exx_split": _EXX}[knob]
+    out = {v for v in ladder if lo <= v <= hi}
+    if knob == "exx_split":
+        out.add(0)  # the heuristic
+    if value == 0 or lo <= value <= hi:
+        out.add(value)
+    return sorted(out)
+
+
+def choose(times, default, min_gain):
+    """The fastest value, unless `default` is within a fraction min_gain of it."""
+    best = min(times, key=lambda v: (times[v], v))
+    if default in times and times[default] <= times[best] * (1.0 + min_gain):
+        return default
+    return best
+
+
+class Autotuner:
+    """Times and sets the launch-shape knobs of contexts; caches by shape.
+
+    cache_path: JSON file read at construction and rewritten after each tune
+    (None: in memory only).  repeats x inner calls are timed per candidate,
+    after one warm-up call; the minimum over repeats counts.
+    """
+
+    def __init__(self, cache_path=None, repeats=3, inner=2, min_gain=0.02, knobs=KNOBS):
+        self.cache_path = cache_path
+        self.repeats = int(repeats)
+        self.inner = int(inner)
+        self.min_gain = float(min_gain)
+        self.knobs = tuple(knobs)
+        self.cache = {}
+        self.reports = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as fh:
+                self.cache = json.load(fh)
+
+    # ------------------------------------------------------------------ cache
+    def _save(self):
+        if not self.cache_path:
+            return
+        os.makedirs(os.path.dirname(os.path.abspath(self.cache_path)), exist_ok=True)
+        tmp = self.cache_path + f".{os.getpid()}.tmp"
+        with open(tmp, "w") as fh:
+            json.dump(self.cache, fh, indent=1, sort_keys=True)
+        os.replace(tmp, self.cache_path)
+
+    def apply(self, ctx, surrogate=False):
+        """Set the cached choices for ctx's shape; returns them, or None if none are cached."""
+        entry = self.cache.get(shape_key(ctx, surrogate))
+        if entry is None:
+            return None
+        for knob, v in entry.items():
+            if knob in self.knobs and ctx.tune_get(knob)[2] >= 1:
+                ctx.tune_set(knob, v)
+        return dict(entry)
+
+    # ------------------------------------------------------------------ timing
+    def _time(self, ctx, fn):
+        fn()
+        ctx.sync()
+        best = float("inf")
+        for _ in range(self.repeats):
+            t0 = time.perf_counter()
+            for _ in range(self.inner):
+                fn()
+            ctx.sync()
+            best = min(best, (time.perf_counter() - t0) / self.inner)
+        return best * 1e3
+
+    def tune(self, ctx, ghalfa, ghalfb, surrogate=False):
+        """Time every candidate of each applicable knob with these Green's
+        functions ([W, n_s, N] complex128 on ctx's device), set the choices on
+        ctx, cache and return them ({knob: value}).
+
+        surrogate: true when the context runs the learned control variate
+        (then only fb_split is tuned, see the module note)."""
+        nw = ghalfa.shape[0]
+        dev = ghalfa.device
+        vbias = torch.empty(nw, ctx.dims.nchol, dtype=torch.complex128, device=dev)
+        energy = torch.empty(nw, 3, dtype=torch.complex128, device=dev)
+        calls = {
+            "fb_split": lambda: ctx.force_bias(ghalfa, ghalfb, vbias),
+            "quad_wgs": lambda: ctx.local_energy(ghalfa, ghalfb, energy),
+            "exx_split": lambda: ctx.local_energy(ghalfa, ghalfb, energy),
+        }
+        chosen, report = {}, {}
+        for knob in self.knobs:
+            if surrogate and knob != "fb_split":
+                continue
+            value, lo, hi = ctx.tune_get(knob)
+            cands = candidates(knob, value, lo, hi)
+            if not cands:
+                continue
+            times = {}
+            for v in cands:
+                ctx.tune_set(knob, v)
+                times[v] = self._time(ctx, calls[knob])
+            pick = choose(times, value, self.min_gain)
+            ctx.tune_set(knob, pick)
+            chosen[knob] = pick
+            report[knob] = {"default": value, "chosen": pick, "ms": {str(k): round(t, 4) for k, t in times.items()}}
+        key = shape_key(ctx, bool(surrogate))
+        self.cache[key] = chosen
+        self.reports[key] = report
+        self._save()
+        return chosen
+
+    def tune_walkers(self, ctx, walkers, surrogate=False, force=False):
+        """Apply the cached choices for ctx's shape, or tune with the walkers'
+        Green's function (computed into temporaries: the walkers are not
+        changed).  surrogate as in tune; force=True tunes even when a cached
+        entry exists."""
+        if not force:
+            got = self.apply(ctx, bool(surrogate))
+            if got is not None:
+                return got
+        dev = walkers.phia.device
+        ga = torch.empty_like(walkers.Ghalfa)
+        gb = torch.empty_like(walkers.Ghalfb)
+        ovlp = torch.empty(walkers.phia.shape[0], dtype=torch.complex128, device=dev)
+        ctx.greens_function(walkers.phia, walkers.phib, ga, gb, ovlp)
+        return self.tune(ctx, ga, gb, surrogate)
+
+
+def as_tuner(spec):
+    """None / False: no tuning; True: the default cache file; a path: that
+    cache file; an Autotuner: itself."""
+    if spec is None or spec is False:
+        return None
+    if isinstance(spec, Autotuner):
+        return spec
+    if spec is True:
+        return Autotuner(cache_path=DEFAULT_CACHE)
+    return Autotuner(cache_path=str(spec))

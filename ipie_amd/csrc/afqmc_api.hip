@@ -71,6 +71,13 @@ struct afqmc_ctx {
   int4* qunits[2] = {nullptr, nullptr};       // BF16 column units per spin, one table per row-block count
   std::vector<int> qunit_off[2], qunit_n[2];  // [mb - 1]: offset and length of the table for mb row blocks
   int xs_used[2] = {1, 1};  // partial-sum count of the last exchange per spin
+  // launch shapes that afqmc_tune_set may change (heuristic defaults)
+  int fb_split_spin = BF16_FB_SPLIT;  // split-BF16 force bias: K ranges per spin
+  int fb_cap = 0;                     // K ranges vbias_part holds (0: not allocated yet)
+  int quad_target = QUAD_TARGET_WGS;  // split-BF16 quadratic form: workgroups the column units aim for
+  int64_t qy_cap = 0;                 // complex entries qy holds (split-BF16 quadratic form)
+  int exx_split = 0;                  // T-form exchange: Cholesky-range split (0: heuristic)
+  std::vector<void*> retired;         // buffers a retune replaced: graphs captured before still use them
   int64_t exx_cap = 0;      // capacity (complex entries) of each exchange partial buffer
   double* gcomp = nullptr;  // [W, nmax, N] complex: Ghalf of sampled walkers (FP64 exchange)
   double* gf_o = nullptr;     // [W, 2, nmax, nmax] complex: overlap matrices
@@ -204,16 +211,16 @@ int refuse_in_capture(afqmc_ctx* c, const char* what) {
 
 // Column units of the block-triangular quadratic-form GEMM with mb row blocks:
 // column tile t sums the K chunks [0, min(5 (t + 1), kchunks)), cut at the
-// multiples of T chunks (T chosen for about 1536 workgroups, at least 32), and
+// multiples of T chunks (T chosen for about target_wgs workgroups, at least 32), and
 // the units are listed by K block, then tile.  The grid takes the row blocks
 // fastest, so the workgroups that run together on one XCD (consecutive logical
 // indices) are a few tiles of the same K block: they read the same d rows and
 // the same K range, which L2 then serves once.
-std::vector<int4> quad_units(int64_t nb, int kchunks, int64_t mb) {
+std::vector<int4> quad_units(int64_t nb, int kchunks, int64_t mb, int target_wgs) {
   const int per_tile = QUAD_TILE / 32;
   int64_t total = 0;
   for (int64_t t = 0; t < nb; ++t) total += std::min<int64_t>(per_tile * (t + 1), kchunks);
-  const int64_t target = std::max<int64_t>(ceil_div(1536, mb), 1);
+  const int64_t target = std::max<int64_t>(ceil_div(target_wgs, mb), 1);
   const int64_t T = std::max<int64_t>(ceil_div(total, target), 32);
   std::vector<int4> u;
   for (int64_t z = 0; z * T < kchunks; ++z)
@@ -244,6 +251,50 @@ int alloc_exchange_planes(afqmc_ctx* c) {
   return AFQMC_OK;
 }
 
+// Column-unit tables of the split-BF16 quadratic form for every row-block
+// count (quad_units with c->quad_target), and the dot-partial buffer qy sized
+// for the largest table.  A rebuild (afqmc_tune_set) keeps the tables and qy it
+// replaces alive until afqmc_destroy: a graph captured before still uses them.
+int build_quad_units(afqmc_ctx* c) {
+  const int64_t W = c->d.nwalkers;
+  int64_t max_units = 0;
+  std::vector<int4> all[2];
+  std::vector<int> off[2], cnt[2];
+  for (int s = 0; s < 2; ++s) {
+    if (!c->qwp[s]) continue;
+    const int64_t ldn = (int64_t)(s == 0 ? c->d.nalpha : c->d.nbeta) * c->kpad;
+    for (int64_t mb = 1; mb <= ceil_div(W, 128); ++mb) {
+      std::vector<int4> u = quad_units(ceil_div(ldn, QUAD_TILE), (int)(ldn / 32), mb, c->quad_target);
+      off[s].push_back((int)all[s].size());
+      cnt[s].push_back((int)u.size());
+      max_units = std::max<int64_t>(max_units, (int64_t)u.size());
+      all[s].insert(all[s].end(), u.begin(), u.end());
+    }
+  }
+  if (max_units == 0) return AFQMC_OK;
+  const int64_t need = max_units * QUAD_SLOTS * W;
+  if (need > c->qy_cap) {
+    double* qy = nullptr;
+    if (dmalloc((void**)&qy, 16 * (size_t)need) != hipSuccess)
+      return fail(c, AFQMC_ERR_ALLOC, "quadratic-form partial buffer allocation failed (%s)", c->describe_dims);
+    if (c->qy) c->retired.push_back(c->qy);
+    c->qy = qy;
+    c->qy_cap = need;
+  }
+  for (int s = 0; s < 2; ++s) {
+    if (all[s].empty()) continue;
+    int4* t = nullptr;
+    if (dmalloc((void**)&t, sizeof(int4) * all[s].size()) != hipSuccess)
+      return fail(c, AFQMC_ERR_ALLOC, "quadratic-form unit table allocation failed (%s)", c->describe_dims);
+    HIP_TRY(c, hipMemcpy(t, all[s].data(), sizeof(int4) * all[s].size(), hipMemcpyHostToDevice));
+    if (c->qunits[s]) c->retired.push_back(c->qunits[s]);
+    c->qunits[s] = t;
+    c->qunit_off[s] = off[s];
+    c->qunit_n[s] = cnt[s];
+  }
+  return AFQMC_OK;
+}
+
 // Workspace is allocated on first use by an entry point that needs it (the
 // Green's function and overlap use only their own O / O^-1 buffers, see do_gf).
 int ensure_ws(afqmc_ctx* c) {
@@ -254,6 +305,7 @@ int ensure_ws(afqmc_ctx* c) {
   const int nmax = d.nalpha > d.nbeta ? d.nalpha : d.nbeta;
   bool ok = true;
   ok = ok && dmalloc((void**)&c->vbias_part, 16 * (size_t)c->fb_ksplit * W * X) == hipSuccess;
+  if (ok) c->fb_cap = c->fb_ksplit;
   ok = ok && dmalloc((void**)&c->xshifted, 16 * (size_t)W * X) == hipSuccess;
   ok = ok && dmalloc((void**)&c->wvec, 16 * (size_t)6 * W) == hipSuccess;
   // partial sums: at most one per Cholesky vector per walker, whatever the split
@@ -284,10 +336,9 @@ int ensure_ws(afqmc_ctx* c) {
       if (mixed) ok = ok && dmalloc((void**)&c->qwp[s], 2 * (size_t)2 * rows * rows) == hipSuccess;
       else ok = ok && dmalloc((void**)&c->qw64[s], 8 * (size_t)rows * rows) == hipSuccess;
     }
-    // BF16: QUAD_SLOTS dot partials per column unit (at most 1536 / mb + 2 nb units)
-    const size_t ybytes = mixed ? 16 * (size_t)(1536 + 2 * ceil_div(nmax * Np, QUAD_TILE)) * QUAD_SLOTS * W
-                                : 16 * (size_t)W * nmax * Np;
-    ok = ok && dmalloc((void**)&c->qy, ybytes) == hipSuccess;
+    // FP64: Y = G Wq; split-BF16: QUAD_SLOTS dot partials per column unit, sized
+    // with the unit tables (build_quad_units)
+    if (!mixed) ok = ok && dmalloc((void**)&c->qy, 16 * (size_t)W * nmax * Np) == hipSuccess;
   }
   ok = ok && dmalloc((void**)&c->vhs, 16 * (size_t)W * N * N) == hipSuccess;
   for (int i = 0; i < 4; ++i) ok = ok && dmalloc((void**)&c->buf[i], 16 * (size_t)W * N * c->ncol) == hipSuccess;
@@ -301,20 +352,8 @@ int ensure_ws(afqmc_ctx* c) {
       tri[P + p] = j;
     }
   HIP_TRY(c, hipMemcpy(c->tri, tri.data(), sizeof(int) * 2 * P, hipMemcpyHostToDevice));
-  for (int s = 0; s < 2; ++s) {
-    if (!c->qwp[s]) continue;
-    const int64_t ldn = (int64_t)(s == 0 ? d.nalpha : d.nbeta) * c->kpad;
-    std::vector<int4> all;
-    for (int64_t mb = 1; mb <= ceil_div(W, 128); ++mb) {
-      std::vector<int4> u = quad_units(ceil_div(ldn, QUAD_TILE), (int)(ldn / 32), mb);
-      c->qunit_off[s].push_back((int)all.size());
-      c->qunit_n[s].push_back((int)u.size());
-      all.insert(all.end(), u.begin(), u.end());
-    }
-    if (dmalloc((void**)&c->qunits[s], sizeof(int4) * all.size()) != hipSuccess)
-      return fail(c, AFQMC_ERR_ALLOC, "workspace allocation failed (%s)", c->describe_dims);
-    HIP_TRY(c, hipMemcpy(c->qunits[s], all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice));
-  }
+  const int r = build_quad_units(c);
+  if (r) return r;
   HIP_TRY(c, hipMemset(c->wvec, 0, 16 * (size_t)6 * W));
   c->ws_ready = true;
   return AFQMC_OK;
@@ -405,11 +444,11 @@ int do_force_bias_bf16(afqmc_ctx* c, int nw, const double* ga, const double* gb)
     b.ldb = (int64_t)n * c->kpad;
     b.nn = c->d.nchol;
     b.kchunks = (int)(b.lda / 32);
-    b.ksplit = BF16_FB_SPLIT;
+    b.ksplit = c->fb_split_spin;
     b.epilogue = BF16_EPI_PARTIAL;
     b.out = c->vbias_part;
     b.out32 = reinterpret_cast<float*>(c->vbias_part);  // read back by fields_kernel as FP32 (part32)
-    b.z_offset = s * BF16_FB_SPLIT;
+    b.z_offset = s * c->fb_split_spin;
     b.scale = (c->d.rhf && c->d.nbeta > 0) ? 2.0 : 1.0;
     HIP_TRY(c, launch_bf16_gemm(b, c->stream));
   }
@@ -839,7 +878,7 @@ int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g, const int* wa
     b.wb = c->bx_wb[spin];
     b.xb = c->bx_xb[spin];
     const int64_t nwb = ceil_div(nw, b.wb), nxb = ceil_div(c->d.nchol, b.xb);
-    int64_t xs = ceil_div(ceil_div(2048, nwb), 8) * 8;
+    int64_t xs = c->exx_split > 0 ? c->exx_split : ceil_div(ceil_div(2048, nwb), 8) * 8;
     if (xs > nxb) xs = nxb;
     if (xs < 1) xs = 1;
     b.xsplit = (int)xs;
@@ -863,7 +902,7 @@ int exchange_spin(afqmc_ctx* c, int nw, int spin, const double* g, const int* wa
   a.nocc = n;
   a.ghalf = g;
   a.rchol = rchol;
-  a.xsplit = exchange_xsplit(nw, c->d.nchol, n);
+  a.xsplit = exchange_xsplit(nw, c->d.nchol, n, c->exx_split);
   a.exx_part = part;
   if ((int64_t)a.xsplit * nw > c->exx_cap) return fail(c, AFQMC_ERR_SHAPE, "exchange partial buffer too small");
   HIP_TRY(c, launch_exchange(a, c->stream));
@@ -1144,7 +1183,7 @@ int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims) {
   if (d.precision == AFQMC_PREC_MIXED) {
     c->xpad = (d.nchol + 31) / 32 * 32;
     c->bf16_fb = d.nchol > 0 && c->bx_ok[0] && (!two || c->bx_ok[1]);
-    if (c->bf16_fb) c->fb_ksplit = BF16_FB_SPLIT * (two ? 2 : 1);
+    if (c->bf16_fb) c->fb_ksplit = c->fb_split_spin * (two ? 2 : 1);
   }
   if (dmalloc((void**)&c->flags, sizeof(int)) != hipSuccess ||
       dmalloc((void**)&c->nfb, sizeof(unsigned long long)) != hipSuccess ||
@@ -1177,6 +1216,7 @@ void afqmc_destroy(afqmc_ctx* c) {
                   c->sur.K, c->sur.et, c->sur.cv, c->sur.mlp, c->sur.gtmp, c->dstate};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (void* p : c->retired) (void)hipFree(p);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
   for (const PendingEvent& pe : c->ev_pending) {
@@ -1888,6 +1928,96 @@ int afqmc_batch_launch(afqmc_batch* b, int32_t replays, const uint64_t* counters
 int afqmc_draw_normals(afqmc_ctx* c, double* out, int64_t n, uint64_t seed, uint64_t counter) {
   if (!c || !out || n < 0) return AFQMC_ERR_INVALID_ARG;
   HIP_TRY(c, launch_draw_normals(out, n, seed, counter, c->stream));
+  return AFQMC_OK;
+}
+
+// Launch shapes (no counterpart in ipie: its kernels' shapes are fixed by
+// CuPy / numba).  Each knob has a heuristic default; ipie_amd/autotune.py
+// times the candidates on the context's own shapes and sets the fastest.
+namespace {
+int tune_fb_max(const afqmc_ctx* c) {
+  if (c->bf16_fb) {
+    int kch = c->d.nalpha * c->kpad / 32;
+    if (two_spins(c) && c->d.nbeta > 0) kch = std::min(kch, c->d.nbeta * c->kpad / 32);
+    return std::max(1, std::min(32, kch));  // at least one K chunk per range
+  }
+  const int64_t N = c->d.nbasis;
+  const int64_t kch = ceil_div((int64_t)c->d.nalpha * N, 16) + (two_spins(c) ? ceil_div((int64_t)c->d.nbeta * N, 16) : 0);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(32, kch / 8));
+}
+int tune_exx_max(const afqmc_ctx* c) {
+  int hi = 1;
+  for (int s = 0; s < (two_spins(c) ? 2 : 1); ++s) {
+    const int n = s == 0 ? c->d.nalpha : c->d.nbeta;
+    if (n < 1) continue;
+    const int xb = (c->d.precision == AFQMC_PREC_MIXED && c->bx_ok[s]) ? c->bx_xb[s] : 1;
+    hi = std::max(hi, (int)ceil_div(c->d.nchol, xb));
+  }
+  return hi;
+}
+}  // namespace
+
+int afqmc_tune_get(afqmc_ctx* c, int32_t knob, int32_t* value, int32_t* lo, int32_t* hi) {
+  if (!c || !value || !lo || !hi) return AFQMC_ERR_INVALID_ARG;
+  *value = 0;
+  *lo = 0;
+  *hi = 0;
+  switch (knob) {
+    case AFQMC_TUNE_FB_SPLIT:
+      if (c->d.nchol < 1) return AFQMC_OK;
+      *value = c->bf16_fb ? c->fb_split_spin : c->fb_ksplit;
+      *lo = 1;
+      *hi = std::max(tune_fb_max(c), *value);
+      return AFQMC_OK;
+    case AFQMC_TUNE_QUAD_WGS:
+      if (!c->quad_on || c->d.precision != AFQMC_PREC_MIXED) return AFQMC_OK;
+      *value = c->quad_target;
+      *lo = 256;
+      *hi = 8192;
+      return AFQMC_OK;
+    case AFQMC_TUNE_EXX_SPLIT:
+      if (c->quad_on || c->d.nchol < 1) return AFQMC_OK;
+      *value = c->exx_split;
+      *lo = 1;
+      *hi = tune_exx_max(c);
+      return AFQMC_OK;
+    default:
+      return fail(c, AFQMC_ERR_INVALID_ARG, "unknown tuning knob %d", knob);
+  }
+}
+
+int afqmc_tune_set(afqmc_ctx* c, int32_t knob, int32_t value) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (c->capturing || t_capturing) return refuse_in_capture(c, "afqmc_tune_set");
+  int32_t cur, lo, hi;
+  const int r = afqmc_tune_get(c, knob, &cur, &lo, &hi);
+  if (r) return r;
+  if (hi < 1) return fail(c, AFQMC_ERR_UNSUPPORTED, "tuning knob %d does not apply to this context", knob);
+  const bool heuristic = knob == AFQMC_TUNE_EXX_SPLIT && value == 0;
+  if (!heuristic && (value < lo || value > hi))
+    return fail(c, AFQMC_ERR_INVALID_ARG, "tuning knob %d: %d outside [%d, %d]", knob, value, lo, hi);
+  (void)hipSetDevice(c->device);
+  if (knob == AFQMC_TUNE_FB_SPLIT) {
+    if (c->bf16_fb) {
+      c->fb_split_spin = value;
+      c->fb_ksplit = value * (two_spins(c) ? 2 : 1);
+    } else {
+      c->fb_ksplit = value;
+    }
+    if (c->ws_ready && c->fb_ksplit > c->fb_cap) {
+      double* part = nullptr;
+      if (dmalloc((void**)&part, 16 * (size_t)c->fb_ksplit * c->d.nwalkers * c->d.nchol) != hipSuccess)
+        return fail(c, AFQMC_ERR_ALLOC, "force-bias partial buffer allocation failed (%s)", c->describe_dims);
+      c->retired.push_back(c->vbias_part);
+      c->vbias_part = part;
+      c->fb_cap = c->fb_ksplit;
+    }
+  } else if (knob == AFQMC_TUNE_QUAD_WGS) {
+    c->quad_target = value;
+    if (c->ws_ready) return build_quad_units(c);
+  } else {
+    c->exx_split = value;
+  }
   return AFQMC_OK;
 }
 

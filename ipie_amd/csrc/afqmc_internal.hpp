@@ -118,7 +118,7 @@ struct ExchangeArgs {
   double* exx_part;     // [xsplit, W] complex, sum_x tr(T_x T_x) (no 1/2)
 };
 hipError_t launch_exchange(const ExchangeArgs& a, hipStream_t stream);
-int exchange_xsplit(int nwalkers, int nchol, int nocc);
+int exchange_xsplit(int nwalkers, int nchol, int nocc, int want = 0);
 
 struct ExchangeBf16Args {
   int nwalkers, nchol, nocc, kpad;  // kpad = N rounded up to 32
@@ -138,7 +138,8 @@ hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream);
 // C = A B^T on split-BF16 planes: A 4 planes [m][lda] (re hi, re lo, im hi,
 // im lo), B 2 planes [nn][ldb] (hi, lo), K = kchunks * 32 (zero padded).
 enum { BF16_EPI_PARTIAL = 0, BF16_EPI_VHS = 1, BF16_EPI_QDOT = 2, BF16_EPI_ADD = 3 };
-constexpr int BF16_FB_SPLIT = 8;  // K ranges per spin of the split-BF16 force bias
+constexpr int BF16_FB_SPLIT = 8;       // K ranges per spin of the split-BF16 force bias (default)
+constexpr int QUAD_TARGET_WGS = 1536;  // workgroups the quadratic-form column units aim for (default)
 struct Bf16GemmArgs {
   const uint16_t* a;
   int64_t a_plane, lda, m;

@@ -358,12 +358,13 @@ size_t ex_lds_bytes(const ExPlan& pl, int nocc) {
 
 }  // namespace
 
-int exchange_xsplit(int nwalkers, int nchol, int nocc) {
+int exchange_xsplit(int nwalkers, int nchol, int nocc, int want) {
   const ExPlan pl = choose_plan(nocc);
   const int nwb = (int)ceil_div(nwalkers, pl.wb);
   const int nxb = (int)ceil_div(nchol, pl.xb);
-  // aim for >= 2048 workgroups, never more splits than x blocks
-  int xs = (int)ceil_div(2048, nwb);
+  // want > 0: that split; otherwise aim for >= 2048 workgroups.  Never more
+  // splits than x blocks.
+  int xs = want > 0 ? want : (int)ceil_div(2048, nwb);
   if (xs > nxb) xs = nxb;
   if (xs < 1) xs = 1;
   return xs;

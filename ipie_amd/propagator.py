@@ -34,6 +34,7 @@ import numpy
 import scipy.linalg
 import torch
 
+from ipie_amd import autotune as _autotune
 from ipie_amd import engine as _engine
 
 
@@ -64,8 +65,11 @@ def construct_one_body_propagator(hamiltonian, mf_shift, dt):
 
 
 class PhaselessGeneric:
-    def __init__(self, time_step, exp_nmax=6, verbose=False, seed=7, precision="fp64", surrogate=None):
+    def __init__(self, time_step, exp_nmax=6, verbose=False, seed=7, precision="fp64", surrogate=None,
+                 autotune=None):
         self.dt = time_step
+        self.autotune = autotune  # None / True / cache path / ipie_amd.autotune.Autotuner
+        self.tuning = None
         self.precision = precision
         self.surrogate = surrogate
         self.verbose = verbose
@@ -105,9 +109,14 @@ class PhaselessGeneric:
             eng = _engine.Engine(trial, walkers, nchol=hamiltonian.nchol, exp_nmax=self.exp_nmax,
                                  precision=self.precision)
             eng.set_hamiltonian(hamiltonian, self.expH1, self.mf_shift, self.dt)
+        opts = None
         if self.surrogate:
             opts = dict(fraction=0.125, warmup=100, train_iters=4, lr=1e-3, seed=self.seed)
             opts.update(self.surrogate if isinstance(self.surrogate, dict) else {})
+        tuner = _autotune.as_tuner(self.autotune)
+        if tuner is not None:  # before the control variate is on: its step trains it
+            self.tuning = tuner.tune_walkers(eng.ctx, walkers, surrogate=opts is not None)
+        if opts is not None:
             eng.ctx.surrogate_configure(True, **opts)
         walkers._engine = eng
         self._engine = eng

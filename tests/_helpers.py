@@ -37,7 +37,7 @@ class Pair:
     """Oracle objects and device objects built from one set of arrays."""
 
     def __init__(self, h1e, chol, psi, nelec, nwalkers, dt=0.005, ecore=0.0, init=None, rhf=False, exp_nmax=6,
-                 precision="fp64", surrogate=None):
+                 precision="fp64", surrogate=None, autotune=None):
         import torch
 
         from ipie_amd import Generic as DevSystem
@@ -59,7 +59,8 @@ class Pair:
         self.walkers = UHFWalkers(psi, nelec[0], nelec[1], self.N, nwalkers)
         self.walkers.rhf = rhf
         self.orc.walkers.rhf = rhf
-        self.prop = PhaselessGeneric(dt, exp_nmax=exp_nmax, precision=precision, surrogate=surrogate)
+        self.prop = PhaselessGeneric(dt, exp_nmax=exp_nmax, precision=precision, surrogate=surrogate,
+                                     autotune=autotune)
         self.prop.build(self.ham, self.trial, self.walkers)
         if init is not None:
             self.set_phi(init[:, :, : nelec[0]], init[:, :, nelec[0]:])

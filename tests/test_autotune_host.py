@@ -1,0 +1,113 @@
+"""Host logic of the launch-shape autotuner: candidate ladders, the choice
+rule (the heuristic stays unless beaten by more than min_gain), the JSON
+cache keyed by shape, and re-application from the cache without timing.  A
+stand-in context with a cost model replaces the device (no GPU here)."""
+
+import json
+
+import pytest
+
+from ipie_amd import autotune
+
+
+class FakeDims:
+    nbasis, nchol, nalpha, nbeta, nwalkers, rhf = 40, 120, 4, 4, 32, 0
+
+
+class FakeCtx:
+    """Knob ranges like a mixed-precision quadratic-form context; the cost of a
+    call is a function of the current knob values."""
+
+    device = "cpu"
+    precision = "mixed"
+    dims = FakeDims()
+
+    def __init__(self):
+        self.v = {"fb_split": 8, "quad_wgs": 1536, "exx_split": 0}
+        self.ranges = {"fb_split": (1, 12), "quad_wgs": (256, 8192), "exx_split": (0, 0)}
+        self.calls = []
+
+    def tune_get(self, knob):
+        lo, hi = self.ranges[knob]
+        return self.v[knob], lo, hi
+
+    def tune_set(self, knob, value):
+        lo, hi = self.ranges[knob]
+        assert hi >= 1 and lo <= value <= hi, (knob, value)
+        self.v[knob] = value
+
+    def cost(self, knob):
+        if knob == "fb_split":
+            return abs(self.v["fb_split"] - 3) + 1.0
+        return abs(self.v["quad_wgs"] - 3072) / 1000.0 + 1.0
+
+
+def test_candidates_cover_the_range_and_the_current_value():
+    assert autotune.candidates("fb_split", 8, 1, 12) == [1, 2, 3, 4, 6, 8, 10, 12]
+    assert autotune.candidates("fb_split", 7, 1, 12) == [1, 2, 3, 4, 6, 7, 8, 10, 12]
+    assert autotune.candidates("quad_wgs", 1536, 256, 8192)[0] == 512
+    assert 0 in autotune.candidates("exx_split", 0, 1, 40)
+    assert max(autotune.candidates("exx_split", 0, 1, 40)) == 32
+    assert autotune.candidates("quad_wgs", 0, 0, 0) == []
+
+
+def test_choice_keeps_the_heuristic_unless_clearly_beaten():
+    assert autotune.choose({1: 1.0, 2: 0.9, 8: 0.99}, 8, 0.02) == 2
+    assert autotune.choose({1: 1.0, 2: 0.99, 8: 1.0}, 8, 0.02) == 8
+    assert autotune.choose({1: 1.0, 2: 1.0}, 5, 0.02) == 1  # default not timed: the fastest, lowest on ties
+
+
+def _fake_tune(monkeypatch, tuner, ctx):
+    def fake_time(c, fn):
+        return ctx.cost(fn())
+
+    monkeypatch.setattr(tuner, "_time", fake_time)
+    # the knob's entry point returns which cost applies
+    ctx.force_bias = lambda ga, gb, vb: "fb_split"
+    ctx.local_energy = lambda ga, gb, en: "quad_wgs"
+
+
+def test_tune_sets_the_fastest_and_caches_by_shape(tmp_path, monkeypatch):
+    torch = pytest.importorskip("torch")
+    path = tmp_path / "cache.json"
+    t = autotune.Autotuner(cache_path=str(path))
+    ctx = FakeCtx()
+    _fake_tune(monkeypatch, t, ctx)
+    g = torch.zeros(32, 4, 40, dtype=torch.complex128)
+    chosen = t.tune(ctx, g, g)
+    assert chosen == {"fb_split": 3, "quad_wgs": 3072}
+    assert ctx.v["fb_split"] == 3 and ctx.v["quad_wgs"] == 3072
+    key = autotune.shape_key(ctx)
+    assert "N=40" in key and "X=120" in key and "W=32" in key and key.endswith("mixed")
+    assert autotune.shape_key(ctx, True) == key + "|surrogate"
+    assert json.loads(path.read_text())[key] == chosen
+    rep = t.reports[key]["fb_split"]
+    assert rep["default"] == 8 and rep["chosen"] == 3 and set(rep["ms"]) == {"1", "2", "3", "4", "6", "8", "10", "12"}
+    # a new tuner applies the cache to a fresh context of the same shape
+    t2 = autotune.Autotuner(cache_path=str(path))
+    fresh = FakeCtx()
+    assert t2.apply(fresh) == chosen
+    assert fresh.v["fb_split"] == 3 and fresh.v["quad_wgs"] == 3072
+    other = FakeCtx()
+    other.dims = type("D", (), dict(vars(FakeDims), nwalkers=64))()
+    assert t2.apply(other) is None  # another shape: nothing cached
+
+
+def test_as_tuner_specs(tmp_path):
+    assert autotune.as_tuner(None) is None and autotune.as_tuner(False) is None
+    t = autotune.Autotuner()
+    assert autotune.as_tuner(t) is t
+    p = str(tmp_path / "x.json")
+    assert autotune.as_tuner(p).cache_path == p
+    assert autotune.as_tuner(True).cache_path == autotune.DEFAULT_CACHE
+
+
+def test_surrogate_contexts_tune_the_force_bias_only(monkeypatch):
+    torch = pytest.importorskip("torch")
+    t = autotune.Autotuner()
+    ctx = FakeCtx()
+    _fake_tune(monkeypatch, t, ctx)
+    g = torch.zeros(32, 4, 40, dtype=torch.complex128)
+    assert t.tune(ctx, g, g, surrogate=True) == {"fb_split": 3}
+    assert ctx.v["quad_wgs"] == 1536
+    assert autotune.shape_key(ctx, True) in t.cache and autotune.shape_key(ctx) not in t.cache
