@@ -3,13 +3,20 @@
 // ipie's CPU path (ipie/walkers/uhf_walkers.py:108-143) takes LAPACK's
 // economic QR and multiplies Q by sign(diag R), i.e. it returns the unique Q
 // whose R has a positive real diagonal.  Here the same Q is produced by
-// shifted Cholesky-QR in three passes (a shifted pass followed by CholeskyQR2,
-// Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput.
-// 42 (2020) A477), which is as accurate as Householder QR for any numerically
-// full-rank walker and maps onto per-row independent work:
-//   G = X^H X (+ shift on pass 1),  G = R^H R,  X <- X R^{-1},
-// as four sweeps over the rows of X on FP64 MFMA: the Gram matrix of pass
-// p + 1 is formed from the rows of X R_p^{-1} while they are written.
+// Cholesky-QR, which maps onto per-row independent work:
+//   G = X^H X (+ shift),  G = R^H R,  X <- X R^{-1},
+// as sweeps over the rows of X on FP64 MFMA: the Gram matrix of pass p + 1 is
+// formed from the rows of X R_p^{-1} while they are written.
+//   - CholeskyQR2 (two passes, three sweeps; Yamamoto, Nakatsukasa, Yanagisawa,
+//     Fukaya, ETNA 44 (2015) 306) when the unshifted Cholesky of X^H X succeeds
+//     and 8 kappa sqrt(N n u + n (n + 1) u) <= 1/2 with kappa bounded by
+//     ||R||_F ||R^-1||_F >= kappa_2(X): then Q is orthogonal to O(u).  Walkers
+//     re-orthogonalised every few steps are far inside this bound.
+//   - otherwise shifted CholeskyQR3 (a shifted pass, then CholeskyQR2; Fukaya,
+//     Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput. 42
+//     (2020) A477), as accurate as Householder QR for any numerically
+//     full-rank walker: the Gram matrix is formed again (the unshifted
+//     attempt overwrote it), then three passes and four sweeps.
 // log det R = sum over passes of sum_i log R_ii, and
 //   detR = exp(log det R_a + log det R_b - detR_shift),
 //   log_detR += log detR,  ovlp /= detR.
@@ -174,6 +181,38 @@ __device__ double qr_cholesky(cplx* G, int ldg, int n) {
   return logdet;
 }
 
+// ||R||_F ||R^{-1}||_F from the upper triangles (an upper bound of kappa_2(R)),
+// the same value in every thread (fixed-order sums).
+__device__ double qr_kappa_f(const cplx* R, const cplx* Ri, int ld, int n) {
+  __shared__ double part[2][QR_WAVES];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double a = 0.0, b = 0.0;
+  for (int e = threadIdx.x; e < n * n; e += QR_THREADS) {
+    const int i = e / n, j = e % n;
+    if (j >= i) {
+      const cplx r = R[i * ld + j], x = Ri[i * ld + j];
+      a += r.re * r.re + r.im * r.im;
+      b += x.re * x.re + x.im * x.im;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  if (lane == 0) {
+    part[0][wave] = a;
+    part[1][wave] = b;
+  }
+  __syncthreads();
+  double sa = 0.0, sb = 0.0;
+  for (int w = 0; w < QR_WAVES; ++w) {
+    sa += part[0][w];
+    sb += part[1][w];
+  }
+  __syncthreads();  // part is rewritten by the next call
+  return sqrt(sa * sb);
+}
+
 // Ri = R^{-1} (upper triangular, zero below and in the padding rows and
 // columns up to np), one column per thread.
 __device__ void qr_upper_inverse(const cplx* R, int ldr, int n, int np, cplx* Ri, int ldi) {
@@ -222,27 +261,43 @@ __global__ void __launch_bounds__(QR_THREADS) reortho_kernel(ReorthoArgs a) {
   const int nspin = a.nbeta > 0 ? 2 : 1;
   double logdet = 0.0;
   bool bad = false;
+  constexpr double u = 1.1102230246251565e-16;  // 2^-53
   for (int s = 0; s < nspin; ++s) {
     const int n = s == 0 ? a.nalpha : a.nbeta;
     double* phi = (s == 0 ? a.phia : a.phib) + 2 * (int64_t)w * N * n;
-    // sweep p: X <- X Ri_{p-1} (p > 0) and, for p < 3, G = X^H X
-    for (int pass = 0; pass <= 3; ++pass) {
-      const double tr = qr_sweep<NT>(phi, N, n, pass > 0 ? Ri : nullptr, ldg, pass < 3 ? G : nullptr, ldg);
-      if (pass == 3) break;
-      if (pass == 0) {
-        // shift = 11 (N n + n (n + 1)) u ||X||_F^2, u = 2^-53
-        const double shift = 11.0 * ((double)N * n + (double)n * (n + 1)) * 1.1102230246251565e-16 * tr;
+    const double mn = (double)N * n + (double)n * (n + 1);
+    // One loop, one call site per stage (the sweep is inlined once):
+    //   apply: X <- X Ri in this sweep;  gram: form G = X^H X in it.
+    // The first Gram pass is unshifted; if its Cholesky fails or R fails the
+    // CholeskyQR2 bound, the Gram matrix is formed again and shifted.  Then
+    // `left` apply + Gram passes (1 for CholeskyQR2, 2 after a shifted pass)
+    // and a final apply.
+    bool apply = false, gram = true, shifted = false;
+    int left = 0;
+    for (;;) {
+      const double tr = qr_sweep<NT>(phi, N, n, apply ? Ri : nullptr, ldg, gram ? G : nullptr, ldg);
+      if (!gram) break;
+      if (shifted && !apply) {
+        // shift = 11 (N n + n (n + 1)) u ||X||_F^2
         if (threadIdx.x == 0)
-          for (int i = 0; i < n; ++i) G[i * ldg + i].re += shift;
+          for (int i = 0; i < n; ++i) G[i * ldg + i].re += 11.0 * mn * u * tr;
         __syncthreads();
       }
       const double ldr = qr_cholesky(G, ldg, n);
-      if (!(ldr == ldr)) {
+      const bool ok = ldr == ldr;  // the same in every thread
+      if (ok) qr_upper_inverse(G, ldg, n, 16 * NT, Ri, ldg);
+      if (!apply && !shifted && !(ok && 8.0 * qr_kappa_f(G, Ri, ldg, n) * sqrt(mn * u) <= 0.5)) {
+        shifted = true;
+        continue;
+      }
+      if (!ok) {
         bad = true;
         break;
       }
       logdet += ldr;
-      qr_upper_inverse(G, ldg, n, 16 * NT, Ri, ldg);
+      left = apply ? left - 1 : (shifted ? 2 : 1);
+      apply = true;
+      gram = left > 0;
     }
     if (bad) break;
   }

@@ -170,6 +170,47 @@ def test_reortho(name):
     assert rel_err(p.walkers.log_detR.cpu().numpy(), numpy.array(p.orc.walkers.log_detR)) < TOL
 
 
+def _ill_conditioned(N, n, W, kappa, rs):
+    """W walkers [N, n] complex with singular values logspaced from 1 to 1/kappa."""
+    out = numpy.empty((W, N, n), dtype=numpy.complex128)
+    s = numpy.logspace(0.0, -numpy.log10(kappa), n)
+    for w in range(W):
+        u, _ = numpy.linalg.qr(rs.normal(size=(N, n)) + 1j * rs.normal(size=(N, n)))
+        v, _ = numpy.linalg.qr(rs.normal(size=(n, n)) + 1j * rs.normal(size=(n, n)))
+        out[w] = (u * s) @ v.conj().T
+    return out
+
+
+@pytest.mark.parametrize("name,kappa", [("n64_n40", 1e6), ("odd", 1e6), ("odd", 1e11)])
+def test_reortho_ill_conditioned(name, kappa):
+    """Walkers beyond the CholeskyQR2 bound (kappa 1e6: the unshifted Cholesky
+    succeeds but R fails the bound; 1e11: the unshifted Cholesky breaks down)
+    take the shifted three-pass path: Q orthonormal to 1e-12, X = Q R with R
+    upper triangular, and Q, detR as numpy's QR to kappa-scaled tolerances."""
+    p = make_pair(name)
+    N, X, D, na, nb, W, scale = CASES[name]
+    print(f"[case] ill-conditioned walkers kappa={kappa:g}", flush=True)
+    rs = numpy.random.RandomState(17)
+    xa, xb = _ill_conditioned(N, na, W, kappa, rs), _ill_conditioned(N, nb, W, kappa, rs)
+    p.set_phi(xa, xb)
+    p.walkers.ovlp.fill_(1.0)
+    p.orc.walkers.ovlp = numpy.ones(W, dtype=numpy.complex128)
+    detR = p.walkers.reortho().cpu().numpy()
+    detR_o = numpy.array(p.orc.walkers.reortho())
+    tol = max(1e-9, 1e3 * kappa * 1.1e-16)
+    for x, q_dev, q_orc in ((xa, p.walkers.phia, p.orc.walkers.phia), (xb, p.walkers.phib, p.orc.walkers.phib)):
+        q = q_dev.cpu().numpy()
+        n = x.shape[2]
+        for w in range(W):
+            qq = q[w].conj().T @ q[w]
+            assert numpy.abs(qq - numpy.eye(n)).max() < 1e-12, (kappa, w, "orthogonality")
+            r = q[w].conj().T @ x[w]
+            assert numpy.abs(numpy.tril(r, -1)).max() < 1e-12 * numpy.abs(r).max(), (kappa, w, "triangular")
+            assert numpy.all(r.diagonal().real > 0.0), (kappa, w, "positive diagonal")
+        assert rel_err(q, q_orc) < tol, (kappa, "Q vs numpy", rel_err(q, q_orc))
+    assert rel_err(detR, detR_o) < tol, (kappa, "detR", rel_err(detR, detR_o))
+
+
 def test_rhf_walkers():
     name = "n24_norm"
     p = make_pair(name, rhf=True)
