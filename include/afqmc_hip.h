@@ -103,6 +103,9 @@ int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims);
 void afqmc_destroy(afqmc_ctx* ctx);
 /* hipStream_t passed as void*; NULL = the default stream */
 int afqmc_set_stream(afqmc_ctx* ctx, void* stream);
+/* Borrows the static tensors.  Reads the trial orbitals once on the context's
+ * stream (and waits for it): when their imaginary parts are all zero, the
+ * overlap matrices take two real products per step instead of four. */
 int afqmc_set_static(afqmc_ctx* ctx, const afqmc_static* s);
 const char* afqmc_last_error(const afqmc_ctx* ctx);
 int afqmc_sync(afqmc_ctx* ctx);

@@ -228,9 +228,8 @@ class Context:
             ptr(chol_packed), ptr(rchola), ptr(rcholb), ptr(rH1a), ptr(rH1b), ptr(expH1), ptr(mf_shift),
             ptr(psia), ptr(psib), float(dt), float(ecore),
         )
-        st = self.lib.afqmc_set_static(self.h, ctypes.byref(s))
-        if st != AFQMC_OK:
-            raise AfqmcError(f"afqmc_set_static: {STATUS_NAMES.get(st, st)}: {self.lib.afqmc_last_error(self.h)}")
+        # on the current stream: the context reads the trial orbitals (real or complex) after their upload
+        self._call("afqmc_set_static", ctypes.byref(s))
         # the context borrows these: keep them alive
         self._static_refs = (chol_packed, rchola, rcholb, rH1a, rH1b, expH1, mf_shift, psia, psib)
 

@@ -43,6 +43,8 @@ struct afqmc_ctx {
   double* buf[4] = {nullptr, nullptr, nullptr, nullptr};  // P, Ta, Tb, Q: [W, N, ncol] complex
   int* tri = nullptr;            // [2, P] packed-index rows / cols
   int* flags = nullptr;
+  int* psi_flag = nullptr;  // scratch of afqmc_set_static's check of the trial orbitals
+  bool psi_real = false;    // the trial orbitals are real (imaginary parts exactly zero)
   unsigned long long* nfb = nullptr;
   bool ws_ready = false;
   char describe_dims[128] = {0};
@@ -412,6 +414,7 @@ int do_gf(afqmc_ctx* c, int nw, const double* phia, const double* phib, double* 
   a.ovlp = ovlp;
   a.log_shift = log_shift;
   a.flags = c->flags;
+  a.psi_real = c->psi_real ? 1 : 0;
   if (!c->gf_o) {
     const size_t bytes = gf_work_bytes((int)c->d.nwalkers, c->d.nalpha, c->d.nbeta);
     if (dmalloc((void**)&c->gf_o, bytes) != hipSuccess || dmalloc((void**)&c->gf_oinv, bytes) != hipSuccess)
@@ -1207,7 +1210,7 @@ void afqmc_destroy(afqmc_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   else (void)hipDeviceSynchronize();
   void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
-                  c->buf[2], c->buf[3], c->tri, c->flags, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
+                  c->buf[2], c->buf[3], c->tri, c->flags, c->psi_flag, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
                   c->lplanes[1], c->xplanes, c->cplanes, c->vplanes, c->tplanes[0], c->dplanes, c->oplanes,
                   c->tplanes[1], c->gcomp, c->gf_o, c->gf_oinv, c->qw64[0], c->qw64[1], c->qwp[0],
                   c->qwp[1], c->qy, c->qunits[0], c->qunits[1], c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
@@ -1242,6 +1245,18 @@ int afqmc_set_static(afqmc_ctx* c, const afqmc_static* s) {
   if (c->d.nbeta > 0 && !s->psib && two) return fail(c, AFQMC_ERR_INVALID_ARG, "null psib");
   c->s = *s;
   c->have_static = true;
+  // real trial orbitals (the usual RHF / UHF trial of a real Hamiltonian): the
+  // overlap matrices skip the products with their zero imaginary parts
+  if (!c->psi_flag && dmalloc((void**)&c->psi_flag, sizeof(int)) != hipSuccess)
+    return fail(c, AFQMC_ERR_ALLOC, "flag allocation failed");
+  int any = 0;
+  HIP_TRY(c, hipMemsetAsync(c->psi_flag, 0, sizeof(int), c->stream));
+  HIP_TRY(c, launch_imag_any(s->psia, (int64_t)c->d.nbasis * c->d.nalpha, c->psi_flag, c->stream));
+  if (two && c->d.nbeta > 0)
+    HIP_TRY(c, launch_imag_any(s->psib, (int64_t)c->d.nbasis * c->d.nbeta, c->psi_flag, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(&any, c->psi_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->psi_real = any == 0;
   c->planes_ready = false;
   c->quad_ready = false;
   c->sur.cv_ready = false;

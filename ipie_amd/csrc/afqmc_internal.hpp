@@ -54,8 +54,12 @@ struct GfArgs {
   double* ovlp;         // [W] complex
   const double* log_shift;  // [W] or null
   int* flags;
+  int psi_real;  // the trial orbitals have zero imaginary parts (O takes two real MFMAs per K step, not four)
 };
 size_t gf_lds_bytes(int nalpha, int nbeta);
+// *flag = 1 if any of the n complex values z has a nonzero imaginary part (flag
+// must be zero before; every writer stores 1)
+hipError_t launch_imag_any(const double* z, int64_t n, int* flag, hipStream_t stream);
 // bytes of each GF work buffer (O, O^{-1}): [W][2][nmax][nmax] complex
 size_t gf_work_bytes(int nwalkers, int nalpha, int nbeta);
 hipError_t launch_gf(const GfArgs& a, bool want_ghalf, double* obuf, double* oinv, hipStream_t stream);

@@ -23,8 +23,9 @@ constexpr int GF_WAVES = 4;  // waves per workgroup of the MFMA kernels
 __device__ __forceinline__ int gf_nspin(const GfArgs& a) { return (a.nbeta > 0 && !a.rhf) ? 2 : 1; }
 
 // O[w][s][i][j] = sum_m phi_s[w][m][i] conj(psi_s[m][j]); wave = (w, s, row tile it).
-// MFMA operands: A[i][k = m] = phi[m][i], B[k = m][j] = conj(psi[m][j]).
-template <int NT>
+// MFMA operands: A[i][k = m] = phi[m][i], B[k = m][j] = conj(psi[m][j]).  REAL:
+// psi has zero imaginary parts, whose two MFMAs per step would add only zeros.
+template <int NT, bool REAL>
 __global__ void __launch_bounds__(64 * GF_WAVES) gf_obuild_kernel(GfArgs a, double* __restrict__ obuf, int nmax) {
   const int gw = blockIdx.x * GF_WAVES + (threadIdx.x >> 6);
   const int nspin = gf_nspin(a);
@@ -56,7 +57,10 @@ __global__ void __launch_bounds__(64 * GF_WAVES) gf_obuild_kernel(GfArgs a, doub
 #pragma unroll
       for (int jt = 0; jt < NT; ++jt) {
         const int j = jt * 16 + r16;
-        bv[u][jt] = (m < N && j < n) ? cconj(ld(psi, (int64_t)m * n + j)) : cmake(0.0, 0.0);
+        if (REAL)
+          bv[u][jt] = cmake((m < N && j < n) ? psi[2 * ((int64_t)m * n + j)] : 0.0, 0.0);
+        else
+          bv[u][jt] = (m < N && j < n) ? cconj(ld(psi, (int64_t)m * n + j)) : cmake(0.0, 0.0);
       }
     }
 #pragma unroll
@@ -65,8 +69,8 @@ __global__ void __launch_bounds__(64 * GF_WAVES) gf_obuild_kernel(GfArgs a, doub
 #pragma unroll
       for (int jt = 0; jt < NT; ++jt) {
         re[jt] = mfma64(av[u].re, bv[u][jt].re, re[jt]);
-        re[jt] = mfma64(-av[u].im, bv[u][jt].im, re[jt]);
-        im[jt] = mfma64(av[u].re, bv[u][jt].im, im[jt]);
+        if (!REAL) re[jt] = mfma64(-av[u].im, bv[u][jt].im, re[jt]);
+        if (!REAL) im[jt] = mfma64(av[u].re, bv[u][jt].im, im[jt]);
         im[jt] = mfma64(av[u].im, bv[u][jt].re, im[jt]);
       }
     }
@@ -326,7 +330,18 @@ __global__ void __launch_bounds__(64 * GF_WAVES) gf_ghalf_kernel(GfArgs a, const
   }
 }
 
+__global__ void __launch_bounds__(256) imag_any_kernel(const double* __restrict__ z, int64_t n, int* flag) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n && z[2 * i + 1] != 0.0) *flag = 1;
+}
+
 }  // namespace
+
+hipError_t launch_imag_any(const double* z, int64_t n, int* flag, hipStream_t stream) {
+  if (n < 1) return hipSuccess;
+  hipLaunchKernelGGL(imag_any_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, z, n, flag);
+  return hipGetLastError();
+}
 
 size_t gf_lds_bytes(int nalpha, int nbeta) {
   const int nmax = nalpha > nbeta ? nalpha : nbeta;
@@ -347,12 +362,22 @@ hipError_t launch_gf(const GfArgs& a, bool want_ghalf, double* obuf, double* oin
   const int nt = (nmax + 15) / 16;
   const int64_t waves = (int64_t)a.nwalkers * nspin * nt;
   const unsigned grid = (unsigned)ceil_div(waves, GF_WAVES);
+#define AFQMC_OBUILD(NT)                                                                                  \
+  do {                                                                                                   \
+    if (a.psi_real)                                                                                      \
+      hipLaunchKernelGGL((gf_obuild_kernel<NT, true>), dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, obuf, \
+                         nmax);                                                                          \
+    else                                                                                                 \
+      hipLaunchKernelGGL((gf_obuild_kernel<NT, false>), dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, obuf, \
+                         nmax);                                                                          \
+  } while (0)
   switch (nt) {
-    case 1: hipLaunchKernelGGL(gf_obuild_kernel<1>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, obuf, nmax); break;
-    case 2: hipLaunchKernelGGL(gf_obuild_kernel<2>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, obuf, nmax); break;
-    case 3: hipLaunchKernelGGL(gf_obuild_kernel<3>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, obuf, nmax); break;
-    default: hipLaunchKernelGGL(gf_obuild_kernel<4>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, obuf, nmax); break;
+    case 1: AFQMC_OBUILD(1); break;
+    case 2: AFQMC_OBUILD(2); break;
+    case 3: AFQMC_OBUILD(3); break;
+    default: AFQMC_OBUILD(4); break;
   }
+#undef AFQMC_OBUILD
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   // the inverse's columns need a second matrix of LDS; the overlap alone does not
