@@ -85,22 +85,58 @@ __global__ void __launch_bounds__(64 * GF_WAVES) gf_obuild_kernel(GfArgs a, doub
     }
 }
 
-// sum_{k0 <= k < k1} row[k] col[k ld] (complex), four independent partial sums.
+// sum_{k0 <= k < k1} row[k] col[k ld] (complex): eight loads of each operand in
+// flight, four independent partial sums.
 __device__ __forceinline__ cplx lu_dot(const cplx* row, const cplx* col, int ld, int k0, int k1) {
   cplx s4[4] = {cmake(0.0, 0.0), cmake(0.0, 0.0), cmake(0.0, 0.0), cmake(0.0, 0.0)};
   int k = k0;
-  for (; k + 4 <= k1; k += 4) {
-    cplx r[4], x[4];
+  for (; k + 8 <= k1; k += 8) {
+    cplx r[8], x[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       r[u] = row[k + u];
       x[u] = col[(k + u) * ld];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) cfma(s4[u], r[u], x[u]);
+    for (int u = 0; u < 8; ++u) cfma(s4[u & 3], r[u], x[u]);
   }
   for (; k < k1; ++k) cfma(s4[0], row[k], col[k * ld]);
   return cadd(cadd(s4[0], s4[1]), cadd(s4[2], s4[3]));
+}
+
+// max over the 64 lanes of a wave of v (every lane active), in every lane:
+// DPP swaps within rows of 16 lanes (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror pair each lane with one of another group), then
+// two shuffles across rows.
+__device__ __forceinline__ double dpp_max(double v, int ctrl) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  switch (ctrl) {  // the control must be a constant of the instruction
+    case 0xB1:
+      lo = __builtin_amdgcn_update_dpp(lo, lo, 0xB1, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(hi, hi, 0xB1, 0xF, 0xF, false);
+      break;
+    case 0x4E:
+      lo = __builtin_amdgcn_update_dpp(lo, lo, 0x4E, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(hi, hi, 0x4E, 0xF, 0xF, false);
+      break;
+    case 0x141:
+      lo = __builtin_amdgcn_update_dpp(lo, lo, 0x141, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(hi, hi, 0x141, 0xF, 0xF, false);
+      break;
+    default:
+      lo = __builtin_amdgcn_update_dpp(lo, lo, 0x140, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(hi, hi, 0x140, 0xF, 0xF, false);
+      break;
+  }
+  return fmax(v, __hiloint2double(hi, lo));
+}
+__device__ __forceinline__ double wave_max(double v) {
+  v = dpp_max(v, 0xB1);
+  v = dpp_max(v, 0x4E);
+  v = dpp_max(v, 0x141);
+  v = dpp_max(v, 0x140);
+  v = fmax(v, __shfl_xor(v, 16, 64));
+  return fmax(v, __shfl_xor(v, 32, 64));
 }
 
 // One wave per walker: LU of O_s (in LDS, row stride nmax + 1), log-det and
@@ -127,21 +163,16 @@ __global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __res
     __syncthreads();
     int swaps = 0;
     for (int k = 0; k < n; ++k) {
+      // pivot: the largest |re| + |im| in rows k..n-1, the first such row on ties
+      const bool cand = lane >= k && lane < n;
       double v = -1.0;
-      int best = lane;
-      if (lane >= k && lane < n) {
+      if (cand) {
         const cplx z = LU[lane * lr + k];
         v = fabs(z.re) + fabs(z.im);
       }
-      for (int off = 1; off < 64; off <<= 1) {
-        const double ov = __shfl_xor(v, off, 64);
-        const int ob = __shfl_xor(best, off, 64);
-        if (ov > v || (ov == v && ob < best)) {
-          v = ov;
-          best = ob;
-        }
-      }
-      const int p = best;  // the same in every lane after the butterfly
+      const double vmax = wave_max(v);
+      const unsigned long long hit = __ballot(cand && v == vmax);
+      const int p = hit ? __ffsll(hit) - 1 : k;  // the same in every lane
       if (lane == 0) piv[k] = p;
       if (p != k) {
         ++swaps;
@@ -162,16 +193,16 @@ __global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __res
         row[k] = l;
         const cplx* urow = LU + k * lr;
         int j = k + 1;
-        // four columns at a time, loads before stores (row k is not this lane's row)
-        for (; j + 4 <= n; j += 4) {
-          cplx u[4], x[4];
+        // eight columns at a time, loads before stores (row k is not this lane's row)
+        for (; j + 8 <= n; j += 8) {
+          cplx u[8], x[8];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < 8; ++q) {
             u[q] = urow[j + q];
             x[q] = row[j + q];
           }
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < 8; ++q) {
             x[q].re -= l.re * u[q].re - l.im * u[q].im;
             x[q].im -= l.re * u[q].im + l.im * u[q].re;
             row[j + q] = x[q];
