@@ -44,6 +44,7 @@ struct afqmc_ctx {
   int* tri = nullptr;            // [2, P] packed-index rows / cols
   int* flags = nullptr;
   int* psi_flag = nullptr;  // scratch of afqmc_set_static's check of the trial orbitals
+  int g_planes_nw = 0;      // > 0: gplanes hold Ghalf of these nw walkers, written by the GF of this step
   bool psi_real = false;    // the trial orbitals are real (imaginary parts exactly zero)
   unsigned long long* nfb = nullptr;
   bool ws_ready = false;
@@ -398,8 +399,8 @@ int ensure_planes(afqmc_ctx* c) {
 
 // ---------------------------------------------------------------- phases
 int do_gf(afqmc_ctx* c, int nw, const double* phia, const double* phib, double* ga, double* gb, double* ovlp,
-          const double* log_shift, bool want_ghalf) {
-  GfArgs a;
+          const double* log_shift, bool want_ghalf, bool fb_planes = false) {
+  GfArgs a{};
   a.nwalkers = nw;
   a.nbasis = c->d.nbasis;
   a.nalpha = c->d.nalpha;
@@ -415,6 +416,15 @@ int do_gf(afqmc_ctx* c, int nw, const double* phia, const double* phib, double* 
   a.log_shift = log_shift;
   a.flags = c->flags;
   a.psi_real = c->psi_real ? 1 : 0;
+  // the split-BF16 force bias that follows in the same step reads Ghalf's planes
+  // from this kernel instead of splitting Ghalf again
+  c->g_planes_nw = 0;
+  if (fb_planes && want_ghalf && c->bf16_fb && c->gplanes[0] && (!two_spins(c) || c->gplanes[1])) {
+    a.gplanes[0] = c->gplanes[0];
+    a.gplanes[1] = two_spins(c) ? c->gplanes[1] : nullptr;
+    a.kpad = c->kpad;
+    c->g_planes_nw = nw;
+  }
   if (!c->gf_o) {
     const size_t bytes = gf_work_bytes((int)c->d.nwalkers, c->d.nalpha, c->d.nbeta);
     if (dmalloc((void**)&c->gf_o, bytes) != hipSuccess || dmalloc((void**)&c->gf_oinv, bytes) != hipSuccess)
@@ -436,7 +446,8 @@ int do_force_bias_bf16(afqmc_ctx* c, int nw, const double* ga, const double* gb)
   const int nspin = two_spins(c) ? 2 : 1;
   for (int s = 0; s < nspin; ++s) {
     const int n = s == 0 ? c->d.nalpha : c->d.nbeta;
-    HIP_TRY(c, launch_split_ghalf(s == 0 ? ga : gb, (int64_t)nw * n, N, c->kpad, c->gplanes[s], c->stream));
+    if (c->g_planes_nw != nw)
+      HIP_TRY(c, launch_split_ghalf(s == 0 ? ga : gb, (int64_t)nw * n, N, c->kpad, c->gplanes[s], c->stream));
     Bf16GemmArgs b{};
     b.a = c->gplanes[s];
     b.a_plane = (int64_t)nw * n * c->kpad;
@@ -455,6 +466,7 @@ int do_force_bias_bf16(afqmc_ctx* c, int nw, const double* ga, const double* gb)
     b.scale = (c->d.rhf && c->d.nbeta > 0) ? 2.0 : 1.0;
     HIP_TRY(c, launch_bf16_gemm(b, c->stream));
   }
+  c->g_planes_nw = 0;  // the exchange reuses the planes
   return AFQMC_OK;
 }
 
@@ -1694,12 +1706,13 @@ int afqmc_propagate(afqmc_ctx* c, int32_t nw, double* phia, double* phib, double
   if (r) return r;
   if (!phia || !ga || !weight || !ovlp || !hybrid_energy || (two_spins(c) && (!phib || !gb)))
     return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
-  r = do_gf(c, nw, phia, phib, ga, gb, wv(c, WV_OVLP_OLD), log_shift, true);
+  r = do_gf(c, nw, phia, phib, ga, gb, wv(c, WV_OVLP_OLD), log_shift, true, true);
   if (r) return r;
   r = gather(c, nw, phia, phib, true);
   if (r) return r;
   r = do_two_body_core(c, nw, ga, gb, xi, seed, counter, nullptr, nullptr, energy_prev != nullptr, energy_prev,
                        wv(c, WV_OVLP_OLD));
+  c->g_planes_nw = 0;
   if (r) return r;
   const double* res = nullptr;
   const uint16_t* resT = nullptr;

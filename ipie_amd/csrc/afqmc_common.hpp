@@ -43,6 +43,18 @@ __device__ __forceinline__ void cfma(cplx& a, cplx b, cplx c) {
   a.im = fma(b.im, c.re, a.im);
 }
 __device__ __forceinline__ double cabs(cplx a) { return hypot(a.re, a.im); }
+
+// Split-BF16 pair of an FP64 value: hi = bf16(x) and lo = bf16(x - hi), each
+// rounded to nearest even from FP32 (|x - hi - lo| <= 2^-17 |x|).
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ void split_bf16_pair(double x, uint16_t& hi, uint16_t& lo) {
+  hi = bf16_rne((float)x);
+  lo = bf16_rne((float)(x - (double)__uint_as_float(((uint32_t)hi) << 16)));
+}
 __device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
   // Smith's algorithm (no overflow for moderate exponents).
   if (fabs(b.re) >= fabs(b.im)) {

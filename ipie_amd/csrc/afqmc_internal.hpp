@@ -55,6 +55,10 @@ struct GfArgs {
   const double* log_shift;  // [W] or null
   int* flags;
   int psi_real;  // the trial orbitals have zero imaginary parts (O takes two real MFMAs per K step, not four)
+  // optional: Ghalf_s also as the 4 split-BF16 planes [W n_s, kpad] of spin s
+  // (re_hi, re_lo, im_hi, im_lo; K padding zero), as split_ghalf writes them
+  uint16_t* gplanes[2];
+  int kpad;
 };
 size_t gf_lds_bytes(int nalpha, int nbeta);
 // *flag = 1 if any of the n complex values z has a nonzero imaginary part (flag

@@ -76,10 +76,7 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-__device__ __forceinline__ void split_bf16(double x, uint16_t& hi, uint16_t& lo) {
-  hi = f2bf((float)x);
-  lo = f2bf((float)(x - (double)bf2f(hi)));
-}
+__device__ __forceinline__ void split_bf16(double x, uint16_t& hi, uint16_t& lo) { split_bf16_pair(x, hi, lo); }
 
 __device__ __forceinline__ v8bf as_v8bf(v4u u) { return __builtin_bit_cast(v8bf, u); }
 

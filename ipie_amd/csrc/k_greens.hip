@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(256) gf_combine_kernel(GfArgs a, const double*
 
 // Ghalf_s[w][i][m] = sum_j Oinv[w][s][i][j] phi_s[w][m][j]; wave = (w, s, row tile it).
 // MFMA operands: A[i][k = j] = Oinv[i][j] (held for all K steps), B[k = j][m] = phi[m][j].
-template <int KS>
+template <int KS, bool PLANES>
 __global__ void __launch_bounds__(64 * GF_WAVES) gf_ghalf_kernel(GfArgs a, const double* __restrict__ oinv, int nmax,
                                                                 int ntile) {
   const int gw = blockIdx.x * GF_WAVES + (threadIdx.x >> 6);
@@ -317,6 +317,9 @@ __global__ void __launch_bounds__(64 * GF_WAVES) gf_ghalf_kernel(GfArgs a, const
   const double* phi = (s == 0 ? a.phia : a.phib) + 2 * (int64_t)w * N * n;
   const double* oi = oinv + 2 * ((int64_t)w * 2 + s) * nmax * nmax;
   double* gh = (s == 0 ? a.ghalfa : a.ghalfb) + 2 * (int64_t)w * n * N;
+  uint16_t* gp = PLANES ? a.gplanes[s] : nullptr;  // split-BF16 planes of Ghalf_s, or null
+  const int64_t pstride = (int64_t)a.nwalkers * n * a.kpad;
+  uint16_t* gprow = gp ? gp + (int64_t)w * n * a.kpad : nullptr;
   const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int i = it * 16 + r16;
   cplx af[KS];
@@ -355,7 +358,32 @@ __global__ void __launch_bounds__(64 * GF_WAVES) gf_ghalf_kernel(GfArgs a, const
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = it * 16 + 4 * r + kq;  // accumulator row = 4 r + lane / 16
-        if (row < n && m < N) st(gh, (int64_t)row * N + m, cmake(re[r], im[r]));
+        if (row < n && m < N) {
+          st(gh, (int64_t)row * N + m, cmake(re[r], im[r]));
+          if (PLANES && gprow) {
+            uint16_t rh, rl, ih, il;
+            split_bf16_pair(re[r], rh, rl);
+            split_bf16_pair(im[r], ih, il);
+            const int64_t o = (int64_t)row * a.kpad + m;
+            gprow[o] = rh;
+            gprow[pstride + o] = rl;
+            gprow[2 * pstride + o] = ih;
+            gprow[3 * pstride + o] = il;
+          }
+        }
+      }
+    }
+  }
+  if (PLANES && gprow) {  // zero K padding of this wave's rows
+    const int pad = a.kpad - N;
+    for (int e = lane; e < 16 * pad; e += 64) {
+      const int row = it * 16 + e / pad, m = N + e % pad;
+      if (row < n) {
+        const int64_t o = (int64_t)row * a.kpad + m;
+        gprow[o] = 0;
+        gprow[pstride + o] = 0;
+        gprow[2 * pstride + o] = 0;
+        gprow[3 * pstride + o] = 0;
       }
     }
   }
@@ -423,12 +451,21 @@ hipError_t launch_gf(const GfArgs& a, bool want_ghalf, double* obuf, double* oin
   hipLaunchKernelGGL(gf_combine_kernel, dim3((unsigned)ceil_div(a.nwalkers, 256)), dim3(256), 0, stream, a, res);
   e = hipGetLastError();
   if (e != hipSuccess || !want_ghalf) return e;
+  const bool planes = a.gplanes[0] != nullptr;
+#define AFQMC_GHALF(KS)                                                                                      \
+  do {                                                                                                       \
+    if (planes)                                                                                              \
+      hipLaunchKernelGGL((gf_ghalf_kernel<KS, true>), dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, oinv, nmax, nt); \
+    else                                                                                                     \
+      hipLaunchKernelGGL((gf_ghalf_kernel<KS, false>), dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, oinv, nmax, nt); \
+  } while (0)
   switch (nt) {  // K steps of 4 cover nmax <= 16 nt
-    case 1: hipLaunchKernelGGL(gf_ghalf_kernel<4>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, oinv, nmax, nt); break;
-    case 2: hipLaunchKernelGGL(gf_ghalf_kernel<8>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, oinv, nmax, nt); break;
-    case 3: hipLaunchKernelGGL(gf_ghalf_kernel<12>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, oinv, nmax, nt); break;
-    default: hipLaunchKernelGGL(gf_ghalf_kernel<16>, dim3(grid), dim3(64 * GF_WAVES), 0, stream, a, oinv, nmax, nt); break;
+    case 1: AFQMC_GHALF(4); break;
+    case 2: AFQMC_GHALF(8); break;
+    case 3: AFQMC_GHALF(12); break;
+    default: AFQMC_GHALF(16); break;
   }
+#undef AFQMC_GHALF
   return hipGetLastError();
 }
 
