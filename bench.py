@@ -438,8 +438,8 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters behind roofline.traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs one GPU only (asked for --gpus {args.gpus}); the walker-sharded multi-GPU "
-              "design is described in DESIGN.md and not built here.", file=sys.stderr)
+        print(f"bench.py runs one GPU only (asked for --gpus {args.gpus}); sharding walkers over several "
+              "GPUs is a design that is not built here.", file=sys.stderr)
         sys.exit(2)
 
     import dataclasses
