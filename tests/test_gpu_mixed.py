@@ -110,3 +110,23 @@ def test_mixed_propagation_step(name):
         err = rel_err(getattr(p.walkers, attr).cpu().numpy(), numpy.asarray(getattr(p.orc.walkers, attr)))
         print(f"{attr} rel err {err:.2e}")
         assert err < t, attr
+
+
+@pytest.mark.parametrize("name", ["n64_w300", "n96_n24_n12"])
+def test_mixed_fused_step_equals_stepwise(name):
+    """afqmc_propagate (the Green's function kernel writes Ghalf's split-BF16
+    planes for the force bias; the one body writes P^T's planes for the Taylor
+    series) against the separate entry points (each splits its own operands),
+    with the same fields: the planes are the same values, so the steps agree to
+    rounding (a stale or misplaced plane would differ by ~1e-3 relative)."""
+    a, b = make(name, "mixed"), make(name, "mixed")
+    N, X, D, na, nb, W, scale = CASES[name]
+    xi = numpy.random.RandomState(5).normal(size=(W, X))
+    for p in (a, b):
+        p.prop.xi_source = lambda w, x: xi
+    for step in range(2):
+        a.prop.propagate_walkers(a.walkers, a.ham, a.trial, 0.0)
+        b.prop.propagate_walkers_stepwise(b.walkers, b.ham, b.trial, 0.0)
+        for attr in ("phia", "phib", "weight", "ovlp", "hybrid_energy"):
+            err = rel_err(getattr(a.walkers, attr).cpu().numpy(), getattr(b.walkers, attr).cpu().numpy())
+            assert err < 1e-12, (step, attr, err)
