@@ -208,3 +208,46 @@ def test_surrogate_context_tunes_the_force_bias_only():
     torch.cuda.synchronize()
     assert p.prop.surrogate_stats()["steps"] == u.prop.surrogate_stats()["steps"]
     assert rel_err(e1.cpu().numpy(), e2.cpu().numpy()) < 1e-4
+
+
+def test_shape_buckets_tune_once_and_batch():
+    """Systems built with one shared tuner: a shape is timed once and later
+    systems of that shape reuse its choices (a bucket); the tuned systems then
+    run in one multi-system graph, equal to stepping them eagerly."""
+    import torch
+
+    from ipie_amd.autotune import Autotuner
+    from ipie_amd.batch import MultiSystemBatch
+
+    t = Autotuner(cache_path=None, repeats=1, inner=1)
+    timed = []
+    real_tune = t.tune
+
+    def counting_tune(ctx, ga, gb, surrogate=False):
+        timed.append(ctx.describe())
+        return real_tune(ctx, ga, gb, surrogate)
+
+    t.tune = counting_tune
+    cases = ["mixed_quad", "mixed_quad", "fp64_t"]
+    tuned = [make(c, autotune=t) for c in cases]
+    assert len(timed) == 2 and len(t.cache) == 2  # two shapes, three systems
+    assert tuned[0].prop.tuning == tuned[1].prop.tuning
+    eager = [make(c) for c in cases]
+    for q, p in zip(eager, tuned):
+        q.walkers.phia.copy_(p.walkers.phia)
+        q.walkers.phib.copy_(p.walkers.phib)
+    e_t = [torch.empty(p.W, 3, dtype=torch.complex128, device=p.walkers.device) for p in tuned]
+    e_e = [torch.empty_like(x) for x in e_t]
+    for p, e in zip(tuned + eager, e_t + e_e):
+        _block(p, e, 0.0)
+    batch = MultiSystemBatch([(p.prop, p.walkers, p.ham, p.trial) for p in tuned])
+    batch.capture(5, reortho_every=5, energies=e_t)
+    batch.replay([0.0] * len(tuned), replays=1)
+    for p, e in zip(eager, e_e):
+        _block(p, e, 0.0)
+    torch.cuda.synchronize()
+    for k, (p, q) in enumerate(zip(tuned, eager)):
+        tol = TOL[CASES[cases[k]][-1]] * 5
+        for attr in FIELDS:
+            err = rel_err(getattr(p.walkers, attr).cpu().numpy(), getattr(q.walkers, attr).cpu().numpy())
+            assert err < tol, (cases[k], attr, err)
