@@ -463,7 +463,7 @@ int do_force_bias_bf16(afqmc_ctx* c, int nw, const double* ga, const double* gb)
     b.out = c->vbias_part;
     b.out32 = reinterpret_cast<float*>(c->vbias_part);  // read back by fields_kernel as FP32 (part32)
     b.z_offset = s * c->fb_split_spin;
-    b.scale = (c->d.rhf && c->d.nbeta > 0) ? 2.0 : 1.0;
+    b.scale = c->d.rhf ? 2.0 : 1.0;  // force_bias.py:101-104 doubles for rhf whatever ndown
     HIP_TRY(c, launch_bf16_gemm(b, c->stream));
   }
   c->g_planes_nw = 0;  // the exchange reuses the planes
@@ -483,7 +483,7 @@ int do_force_bias(afqmc_ctx* c, int nw, const double* ga, const double* gb) {
   if (two_spins(c)) {
     a.nseg = 2;
     a.seg[1] = NtSegment{gb, (int64_t)nb * N, c->s.rcholb, (int64_t)nb * N, nb * N};
-  } else if (c->d.rhf && nb > 0) {
+  } else if (c->d.rhf) {  // force_bias.py:101-104 doubles for rhf whatever ndown
     a.scale = 2.0;
   }
   a.vbias_part = c->vbias_part;

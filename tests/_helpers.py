@@ -14,6 +14,28 @@ def rel_err(a, b):
     return float(numpy.abs(a - b).max(initial=0.0) / scale)
 
 
+def rel_err_elem(a, b, floor=1e-12):
+    """max over entries of |a - b| / max(|b|, floor * max |b|): the relative
+    error of every walker's value on its own scale (overlaps and weights span
+    orders of magnitude), with an absolute floor so that entries the reference
+    makes exactly zero (cosine-clipped weights) compare against a small
+    multiple of the largest entry instead of zero."""
+    a = numpy.asarray(a)
+    b = numpy.asarray(b)
+    big = max(numpy.abs(b).max(initial=0.0), 1e-300)
+    den = numpy.maximum(numpy.abs(b), floor * big)
+    return float((numpy.abs(a - b) / den).max(initial=0.0))
+
+
+PER_WALKER = ("weight", "ovlp", "hybrid_energy")
+
+
+def walker_err(attr, a, b):
+    """Per-walker relative error for the per-walker scalars, max-norm relative
+    error for arrays of matrices (phi, Ghalf)."""
+    return rel_err_elem(a, b) if attr in PER_WALKER else rel_err(a, b)
+
+
 def synthetic_arrays(N, X, decay, nalpha, nbeta, seed=7, scale=1.0):
     from ipie_amd import synthetic
 
