@@ -3,7 +3,10 @@
  *
  * Plain C: device pointers, sizes and scalars only.  Complex arrays are
  * interleaved (re, im) float64 pairs, the memory layout of complex128; every
- * array is dense row-major with the shape given beside it.  A context borrows
+ * array is dense row-major (C-contiguous) with the shape given beside it: no
+ * entry point takes a leading dimension or stride, so a caller holding a
+ * strided view copies it to a dense buffer first (the Python binding,
+ * ipie_amd/_lib.py, refuses non-contiguous tensors).  A context borrows
  * the static tensors passed to afqmc_set_static for its whole lifetime and
  * borrows walker arrays only for the duration of a call.  All work is
  * enqueued on the context's stream; nothing blocks the host except
@@ -111,6 +114,17 @@ const char* afqmc_last_error(const afqmc_ctx* ctx);
 int afqmc_sync(afqmc_ctx* ctx);
 /* synchronises, returns the OR of device flags since the last clear */
 int afqmc_get_flags(afqmc_ctx* ctx, int32_t* flags, int clear);
+/* Per-walker status (the optional per-walker flag array of the error
+ * conventions; ipie raises numpy.linalg.LinAlgError on a singular overlap,
+ * greens_function_single_det.py:79-97, and lets NaN weights propagate):
+ * flags [nw] int32 device array receives, for walkers 0..nw-1, the OR of the
+ * AFQMC_FLAG_* conditions each walker met since the last take (singular
+ * overlap matrix in the Green's function / overlap, non-finite weight in the
+ * weight update, rank-deficient walker in reortho); clear = 1 resets them.
+ * flags may be NULL with clear = 1.  Enqueued on the context's stream with a
+ * kernel (no memset, so it can be recorded into a step graph); no
+ * synchronisation. */
+int afqmc_walker_flags(afqmc_ctx* ctx, int32_t nwalkers, int32_t* flags, int clear);
 /* total force-bias caps accumulated on the device (synchronises) */
 int afqmc_get_nfb(afqmc_ctx* ctx, uint64_t* nfb);
 /* How this context evaluates the exchange sum_x tr(T_x T_x) (no counterpart in

@@ -74,6 +74,7 @@ SIGNATURES = {
     "afqmc_sync": (ctypes.c_int, [_dp]),
     "afqmc_get_flags": (ctypes.c_int, [_dp, ctypes.POINTER(_i32), ctypes.c_int]),
     "afqmc_get_nfb": (ctypes.c_int, [_dp, ctypes.POINTER(_u64)]),
+    "afqmc_walker_flags": (ctypes.c_int, [_dp, _i32, _dp, ctypes.c_int]),
     "afqmc_greens_function": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _dp, _dp, _dp]),
     "afqmc_overlap": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp, _dp]),
     "afqmc_force_bias": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp]),
@@ -438,6 +439,16 @@ class Context:
         if st != AFQMC_OK:
             raise AfqmcError(f"afqmc_get_flags: {STATUS_NAMES.get(st, st)}")
         return int(f.value)
+
+    def walker_flags(self, nwalkers, out=None, clear=True):
+        """Per-walker flags (int32 [nwalkers], OR of FLAG_* since the last take):
+        a singular overlap matrix, a non-finite weight, a rank-deficient walker
+        in reortho.  Enqueued on the current stream; no synchronisation."""
+        if out is None:
+            out = torch.empty(nwalkers, dtype=torch.int32, device=self.device)
+        _check_tensor("out", out, torch.int32, (nwalkers,), self.device)
+        self._call("afqmc_walker_flags", int(nwalkers), ptr(out), int(bool(clear)))
+        return out
 
     def nfb(self):
         v = _u64(0)

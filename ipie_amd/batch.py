@@ -60,6 +60,7 @@ class MultiSystemBatch:
             s.wait_stream(cur)
         for c, s in zip(self.ctx, self.streams):
             c.bind_stream(s)
+        self.steps = None  # a failed capture leaves nothing to replay
         self._check(self.lib.afqmc_batch_begin(self.h), "afqmc_batch_begin")
         try:
             for j in range(nsteps):
@@ -74,6 +75,8 @@ class MultiSystemBatch:
                         if reortho_every and (j + 1) % reortho_every == 0:
                             walkers.reortho()
         except BaseException:
+            # close the capture (joins every forked stream) before the error propagates,
+            # so no stream is left recording into a graph that will never be launched
             self.lib.afqmc_batch_end(self.h)
             raise
         self._check(self.lib.afqmc_batch_end(self.h), "afqmc_batch_end")

@@ -43,6 +43,7 @@ struct afqmc_ctx {
   double* buf[4] = {nullptr, nullptr, nullptr, nullptr};  // P, Ta, Tb, Q: [W, N, ncol] complex
   int* tri = nullptr;            // [2, P] packed-index rows / cols
   int* flags = nullptr;
+  int* wflags = nullptr;    // [W] per-walker flags (OR of AFQMC_FLAG_*) since the last take
   int* psi_flag = nullptr;  // scratch of afqmc_set_static's check of the trial orbitals
   int g_planes_nw = 0;      // > 0: gplanes hold Ghalf of these nw walkers, written by the GF of this step
   bool psi_real = false;    // the trial orbitals are real (imaginary parts exactly zero)
@@ -164,6 +165,25 @@ hipEvent_t take_event(afqmc_ctx* c) {
   return e;
 }
 
+// Adds the phases whose stop event has completed, oldest first, to the totals
+// and returns their events to the pool (no wait): keeps the pending list short
+// when profiling stays on and the totals are read rarely.
+void drain_completed(afqmc_ctx* c) {
+  size_t k = 0;
+  for (; k < c->ev_pending.size(); ++k) {
+    const PendingEvent& pe = c->ev_pending[k];
+    if (hipEventQuery(pe.stop) != hipSuccess) break;
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, pe.start, pe.stop) == hipSuccess) {
+      c->prof_ms[pe.phase] += t;
+      c->prof_count[pe.phase] += 1;
+    }
+    c->ev_free.push_back(pe.start);
+    c->ev_free.push_back(pe.stop);
+  }
+  c->ev_pending.erase(c->ev_pending.begin(), c->ev_pending.begin() + (std::ptrdiff_t)k);
+}
+
 // Records a start/stop event pair around one phase when profiling is on.
 struct PhaseScope {
   afqmc_ctx* c;
@@ -171,6 +191,7 @@ struct PhaseScope {
   hipEvent_t start = nullptr;
   PhaseScope(afqmc_ctx* ctx, int ph) : c(ctx), phase(ph) {
     if (c->prof && !c->capturing) {
+      if (c->ev_pending.size() >= 1024) drain_completed(c);
       start = take_event(c);
       if (start) (void)hipEventRecord(start, c->stream);
     }
@@ -415,6 +436,7 @@ int do_gf(afqmc_ctx* c, int nw, const double* phia, const double* phib, double* 
   a.ovlp = ovlp;
   a.log_shift = log_shift;
   a.flags = c->flags;
+  a.wflags = c->wflags;
   a.psi_real = c->psi_real ? 1 : 0;
   // the split-BF16 force bias that follows in the same step reads Ghalf's planes
   // from this kernel instead of splitting Ghalf again
@@ -1201,12 +1223,14 @@ int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims) {
     if (c->bf16_fb) c->fb_ksplit = c->fb_split_spin * (two ? 2 : 1);
   }
   if (dmalloc((void**)&c->flags, sizeof(int)) != hipSuccess ||
+      dmalloc((void**)&c->wflags, sizeof(int) * (size_t)d.nwalkers) != hipSuccess ||
       dmalloc((void**)&c->nfb, sizeof(unsigned long long)) != hipSuccess ||
       dmalloc((void**)&c->dstate, sizeof(StepState)) != hipSuccess) {
     afqmc_destroy(c);
     return AFQMC_ERR_ALLOC;
   }
   if (hipMemset(c->flags, 0, sizeof(int)) != hipSuccess ||
+      hipMemset(c->wflags, 0, sizeof(int) * (size_t)d.nwalkers) != hipSuccess ||
       hipMemset(c->nfb, 0, sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(c->dstate, 0, sizeof(StepState)) != hipSuccess) {
     afqmc_destroy(c);
@@ -1222,7 +1246,7 @@ void afqmc_destroy(afqmc_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   else (void)hipDeviceSynchronize();
   void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
-                  c->buf[2], c->buf[3], c->tri, c->flags, c->psi_flag, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
+                  c->buf[2], c->buf[3], c->tri, c->flags, c->wflags, c->psi_flag, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
                   c->lplanes[1], c->xplanes, c->cplanes, c->vplanes, c->tplanes[0], c->dplanes, c->oplanes,
                   c->tplanes[1], c->gcomp, c->gf_o, c->gf_oinv, c->qw64[0], c->qw64[1], c->qwp[0],
                   c->qwp[1], c->qy, c->qunits[0], c->qunits[1], c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
@@ -1294,6 +1318,15 @@ int afqmc_get_flags(afqmc_ctx* c, int32_t* flags, int clear) {
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   *flags = h;
   if (clear) HIP_TRY(c, hipMemsetAsync(c->flags, 0, sizeof(int), c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_walker_flags(afqmc_ctx* c, int32_t nw, int32_t* flags, int clear) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (nw < 0 || nw > c->d.nwalkers)
+    return fail(c, AFQMC_ERR_SHAPE, "nwalkers=%d outside [0, %d]", nw, c->d.nwalkers);
+  if (!flags && !clear) return fail(c, AFQMC_ERR_INVALID_ARG, "null flags array and nothing to clear");
+  HIP_TRY(c, launch_take_flags(c->wflags, flags, nw, clear ? 1 : 0, c->stream));
   return AFQMC_OK;
 }
 
@@ -1692,6 +1725,7 @@ int afqmc_update_weight(afqmc_ctx* c, int32_t nw, const double* ovlp, const doub
   a.hybrid_energy = hybrid_energy;
   a.walker_ovlp = walker_ovlp;
   a.flags = c->flags;
+  a.wflags = c->wflags;
   PhaseScope ps(c, AFQMC_PHASE_WEIGHT);
   HIP_TRY(c, launch_update_weight(a, c->stream));
   return AFQMC_OK;
@@ -1760,6 +1794,7 @@ int afqmc_reortho(afqmc_ctx* c, int32_t nw, double* phia, double* phib, double* 
   a.log_detR = log_detR;
   a.detR_shift = detR_shift;
   a.flags = c->flags;
+  a.wflags = c->wflags;
   PhaseScope ps(c, AFQMC_PHASE_REORTHO);
   HIP_TRY(c, launch_reortho(a, c->stream));
   return AFQMC_OK;
@@ -1784,6 +1819,7 @@ int afqmc_graph_end(afqmc_ctx* c, int32_t* steps) {
   if (!c->capturing) return fail(c, AFQMC_ERR_INVALID_ARG, "no step-graph capture is open");
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess) g = nullptr;  // a failed end hands back no graph of ours to destroy
   c->capturing = false;
   t_capturing = false;
   // the recorded control-variate energies did not run
@@ -1898,13 +1934,19 @@ int afqmc_batch_end(afqmc_batch* b) {
   afqmc_ctx* c0 = b->ctx[0];
   if (!b->capturing) return fail(c0, AFQMC_ERR_INVALID_ARG, "no capture is open");
   const int n = (int)b->ctx.size();
+  // Join every forked stream, whatever failed before: a stream left unjoined
+  // makes hipStreamEndCapture fail with hipErrorStreamCaptureUnjoined and stays
+  // bound to the ended capture, so the next call on it would act on capture
+  // state that no longer exists.
   hipError_t e = hipSuccess;
-  for (int i = 1; i < n && e == hipSuccess; ++i) {
-    e = hipEventRecord(b->ev[i], b->ctx[i]->stream);
-    if (e == hipSuccess) e = hipStreamWaitEvent(c0->stream, b->ev[i], 0);
+  for (int i = 1; i < n; ++i) {
+    hipError_t ej = hipEventRecord(b->ev[i], b->ctx[i]->stream);
+    if (ej == hipSuccess) ej = hipStreamWaitEvent(c0->stream, b->ev[i], 0);
+    if (e == hipSuccess) e = ej;
   }
   hipGraph_t g = nullptr;
   const hipError_t e2 = hipStreamEndCapture(c0->stream, &g);
+  if (e2 != hipSuccess) g = nullptr;  // a failed end hands back no graph of ours to destroy
   if (e == hipSuccess) e = e2;
   b->capturing = false;
   t_capturing = false;

@@ -54,6 +54,7 @@ struct GfArgs {
   double* ovlp;         // [W] complex
   const double* log_shift;  // [W] or null
   int* flags;
+  int* wflags;   // [W] per-walker flags (OR of AFQMC_FLAG_*) or null
   int psi_real;  // the trial orbitals have zero imaginary parts (O takes two real MFMAs per K step, not four)
   // optional: Ghalf_s also as the 4 split-BF16 planes [W n_s, kpad] of spin s
   // (re_hi, re_lo, im_hi, im_lo; K padding zero), as split_ghalf writes them
@@ -279,8 +280,11 @@ struct WeightArgs {
   double* hybrid_energy;   // [W] complex (in: previous, out: new)
   double* walker_ovlp;     // [W] complex (out: ovlp_new)
   int* flags;
+  int* wflags;             // [W] per-walker flags or null
 };
 hipError_t launch_update_weight(const WeightArgs& a, hipStream_t stream);
+// dst[0..n) <- src[0..n) (dst may be null), then src <- 0 when clear
+hipError_t launch_take_flags(int* src, int* dst, int n, int clear, hipStream_t stream);
 
 struct EnergyArgs {
   int nwalkers, nbasis, nalpha, nbeta, rhf;
@@ -424,6 +428,7 @@ struct ReorthoArgs {
   double* log_detR;      // [W] real, += log detR
   const double* detR_shift;  // [W] or null
   int* flags;
+  int* wflags;               // [W] per-walker flags or null
 };
 hipError_t launch_reortho(const ReorthoArgs& a, hipStream_t stream);
 

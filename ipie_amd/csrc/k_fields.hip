@@ -138,6 +138,7 @@ __global__ void update_weight_kernel(WeightArgs a) {
   st(a.hybrid_energy, w, eh);
   st(a.walker_ovlp, w, o_new);
   if (!isfinite(wnew) && a.flags) atomicOr(a.flags, AFQMC_FLAG_NONFINITE);
+  if (!isfinite(wnew) && a.wflags) atomicOr(a.wflags + w, AFQMC_FLAG_NONFINITE);
 }
 
 __global__ void __launch_bounds__(FLD_THREADS) energy_kernel(EnergyArgs a) {
@@ -226,6 +227,22 @@ hipError_t launch_step_set(StepState* state, unsigned long long counter, double 
 
 hipError_t launch_step_advance(StepState* state, long long dcounter, long long dadam, hipStream_t stream) {
   hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(64), 0, stream, state, dcounter, dadam);
+  return hipGetLastError();
+}
+
+// dst[i] = src[i] for i < n, then src[i] = 0 when clear (a kernel, not a
+// memset, so that a captured graph replays it correctly).
+__global__ void take_flags_kernel(int* src, int* dst, int n, int clear) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int v = src[i];
+  if (dst) dst[i] = v;
+  if (clear) src[i] = 0;
+}
+
+hipError_t launch_take_flags(int* src, int* dst, int n, int clear, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(take_flags_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, src, dst, n, clear);
   return hipGetLastError();
 }
 

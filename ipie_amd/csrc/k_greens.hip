@@ -299,6 +299,7 @@ __global__ void __launch_bounds__(256) gf_combine_kernel(GfArgs a, const double*
   const double e = exp(lg - lsh);
   st(a.ovlp, w, cmake(ph.re * e, ph.im * e));
   if (singular && a.flags) atomicOr(a.flags, AFQMC_FLAG_SINGULAR);
+  if (singular && a.wflags) atomicOr(a.wflags + w, AFQMC_FLAG_SINGULAR);
 }
 
 // Ghalf_s[w][i][m] = sum_j Oinv[w][s][i][j] phi_s[w][m][j]; wave = (w, s, row tile it).

@@ -304,6 +304,7 @@ __global__ void __launch_bounds__(QR_THREADS) reortho_kernel(ReorthoArgs a) {
   if (threadIdx.x == 0) {
     if (bad) {
       if (a.flags) atomicOr(a.flags, AFQMC_FLAG_CHOLESKY);
+      if (a.wflags) atomicOr(a.wflags + w, AFQMC_FLAG_CHOLESKY);
       a.detR[w] = NAN;
       return;
     }

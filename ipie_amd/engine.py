@@ -25,6 +25,7 @@ class Engine:
         self.nchol = nchol
         self.exp_nmax = exp_nmax
         self.device = walkers.device
+        self.nwalkers = walkers.nwalkers
         self.rhf = bool(walkers.rhf)
         self.ctx = _lib.Context(
             walkers.nbasis, nchol, walkers.nup, walkers.ndown, walkers.nwalkers, exp_nmax=exp_nmax, rhf=walkers.rhf,
@@ -82,7 +83,9 @@ class Engine:
             return 0
         f = self.ctx.flags(clear=True)
         if f & _lib.FLAG_SINGULAR:
-            raise SingularOverlapError("Singular matrix: a walker has zero overlap with the trial")
+            wf = self.ctx.walker_flags(self.nwalkers, clear=False).cpu()
+            bad = torch.nonzero(wf & _lib.FLAG_SINGULAR).flatten().tolist()
+            raise SingularOverlapError(f"Singular matrix: walkers {bad[:16]} have zero overlap with the trial")
         return f
 
 

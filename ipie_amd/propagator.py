@@ -28,8 +28,6 @@ without it the fields come from the device generator (Philox4x32-10, key
 ``seed``, one counter value per call).
 """
 
-import time
-
 import numpy
 import scipy.linalg
 import torch
@@ -39,15 +37,58 @@ from ipie_amd import engine as _engine
 
 
 class PropagatorTimer:
-    """Restates ipie/propagation/continuous_base.py:4-11."""
+    """The six fields of ipie/propagation/continuous_base.py:4-11, in seconds.
+
+    ipie adds host wall time around each phase, with a device synchronisation
+    after it (phaseless_base.py:274-338, phaseless_generic.py:39-58):
+    tgf the Green's function, tgemm the two one-body halves and the Taylor
+    series, tfbias the force bias and xbar, tvhs the VHS construction, tovlp the
+    new overlap, tupdate the weight update.  Here the step is one fused C-ABI
+    call, so the fields are read from the context's HIP-event time of the same
+    phases (afqmc_profile_read; reading one synchronises the context's
+    stream, as ipie's own timing does after every phase).  A field may be
+    assigned (e.g. reset to 0); later reads add the device time since."""
+
+    PHASES = {
+        "tgf": ("gf",),
+        "tgemm": ("one_body", "taylor"),
+        "tfbias": ("force_bias", "fields"),
+        "tvhs": ("vhs",),
+        "tovlp": ("overlap",),
+        "tupdate": ("weight",),
+    }
 
     def __init__(self):
-        self.tfbias = 0.0
-        self.tovlp = 0.0
-        self.tupdate = 0.0
-        self.tgf = 0.0
-        self.tvhs = 0.0
-        self.tgemm = 0.0
+        self._ctx = None
+        self._host = dict.fromkeys(self.PHASES, 0.0)
+        self._zero = dict.fromkeys(self.PHASES, 0.0)
+
+    def bind(self, ctx):
+        """Time the phases of this device context (turns its profiling on)."""
+        ctx.profile(True)
+        self._ctx = ctx
+        self._zero = dict.fromkeys(self.PHASES, 0.0)
+
+    def _device(self, name):
+        if self._ctx is None:
+            return 0.0
+        prof = self._ctx.profile_read()
+        return 1e-3 * sum(prof[p][0] for p in self.PHASES[name])
+
+
+def _timer_field(name):
+    def get(self):
+        return self._host[name] + self._device(name) - self._zero[name]
+
+    def set_(self, value):
+        self._host[name] = float(value)
+        self._zero[name] = self._device(name)
+
+    return property(get, set_)
+
+
+for _name in PropagatorTimer.PHASES:
+    setattr(PropagatorTimer, _name, _timer_field(_name))
 
 
 def construct_mean_field_shift(hamiltonian, trial):
@@ -85,7 +126,6 @@ class PhaselessGeneric:
         self.xi_source = None
         self.seed = int(seed)
         self.counter = 0
-        self.timing = False
         self.last_xi = None
         self.keep_xi = False
         self._engine = None
@@ -120,6 +160,7 @@ class PhaselessGeneric:
             eng.ctx.surrogate_configure(True, **opts)
         walkers._engine = eng
         self._engine = eng
+        self.timer.bind(eng.ctx)
         self.mpi_handler = mpi_handler
 
     def _setup_on_device(self, ham, trial, walkers):
@@ -207,17 +248,11 @@ class PhaselessGeneric:
         """
         eng = self._eng(walkers)
         xi = self._next_xi(walkers, hamiltonian)
-        if self.timing:
-            torch.cuda.synchronize(walkers.device)
-            t0 = time.time()
         eng.ctx.propagate(
             walkers.phia, walkers.phib, walkers.Ghalfa, walkers.Ghalfb, walkers.weight, walkers.ovlp,
             walkers.hybrid_energy, float(numpy.real(eshift)), xi=xi, seed=self.seed, counter=self.counter,
             log_shift=walkers.log_shift, energy_prev=energy_prev,
         )
-        if self.timing:
-            torch.cuda.synchronize(walkers.device)
-            self.timer.tgemm += time.time() - t0
 
     # ------------------------------------------------------------ step graphs
     def capture_steps(self, walkers, hamiltonian, trial, nsteps, reortho_every=0, energy_prev=None):
@@ -270,11 +305,7 @@ class PhaselessGeneric:
 
     def propagate_walkers_one_body(self, walkers):
         """phi_s <- expH1[s] phi_s (phaseless_base.py:274-280)."""
-        t0 = time.time()
         self._eng(walkers).ctx.one_body(walkers.phia, walkers.phib)
-        if self.timing:
-            torch.cuda.synchronize(walkers.device)
-            self.timer.tgemm += time.time() - t0
 
     def propagate_walkers_two_body(self, walkers, hamiltonian, trial):
         """Force bias, fields and exp(VHS) from walkers.Ghalf (phaseless_base.py:282-310); returns (cmf, cfb)."""
@@ -287,14 +318,10 @@ class PhaselessGeneric:
         xi_out = None
         if self.keep_xi and xi is None:
             xi_out = torch.empty(nw, hamiltonian.nchol, dtype=torch.float64, device=walkers.device)
-        t0 = time.time()
         eng.ctx.two_body(walkers.phia, walkers.phib, walkers.Ghalfa, walkers.Ghalfb, cmf, cfb, xi=xi, seed=self.seed,
                          counter=self.counter, xi_out=xi_out, vbias_out=self.vbias)
         if xi_out is not None:
             self.last_xi = xi_out
-        if self.timing:
-            torch.cuda.synchronize(walkers.device)
-            self.timer.tvhs += time.time() - t0
         return cmf, cfb
 
     def update_weight(self, walkers, ovlp, ovlp_new, cfb, cmf, eshift):

@@ -51,6 +51,15 @@ class UHFWalkers:
         self.rhf = False
         self._engine = None
 
+    def status_flags(self, clear=True):
+        """Per-walker status, int32 [W] on the device: the OR of
+        ipie_amd._lib.FLAG_SINGULAR (zero overlap with the trial), FLAG_NONFINITE
+        (non-finite weight) and FLAG_CHOLESKY (rank-deficient in reortho) each
+        walker met since the last call (afqmc_walker_flags; no synchronisation)."""
+        if self._engine is None:
+            return torch.zeros(self.nwalkers, dtype=torch.int32, device=self.device)
+        return self._engine.ctx.walker_flags(self.nwalkers, clear=clear)
+
     def build(self, trial):
         """Restates uhf_walkers.py:101-102."""
         self.ovlp = trial.calc_greens_function(self)
