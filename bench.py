@@ -28,6 +28,13 @@ PEAK = {"f64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
 REORTHO_EVERY = 5
 SURROGATE_WARMUP = 100
 BIAS_BUDGET_HA = 1e-4
+# modes run by default per workload: the headline compares the FP64 step with the
+# mixed-precision modes; the surrogate workload compares the control variate
+# (with and without its MLP term) with the all-exact mixed-precision estimator
+DEFAULT_MODES = {
+    "n400": ("fp64", "mixed", "mixed+surrogate"),
+    "n300": ("mixed", "mixed+surrogate", "mixed+surrogate-nomlp"),
+}
 
 
 def log(*a):
@@ -95,7 +102,11 @@ def build_device(host, w, mode, tuner=None):
     from ipie_amd.walkers import UHFWalkers
 
     precision = "fp64" if mode == "fp64" else "mixed"
-    surrogate = dict(fraction=0.125, warmup=SURROGATE_WARMUP, train_iters=4, lr=1e-3) if "surrogate" in mode else None
+    surrogate = None
+    if "surrogate" in mode:
+        # "-nomlp": the physical control variate alone (train_iters = 0 turns the MLP term off)
+        surrogate = dict(fraction=0.125, warmup=SURROGATE_WARMUP, train_iters=0 if mode.endswith("-nomlp") else 4,
+                         lr=1e-3)
     t0 = time.time()
     walkers = UHFWalkers(host["psi"], w.nalpha, w.nbeta, w.nbasis, w.nwalkers)
     walkers.build(host["trial"])
@@ -158,7 +169,7 @@ def make_tuner(args):
     return Autotuner(cache_path=None)
 
 
-def run_multi(args):
+def run_multi(args, emit=True):
     """The uneven multi-system batch: 16 independent systems (N 64-256, X = 5 N,
     N / 10 occupied per spin, 64-512 walkers, mixed precision, local energy
     every step, QR every 5 steps), stepped (a) eagerly one system after
@@ -252,7 +263,50 @@ def run_multi(args):
         "finite": finite,
         "launch_shapes": [st["prop"].tuning for st in systems] if tuner is not None else "heuristic",
     }
-    print(json.dumps(out), flush=True)
+    if tuner is not None:
+        out["autotune_buckets"] = len(tuner.reports)
+    del systems, batch
+    torch.cuda.empty_cache()
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
+
+
+def run_surrogate_workload(args, tuner):
+    """The surrogate workload (N=300, X=1800, n=30, W=2048): the all-exact
+    mixed-precision estimator, the control variate with its MLP term and the
+    control variate alone, on one trajectory, over one block of at least 100
+    steps after the 100 training steps.  Returns one record: rates, and for
+    each control-variate mode its bias against the all-exact run, the variance
+    it adds to the per-step energy, and the share of that variance the MLP
+    removes."""
+    import torch
+
+    from ipie_amd import synthetic
+
+    w = synthetic.WORKLOADS["n300"]
+    host = build_host(w, math.sqrt(12.0 / (w.nbasis * w.decay)))
+    K = max(args.steps, 100)
+    modes = run_modes(args, w, host, list(DEFAULT_MODES["n300"]), K, SURROGATE_WARMUP + 5, tuner)
+    compare_modes(modes, "mixed")
+    keep = ("value", "ms_per_step", "timed", "energy_bias_Ha", "energy_bias_sem_Ha", "added_variance_per_step_Ha2",
+            "reference_step_variance_Ha2", "within_0p1_mHa", "surrogate", "roofline_kernel")
+    rec = {
+        "workload": f"{w.name}: N={w.nbasis} X={w.nchol} decay={w.decay} nalpha={w.nalpha} nbeta={w.nbeta} "
+                    f"W={w.nwalkers} dt={w.dt}, normalised Cholesky scale",
+        "steps": K,
+        "warmup": SURROGATE_WARMUP + 5,
+        "modes": {m: {k: r[k] for k in keep if k in r} for m, r in modes.items()},
+    }
+    a = modes.get("mixed+surrogate", {}).get("added_variance_per_step_Ha2")
+    b = modes.get("mixed+surrogate-nomlp", {}).get("added_variance_per_step_Ha2")
+    if a is not None and b:
+        rec["mlp_variance_reduction"] = 1.0 - a / b
+    if "mixed" in modes:
+        rec["surrogate_speedup_vs_exact"] = modes.get("mixed+surrogate", {}).get("value", 0.0) / modes["mixed"]["value"]
+    del host
+    torch.cuda.empty_cache()
+    return rec
 
 
 def run_graph(st, w, steps, one_step, counter, energy, eshift):
@@ -356,7 +410,21 @@ def run_gpu(st, w, steps, warmup, graph=True):
 
 
 def cpu_baseline(st, w, sample_walkers, sample_steps):
-    """Time the oracle (numpy restatement of ipie's CPU path) on a bounded sample."""
+    """Time the oracle (numpy restatement of ipie's CPU path) on a bounded sample,
+    with every host thread the BLAS has and again with one thread."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except Exception:
+        threadpool_limits = None
+    rec = _cpu_baseline_once(st, w, sample_walkers, sample_steps)
+    if threadpool_limits is not None:
+        with threadpool_limits(limits=1):
+            one = _cpu_baseline_once(st, w, sample_walkers, sample_steps)
+        rec["single_core"] = {k: one[k] for k in ("value", "unit", "cores", "sample")}
+    return rec
+
+
+def _cpu_baseline_once(st, w, sample_walkers, sample_steps):
     from oracle import afqmc
 
     try:
@@ -412,79 +480,41 @@ def traffic_per_launch(mode, w):
     return None
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="n400")
-    ap.add_argument("--nwalkers", type=int, default=None)
-    ap.add_argument("--cholesky-scale", default="normalised", choices=["normalised", "as-specified"])
-    ap.add_argument("--modes", default="fp64,mixed,mixed+surrogate",
-                    help="comma list of fp64, mixed, mixed+surrogate; the headline value is the fastest mode "
-                    "whose block energy is within 0.1 mHa of the FP64 mode on the same trajectory")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="time eager steps only (default: the value is the step-graph replay, the per-phase "
-                         "profile and the roofline kernel time come from the eager pass before it)")
-    ap.add_argument("--setup-timing", action="store_true",
-                    help="only time the set-up of the workload on the host and on the device, then exit")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the heuristic launch shapes (default: tune them per context at build time)")
-    ap.add_argument("--cpu-walkers", type=int, default=4)
-    ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--calibration-copy-mb", type=int, default=0,
-                    help="only copy a device buffer of this many MiB once and exit (the reference for the "
-                         "FETCH_SIZE / WRITE_SIZE counters behind roofline.traffic)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs one GPU only (asked for --gpus {args.gpus}); sharding walkers over several "
-              "GPUs is a design that is not built here.", file=sys.stderr)
-        sys.exit(2)
+def compare_modes(modes, ref_mode):
+    """Per-step weighted energies of every mode against ref_mode on the same
+    trajectory: bias (mean difference) with its standard error, the variance
+    the mode adds to the per-step estimate, and whether the bias is within
+    0.1 mHa."""
+    if ref_mode not in modes:
+        return
+    ref = modes[ref_mode]["_e_steps"]
+    ref_var = float(numpy.var(ref.real))
+    for mode, rec in modes.items():
+        if mode == ref_mode:
+            continue
+        d = (rec["_e_steps"] - ref).real
+        rec["bias_reference_mode"] = ref_mode
+        rec["energy_bias_Ha"] = float(numpy.mean(d))
+        rec["energy_bias_sem_Ha"] = float(numpy.std(d) / numpy.sqrt(len(d)))
+        rec["energy_maxdiff_per_step_Ha"] = float(numpy.abs(d).max())
+        rec["added_variance_per_step_Ha2"] = float(numpy.var(d))
+        rec["reference_step_variance_Ha2"] = ref_var
+        rec["within_0p1_mHa"] = bool(abs(rec["energy_bias_Ha"]) <= BIAS_BUDGET_HA)
+        if ref_mode == "fp64":
+            rec["energy_bias_vs_fp64_Ha"] = rec["energy_bias_Ha"]
 
-    import dataclasses
 
+def run_modes(args, w, host, mode_list, K, warmup, tuner):
+    """Build and step every mode on one workload (same trajectory: the fields come
+    from one counter-based stream and every mode runs the same steps before its
+    timed block); per-mode records, each with its per-step energies (_e_steps)."""
     import torch
 
-    from ipie_amd import synthetic
-
-    if args.calibration_copy_mb > 0:
-        n = args.calibration_copy_mb * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"calibration_copy_bytes": 8 * n}), flush=True)
-        return
-
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    if args.workload == "multi16":
-        run_multi(args)
-        return
-    w = synthetic.WORKLOADS[args.workload]
-    if args.nwalkers:
-        w = dataclasses.replace(w, nwalkers=args.nwalkers)
-    scale = 1.0 if args.cholesky_scale == "as-specified" else math.sqrt(12.0 / (w.nbasis * w.decay))
-    if args.setup_timing:
-        run_setup_timing(w, scale)
-        return
-    host = build_host(w, scale)
-    K = args.steps
-    mode_list = [m for m in args.modes.split(",") if m]
-    # every mode runs the same number of steps before its timed block, so all
-    # modes time the same stretch of one trajectory (the fields come from the
-    # same counter-based stream; the mixed modes' split-BF16 force bias and VHS
-    # move the walkers by ~1e-6 relative per step, so the trajectories stay
-    # close and the per-step energy difference measures the precision bias)
-    warmup = args.warmup
-    if any("surrogate" in m for m in mode_list):
-        warmup = max(warmup, SURROGATE_WARMUP + 5)
     modes = {}
-    tuner = make_tuner(args)
     for mode in mode_list:
         st = build_device(host, w, mode, tuner)
         elapsed, prof, e_steps, finite, g = run_gpu(st, w, K, warmup, graph=not args.no_graph)
+        log(f"[bench] {w.name} {mode}: done")
         ex_ms, ex_n = prof["exchange"]
         form = st["walkers"]._engine.ctx.exchange_form()
         if form == "quad" and mode != "fp64":
@@ -562,22 +592,92 @@ def main():
         modes[mode] = rec
         del st
         torch.cuda.empty_cache()
-    if "fp64" in modes:
-        ref = modes["fp64"]["_e_steps"]
-        for mode, rec in modes.items():
-            if mode == "fp64":
-                continue
-            d = (rec["_e_steps"] - ref).real
-            rec["energy_bias_vs_fp64_Ha"] = float(numpy.mean(d))
-            rec["energy_bias_sem_Ha"] = float(numpy.std(d) / numpy.sqrt(len(d)))
-            rec["energy_maxdiff_per_step_Ha"] = float(numpy.abs(d).max())
-            rec["within_0p1_mHa"] = bool(abs(rec["energy_bias_vs_fp64_Ha"]) <= BIAS_BUDGET_HA)
+    return modes
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="n400")
+    ap.add_argument("--nwalkers", type=int, default=None)
+    ap.add_argument("--cholesky-scale", default="normalised", choices=["normalised", "as-specified"])
+    ap.add_argument("--modes", default="",
+                    help="comma list of fp64, mixed, mixed+surrogate, mixed+surrogate-nomlp (default: per workload, "
+                    "n400 fp64,mixed,mixed+surrogate; n300 mixed,mixed+surrogate,mixed+surrogate-nomlp); the "
+                    "headline value is the fastest mode whose block energy is within 0.1 mHa of the FP64 mode "
+                    "(or, without it, the all-exact mixed mode) on the same trajectory")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eager steps only (default: the value is the step-graph replay, the per-phase "
+                         "profile and the roofline kernel time come from the eager pass before it)")
+    ap.add_argument("--setup-timing", action="store_true",
+                    help="only time the set-up of the workload on the host and on the device, then exit")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="with the n400 headline, skip the secondary records (the surrogate workload n300 at 2048 "
+                    "walkers and the 16-system batch multi16)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the heuristic launch shapes (default: tune them per context at build time)")
+    ap.add_argument("--cpu-walkers", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--calibration-copy-mb", type=int, default=0,
+                    help="only copy a device buffer of this many MiB once and exit (the reference for the "
+                         "FETCH_SIZE / WRITE_SIZE counters behind roofline.traffic)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs one GPU only (asked for --gpus {args.gpus}); sharding walkers over several "
+              "GPUs is a design that is not built here.", file=sys.stderr)
+        sys.exit(2)
+
+    import dataclasses
+
+    import torch
+
+    from ipie_amd import synthetic
+
+    if args.calibration_copy_mb > 0:
+        n = args.calibration_copy_mb * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": 8 * n}), flush=True)
+        return
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload == "multi16":
+        run_multi(args)
+        return
+    w = synthetic.WORKLOADS[args.workload]
+    if args.nwalkers:
+        w = dataclasses.replace(w, nwalkers=args.nwalkers)
+    scale = 1.0 if args.cholesky_scale == "as-specified" else math.sqrt(12.0 / (w.nbasis * w.decay))
+    if args.setup_timing:
+        run_setup_timing(w, scale)
+        return
+    host = build_host(w, scale)
+    K = args.steps
+    mode_list = [m for m in args.modes.split(",") if m] if args.modes else list(DEFAULT_MODES.get(w.name, DEFAULT_MODES["n400"]))
+    # every mode runs the same number of steps before its timed block, so all
+    # modes time the same stretch of one trajectory (the fields come from the
+    # same counter-based stream; the mixed modes' split-BF16 force bias and VHS
+    # move the walkers by ~1e-6 relative per step, so the trajectories stay
+    # close and the per-step energy difference measures the precision bias)
+    warmup = args.warmup
+    if any("surrogate" in m for m in mode_list):
+        warmup = max(warmup, SURROGATE_WARMUP + 5)
+    tuner = make_tuner(args)
+    modes = run_modes(args, w, host, mode_list, K, warmup, tuner)
+    compare_modes(modes, "fp64" if "fp64" in modes else "mixed")
     for rec in modes.values():
         rec.pop("_e_steps")
     # headline: the fastest mode whose block-energy bias vs FP64 is within 0.1 mHa
     # (without an FP64 run in the same invocation the bias is not measured and
     # the fastest mode is reported, flagged as such)
-    ok = [m for m, r in modes.items() if m == "fp64" or r.get("within_0p1_mHa")] if "fp64" in modes else list(modes)
+    ref_mode = "fp64" if "fp64" in modes else ("mixed" if "mixed" in modes else None)
+    ok = [m for m, r in modes.items() if m == ref_mode or r.get("within_0p1_mHa")] if ref_mode else list(modes)
     head = max(ok, key=lambda m: modes[m]["value"])
     m = modes[head]
     rk = m["roofline_kernel"]
@@ -585,9 +685,14 @@ def main():
     dtype = {"fp64": "f64",
              "mixed": "f64 (Green's function, weights, one-body, QR) + bf16x3 force bias, VHS, Taylor, exchange "
                       "(fp32 accumulate)",
-             "mixed+surrogate": "mixed precision as above; exchange estimated per walker as g + [sampled] 8 (exact - g) "
-                                "with g = E_T + 2<K,dG> (f64) + Q(dG) (bf16 x1) + gain * MLP (bf16 weights), exact "
-                                "bf16x3 exchange on a random 1/8 of walkers"}[head]
+             "mixed+surrogate": "f64 (Green's function, weights, one-body, QR) + bf16x3 force bias, VHS, Taylor (fp32 "
+                                "accumulate); exchange estimated per walker as g + [sampled] 8 (exact - g) with "
+                                "g = E_T + 2<K,dG> (f64) + Q(dG) (bf16 x1) + gain * MLP (bf16 weights), exact "
+                                "bf16x3 exchange on a random 1/8 of walkers",
+             "mixed+surrogate-nomlp": "f64 (Green's function, weights, one-body, QR) + bf16x3 force bias, VHS, Taylor "
+                                      "(fp32 accumulate); exchange estimated per walker as g + [sampled] 8 "
+                                      "(exact - g) with g = E_T + 2<K,dG> (f64) + Q(dG) (bf16 x1), exact bf16x3 "
+                                      "exchange on a random 1/8 of walkers"}[head]
     out = {
         "metric": METRIC,
         "value": m["value"],
@@ -595,6 +700,9 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": warmup,
+        "warmup_note": (f"--warmup {args.warmup} raised to {warmup}: the control variate trains on "
+                        f"{SURROGATE_WARMUP} all-exact steps before its timed block, and every mode runs the same "
+                        "steps so all time one trajectory") if warmup != args.warmup else None,
         "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
@@ -642,6 +750,10 @@ def main():
         log(f"[bench] cpu baseline {out['cpu_baseline']['value']:.3f} walker-steps/s")
     else:
         out["cpu_baseline"] = None
+    if w.name == "n400" and not args.no_secondary:
+        del host
+        out["secondary"] = {"n300_surrogate": run_surrogate_workload(args, tuner),
+                            "multi16": run_multi(args, emit=False)}
     print(json.dumps(out), flush=True)
 
 

@@ -323,7 +323,9 @@ int afqmc_tune_set(afqmc_ctx* ctx, int32_t knob, int32_t value);
  *   E_x = g + [w in S] W / k (E_x_exact - g),
  * which is unbiased per walker.  The first warmup_steps energy evaluations are
  * all exact and train the model (train_iters Adam steps each, learning rate
- * lr); afterwards it keeps learning from the sampled walkers.  enable = 0
+ * lr); afterwards it keeps learning from the sampled walkers.  train_iters = 0
+ * turns the MLP term off (gain held at 0, no training): the predictor is then
+ * the physical control variate E_T + 2 <K, G - G_T> + Q_bf16 alone.  enable = 0
  * restores the plain exact estimator.  afqmc_local_energy is always exact. */
 int afqmc_surrogate_configure(afqmc_ctx* ctx, int enable, double fraction, int warmup_steps, int train_iters,
                               double lr, uint64_t seed);

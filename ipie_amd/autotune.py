@@ -35,6 +35,9 @@ import time
 import torch
 
 KNOBS = ("fb_split", "quad_wgs", "exx_split")
+# a context running the learned control variate evaluates its exchange inside
+# afqmc_propagate; only the force-bias split is timed through a state-free call
+SURROGATE_KNOBS = ("fb_split",)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "ipie_amd", "autotune.json")
 
 _FB = (1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 24, 32)
@@ -42,13 +45,43 @@ _QUAD = (512, 768, 1024, 1536, 2048, 3072, 4096, 6144)
 _EXX = (1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 192, 256)
 
 
+def _ceil(a, b):
+    return -(-int(a) // int(b))
+
+
+def _pow2(v):
+    p = 1
+    while p < v:
+        p *= 2
+    return p
+
+
+def library_tag():
+    """Size and modification time of the loaded library: cached choices of an
+    older build (other kernels, other admissible ranges) are not reused."""
+    from ipie_amd import _lib
+
+    try:
+        st = os.stat(_lib.LIB_PATH)
+        return f"{st.st_size:x}-{int(st.st_mtime):x}"
+    except OSError:
+        return "none"
+
+
 def shape_key(ctx, surrogate=False):
-    """Cache key: the device name, every dimension the launch shapes depend on,
-    and whether the context runs the learned control variate."""
+    """Cache key of a shape bucket: the device, the library build, and the
+    classes of the dimensions the launch shapes depend on -- the basis padded
+    to 64 (the K chunks of every contraction), the occupied orbitals per spin
+    rounded up to 8, the Cholesky vectors in 256-column groups (the force-bias
+    and VHS column tiles), the 128-walker row blocks rounded up to a power of
+    two -- the precision, rhf and whether the context runs the learned control
+    variate.  Systems of one bucket share tuned values (each value is checked
+    against the context's own admissible range when applied)."""
     d = ctx.dims
     name = torch.cuda.get_device_name(ctx.device) if torch.cuda.is_available() else "cpu"
-    return (f"{name}|N={d.nbasis}|X={d.nchol}|na={d.nalpha}|nb={d.nbeta}|W={d.nwalkers}|rhf={d.rhf}"
-            f"|{ctx.precision}" + ("|surrogate" if surrogate else ""))
+    return (f"{name}|lib={library_tag()}|N64={_ceil(d.nbasis, 64)}|n8={_ceil(d.nalpha, 8)},{_ceil(d.nbeta, 8)}"
+            f"|X256={_ceil(d.nchol, 256)}|rb={_pow2(_ceil(d.nwalkers, 128))}|rhf={d.rhf}|{ctx.precision}"
+            + ("|surrogate" if surrogate else ""))
 
 
 def candidates(knob, value, lo, hi):
@@ -103,14 +136,29 @@ class Autotuner:
         os.replace(tmp, self.cache_path)
 
     def apply(self, ctx, surrogate=False):
-        """Set the cached choices for ctx's shape; returns them, or None if none are cached."""
+        """Set the cached choices of ctx's shape bucket; returns them, or None when
+        nothing usable is cached (no entry, or a cached value outside this
+        context's admissible range or for a knob the entry lacks: then the
+        caller tunes afresh)."""
         entry = self.cache.get(shape_key(ctx, surrogate))
         if entry is None:
             return None
-        for knob, v in entry.items():
-            if knob in self.knobs and ctx.tune_get(knob)[2] >= 1:
-                ctx.tune_set(knob, v)
-        return dict(entry)
+        plan = {}
+        for knob in self.knobs:
+            if surrogate and knob not in SURROGATE_KNOBS:
+                continue
+            value, lo, hi = ctx.tune_get(knob)
+            if hi < 1:
+                continue
+            if knob not in entry:
+                return None
+            v = int(entry[knob])
+            if not (lo <= v <= hi or (knob == "exx_split" and v == 0)):
+                return None
+            plan[knob] = v
+        for knob, v in plan.items():
+            ctx.tune_set(knob, v)
+        return dict(plan)
 
     # ------------------------------------------------------------------ timing
     def _time(self, ctx, fn):
@@ -143,7 +191,7 @@ class Autotuner:
         }
         chosen, report = {}, {}
         for knob in self.knobs:
-            if surrogate and knob != "fb_split":
+            if surrogate and knob not in SURROGATE_KNOBS:
                 continue
             value, lo, hi = ctx.tune_get(knob)
             cands = candidates(knob, value, lo, hi)

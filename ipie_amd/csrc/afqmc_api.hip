@@ -1127,9 +1127,10 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   cb.resid = S.resid;
   cb.cv = S.cv;
   HIP_TRY(c, launch_sur_combine(cb, c->stream));
-  const SurrogateGainArgs gain{k, S.walker_of, S.target, S.mlp, S.state, 0.8};
+  const SurrogateGainArgs gain{k, S.walker_of, S.target, S.mlp, S.state, 0.8, S.train_iters > 0 ? 1.0 : 0.0};
   HIP_TRY(c, launch_sur_gain(gain, c->stream));
-  const int iters = warm ? S.train_iters : S.online_iters;
+  // train_iters = 0: the physical control variate alone (no MLP term, no training)
+  const int iters = S.train_iters == 0 ? 0 : (warm ? S.train_iters : S.online_iters);
   if (warm) {
     SurrogateStatsArgs sa{k, S.feat, S.walker_of, S.target, S.state};
     HIP_TRY(c, launch_sur_stats(sa, c->stream));

@@ -398,6 +398,7 @@ struct SurrogateGainArgs {
   const double* mlp;       // [W] complex
   SurrogateState* state;
   double decay;
+  double gain_max;  // upper clamp of the gain (0: the MLP term is off)
 };
 hipError_t launch_cv_static(const double* L, const double* gT, int X, int n, int N, double* T0, double* K,
                             double* part, double* et, hipStream_t s);

@@ -511,7 +511,7 @@ __global__ void __launch_bounds__(256) sur_gain_kernel(SurrogateGainArgs a) {
     s->gain_rm = a.decay * s->gain_rm + rm;
     s->gain_mm = a.decay * s->gain_mm + mm;
     const double g = s->gain_mm > 0.0 ? s->gain_rm / s->gain_mm : 0.0;
-    s->gain = fmin(fmax(g, 0.0), 1.0);
+    s->gain = fmin(fmax(g, 0.0), a.gain_max);
   }
 }
 

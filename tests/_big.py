@@ -52,6 +52,34 @@ def spread_subset(W, k=8):
     return out[:k]
 
 
+def device_state(name, precision="mixed", surrogate=None, amp=0.01, seed=3):
+    """Device objects only (no oracle) for a full-size workload, the static
+    tensors built by the device set-up kernels: (system, ham, trial, walkers, prop)."""
+    import torch
+
+    from ipie_amd import Generic as DevSystem
+    from ipie_amd.hamiltonian import GenericRealChol
+    from ipie_amd.propagator import PhaselessGeneric
+    from ipie_amd.trial import SingleDet
+    from ipie_amd.walkers import UHFWalkers
+
+    N, X, D, n, W = SHAPES[name]
+    print(f"[case] {name} device only N={N} X={X} n={n} W={W} precision={precision} surrogate={surrogate}",
+          flush=True)
+    h1e, chol, ecore, psi = arrays(name)
+    ham = GenericRealChol(h1e, chol, ecore=ecore, host_setup=False)
+    trial = SingleDet(psi, (n, n), N)
+    walkers = UHFWalkers(psi, n, n, N, W)
+    prop = PhaselessGeneric(0.005, precision=precision, surrogate=surrogate)
+    prop.build(ham, trial, walkers, device_setup=True)
+    trial.calculate_energy(DevSystem((n, n)), ham)
+    init = random_walkers(N, n, n, W, seed=seed, amp=amp, base=psi)
+    walkers.phia.copy_(torch.from_numpy(numpy.ascontiguousarray(init[:, :, :n])))
+    walkers.phib.copy_(torch.from_numpy(numpy.ascontiguousarray(init[:, :, n:])))
+    walkers.ovlp.copy_(trial.calc_greens_function(walkers))
+    return DevSystem((n, n)), ham, trial, walkers, prop
+
+
 class SubsetPair:
     """Device objects over all W walkers, oracle objects over a subset."""
 

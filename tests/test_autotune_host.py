@@ -78,7 +78,7 @@ def test_tune_sets_the_fastest_and_caches_by_shape(tmp_path, monkeypatch):
     assert chosen == {"fb_split": 3, "quad_wgs": 3072}
     assert ctx.v["fb_split"] == 3 and ctx.v["quad_wgs"] == 3072
     key = autotune.shape_key(ctx)
-    assert "N=40" in key and "X=120" in key and "W=32" in key and key.endswith("mixed")
+    assert "N64=1" in key and "X256=1" in key and "rb=1" in key and "lib=" in key and key.endswith("mixed")
     assert autotune.shape_key(ctx, True) == key + "|surrogate"
     assert json.loads(path.read_text())[key] == chosen
     rep = t.reports[key]["fb_split"]
@@ -89,8 +89,48 @@ def test_tune_sets_the_fastest_and_caches_by_shape(tmp_path, monkeypatch):
     assert t2.apply(fresh) == chosen
     assert fresh.v["fb_split"] == 3 and fresh.v["quad_wgs"] == 3072
     other = FakeCtx()
-    other.dims = type("D", (), dict(vars(FakeDims), nwalkers=64))()
-    assert t2.apply(other) is None  # another shape: nothing cached
+    other.dims = type("D", (), dict(vars(FakeDims), nwalkers=300))()
+    assert t2.apply(other) is None  # another bucket (three 128-walker row blocks): nothing cached
+
+
+def _dims(**kw):
+    return type("D", (), dict(vars(FakeDims), **kw))()
+
+
+def test_shape_buckets_group_similar_systems():
+    a, b = FakeCtx(), FakeCtx()
+    b.dims = _dims(nbasis=60, nchol=200, nalpha=6, nbeta=6, nwalkers=100)
+    assert autotune.shape_key(a) == autotune.shape_key(b)  # same classes: one bucket
+    for kw in (dict(nbasis=70), dict(nalpha=9), dict(nchol=300), dict(nwalkers=129), dict(rhf=1)):
+        c = FakeCtx()
+        c.dims = _dims(**kw)
+        assert autotune.shape_key(c) != autotune.shape_key(a), kw
+    # W = 300 and W = 500 both round up to four 128-walker row blocks
+    c, d = FakeCtx(), FakeCtx()
+    c.dims, d.dims = _dims(nwalkers=300), _dims(nwalkers=500)
+    assert autotune.shape_key(c) == autotune.shape_key(d)
+
+
+def test_cached_values_outside_the_admissible_range_force_a_retune(tmp_path):
+    path = tmp_path / "cache.json"
+    ctx = FakeCtx()
+    key = autotune.shape_key(ctx)
+    path.write_text(json.dumps({key: {"fb_split": 40, "quad_wgs": 3072}}))  # 40 > this context's 12
+    t = autotune.Autotuner(cache_path=str(path))
+    assert t.apply(ctx) is None
+    assert ctx.v == {"fb_split": 8, "quad_wgs": 1536, "exx_split": 0}  # nothing was set
+    path.write_text(json.dumps({key: {"fb_split": 4}}))  # a knob missing from the entry
+    assert autotune.Autotuner(cache_path=str(path)).apply(ctx) is None
+    path.write_text(json.dumps({key: {"fb_split": 4, "quad_wgs": 2048}}))
+    assert autotune.Autotuner(cache_path=str(path)).apply(ctx) == {"fb_split": 4, "quad_wgs": 2048}
+    assert ctx.v["fb_split"] == 4 and ctx.v["quad_wgs"] == 2048
+
+
+def test_key_changes_with_the_library_build(monkeypatch):
+    ctx = FakeCtx()
+    k1 = autotune.shape_key(ctx)
+    monkeypatch.setattr(autotune, "library_tag", lambda: "other-build")
+    assert autotune.shape_key(ctx) != k1
 
 
 def test_as_tuner_specs(tmp_path):

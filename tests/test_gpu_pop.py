@@ -58,7 +58,7 @@ def _rel(a, b):
     return float(numpy.abs(a - b).max() / max(numpy.abs(b).max(), 1e-300))
 
 
-@pytest.mark.parametrize("nw", [2, 7, 64, 301, 1024, 4096])
+@pytest.mark.parametrize("nw", [2, 7, 64, 301, 1024, 2048, 4096])
 @pytest.mark.parametrize("kind", ["lognormal", "ties", "signed"])
 def test_pair_branch_plan(nw, kind):
     print(f"[case] pair_branch nw={nw} kind={kind}", flush=True)

@@ -86,3 +86,43 @@ def test_estimator_is_unbiased(precision):
     # the pooled mean over walkers is unbiased too
     pooled = (mean - ref[:, 2]).mean() / (numpy.sqrt((sem**2).sum()) / W)
     assert abs(pooled) < 5.0
+
+
+@pytest.mark.timeout(900)
+def test_unbiased_at_surrogate_workload():
+    """N=300, X=1800, n=30, W=2048 (the surrogate workload: 16 walker row blocks
+    in the quadratic-form GEMMs, a subset of k = 256 exact walkers): after a
+    short warm-up, R = 48 re-evaluations of one state; every walker's mean
+    estimate within 6 standard errors of its exact exchange, the pooled mean
+    within 5, weights finite."""
+    import torch
+
+    from ipie_amd.estimators import local_energy_batch
+    from tests._big import device_state
+
+    system, ham, trial, walkers, prop = device_state(
+        "n300", precision="mixed", surrogate=dict(warmup=6, fraction=0.125, train_iters=3, lr=2e-3, seed=5))
+    Wn = walkers.nwalkers
+    e = torch.empty(Wn, 3, dtype=torch.complex128, device=walkers.device)
+    for _ in range(6):
+        prop.propagate_walkers(walkers, ham, trial, 0.0, energy_prev=e)
+    snap = snapshot(walkers)
+    trial.calc_greens_function(walkers)
+    ref = local_energy_batch(system, ham, walkers, trial).cpu().numpy()
+    R = 48
+    est = numpy.empty((R, Wn), dtype=numpy.complex128)
+    for r in range(R):
+        restore(walkers, snap)
+        prop.propagate_walkers(walkers, ham, trial, 0.0, energy_prev=e)
+        est[r] = e[:, 2].cpu().numpy()
+    assert prop.surrogate_stats()["last_k"] == round(Wn / 8)
+    assert bool(torch.isfinite(walkers.weight).all())
+    mean = est.mean(axis=0)
+    sem = est.std(axis=0) / numpy.sqrt(R) + 1e-12 * numpy.abs(ref[:, 2])
+    err = numpy.abs(mean - ref[:, 2])
+    z = err / sem
+    tol_abs = 1e-5 * numpy.abs(ref[:, 2]).max()  # mixed-precision exchange error of the exact samples
+    print(f"max z {z.max():.2f} mean z {z.mean():.2f} stats {prop.surrogate_stats()}", flush=True)
+    assert numpy.all((z < 6.0) | (err < tol_abs))
+    pooled = (mean - ref[:, 2]).real.mean() / (numpy.sqrt((sem**2).sum()) / Wn)
+    assert abs(pooled) < 5.0
