@@ -33,6 +33,7 @@
 #ifndef AFQMC_HIP_H
 #define AFQMC_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -337,6 +338,28 @@ int afqmc_surrogate_configure(afqmc_ctx* ctx, int enable, double fraction, int w
 int afqmc_surrogate_stats(afqmc_ctx* ctx, double* out, int n);
 /* n standard normals (Philox4x32-10, Box-Muller) for stream `counter`. */
 int afqmc_draw_normals(afqmc_ctx* ctx, double* out, int64_t n, uint64_t seed, uint64_t counter);
+
+/* Green's function that allocates nothing.  No counterpart in ipie: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same results as
+ * afqmc_greens_function (afqmc_overlap when ghalfa is NULL), without a
+ * context: psia [N, n_a] / psib [N, n_b] complex are the trial orbitals, the
+ * other arrays as in afqmc_greens_function, walker_flags [W] int32 (optional)
+ * receives AFQMC_FLAG_SINGULAR OR-ed per walker, and the work is enqueued on
+ * `stream` (a hipStream_t, NULL for the null stream).  The LU and inverse work
+ * buffers live in a static device array of AFQMC_NOALLOC_SCRATCH_BYTES per
+ * GPU (held until the process ends); afqmc_noalloc_scratch_bytes gives what a
+ * call needs, and a call that needs more returns AFQMC_ERR_SHAPE.  Every call
+ * on one GPU shares that GPU's array: calls that overlap in time on one GPU
+ * race.  afqmc_noalloc_last_error returns the calling thread's message of its
+ * last failed call. */
+#define AFQMC_NOALLOC_SCRATCH_BYTES 134217728ULL /* 128 MiB */
+size_t afqmc_noalloc_scratch_bytes(int32_t nwalkers, int32_t nalpha, int32_t nbeta);
+const char* afqmc_noalloc_last_error(void);
+int afqmc_greens_function_noalloc(int32_t nwalkers, int32_t nbasis, int32_t nalpha, int32_t nbeta, int32_t rhf,
+                                  const double* psia, const double* psib, const double* phia, const double* phib,
+                                  double* ghalfa, double* ghalfb, double* ovlp, const double* log_shift,
+                                  int32_t* walker_flags, void* stream);
 
 #ifdef __cplusplus
 }

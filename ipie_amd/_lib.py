@@ -113,6 +113,12 @@ SIGNATURES = {
     "afqmc_batch_end": (ctypes.c_int, [_dp]),
     "afqmc_batch_steps": (ctypes.c_int, [_dp, ctypes.POINTER(_i32)]),
     "afqmc_batch_launch": (ctypes.c_int, [_dp, _i32, ctypes.POINTER(_u64), ctypes.POINTER(_f64)]),
+    "afqmc_noalloc_scratch_bytes": (ctypes.c_size_t, [_i32, _i32, _i32]),
+    "afqmc_noalloc_last_error": (ctypes.c_char_p, []),
+    "afqmc_greens_function_noalloc": (
+        ctypes.c_int,
+        [_i32, _i32, _i32, _i32, _i32, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp],
+    ),
 }
 
 POP_METHODS = {"pair_branch": 0, "comb": 1, "stochastic_reconfiguration": 2, "rescale": 3}

@@ -22,6 +22,7 @@ SOURCES = [
     "k_fields.hip",
     "k_qr.hip",
     "k_pop.hip", "k_setup.hip",
+    "afqmc_noalloc.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -13,7 +13,7 @@ HEADER = os.path.join(ROOT, "include", "afqmc_hip.h")
 
 def declared_symbols():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:int|void|const char\*)\s+(afqmc_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:int|void|size_t|const char\*)\s+(afqmc_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_declares_entry_points():
