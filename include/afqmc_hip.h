@@ -299,6 +299,17 @@ int afqmc_profile_read(afqmc_ctx* ctx, double* ms, int64_t* launches);
  *                        length of a unit; mixed precision, quadratic form)
  *   AFQMC_TUNE_EXX_SPLIT Cholesky-range split of the T-form exchange
  *                        (0 restores the heuristic)
+ *   AFQMC_TUNE_QUAD_RGROUP
+ *                        walker row blocks (128 walkers each) whose workgroups
+ *                        are grouped over the column units of the split-BF16
+ *                        quadratic form (mixed precision, quadratic form; the
+ *                        default, all of them, puts every row block of a unit
+ *                        on one XCD together)
+ *   AFQMC_TUNE_TILE_FB, AFQMC_TUNE_TILE_VHS, AFQMC_TUNE_TILE_ONE_BODY
+ *                        workgroup tile of the split-BF16 force-bias, VHS and
+ *                        one-body GEMMs (mixed precision): 0 = 128 x 160,
+ *                        1 = 128 x 128, 2 = 128 x 80, 3 = 64 x 160 (walker
+ *                        rows x output columns, 8 waves each)
  * afqmc_tune_get returns the current value and the admissible range [lo, hi];
  * hi = 0 when the knob does not apply to this context.  afqmc_tune_set
  * changes only launch shapes and summation grouping (results agree to
@@ -308,6 +319,10 @@ int afqmc_profile_read(afqmc_ctx* ctx, double* ms, int64_t* launches);
 #define AFQMC_TUNE_FB_SPLIT 1
 #define AFQMC_TUNE_QUAD_WGS 2
 #define AFQMC_TUNE_EXX_SPLIT 3
+#define AFQMC_TUNE_TILE_FB 4
+#define AFQMC_TUNE_TILE_VHS 5
+#define AFQMC_TUNE_TILE_ONE_BODY 6
+#define AFQMC_TUNE_QUAD_RGROUP 7
 int afqmc_tune_get(afqmc_ctx* ctx, int32_t knob, int32_t* value, int32_t* lo, int32_t* hi);
 int afqmc_tune_set(afqmc_ctx* ctx, int32_t knob, int32_t value);
 /* Learned control variate for the exchange energy of afqmc_propagate's
@@ -336,6 +351,14 @@ int afqmc_surrogate_configure(afqmc_ctx* ctx, int enable, double fraction, int w
  * the MLP's target Re(exact - control variate), and (n >= 7) the current gain of
  * the MLP term.  Synchronises. */
 int afqmc_surrogate_stats(afqmc_ctx* ctx, double* out, int n);
+/* cv [W] complex = E_T + 2 <K, G - G_T> + Q_bf16(G - G_T) from Ghalf: the
+ * physical control variate of each walker's exchange sum (the predictor above
+ * without the MLP term), through the kernels afqmc_propagate runs for it.
+ * Needs afqmc_surrogate_configure(enable = 1); changes no walker or model
+ * state.  No counterpart in ipie; it lets the autotuner time the control
+ * variate's exchange. */
+int afqmc_control_variate(afqmc_ctx* ctx, int32_t nwalkers, const double* ghalfa, const double* ghalfb,
+                          double* cv);
 /* n standard normals (Philox4x32-10, Box-Muller) for stream `counter`. */
 int afqmc_draw_normals(afqmc_ctx* ctx, double* out, int64_t n, uint64_t seed, uint64_t counter);
 

@@ -113,6 +113,7 @@ SIGNATURES = {
     "afqmc_batch_end": (ctypes.c_int, [_dp]),
     "afqmc_batch_steps": (ctypes.c_int, [_dp, ctypes.POINTER(_i32)]),
     "afqmc_batch_launch": (ctypes.c_int, [_dp, _i32, ctypes.POINTER(_u64), ctypes.POINTER(_f64)]),
+    "afqmc_control_variate": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp]),
     "afqmc_noalloc_scratch_bytes": (ctypes.c_size_t, [_i32, _i32, _i32]),
     "afqmc_noalloc_last_error": (ctypes.c_char_p, []),
     "afqmc_greens_function_noalloc": (
@@ -124,7 +125,8 @@ SIGNATURES = {
 POP_METHODS = {"pair_branch": 0, "comb": 1, "stochastic_reconfiguration": 2, "rescale": 3}
 
 # launch-shape knobs of afqmc_tune_get / afqmc_tune_set
-TUNE_KNOBS = {"fb_split": 1, "quad_wgs": 2, "exx_split": 3}
+TUNE_KNOBS = {"fb_split": 1, "quad_wgs": 2, "exx_split": 3, "tile_fb": 4, "tile_vhs": 5, "tile_one_body": 6,
+              "quad_rgroup": 7}
 
 PHASES = ["gf", "one_body", "force_bias", "fields", "vhs", "taylor", "overlap", "weight", "exchange", "energy",
           "reortho", "exchange_cv", "split"]
@@ -332,6 +334,14 @@ class Context:
         self._chk_ghalf(ghalfa, ghalfb, nw)
         _check_tensor("energy", energy, torch.complex128, (nw, 3), self.device)
         self._call("afqmc_local_energy", nw, ptr(ghalfa), ptr(ghalfb), ptr(energy))
+
+    def control_variate(self, ghalfa, ghalfb, cv):
+        """cv [W] complex: the physical control variate of each walker's exchange
+        sum (afqmc_control_variate); needs the control variate configured."""
+        nw = ghalfa.shape[0]
+        self._chk_ghalf(ghalfa, ghalfb, nw)
+        self._chk_vec("cv", cv, nw, torch.complex128)
+        self._call("afqmc_control_variate", nw, ptr(ghalfa), ptr(ghalfb), ptr(cv))
 
     def reortho(self, phia, phib, ovlp, detR, log_detR, detR_shift=None):
         nw = self._chk_phi(phia, phib)

@@ -1,29 +1,38 @@
 """Autotuner of a device context's launch shapes.
 
-Three launch shapes of the step depend on the system's dimensions:
+Seven launch shapes of the step depend on the system's dimensions:
 
-  fb_split   K ranges of the force-bias contraction (split-K; per spin in
-             mixed precision, in total in FP64)
-  quad_wgs   workgroups the column units of the split-BF16 quadratic-form
-             exchange aim for, which sets the K-block length of a unit
-  exx_split  Cholesky-range split of the T-form exchange (0: heuristic)
+  fb_split       K ranges of the force-bias contraction (split-K; per spin in
+                 mixed precision, in total in FP64)
+  quad_wgs       workgroups the column units of the split-BF16 quadratic-form
+                 exchange aim for, which sets the K-block length of a unit
+  exx_split      Cholesky-range split of the T-form exchange (0: heuristic)
+  quad_rgroup    walker row blocks grouped over the quadratic form's column
+                 units (which workgroups run together and share L2)
+  tile_fb        workgroup tile of the split-BF16 force-bias GEMM
+  tile_vhs       workgroup tile of the split-BF16 VHS GEMM
+  tile_one_body  workgroup tile of the split-BF16 one-body GEMM
+                 (tiles: 0 = 128 x 160, 1 = 128 x 128, 2 = 128 x 80,
+                 3 = 64 x 160 walker rows x output columns)
 
 A context starts from heuristics (``afqmc_create``).  ``Autotuner.tune`` times
 every admissible candidate of each knob that applies on the context's own
 shapes, through entry points that change no walker state
-(``afqmc_force_bias`` for fb_split, ``afqmc_local_energy`` for the exchange
-knobs), and keeps the fastest unless the heuristic is within ``min_gain`` of
-it.  The choices are cached per device and shape: a multi-system batch whose
+(``afqmc_force_bias`` for fb_split and tile_fb, ``afqmc_local_energy`` for the
+exchange knobs, or ``afqmc_control_variate`` for them in a context that runs
+the learned control variate, whose step evaluates the single-product quadratic
+form of every walker) or act on scratch copies of the walkers
+(``afqmc_two_body`` for tile_vhs, ``afqmc_one_body`` for tile_one_body), and
+keeps the fastest unless the heuristic is within ``min_gain`` of it.  The choices are cached per device and shape: a multi-system batch whose
 systems share a shape tunes that shape once, and a JSON cache file carries the
 choices to the next run.  The knobs change launch shapes and the grouping of
 partial sums only, so results agree with the untuned context to rounding.
 
 A context that runs the learned control variate computes its exchange inside
-``afqmc_propagate`` (the single-product quadratic form of every walker and the
-exact form of a sampled subset), a path no state-free entry point exposes:
-for it only fb_split is tuned and the exchange shapes keep their heuristic
-(its cache entries are keyed apart).  ``PhaselessGeneric(autotune=...)``
-tunes at build time.
+``afqmc_propagate``: the single-product quadratic form of every walker, timed
+here through ``afqmc_control_variate``, and the exact form of a sampled
+subset, which shares the quadratic form's launch shapes (its cache entries are
+keyed apart).  ``PhaselessGeneric(autotune=...)`` tunes at build time.
 
 No counterpart in ipie, whose GPU kernels take CuPy's and numba's shapes.
 """
@@ -34,15 +43,19 @@ import time
 
 import torch
 
-KNOBS = ("fb_split", "quad_wgs", "exx_split")
-# a context running the learned control variate evaluates its exchange inside
-# afqmc_propagate; only the force-bias split is timed through a state-free call
-SURROGATE_KNOBS = ("fb_split",)
+KNOBS = ("fb_split", "quad_wgs", "exx_split", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body")
+# a context running the learned control variate: the quadratic-form knobs are
+# timed on its control variate (the T-form split is not used by it)
+SURROGATE_KNOBS = ("fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body")
+# knobs timed on scratch copies of the walkers
+PHI_KNOBS = ("tile_vhs", "tile_one_body")
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "ipie_amd", "autotune.json")
 
 _FB = (1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 24, 32)
 _QUAD = (512, 768, 1024, 1536, 2048, 3072, 4096, 6144)
 _EXX = (1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 192, 256)
+_TILE = (0, 1, 2, 3)
+_RGROUP = (1, 2, 4, 8, 16, 32)
 
 
 def _ceil(a, b):
@@ -88,7 +101,8 @@ def candidates(knob, value, lo, hi):
     """Values of `knob` to time: a fixed ladder within [lo, hi], plus the current value."""
     if hi < 1:
         return []
-    ladder = {"fb_split": _FB, "quad_wgs": _QUAD, "exx_split": _EXX}[knob]
+    ladder = {"fb_split": _FB, "quad_wgs": _QUAD, "exx_split": _EXX, "tile_fb": _TILE, "tile_vhs": _TILE,
+              "tile_one_body": _TILE, "quad_rgroup": _RGROUP}[knob]
     out = {v for v in ladder if lo <= v <= hi}
     if knob == "exx_split":
         out.add(0)  # the heuristic
@@ -173,25 +187,40 @@ class Autotuner:
             best = min(best, (time.perf_counter() - t0) / self.inner)
         return best * 1e3
 
-    def tune(self, ctx, ghalfa, ghalfb, surrogate=False):
+    def tune(self, ctx, ghalfa, ghalfb, surrogate=False, phia=None, phib=None):
         """Time every candidate of each applicable knob with these Green's
         functions ([W, n_s, N] complex128 on ctx's device), set the choices on
         ctx, cache and return them ({knob: value}).
 
         surrogate: true when the context runs the learned control variate
-        (then only fb_split is tuned, see the module note)."""
+        (then the exchange knobs are not tuned, see the module note).
+        phia / phib: walkers ([W, N, n_s] complex128); the tile_vhs and
+        tile_one_body knobs are timed on copies of them (skipped without)."""
         nw = ghalfa.shape[0]
         dev = ghalfa.device
         vbias = torch.empty(nw, ctx.dims.nchol, dtype=torch.complex128, device=dev)
         energy = torch.empty(nw, 3, dtype=torch.complex128, device=dev)
+        cvbuf = torch.empty(nw, dtype=torch.complex128, device=dev)
+        exchange = ((lambda: ctx.control_variate(ghalfa, ghalfb, cvbuf)) if surrogate
+                    else (lambda: ctx.local_energy(ghalfa, ghalfb, energy)))
         calls = {
             "fb_split": lambda: ctx.force_bias(ghalfa, ghalfb, vbias),
-            "quad_wgs": lambda: ctx.local_energy(ghalfa, ghalfb, energy),
-            "exx_split": lambda: ctx.local_energy(ghalfa, ghalfb, energy),
+            "tile_fb": lambda: ctx.force_bias(ghalfa, ghalfb, vbias),
+            "quad_wgs": exchange,
+            "quad_rgroup": exchange,
+            "exx_split": exchange,
         }
+        if phia is not None:
+            pa, pb = phia.clone(), phib.clone()
+            cmf = torch.empty(nw, dtype=torch.complex128, device=dev)
+            cfb = torch.empty_like(cmf)
+            calls["tile_vhs"] = lambda: ctx.two_body(pa, pb, ghalfa, ghalfb, cmf, cfb, seed=1, counter=0)
+            calls["tile_one_body"] = lambda: ctx.one_body(pa, pb)
         chosen, report = {}, {}
         for knob in self.knobs:
             if surrogate and knob not in SURROGATE_KNOBS:
+                continue
+            if knob not in calls:
                 continue
             value, lo, hi = ctx.tune_get(knob)
             cands = candidates(knob, value, lo, hi)
@@ -225,7 +254,7 @@ class Autotuner:
         gb = torch.empty_like(walkers.Ghalfb)
         ovlp = torch.empty(walkers.phia.shape[0], dtype=torch.complex128, device=dev)
         ctx.greens_function(walkers.phia, walkers.phib, ga, gb, ovlp)
-        return self.tune(ctx, ga, gb, surrogate)
+        return self.tune(ctx, ga, gb, surrogate, phia=walkers.phia, phib=walkers.phib)
 
 
 def as_tuner(spec):

@@ -59,8 +59,9 @@ struct afqmc_ctx {
   uint16_t* gplanes[2] = {nullptr, nullptr};  // per spin: 4 x [W * n_s, kpad] (Ghalf)
   uint16_t* lplanes[2] = {nullptr, nullptr};  // per spin: 2 x [X * n_s, kpad] (half-rotated Cholesky)
   uint16_t* xplanes = nullptr;                // 4 x [W, xpad] (shifted fields)
-  uint16_t* cplanes = nullptr;                // 2 x [P, xpad] (packed Cholesky)
-  uint16_t* vplanes = nullptr;                // VHS split planes in Taylor tiles (W x 80-row blocks x kpad x 4)
+  uint16_t* cplanes = nullptr;                // 2 x [NAT 256, xpad] (Cholesky rows in VHS atom order)
+  uint16_t* vatoms = nullptr;                 // VHS split planes, upper atoms per walker ([W][NAT][4][256])
+  uint16_t* zatom = nullptr;                  // one all-zero atom (4 x 256)
   uint16_t* tplanes[2] = {nullptr, nullptr};  // 4 x [W, rpad, kpad] each (Taylor terms, transposed)
   uint16_t* dplanes = nullptr;                // split-BF16 expH1_s - I per spin, 2 x [N, kpad] each
   uint16_t* oplanes = nullptr;                // 4 x [W n_s, kpad]: transposed planes of phi_s (one body)
@@ -79,6 +80,8 @@ struct afqmc_ctx {
   int fb_split_spin = BF16_FB_SPLIT;  // split-BF16 force bias: K ranges per spin
   int fb_cap = 0;                     // K ranges vbias_part holds (0: not allocated yet)
   int quad_target = QUAD_TARGET_WGS;  // split-BF16 quadratic form: workgroups the column units aim for
+  int tile_fb = 0, tile_vhs = 0, tile_ob = 0;  // split-BF16 GEMM tile shapes (force bias, VHS, one body)
+  int quad_rgroup = 0;  // split-BF16 quadratic form: row blocks per workgroup group (0: all)
   int64_t qy_cap = 0;                 // complex entries qy holds (split-BF16 quadratic form)
   int exx_split = 0;                  // T-form exchange: Cholesky-range split (0: heuristic)
   std::vector<void*> retired;         // buffers a retune replaced: graphs captured before still use them
@@ -340,9 +343,12 @@ int ensure_ws(afqmc_ctx* c) {
   if (d.precision == AFQMC_PREC_MIXED) ok = ok && alloc_exchange_planes(c) == AFQMC_OK;
   if (d.precision == AFQMC_PREC_MIXED) {
     ok = ok && dmalloc((void**)&c->xplanes, 2 * (size_t)4 * W * c->xpad) == hipSuccess;
-    ok = ok && dmalloc((void**)&c->cplanes, 2 * (size_t)2 * P * c->xpad) == hipSuccess;
+    const int64_t NAT = vhs_atom_count((int)N);
+    ok = ok && dmalloc((void**)&c->cplanes, 2 * (size_t)2 * NAT * 256 * c->xpad) == hipSuccess;
     const size_t tbytes = 2 * (size_t)4 * W * taylor_rows_padded(c->ncol) * c->kpad;
-    ok = ok && dmalloc((void**)&c->vplanes, 2 * (size_t)4 * W * vhs_tiled_rows(N) * c->kpad) == hipSuccess;
+    ok = ok && dmalloc((void**)&c->vatoms, 2 * (size_t)W * NAT * 1024) == hipSuccess;
+    ok = ok && dmalloc((void**)&c->zatom, 2 * (size_t)1024) == hipSuccess;
+    ok = ok && hipMemset(c->zatom, 0, 2 * (size_t)1024) == hipSuccess;
     for (int i = 0; i < 2; ++i) {
       ok = ok && dmalloc((void**)&c->tplanes[i], tbytes) == hipSuccess;
       ok = ok && hipMemset(c->tplanes[i], 0, tbytes) == hipSuccess;  // padding stays zero
@@ -364,7 +370,8 @@ int ensure_ws(afqmc_ctx* c) {
     // with the unit tables (build_quad_units)
     if (!mixed) ok = ok && dmalloc((void**)&c->qy, 16 * (size_t)W * nmax * Np) == hipSuccess;
   }
-  ok = ok && dmalloc((void**)&c->vhs, 16 * (size_t)W * N * N) == hipSuccess;
+  // FP64 VHS [W, N, N] complex (2.6 GB at N=400); the split-BF16 path keeps VHS as atoms only
+  if (d.precision != AFQMC_PREC_MIXED) ok = ok && dmalloc((void**)&c->vhs, 16 * (size_t)W * N * N) == hipSuccess;
   for (int i = 0; i < 4; ++i) ok = ok && dmalloc((void**)&c->buf[i], 16 * (size_t)W * N * c->ncol) == hipSuccess;
   ok = ok && dmalloc((void**)&c->tri, sizeof(int) * 2 * (size_t)P) == hipSuccess;
   if (!ok) return fail(c, AFQMC_ERR_ALLOC, "workspace allocation failed (%s)", c->describe_dims);
@@ -404,10 +411,8 @@ int ensure_planes(afqmc_ctx* c) {
       HIP_TRY(c, launch_split_real(s == 0 ? c->s.rchola : c->s.rcholb, (int64_t)c->d.nchol * n, N, c->kpad,
                                    c->lplanes[s], c->stream));
     }
-  if (c->cplanes) {
-    const int64_t P = (int64_t)N * (N + 1) / 2;
-    HIP_TRY(c, launch_split_real(c->s.chol_packed, P, c->d.nchol, c->xpad, c->cplanes, c->stream));
-  }
+  if (c->cplanes)
+    HIP_TRY(c, launch_split_chol_atoms(c->s.chol_packed, N, c->d.nchol, c->xpad, c->cplanes, c->stream));
   if (c->dplanes) {
     HIP_TRY(c, launch_split_eye(c->s.expH1, N, c->kpad, c->dplanes, c->stream));
     if (two_spins(c))
@@ -482,6 +487,7 @@ int do_force_bias_bf16(afqmc_ctx* c, int nw, const double* ga, const double* gb)
     b.kchunks = (int)(b.lda / 32);
     b.ksplit = c->fb_split_spin;
     b.epilogue = BF16_EPI_PARTIAL;
+    b.tile = c->tile_fb;
     b.out = c->vbias_part;
     b.out32 = reinterpret_cast<float*>(c->vbias_part);  // read back by fields_kernel as FP32 (part32)
     b.z_offset = s * c->fb_split_spin;
@@ -545,12 +551,11 @@ int do_fields(afqmc_ctx* c, int nw, bool want_fields, const double* xi, uint64_t
 }
 
 int do_vhs(afqmc_ctx* c, int nw) {
-  if (c->cplanes) {  // split-BF16: VHS[w] = i sqrt(dt) sum_x xshifted[w, x] L_x
+  if (c->cplanes) {  // split-BF16: VHS[w] = i sqrt(dt) sum_x xshifted[w, x] L_x, as atoms
     int r = ensure_planes(c);
     if (r) return r;
     PhaseScope ps(c, AFQMC_PHASE_VHS);
-    const int N = c->d.nbasis;
-    const int64_t P = (int64_t)N * (N + 1) / 2;
+    const int64_t ncols = vhs_atom_count(c->d.nbasis) * 256;
     HIP_TRY(c, launch_split_ghalf(c->xshifted, nw, c->d.nchol, c->xpad, c->xplanes, c->stream));
     Bf16GemmArgs b{};
     b.a = c->xplanes;
@@ -558,18 +563,17 @@ int do_vhs(afqmc_ctx* c, int nw) {
     b.lda = c->xpad;
     b.m = nw;
     b.b = c->cplanes;
-    b.b_plane = P * c->xpad;
+    b.b_plane = ncols * c->xpad;
     b.ldb = c->xpad;
-    b.nn = P;
+    b.nn = ncols;
     b.kchunks = c->xpad / 32;
     b.ksplit = 1;
-    b.epilogue = BF16_EPI_VHS;
-    b.out = c->vhs;
-    b.out32 = reinterpret_cast<float*>(c->vhs);  // complex FP32 in the same buffer (only the Taylor planes read it)
+    b.epilogue = BF16_EPI_VATOM;
+    b.tile = c->tile_vhs;
+    b.out16 = c->vatoms;
+    b.atom_stride = ncols * 4;
     b.scale = sqrt(c->s.dt);
-    b.tri_i = c->tri;
-    b.tri_j = c->tri + P;
-    b.nbasis = N;
+    b.nbasis = c->d.nbasis;
     HIP_TRY(c, launch_bf16_gemm(b, c->stream));
     return AFQMC_OK;
   }
@@ -636,6 +640,7 @@ int one_body_bf16(afqmc_ctx* c, int nw, int s, const uint16_t* aplanes, int64_t 
   b.kchunks = c->kpad / 32;
   b.ksplit = 1;
   b.epilogue = BF16_EPI_ADD;
+  b.tile = c->tile_ob;
   b.scale = 1.0;
   b.group = n;
   b.a_gstride = a_gstride;
@@ -746,7 +751,6 @@ int taylor_bf16(afqmc_ctx* c, int nw, const double** result, const uint16_t** re
   const int N = c->d.nbasis, nc = c->ncol, rpad = taylor_rows_padded(nc);
   double* P = c->buf[0];
   double* Q = c->buf[3];
-  HIP_TRY(c, launch_split_vhs_tiled(reinterpret_cast<const float*>(c->vhs), nw, N, c->kpad, c->vplanes, c->stream));
   if (!c->ptp_ready)
     HIP_TRY(c, launch_split_transpose(P, nw, (int)c->d.nwalkers, N, nc, rpad, c->kpad, c->tplanes[0], c->stream));
   c->ptp_ready = false;
@@ -760,7 +764,10 @@ int taylor_bf16(afqmc_ctx* c, int nw, const double** result, const uint16_t** re
     t.kpad = c->kpad;
     t.tin = c->tplanes[(k - 1) & 1];
     t.tout = k < c->d.exp_nmax ? c->tplanes[k & 1] : nullptr;
-    t.vplanes = c->vplanes;
+    t.vatoms = c->vatoms;
+    t.zatom = c->zatom;
+    t.natom_side = vhs_atom_side(N);
+    t.atom_stride = vhs_atom_count(N) * 1024;
     t.init = P;
     t.q = Q;
     t.dacc = reinterpret_cast<float*>(c->buf[1]);  // unused by the split-BF16 path otherwise
@@ -780,7 +787,7 @@ int taylor_bf16(afqmc_ctx* c, int nw, const double** result, const uint16_t** re
 int taylor(afqmc_ctx* c, int nw, const double** result, const uint16_t** resultT = nullptr) {
   PhaseScope ps(c, AFQMC_PHASE_TAYLOR);
   if (resultT) *resultT = nullptr;
-  if (c->vplanes && c->d.exp_nmax > 0) return taylor_bf16(c, nw, result, resultT);
+  if (c->vatoms && c->d.exp_nmax > 0) return taylor_bf16(c, nw, result, resultT);
   const int N = c->d.nbasis, nc = c->ncol;
   const int64_t NN = (int64_t)N * N, stride = (int64_t)N * nc;
   double* P = c->buf[0];
@@ -849,6 +856,7 @@ int quad_bf16(afqmc_ctx* c, int nw, int spin, const double* g, const int* walker
   b.scale = 1.0;
   b.units = c->qunits[spin] + c->qunit_off[spin][mb - 1];
   b.nunits = nunits;
+  b.rgroup = c->quad_rgroup;
   b.dg = g;
   b.dref = gref;
   b.walker_of = walker_of;
@@ -1248,7 +1256,7 @@ void afqmc_destroy(afqmc_ctx* c) {
   else (void)hipDeviceSynchronize();
   void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
                   c->buf[2], c->buf[3], c->tri, c->flags, c->wflags, c->psi_flag, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
-                  c->lplanes[1], c->xplanes, c->cplanes, c->vplanes, c->tplanes[0], c->dplanes, c->oplanes,
+                  c->lplanes[1], c->xplanes, c->cplanes, c->vatoms, c->zatom, c->tplanes[0], c->dplanes, c->oplanes,
                   c->tplanes[1], c->gcomp, c->gf_o, c->gf_oinv, c->qw64[0], c->qw64[1], c->qwp[0],
                   c->qwp[1], c->qy, c->qunits[0], c->qunits[1], c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
                   c->sur.resid, c->sur.walker_of, c->sur.xbar, c->sur.e1, c->sur.master, c->sur.m,
@@ -1776,6 +1784,48 @@ int afqmc_local_energy(afqmc_ctx* c, int32_t nw, const double* ga, const double*
   return do_energy(c, nw, ga, gb, energy);
 }
 
+int afqmc_control_variate(afqmc_ctx* c, int32_t nw, const double* ga, const double* gb, double* cv) {
+  int r = check_ready(c, nw);
+  if (r) return r;
+  r = need_static(c);
+  if (r) return r;
+  if (!ga || !cv || (two_spins(c) && !gb)) return fail(c, AFQMC_ERR_INVALID_ARG, "null array");
+  if (!c->sur.enabled) return fail(c, AFQMC_ERR_INVALID_ARG, "the control variate is not configured");
+  r = sur_prepare_cv(c);
+  if (r) return r;
+  r = ensure_planes(c);
+  if (r) return r;
+  auto& S = c->sur;
+  const int nspin = two_spins(c) ? 2 : 1;
+  const int N = c->d.nbasis;
+  const int64_t gstride = 2 * (int64_t)std::max(c->d.nalpha, c->d.nbeta) * N;
+  for (int s = 0; s < nspin; ++s) {
+    r = exchange_q_spin(c, nw, s, s == 0 ? ga : gb);
+    if (r) return r;
+  }
+  SurrogateCvArgs cva{};
+  cva.nwalkers = nw;
+  cva.nbasis = N;
+  cva.nspin = nspin;
+  for (int s = 0; s < nspin; ++s) {
+    cva.nocc[s] = s == 0 ? c->d.nalpha : c->d.nbeta;
+    cva.g[s] = s == 0 ? ga : gb;
+    cva.gT[s] = S.gT + s * gstride;
+    cva.K[s] = S.K + s * gstride;
+    cva.part[s] = s == 0 ? c->exx_a : c->exx_b;
+    cva.xs[s] = c->xs_used[s];
+  }
+  cva.et = S.et;
+  cva.cv = cv;
+  // the kernel also folds the MLP term into the step's predictor buffers; the
+  // step rewrites both before it reads them, so its state is not changed
+  cva.pred = S.pred;
+  cva.mlp = S.mlp;
+  cva.state = S.state;
+  HIP_TRY(c, launch_sur_cv(cva, c->stream));
+  return AFQMC_OK;
+}
+
 int afqmc_reortho(afqmc_ctx* c, int32_t nw, double* phia, double* phib, double* ovlp, double* detR,
                   double* log_detR, const double* detR_shift) {
   int r = check_ready(c, nw);
@@ -2052,6 +2102,21 @@ int afqmc_tune_get(afqmc_ctx* c, int32_t knob, int32_t* value, int32_t* lo, int3
       *lo = 1;
       *hi = tune_exx_max(c);
       return AFQMC_OK;
+    case AFQMC_TUNE_QUAD_RGROUP:
+      if (!c->quad_on || c->d.precision != AFQMC_PREC_MIXED) return AFQMC_OK;
+      *hi = (int32_t)ceil_div(c->d.nwalkers, 128);
+      *lo = 1;
+      *value = c->quad_rgroup > 0 ? std::min<int32_t>(c->quad_rgroup, *hi) : *hi;
+      return AFQMC_OK;
+    case AFQMC_TUNE_TILE_FB:
+    case AFQMC_TUNE_TILE_VHS:
+    case AFQMC_TUNE_TILE_ONE_BODY:
+      // split-BF16 GEMM tile shapes (mixed precision; the force bias needs Cholesky vectors)
+      if (c->d.precision != AFQMC_PREC_MIXED || c->d.nchol < 1) return AFQMC_OK;
+      *value = knob == AFQMC_TUNE_TILE_FB ? c->tile_fb : knob == AFQMC_TUNE_TILE_VHS ? c->tile_vhs : c->tile_ob;
+      *lo = 0;
+      *hi = BF16_TILES - 1;
+      return AFQMC_OK;
     default:
       return fail(c, AFQMC_ERR_INVALID_ARG, "unknown tuning knob %d", knob);
   }
@@ -2086,6 +2151,14 @@ int afqmc_tune_set(afqmc_ctx* c, int32_t knob, int32_t value) {
   } else if (knob == AFQMC_TUNE_QUAD_WGS) {
     c->quad_target = value;
     if (c->ws_ready) return build_quad_units(c);
+  } else if (knob == AFQMC_TUNE_QUAD_RGROUP) {
+    c->quad_rgroup = value;
+  } else if (knob == AFQMC_TUNE_TILE_FB) {
+    c->tile_fb = value;
+  } else if (knob == AFQMC_TUNE_TILE_VHS) {
+    c->tile_vhs = value;
+  } else if (knob == AFQMC_TUNE_TILE_ONE_BODY) {
+    c->tile_ob = value;
   } else {
     c->exx_split = value;
   }

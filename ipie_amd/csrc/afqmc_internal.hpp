@@ -146,7 +146,7 @@ hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream);
 
 // C = A B^T on split-BF16 planes: A 4 planes [m][lda] (re hi, re lo, im hi,
 // im lo), B 2 planes [nn][ldb] (hi, lo), K = kchunks * 32 (zero padded).
-enum { BF16_EPI_PARTIAL = 0, BF16_EPI_VHS = 1, BF16_EPI_QDOT = 2, BF16_EPI_ADD = 3 };
+enum { BF16_EPI_PARTIAL = 0, BF16_EPI_VHS = 1, BF16_EPI_QDOT = 2, BF16_EPI_ADD = 3, BF16_EPI_VATOM = 4 };
 constexpr int BF16_FB_SPLIT = 8;       // K ranges per spin of the split-BF16 force bias (default)
 constexpr int QUAD_TARGET_WGS = 1536;  // workgroups the quadratic-form column units aim for (default)
 struct Bf16GemmArgs {
@@ -187,8 +187,23 @@ struct Bf16GemmArgs {
   int64_t tp_plane;
   int tp_rows, tp_off, tp_ld;
   float* out32;      // VHS: when set, the output as complex FP32 [m][N][N] (the accumulators' precision)
+  // VATOM: columns in VHS atom order (vhs_atom_count); out16[row atom_stride + atom 1024
+  // + plane 256 + r 16 + c] = the split planes (re_hi, re_lo, im_hi, im_lo) of
+  // i scale C[row][col], col = atom 256 + r 16 + c, rounded to FP32 first
+  uint16_t* out16;
+  int64_t atom_stride;
+  // workgroup tile shape 0 .. BF16_TILES - 1 (bf16_tile_dims); the quadratic form
+  // (QDOT, unit tables) and the single product take shape 0
+  int tile;
+  // with a unit table: consecutive workgroups cover rgroup row blocks x units
+  // (row blocks fastest), group after group of row blocks; 0 or >= the row-block
+  // count: all row blocks x units
+  int rgroup;
 };
 hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream);
+constexpr int BF16_TILES = 4;
+// rows x columns of a workgroup tile shape (false for an unknown shape)
+bool bf16_tile_dims(int tile, int* rows, int* cols);
 
 // ---------------------------------------------------------------- exchange as g^T Wq g
 constexpr double QUAD_BUDGET_BYTES = 24e9;  // largest Wq per spin held on the device
@@ -218,7 +233,10 @@ struct TaylorBf16Args {
   int nwalkers, wcap, nbasis, ncol, rpad, kpad;
   const uint16_t* tin;      // [4][wcap][rpad][kpad] planes of out_{k-1}^T
   uint16_t* tout;           // same layout for out_k^T, or null (last term)
-  const uint16_t* vplanes;  // tiled split planes of VHS (launch_split_vhs_tiled)
+  const uint16_t* vatoms;   // split planes of the upper VHS atoms of every walker ([W][NAT][4][16][16])
+  const uint16_t* zatom;    // one all-zero atom (4 planes), read for atoms outside the matrix
+  int natom_side;           // NA = ceil(N / 16) atoms per side
+  int64_t atom_stride;      // bf16 values per walker in vatoms (NAT 1024)
   const double* init;       // [W][N][ncol] complex: the series' input (read by the last term)
   double* q;                // [W][N][ncol] complex: init + sum of the terms (written by the last term)
   float* dacc;              // [W][N][ncol] complex FP32: sum of the terms so far
@@ -237,10 +255,19 @@ hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream);
 constexpr int TAYLOR_SPLIT_TERMS = 3;
 hipError_t launch_split_transpose(const double* phi, int nwalkers, int wcap, int N, int ncol, int rpad, int Kp,
                                   uint16_t* planes, hipStream_t stream);
-// VHS [W][N][N] complex FP32 -> the Taylor kernel's tiled split planes (W x
-// vhs_tiled_rows(N) x Kp x 4 bf16 values, zero padding)
-hipError_t launch_split_vhs_tiled(const float* x, int nwalkers, int N, int Kp, uint16_t* planes, hipStream_t stream);
-int64_t vhs_tiled_rows(int N);
+// VHS of the split-BF16 path in 16 x 16 atoms: atom (a, b), a <= b < NA = ceil(N / 16),
+// holds VHS[16 a + r][16 b + c] (the diagonal atoms both triangles, zero outside
+// N x N), listed by a, then b: index a NA - a (a - 1) / 2 + b - a.  VHS is symmetric,
+// so the atoms below the diagonal are these, transposed.
+inline int vhs_atom_side(int N) { return (N + 15) / 16; }
+inline int64_t vhs_atom_count(int N) {
+  const int64_t na = vhs_atom_side(N);
+  return na * (na + 1) / 2;
+}
+// packed Cholesky [N(N+1)/2, X] -> 2 bf16 planes [vhs_atom_count(N) 256, Xp] in atom order
+// (row atom 256 + r 16 + c holds L[(16 a + r, 16 b + c)], zero outside N x N)
+hipError_t launch_split_chol_atoms(const double* chol_packed, int N, int X, int Xp, uint16_t* planes,
+                                   hipStream_t stream);
 // D = M - I (M real [N][N]) -> 2 bf16 planes [N][Kp] (zero padding)
 hipError_t launch_split_eye(const double* m, int N, int Kp, uint16_t* planes, hipStream_t stream);
 int taylor_rows_padded(int ncol);

@@ -11,7 +11,9 @@
 //                         control variate
 //   bf16_gemm_kernel      force bias (ipie/propagation/force_bias.py:79-121) and
 //                         VHS (ipie/propagation/phaseless_generic.py:67-97)
-//   taylor_bf16_kernel    one term of the Taylor series of exp(VHS) phi
+//   taylor_bf16_kernel    one term of the Taylor series of exp(VHS) phi, VHS read as
+//                         16 x 16 atoms of its upper triangle (VHS epilogue of
+//                         bf16_gemm_kernel), mirrors by transposing LDS loads
 //                         (ipie/propagation/operations.py:58-117)
 //
 // Operand planes are bf16 [rows, Kp] with Kp = K rounded up to 32 and zero
@@ -36,8 +38,23 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 // struct copy lowers to a memcpy that pins the staging arrays to scratch)
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
-// 8 waves in a BX_GR x BX_GC grid, each owning BX_MR x BX_MC tiles of 16 x 16
-constexpr int BX_GR = 4, BX_GC = 2, BX_MR = 2, BX_MC = 5;
+// Workgroup tile of the BF16 GEMM: GR x GC waves, each owning MR x MC output
+// tiles of 16 x 16, so R = 16 GR MR rows by C = 16 GC MC columns.
+template <int GR_, int GC_, int MR_, int MC_>
+struct Tile {
+  static constexpr int GR = GR_, GC = GC_, MR = MR_, MC = MC_;
+  static constexpr int R = 16 * GR * MR, C = 16 * GC * MC, THREADS = 64 * GR * GC;
+};
+// The shapes bf16_gemm_kernel is built for (afqmc_tune_set picks one per
+// contraction; the exchange and the quadratic form use shape 0).  All have 8
+// waves and three LDS stages of the 3-product form within 160 KiB.
+using Tile0 = Tile<4, 2, 2, 5>;  // 128 x 160
+using Tile1 = Tile<4, 2, 2, 4>;  // 128 x 128
+using Tile2 = Tile<8, 1, 1, 5>;  // 128 x 80
+using Tile3 = Tile<4, 2, 1, 5>;  // 64 x 160
+
+// 8 waves in a BX_GR x BX_GC grid, each owning BX_MR x BX_MC tiles of 16 x 16 (shape 0)
+constexpr int BX_GR = Tile0::GR, BX_GC = Tile0::GC, BX_MR = Tile0::MR, BX_MC = Tile0::MC;
 constexpr int BX_R = 16 * BX_GR * BX_MR;  // rows of a workgroup tile (walker orbitals)
 constexpr int BX_C = 16 * BX_GC * BX_MC;  // columns (Cholesky-vector orbitals)
 constexpr int BX_THREADS = 64 * BX_GR * BX_GC;
@@ -50,19 +67,19 @@ constexpr size_t BX_LDS_MAX = 106496;  // LDS of a workgroup: two stages of the 
 // re_hi, im_hi (physical planes 0 and 2) and B plane hi.  An LDS stage holds
 // KS sub-chunks of 32 K values (the 1-product form uses two, halving the
 // barriers per unit of MFMA work; the 3-product stage already fills the LDS).
-template <bool SPLIT>
+template <bool SPLIT, class TL = Tile0>
 struct Bx {
   static constexpr int NA = SPLIT ? 4 : 2, NB = SPLIT ? 2 : 1;
   static constexpr int KS = SPLIT ? 1 : 2;
-  static constexpr int A_VEC = NA * BX_R * 4, B_VEC = NB * BX_C * 4;  // per sub-chunk
+  static constexpr int A_VEC = NA * TL::R * 4, B_VEC = NB * TL::C * 4;  // per sub-chunk
   static constexpr int SUB_VEC = A_VEC + B_VEC;
   static constexpr int STAGE_VEC = KS * SUB_VEC;
-  static constexpr int A_PER = A_VEC / BX_THREADS;
-  static constexpr int B_PER = (B_VEC + BX_THREADS - 1) / BX_THREADS;
+  static constexpr int A_PER = A_VEC / TL::THREADS;
+  static constexpr int B_PER = (B_VEC + TL::THREADS - 1) / TL::THREADS;
   static constexpr size_t STAGES = 2 * 16 * (size_t)STAGE_VEC;  // double-buffered, bytes
   static constexpr size_t STAGES3 = 3 * 16 * (size_t)STAGE_VEC;  // triple-buffered (bf16_gemm_kernel)
-  static_assert(A_VEC % BX_THREADS == 0, "A staging is split evenly over the threads");
-  static_assert((B_VEC % BX_THREADS) % 64 == 0, "the partial B vector round is wave-uniform");
+  static_assert(A_VEC % TL::THREADS == 0, "A staging is split evenly over the threads");
+  static_assert((B_VEC % TL::THREADS) % 64 == 0, "the partial B vector round is wave-uniform");
   static_assert(STAGES <= BX_LDS_MAX, "staging fits the workgroup LDS");
   static_assert(STAGES3 <= 163840, "three stages fit the 160 KiB of LDS");
   __device__ static int a_plane(int p) { return SPLIT ? p : 2 * p; }
@@ -160,56 +177,84 @@ __global__ void split_real_kernel(const double* __restrict__ x, int64_t rows, in
   planes[rows * Kp + idx] = l;
 }
 
+// Packed Cholesky rows [P][X] -> 2 bf16 planes [NAT 256][Xp] in VHS atom order:
+// row t 256 + r 16 + c of atom t = (a, b) holds L[(16 a + r, 16 b + c)] (L is
+// symmetric, so the diagonal atoms' lower halves repeat the packed rows of their
+// mirror), zero outside N x N and for x >= X.
+__global__ void split_chol_atoms_kernel(const double* __restrict__ chol, int N, int X, int Xp, int NA,
+                                        uint16_t* __restrict__ planes) {
+  const int64_t rows = (int64_t)NA * (NA + 1) / 2 * 256;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * Xp) return;
+  const int64_t row = idx / Xp;
+  const int x = (int)(idx % Xp);
+  int t = (int)(row >> 8), a = 0;
+  while (t >= NA - a) {  // atoms of row a: (a, a), ..., (a, NA - 1)
+    t -= NA - a;
+    ++a;
+  }
+  const int b = a + t, rc = (int)(row & 255);
+  const int i = 16 * a + (rc >> 4), j = 16 * b + (rc & 15);
+  uint16_t h = 0, l = 0;
+  if (i < N && j < N && x < X) {
+    const int lo = min(i, j), hi = max(i, j);
+    const int64_t p = (int64_t)lo * N - (int64_t)lo * (lo - 1) / 2 + (hi - lo);
+    split_bf16(chol[p * X + x], h, l);
+  }
+  planes[idx] = h;
+  planes[rows * Xp + idx] = l;
+}
+
 
 // Source offsets (bf16 units, K chunk 0) of the B vectors thread tid stages
 // for Cholesky block xblk; rows past the end are clamped to the last row.
-template <bool SPLIT>
-__device__ __forceinline__ void bx_set_b(int64_t (&b_src)[Bx<SPLIT>::B_PER], int tid, int xblk, int xb, int n,
+template <bool SPLIT, class TL = Tile0>
+__device__ __forceinline__ void bx_set_b(int64_t (&b_src)[(Bx<SPLIT, TL>::B_PER)], int tid, int xblk, int xb, int n,
                                          int64_t brows, int64_t bplane, int Kp) {
 #pragma unroll
-  for (int i = 0; i < Bx<SPLIT>::B_PER; ++i) {
-    const int v = (tid + i * BX_THREADS) % Bx<SPLIT>::B_VEC;  // threads past the end load a duplicate, never stored
-    const int p = v / (BX_C * 4), rem = v % (BX_C * 4);
+  for (int i = 0; i < Bx<SPLIT, TL>::B_PER; ++i) {
+    const int v = (tid + i * TL::THREADS) % Bx<SPLIT, TL>::B_VEC;  // threads past the end load a duplicate, never stored
+    const int p = v / (TL::C * 4), rem = v % (TL::C * 4);
     int64_t r = (int64_t)xblk * xb * n + rem / 4;
     if (r > brows - 1) r = brows - 1;
-    b_src[i] = Bx<SPLIT>::b_plane(p) * bplane + r * Kp + (rem % 4) * 8;
+    b_src[i] = Bx<SPLIT, TL>::b_plane(p) * bplane + r * Kp + (rem % 4) * 8;
   }
 }
 
 // Sub-chunks c32 .. c32 + nsub - 1 (32 K values each) into registers.
-template <bool SPLIT>
-__device__ __forceinline__ void bx_fetch(v4u (&pa)[Bx<SPLIT>::KS][Bx<SPLIT>::A_PER],
-                                         v4u (&pb)[Bx<SPLIT>::KS][Bx<SPLIT>::B_PER], const uint16_t* gp,
-                                         const uint16_t* lp, const int64_t (&a_src)[Bx<SPLIT>::A_PER],
-                                         const int64_t (&b_src)[Bx<SPLIT>::B_PER], int c32, int nsub) {
+template <bool SPLIT, class TL = Tile0>
+__device__ __forceinline__ void bx_fetch(v4u (&pa)[(Bx<SPLIT, TL>::KS)][(Bx<SPLIT, TL>::A_PER)],
+                                         v4u (&pb)[(Bx<SPLIT, TL>::KS)][(Bx<SPLIT, TL>::B_PER)], const uint16_t* gp,
+                                         const uint16_t* lp, const int64_t (&a_src)[(Bx<SPLIT, TL>::A_PER)],
+                                         const int64_t (&b_src)[(Bx<SPLIT, TL>::B_PER)], int c32, int nsub) {
 #pragma unroll
-  for (int sc = 0; sc < Bx<SPLIT>::KS; ++sc) {
+  for (int sc = 0; sc < Bx<SPLIT, TL>::KS; ++sc) {
     if (sc >= nsub) break;
     const int64_t k0 = (int64_t)(c32 + sc) * BX_KC;
 #pragma unroll
-    for (int i = 0; i < Bx<SPLIT>::A_PER; ++i) pa[sc][i] = *reinterpret_cast<const v4u*>(gp + a_src[i] + k0);
+    for (int i = 0; i < Bx<SPLIT, TL>::A_PER; ++i) pa[sc][i] = *reinterpret_cast<const v4u*>(gp + a_src[i] + k0);
 #pragma unroll
-    for (int i = 0; i < Bx<SPLIT>::B_PER; ++i) pb[sc][i] = *reinterpret_cast<const v4u*>(lp + b_src[i] + k0);
+    for (int i = 0; i < Bx<SPLIT, TL>::B_PER; ++i) pb[sc][i] = *reinterpret_cast<const v4u*>(lp + b_src[i] + k0);
   }
 }
 
-template <bool SPLIT>
-__device__ __forceinline__ void bx_stash(uint4* lds_u4, const v4u (&pa)[Bx<SPLIT>::KS][Bx<SPLIT>::A_PER],
-                                         const v4u (&pb)[Bx<SPLIT>::KS][Bx<SPLIT>::B_PER], int tid, int buf,
+template <bool SPLIT, class TL = Tile0>
+__device__ __forceinline__ void bx_stash(uint4* lds_u4, const v4u (&pa)[(Bx<SPLIT, TL>::KS)][(Bx<SPLIT, TL>::A_PER)],
+                                         const v4u (&pb)[(Bx<SPLIT, TL>::KS)][(Bx<SPLIT, TL>::B_PER)], int tid, int buf,
                                          int nsub) {
 #pragma unroll
-  for (int sc = 0; sc < Bx<SPLIT>::KS; ++sc) {
+  for (int sc = 0; sc < Bx<SPLIT, TL>::KS; ++sc) {
     if (sc >= nsub) break;
-    v4u* base = reinterpret_cast<v4u*>(lds_u4) + buf * Bx<SPLIT>::STAGE_VEC + sc * Bx<SPLIT>::SUB_VEC;
+    v4u* base = reinterpret_cast<v4u*>(lds_u4) + buf * Bx<SPLIT, TL>::STAGE_VEC + sc * Bx<SPLIT, TL>::SUB_VEC;
 #pragma unroll
-    for (int i = 0; i < Bx<SPLIT>::A_PER; ++i) {
-      const int v = tid + i * BX_THREADS;
+    for (int i = 0; i < Bx<SPLIT, TL>::A_PER; ++i) {
+      const int v = tid + i * TL::THREADS;
       base[lds_vec(v >> 2, v & 3)] = pa[sc][i];
     }
 #pragma unroll
-    for (int i = 0; i < Bx<SPLIT>::B_PER; ++i) {
-      const int v = tid + i * BX_THREADS;
-      if (v < Bx<SPLIT>::B_VEC) base[Bx<SPLIT>::A_VEC + lds_vec(v >> 2, v & 3)] = pb[sc][i];
+    for (int i = 0; i < Bx<SPLIT, TL>::B_PER; ++i) {
+      const int v = tid + i * TL::THREADS;
+      if (v < Bx<SPLIT, TL>::B_VEC) base[Bx<SPLIT, TL>::A_VEC + lds_vec(v >> 2, v & 3)] = pb[sc][i];
     }
   }
 }
@@ -223,15 +268,15 @@ __device__ __forceinline__ void bx_stash(uint4* lds_u4, const v4u (&pa)[Bx<SPLIT
 // share of the A and B vectors with the same instruction count (the last
 // instruction of a share may have idle lanes), so a wait on vmcnt can name
 // the loads of a later stage that may stay in flight.
-template <bool SPLIT>
+template <bool SPLIT, class TL = Tile0>
 struct Dl {
-  static constexpr int WAVES = BX_THREADS / 64;
-  static constexpr int A_WAVE = Bx<SPLIT>::A_VEC / WAVES, B_WAVE = Bx<SPLIT>::B_VEC / WAVES;
+  static constexpr int WAVES = TL::THREADS / 64;
+  static constexpr int A_WAVE = Bx<SPLIT, TL>::A_VEC / WAVES, B_WAVE = Bx<SPLIT, TL>::B_VEC / WAVES;
   static constexpr int A_INS = (A_WAVE + 63) / 64, B_INS = (B_WAVE + 63) / 64;
   static constexpr int INS_SUB = A_INS + B_INS;  // load instructions per sub-chunk
-  static_assert(Bx<SPLIT>::A_VEC % WAVES == 0 && Bx<SPLIT>::B_VEC % WAVES == 0, "equal shares per wave");
+  static_assert(Bx<SPLIT, TL>::A_VEC % WAVES == 0 && Bx<SPLIT, TL>::B_VEC % WAVES == 0, "equal shares per wave");
   static_assert(A_WAVE % 64 == 0, "A shares are whole instructions");
-  static_assert(Bx<SPLIT>::KS * INS_SUB <= 63, "vmcnt counts the loads of one stage");
+  static_assert(Bx<SPLIT, TL>::KS * INS_SUB <= 63, "vmcnt counts the loads of one stage");
 };
 
 template <int N>
@@ -240,39 +285,39 @@ __device__ __forceinline__ void wait_vm() {
 }
 
 // source offsets (in bf16 elements, K offset 0) of this lane's slots
-template <bool SPLIT>
-__device__ __forceinline__ void dl_sources(int64_t (&a_src)[Dl<SPLIT>::A_INS], int64_t (&b_src)[Dl<SPLIT>::B_INS],
+template <bool SPLIT, class TL = Tile0>
+__device__ __forceinline__ void dl_sources(int64_t (&a_src)[(Dl<SPLIT, TL>::A_INS)], int64_t (&b_src)[(Dl<SPLIT, TL>::B_INS)],
                                            const Bf16GemmArgs& a, int rb, int cb, int wave, int lane) {
-  using C = Bx<SPLIT>;
+  using C = Bx<SPLIT, TL>;
 #pragma unroll
-  for (int j = 0; j < Dl<SPLIT>::A_INS; ++j) {
-    const int s = wave * Dl<SPLIT>::A_WAVE + j * 64 + lane;
+  for (int j = 0; j < Dl<SPLIT, TL>::A_INS; ++j) {
+    const int s = wave * Dl<SPLIT, TL>::A_WAVE + j * 64 + lane;
     const int row = s >> 2, seg = (s & 3) ^ lds_swz(row);
-    const int p = row / BX_R;
-    int64_t r = (int64_t)rb * BX_R + row % BX_R;
+    const int p = row / TL::R;
+    int64_t r = (int64_t)rb * TL::R + row % TL::R;
     if (r > a.m - 1) r = a.m - 1;
     if (a.a_gstride > 0) r = (r / a.group) * a.a_gstride + a.a_goff + r % a.group;
     a_src[j] = C::a_plane(p) * a.a_plane + r * a.lda + seg * 8;
   }
 #pragma unroll
-  for (int j = 0; j < Dl<SPLIT>::B_INS; ++j) {
-    int s = wave * Dl<SPLIT>::B_WAVE + j * 64 + lane;
-    if (j * 64 + lane >= Dl<SPLIT>::B_WAVE) s = wave * Dl<SPLIT>::B_WAVE;  // idle lane: any valid address
+  for (int j = 0; j < Dl<SPLIT, TL>::B_INS; ++j) {
+    int s = wave * Dl<SPLIT, TL>::B_WAVE + j * 64 + lane;
+    if (j * 64 + lane >= Dl<SPLIT, TL>::B_WAVE) s = wave * Dl<SPLIT, TL>::B_WAVE;  // idle lane: any valid address
     const int row = s >> 2, seg = (s & 3) ^ lds_swz(row);
-    const int p = row / BX_C;
-    int64_t r = (int64_t)cb * BX_C + row % BX_C;
+    const int p = row / TL::C;
+    int64_t r = (int64_t)cb * TL::C + row % TL::C;
     if (r > a.nn - 1) r = a.nn - 1;
     b_src[j] = C::b_plane(p) * a.b_plane + r * a.ldb + seg * 8;
   }
 }
 
 // issue the loads of sub-chunks c32 .. c32 + nsub - 1 into LDS stage `buf`
-template <bool SPLIT>
+template <bool SPLIT, class TL = Tile0>
 __device__ __forceinline__ void dl_issue(uint4* lds_u4, int buf, int nsub, const uint16_t* ga, const uint16_t* gb,
-                                         const int64_t (&a_src)[Dl<SPLIT>::A_INS],
-                                         const int64_t (&b_src)[Dl<SPLIT>::B_INS], int c32, int wave, int lane) {
-  using C = Bx<SPLIT>;
-  using D = Dl<SPLIT>;
+                                         const int64_t (&a_src)[(Dl<SPLIT, TL>::A_INS)],
+                                         const int64_t (&b_src)[(Dl<SPLIT, TL>::B_INS)], int c32, int wave, int lane) {
+  using C = Bx<SPLIT, TL>;
+  using D = Dl<SPLIT, TL>;
 #pragma unroll
   for (int sc = 0; sc < C::KS; ++sc) {
     if (sc >= nsub) break;
@@ -299,15 +344,15 @@ __device__ __forceinline__ void dl_issue(uint4* lds_u4, int buf, int nsub, const
 __device__ __forceinline__ void dl_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // wait until at most `pend` sub-chunks of direct loads are outstanding
-template <bool SPLIT>
+template <bool SPLIT, class TL = Tile0>
 __device__ __forceinline__ void dl_wait(int pend) {
-  constexpr int I = Dl<SPLIT>::INS_SUB;
+  constexpr int I = Dl<SPLIT, TL>::INS_SUB;
   if (pend >= 2) wait_vm<2 * I>();
   else if (pend == 1) wait_vm<I>();
   else wait_vm<0>();
 }
 
-// One K chunk from LDS stage `stage`: the wave's BX_MR x BX_MC tiles,
+// One K chunk from LDS stage `stage`: the wave's TL::MR x TL::MC tiles,
 // re += Re(A) B and im += Im(A) B, each as hi*hi + hi*lo + lo*hi (SPLIT) or
 // hi*hi.  TR swaps the MFMA operands, so the accumulators hold the transposed
 // tiles: element r of tile (i, j) is C[i 16 + l16][j 16 + 4 q + r] instead of
@@ -316,28 +361,28 @@ __device__ __forceinline__ v4f mfma_ab(v8bf a, v8bf b, v4f c, bool tr) {
   return tr ? mfma_bf16(b, a, c) : mfma_bf16(a, b, c);
 }
 
-template <bool SPLIT, bool TR = false>
+template <bool SPLIT, bool TR = false, class TL = Tile0>
 __device__ __forceinline__ void bx_sub(const uint4* sub, int wr, int wc, int l16, int q,
-                                       v4f (&re)[BX_MR][BX_MC], v4f (&im)[BX_MR][BX_MC]) {
+                                       v4f (&re)[(TL::MR)][(TL::MC)], v4f (&im)[(TL::MR)][(TL::MC)]) {
   const uint16_t* As = reinterpret_cast<const uint16_t*>(sub);
-  const uint16_t* Bs = As + 8 * Bx<SPLIT>::A_VEC;
-  v8bf bh[BX_MC], bl[BX_MC];
+  const uint16_t* Bs = As + 8 * Bx<SPLIT, TL>::A_VEC;
+  v8bf bh[(TL::MC)], bl[(TL::MC)];
 #pragma unroll
-  for (int j = 0; j < BX_MC; ++j) {
-    const int col = (wc * BX_MC + j) * 16 + l16;
+  for (int j = 0; j < TL::MC; ++j) {
+    const int col = (wc * TL::MC + j) * 16 + l16;
     bh[j] = lds_frag(Bs, col, q);
-    if (SPLIT) bl[j] = lds_frag(Bs, BX_C + col, q);
+    if (SPLIT) bl[j] = lds_frag(Bs, TL::C + col, q);
   }
 #pragma unroll
-  for (int i = 0; i < BX_MR; ++i) {
-    const int row = (wr * BX_MR + i) * 16 + l16;
+  for (int i = 0; i < TL::MR; ++i) {
+    const int row = (wr * TL::MR + i) * 16 + l16;
     if (SPLIT) {
       const v8bf arh = lds_frag(As, row, q);
-      const v8bf arl = lds_frag(As, BX_R + row, q);
-      const v8bf aih = lds_frag(As, 2 * BX_R + row, q);
-      const v8bf ail = lds_frag(As, 3 * BX_R + row, q);
+      const v8bf arl = lds_frag(As, TL::R + row, q);
+      const v8bf aih = lds_frag(As, 2 * TL::R + row, q);
+      const v8bf ail = lds_frag(As, 3 * TL::R + row, q);
 #pragma unroll
-      for (int j = 0; j < BX_MC; ++j) {
+      for (int j = 0; j < TL::MC; ++j) {
         re[i][j] = mfma_ab(arh, bh[j], re[i][j], TR);
         re[i][j] = mfma_ab(arh, bl[j], re[i][j], TR);
         re[i][j] = mfma_ab(arl, bh[j], re[i][j], TR);
@@ -347,9 +392,9 @@ __device__ __forceinline__ void bx_sub(const uint4* sub, int wr, int wc, int l16
       }
     } else {
       const v8bf arh = lds_frag(As, row, q);
-      const v8bf aih = lds_frag(As, BX_R + row, q);
+      const v8bf aih = lds_frag(As, TL::R + row, q);
 #pragma unroll
-      for (int j = 0; j < BX_MC; ++j) {
+      for (int j = 0; j < TL::MC; ++j) {
         re[i][j] = mfma_ab(arh, bh[j], re[i][j], TR);
         im[i][j] = mfma_ab(aih, bh[j], im[i][j], TR);
       }
@@ -358,13 +403,13 @@ __device__ __forceinline__ void bx_sub(const uint4* sub, int wr, int wc, int l16
 }
 
 // The nsub sub-chunks of LDS stage `stage`.
-template <bool SPLIT, bool TR = false>
+template <bool SPLIT, bool TR = false, class TL = Tile0>
 __device__ __forceinline__ void bx_chunk(const uint4* stage, int wr, int wc, int l16, int q,
-                                         v4f (&re)[BX_MR][BX_MC], v4f (&im)[BX_MR][BX_MC], int nsub) {
+                                         v4f (&re)[(TL::MR)][(TL::MC)], v4f (&im)[(TL::MR)][(TL::MC)], int nsub) {
 #pragma unroll
-  for (int sc = 0; sc < Bx<SPLIT>::KS; ++sc) {
+  for (int sc = 0; sc < Bx<SPLIT, TL>::KS; ++sc) {
     if (sc >= nsub) break;
-    bx_sub<SPLIT, TR>(stage + sc * Bx<SPLIT>::SUB_VEC, wr, wc, l16, q, re, im);
+    bx_sub<SPLIT, TR, TL>(stage + sc * Bx<SPLIT, TL>::SUB_VEC, wr, wc, l16, q, re, im);
   }
 }
 
@@ -523,6 +568,8 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
 //   BF16_EPI_VHS      out[w][i][j] = out[w][j][i] = i scale C[w][p],
 //                     (i, j) = (tri_i[p], tri_j[p])              (VHS); with
 //                     out32, complex FP32 and the upper triangle i <= j only
+//   BF16_EPI_VATOM    the split planes of i scale C[w][col], rounded to FP32, at out16[w atom_stride
+//                     + atom 1024 + plane 256 + r 16 + c] for col = atom 256 + r 16 + c (VHS atoms)
 //   BF16_EPI_ADD      out[w][col][c] = base[w][col][c] + scale C[r][col], r = w group + c,
 //                     optionally with the split planes of the sum, transposed (one body)
 //   BF16_EPI_QDOT     out[((cb ksplit + z) BX_GC + wc) m + row] = sum over the
@@ -531,16 +578,25 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
 // With a unit table (block-triangular B), the workgroup's column unit u gives
 // (column tile, K chunk range) = (units[u].x, [units[u].y, units[u].z)) and the
 // QDOT slot is u BX_GC + wc; otherwise column tile cb and split z of ksplit.
-// TR (transposed accumulators, see bx_sub) is used with the ADD epilogue only.
+// TR (transposed accumulators, see bx_sub) is used with the ADD and VATOM epilogues only.
 // Workgroups that run together share the B tile (row blocks vary fastest).
-template <bool SPLIT, bool TR, bool DL>
-__global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
-  using C = Bx<SPLIT>;
+template <bool SPLIT, bool TR, bool DL, class TL>
+__global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
+  using C = Bx<SPLIT, TL>;
   extern __shared__ uint4 lds_u4[];
-  const int mb = (int)ceil_div(a.m, BX_R), nb = (int)ceil_div(a.nn, BX_C);
+  const int mb = (int)ceil_div(a.m, TL::R), nb = (int)ceil_div(a.nn, TL::C);
   const int ncu = a.units ? a.nunits : nb;  // column units of the grid
   const int lin = xcd_remap(blockIdx.x, gridDim.x);
-  const int rb = lin % mb, cu = (lin / mb) % ncu, z = lin / mb / ncu;
+  int rb = lin % mb, cu = (lin / mb) % ncu, z = lin / mb / ncu;
+  if (a.units && a.rgroup > 0 && a.rgroup < mb) {
+    // groups of rgroup row blocks, each over every unit (the workgroups that run
+    // together on one XCD: rgroup row blocks x several units of one K block)
+    const int R = a.rgroup, rem = lin % (ncu * R);
+    rb = (lin / (ncu * R)) * R + rem % R;
+    cu = rem / R;
+    z = 0;
+    if (rb >= mb) return;
+  }
   if (z >= (a.units ? 1 : a.ksplit)) return;
   int cb = cu, c0, c1;
   if (a.units) {
@@ -554,14 +610,14 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
   }
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
-  const int wr = wave % BX_GR, wc = wave / BX_GR;
+  const int wr = wave % TL::GR, wc = wave / TL::GR;
   // staged rows past the end are clamped to the last row (their outputs are not written)
   int64_t a_src[C::A_PER], b_src[C::B_PER];
 #pragma unroll
   for (int i = 0; i < C::A_PER; ++i) {
-    const int v = tid + i * BX_THREADS;
-    const int p = v / (BX_R * 4), rem = v % (BX_R * 4);
-    int64_t r = (int64_t)rb * BX_R + rem / 4;
+    const int v = tid + i * TL::THREADS;
+    const int p = v / (TL::R * 4), rem = v % (TL::R * 4);
+    int64_t r = (int64_t)rb * TL::R + rem / 4;
     if (r > a.m - 1) r = a.m - 1;
     if (a.a_gstride > 0) r = (r / a.group) * a.a_gstride + a.a_goff + r % a.group;
     a_src[i] = C::a_plane(p) * a.a_plane + r * a.lda + (rem % 4) * 8;
@@ -569,17 +625,17 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
   {
 #pragma unroll
     for (int i = 0; i < C::B_PER; ++i) {
-      const int v = (tid + i * BX_THREADS) % C::B_VEC;
-      const int p = v / (BX_C * 4), rem = v % (BX_C * 4);
-      int64_t r = (int64_t)cb * BX_C + rem / 4;
+      const int v = (tid + i * TL::THREADS) % C::B_VEC;
+      const int p = v / (TL::C * 4), rem = v % (TL::C * 4);
+      int64_t r = (int64_t)cb * TL::C + rem / 4;
       if (r > a.nn - 1) r = a.nn - 1;
       b_src[i] = C::b_plane(p) * a.b_plane + r * a.ldb + (rem % 4) * 8;
     }
-    v4f re[BX_MR][BX_MC], im[BX_MR][BX_MC];
+    v4f re[(TL::MR)][(TL::MC)], im[(TL::MR)][(TL::MC)];
 #pragma unroll
-    for (int i = 0; i < BX_MR; ++i)
+    for (int i = 0; i < TL::MR; ++i)
 #pragma unroll
-      for (int j = 0; j < BX_MC; ++j) {
+      for (int j = 0; j < TL::MC; ++j) {
         re[i][j] = zero4f();
         im[i][j] = zero4f();
       }
@@ -591,43 +647,43 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
     if constexpr (DL) {
       // direct loads: stage t + 2 is issued before stage t is computed; the
       // wait after it leaves stage t + 2 in flight and lands stage t + 1
-      int64_t da[Dl<SPLIT>::A_INS], db[Dl<SPLIT>::B_INS];
-      dl_sources<SPLIT>(da, db, a, rb, cb, wave, lane);
-      if (nst > 0) dl_issue<SPLIT>(lds_u4, 0, nsub_of(0), a.a, a.b, da, db, c0, wave, lane);
-      if (nst > 1) dl_issue<SPLIT>(lds_u4, 1, nsub_of(1), a.a, a.b, da, db, c0 + C::KS, wave, lane);
-      dl_wait<SPLIT>(nst > 1 ? nsub_of(1) : 0);
+      int64_t da[(Dl<SPLIT, TL>::A_INS)], db[(Dl<SPLIT, TL>::B_INS)];
+      dl_sources<SPLIT, TL>(da, db, a, rb, cb, wave, lane);
+      if (nst > 0) dl_issue<SPLIT, TL>(lds_u4, 0, nsub_of(0), a.a, a.b, da, db, c0, wave, lane);
+      if (nst > 1) dl_issue<SPLIT, TL>(lds_u4, 1, nsub_of(1), a.a, a.b, da, db, c0 + C::KS, wave, lane);
+      dl_wait<SPLIT, TL>(nst > 1 ? nsub_of(1) : 0);
       dl_barrier();
       int buf = 0;
       for (int t = 0; t < nst; ++t) {
         int pend = 0;
         if (t + 2 < nst) {
-          dl_issue<SPLIT>(lds_u4, buf == 0 ? 2 : buf - 1, nsub_of(t + 2), a.a, a.b, da, db, c0 + (t + 2) * C::KS,
+          dl_issue<SPLIT, TL>(lds_u4, buf == 0 ? 2 : buf - 1, nsub_of(t + 2), a.a, a.b, da, db, c0 + (t + 2) * C::KS,
                           wave, lane);
           pend = nsub_of(t + 2);
         }
-        bx_chunk<SPLIT, TR>(lds_u4 + buf * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub_of(t));
-        dl_wait<SPLIT>(pend);
+        bx_chunk<SPLIT, TR, TL>(lds_u4 + buf * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub_of(t));
+        dl_wait<SPLIT, TL>(pend);
         dl_barrier();
         buf = buf == 2 ? 0 : buf + 1;
       }
     } else {
     v4u pa[C::KS][C::A_PER], pb[C::KS][C::B_PER];
     if (nst > 0) {
-      bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0, nsub_of(0));
-      bx_stash<SPLIT>(lds_u4, pa, pb, tid, 0, nsub_of(0));
+      bx_fetch<SPLIT, TL>(pa, pb, a.a, a.b, a_src, b_src, c0, nsub_of(0));
+      bx_stash<SPLIT, TL>(lds_u4, pa, pb, tid, 0, nsub_of(0));
     }
     if (nst > 1) {
-      bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0 + C::KS, nsub_of(1));
-      bx_stash<SPLIT>(lds_u4, pa, pb, tid, 1, nsub_of(1));
+      bx_fetch<SPLIT, TL>(pa, pb, a.a, a.b, a_src, b_src, c0 + C::KS, nsub_of(1));
+      bx_stash<SPLIT, TL>(lds_u4, pa, pb, tid, 1, nsub_of(1));
     }
-    if (nst > 2) bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0 + 2 * C::KS, nsub_of(2));
+    if (nst > 2) bx_fetch<SPLIT, TL>(pa, pb, a.a, a.b, a_src, b_src, c0 + 2 * C::KS, nsub_of(2));
     __syncthreads();
     int buf = 0;
     for (int t = 0; t < nst; ++t) {
-      bx_chunk<SPLIT, TR>(lds_u4 + buf * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub_of(t));
+      bx_chunk<SPLIT, TR, TL>(lds_u4 + buf * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub_of(t));
       const int nbuf = buf == 0 ? 2 : buf - 1;  // (t + 2) % 3: the buffer computed at t - 1
-      if (t + 2 < nst) bx_stash<SPLIT>(lds_u4, pa, pb, tid, nbuf, nsub_of(t + 2));
-      if (t + 3 < nst) bx_fetch<SPLIT>(pa, pb, a.a, a.b, a_src, b_src, c0 + (t + 3) * C::KS, nsub_of(t + 3));
+      if (t + 2 < nst) bx_stash<SPLIT, TL>(lds_u4, pa, pb, tid, nbuf, nsub_of(t + 2));
+      if (t + 3 < nst) bx_fetch<SPLIT, TL>(pa, pb, a.a, a.b, a_src, b_src, c0 + (t + 3) * C::KS, nsub_of(t + 3));
       __syncthreads();
       buf = buf == 2 ? 0 : buf + 1;
     }
@@ -637,16 +693,16 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
       // of a row; lane l16 = 4 i + r keeps the sum of row (i, r)
       double myr = 0.0, myi = 0.0;
 #pragma unroll
-      for (int i = 0; i < BX_MR; ++i)
+      for (int i = 0; i < TL::MR; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int64_t row = (int64_t)rb * BX_R + (wr * BX_MR + i) * 16 + 4 * q + r;
+          int64_t row = (int64_t)rb * TL::R + (wr * TL::MR + i) * 16 + 4 * q + r;
           if (row > a.m - 1) row = a.m - 1;  // clamped rows are computed, never stored
           const double* drow = a.dg + 2 * (a.walker_of ? (int64_t)a.walker_of[row] : row) * a.dstride;
           double er = 0.0, ei = 0.0;
 #pragma unroll
-          for (int j = 0; j < BX_MC; ++j) {
-            const int64_t col = (int64_t)cb * BX_C + (wc * BX_MC + j) * 16 + l16;
+          for (int j = 0; j < TL::MC; ++j) {
+            const int64_t col = (int64_t)cb * TL::C + (wc * TL::MC + j) * 16 + l16;
             const int io = (int)(col / a.dkp), mm = (int)(col % a.dkp);
             if (col < a.nn && mm < a.dn) {
               const int64_t off = (int64_t)io * a.dn + mm;
@@ -667,24 +723,24 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
             myi = ei;
           }
         }
-      if (l16 < 4 * BX_MR) {
+      if (l16 < 4 * TL::MR) {
         const int i = l16 >> 2, r = l16 & 3;
-        const int64_t row = (int64_t)rb * BX_R + (wr * BX_MR + i) * 16 + 4 * q + r;
-        const int64_t slot = (a.units ? (int64_t)cu : (int64_t)cb * a.ksplit + z) * BX_GC + wc;
+        const int64_t row = (int64_t)rb * TL::R + (wr * TL::MR + i) * 16 + 4 * q + r;
+        const int64_t slot = (a.units ? (int64_t)cu : (int64_t)cb * a.ksplit + z) * TL::GC + wc;
         if (row < a.m) st(a.out, slot * a.m + row, cmake(a.scale * myr, a.scale * myi));
       }
     } else if (a.epilogue == BF16_EPI_ADD) {
       // transposed accumulators (TR): lane l16 walks the rows, so the FP64
       // reads and writes of 16 lanes are consecutive columns c of one walker
 #pragma unroll
-      for (int i = 0; i < BX_MR; ++i) {
-        const int64_t row = (int64_t)rb * BX_R + (wr * BX_MR + i) * 16 + l16;
+      for (int i = 0; i < TL::MR; ++i) {
+        const int64_t row = (int64_t)rb * TL::R + (wr * TL::MR + i) * 16 + l16;
         if (row >= a.m) continue;
         const int64_t w = row / a.group;
         const int c = (int)(row % a.group);
 #pragma unroll
-        for (int j = 0; j < BX_MC; ++j) {
-          const int64_t col0 = (int64_t)cb * BX_C + (wc * BX_MC + j) * 16 + 4 * q;
+        for (int j = 0; j < TL::MC; ++j) {
+          const int64_t col0 = (int64_t)cb * TL::C + (wc * TL::MC + j) * 16 + 4 * q;
           uint16_t h[4][4];  // [plane][r]: re_hi, re_lo, im_hi, im_lo
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
@@ -714,11 +770,37 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
           }
         }
       }
+    } else if (a.epilogue == BF16_EPI_VATOM) {
+      // transposed accumulators (TR): each lane holds 4 consecutive columns of one
+      // row, so each plane takes one 8-byte store (4 entries of an atom row)
+#pragma unroll
+      for (int i = 0; i < (TL::MR); ++i) {
+        const int64_t row = (int64_t)rb * TL::R + (wr * TL::MR + i) * 16 + l16;
+        if (!TR || row >= a.m) continue;
+        uint16_t* drow = a.out16 + row * a.atom_stride;
+#pragma unroll
+        for (int j = 0; j < (TL::MC); ++j) {
+          const int64_t col0 = (int64_t)cb * TL::C + (wc * TL::MC + j) * 16 + 4 * q;  // nn % 256 == 0: all in range
+          if (col0 >= a.nn) continue;
+          uint16_t h[4][4];  // [plane][r]: re_hi, re_lo, im_hi, im_lo
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            // i scale (re + i im) = scale (-im + i re), rounded to FP32 as the accumulators hold it
+            const float vre = (float)(-(a.scale * im[i][j][r])), vim = (float)(a.scale * re[i][j][r]);
+            split_bf16((double)vre, h[0][r], h[1][r]);
+            split_bf16((double)vim, h[2][r], h[3][r]);
+          }
+          uint16_t* dst = drow + (col0 >> 8) * 1024 + (col0 & 255);
+#pragma unroll
+          for (int p = 0; p < 4; ++p)
+            *reinterpret_cast<ushort4*>(dst + 256 * p) = make_ushort4(h[p][0], h[p][1], h[p][2], h[p][3]);
+        }
+      }
     } else {
       const int64_t NN = (int64_t)a.nbasis * a.nbasis;
 #pragma unroll
-      for (int j = 0; j < BX_MC; ++j) {
-        const int64_t col = (int64_t)cb * BX_C + (wc * BX_MC + j) * 16 + l16;
+      for (int j = 0; j < TL::MC; ++j) {
+        const int64_t col = (int64_t)cb * TL::C + (wc * TL::MC + j) * 16 + l16;
         if (col >= a.nn) continue;
         int ti = 0, tj = 0;
         if (a.epilogue == BF16_EPI_VHS) {
@@ -726,10 +808,10 @@ __global__ void __launch_bounds__(BX_THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
           tj = a.tri_j[col];
         }
 #pragma unroll
-        for (int i = 0; i < BX_MR; ++i)
+        for (int i = 0; i < TL::MR; ++i)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int64_t row = (int64_t)rb * BX_R + (wr * BX_MR + i) * 16 + 4 * q + r;
+            const int64_t row = (int64_t)rb * TL::R + (wr * TL::MR + i) * 16 + 4 * q + r;
             if (row >= a.m) continue;
             const double vr = a.scale * re[i][j][r], vi = a.scale * im[i][j][r];
             if (a.epilogue == BF16_EPI_VHS) {
@@ -770,11 +852,20 @@ size_t bf16_lds_bytes(int wb, int xb, int nocc) {
 // ---------------------------------------------------------------- Taylor series
 // out_k^T = out_{k-1}^T VHS / k for each walker (VHS is symmetric), complex x
 // complex in split-BF16: A = out_{k-1}^T planes [4][W][rpad][kpad] (row = column
-// of the walker matrix, K = basis index), B = VHS planes [4][W N][kpad] (row =
-// output basis index).  re = Ar Br - Ai Bi, im = Ar Bi + Ai Br, each real
-// product as hi*hi + hi*lo + lo*hi (SPLIT: 12 MFMAs per tile and K chunk) or
-// hi*hi (4 MFMAs; the hi planes only); -Ai comes from sign-flipped fragments.  Workgroup: one walker, 80 rows x 80 output
-// basis indices, 5 waves, wave v owns output column tile v and all 5 row tiles.
+// of the walker matrix, K = basis index), B = the VHS atoms (row = output basis
+// index).  re = Ar Br - Ai Bi, im = Ar Bi + Ai Br, each real product as
+// hi*hi + hi*lo + lo*hi (SPLIT: 12 MFMAs per tile and K chunk) or hi*hi (4
+// MFMAs; the hi planes only); -Ai comes from sign-flipped fragments.
+// Workgroup: one walker, 80 rows x 80 output basis indices, 5 waves, wave v
+// owns output column tile v and all 5 row tiles.
+// B of a K chunk is the 5 x 2 atoms (aj, ak), aj = 5 cb + v, ak = 2 c + h.  Only
+// the upper atoms are stored: an atom with aj <= ak is read as stored (row =
+// output index, 16 K values per row), one with aj > ak is its mirror (ak, aj),
+// whose rows are K values; the MFMA operand of a mirror atom (8 consecutive K
+// of one output index per lane) is read with the transposing LDS load
+// ds_read_b64_tr_b16, two per plane.  Each stored atom is thus read by the
+// workgroups of both its row and its column block, which run together on one
+// XCD, so the second read is served by L2.
 // Epilogue: the terms are summed in FP32 in dacc (the first term writes it,
 // later ones add to it) and the last term writes q = init + dacc + out_k (FP64)
 // with the split planes of q^T (qout); every term but the last writes the
@@ -782,55 +873,33 @@ size_t bf16_lds_bytes(int wb, int xb, int nocc) {
 // buffer capacity wcap; their padding rows and K entries are never written
 // and must be zero.
 constexpr int TY_WAVES = 5, TY_R = 80, TY_C = 80, TY_THREADS = 64 * TY_WAVES;
-constexpr int TY_TILE = 4 * TY_C * BX_KC;  // bf16 values of one VHS tile (4 planes x 80 rows x 32 K)
+static_assert(TY_C == 5 * 16 && BX_KC == 2 * 16, "a workgroup's B chunk is 5 x 2 atoms");
 
-// VHS [W][N][N] complex FP32, upper triangle (n <= k) only, -> split planes in
-// Taylor tiles: tile (w, cb, c) at ((w ncb + cb) nch + c) TY_TILE holds planes
-// re_hi, re_lo, im_hi, im_lo of rows cb 80 .. cb 80 + 79 and K = c 32 .. c 32 + 31,
-// each [80][32], so that a workgroup's K chunk is one contiguous 20 KiB block.
-// Rows >= N and K >= N are zero.  One workgroup per (walker, 32 x 32 block of
-// (n, k)): blocks below the diagonal are read from their mirror above it and
-// transposed through LDS, so reads and writes are coalesced.
-__global__ void __launch_bounds__(256) split_vhs_tiled_kernel(const float2* __restrict__ x, int nwalkers, int N,
-                                                             int Kp, uint16_t* __restrict__ planes) {
-  __shared__ float2 blk[32][33];
-  const int ncb = (int)ceil_div(N, TY_C), nch = Kp / BX_KC;
-  const int rows = ncb * TY_C, nbn = (int)ceil_div(rows, 32), nbk = Kp / 32;
-  int b = blockIdx.x;
-  const int bk = b % nbk;
-  b /= nbk;
-  const int bn = b % nbn, w = b / nbn;
-  if (w >= nwalkers) return;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  const float2* xw = x + (int64_t)w * N * N;
-  // stage the block that holds the values: (bn, bk) itself when bn <= bk, else its mirror (bk, bn)
-  const int r0 = 32 * min(bn, bk), c0 = 32 * max(bn, bk);
-  for (int yy = ty; yy < 32; yy += 8) {
-    const int r = r0 + yy, c = c0 + tx;
-    blk[yy][tx] = (r < N && c < N) ? xw[(int64_t)r * N + c] : make_float2(0.0f, 0.0f);
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+
+// LDS slot (16-byte vector) of half h of row r of an atom plane: the halves of
+// rows 4..11 are swapped, so that 8 or 16 lanes reading consecutive rows with
+// 16-byte loads cover distinct slots modulo 8 and modulo 16 (no bank conflicts
+// with 32 or 64 banks of 4 bytes), and the 16 lanes of a transposing load (4
+// aligned rows x 32 bytes) cover one 128-byte span
+__device__ __forceinline__ int atom_vec(int r, int h) { return r * 2 + (h ^ (((r >> 2) ^ (r >> 3)) & 1)); }
+
+// B fragment of an atom plane at LDS vector offset vb: lane (l16, q) gets 8
+// consecutive K values (8 (q & 1) .. + 7 within the atom) of output index l16.
+// Stored atom: row l16, one 16-byte load.  Mirror atom: column l16 of rows
+// 8 (q & 1) .. + 7, two transposing loads of 4 rows each (lane r' of a 16-lane
+// group supplies row r' / 4, quarter r' % 4; lane i receives element i of each
+// of the 4 rows).
+__device__ __forceinline__ v8bf ty_bfrag(const uint4* lds, int vb, int l16, int q, bool mirror) {
+  if (!mirror) return as_v8bf(reinterpret_cast<const v4u*>(lds)[vb + atom_vec(l16, q & 1)]);
+  v4bf t[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int r = 8 * (q & 1) + 4 * u + (l16 >> 2), qt = l16 & 3;
+    const char* p = reinterpret_cast<const char*>(lds + vb + atom_vec(r, qt >> 1)) + 8 * (qt & 1);
+    t[u] = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((__attribute__((address_space(3))) v4bf*)p);
   }
-  __syncthreads();
-  constexpr int P = TY_C * BX_KC;
-  for (int yy = ty; yy < 32; yy += 8) {
-    const int n = 32 * bn + yy, k = 32 * bk + tx;
-    if (n >= rows) continue;
-    // value (n, k) = VHS[min][max]: in the staged block at (n - r0, k - c0) or its transpose
-    float2 v;
-    if (bn < bk) v = blk[yy][tx];
-    else if (bn > bk) v = blk[tx][yy];
-    else v = yy <= tx ? blk[yy][tx] : blk[tx][yy];
-    uint16_t rh = 0, rl = 0, ih = 0, il = 0;
-    if (n < N && k < N) {
-      split_bf16(v.x, rh, rl);
-      split_bf16(v.y, ih, il);
-    }
-    const int cb = n / TY_C, rr = n % TY_C;
-    const int64_t o = (((int64_t)w * ncb + cb) * nch + bk) * TY_TILE + rr * BX_KC + tx;
-    planes[o] = rh;
-    planes[o + P] = rl;
-    planes[o + 2 * P] = ih;
-    planes[o + 3 * P] = il;
-  }
+  return __builtin_shufflevector(t[0], t[1], 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
 __device__ __forceinline__ v8bf neg_bf16(v8bf x) {
@@ -852,6 +921,25 @@ struct Ty {
   __device__ static int phys(int pi) { return SPLIT ? pi : 2 * pi; }
 };
 
+// 4 consecutive K entries (of which `valid` are < N) of planes 0, pstep, .. of a
+// transposed Taylor operand at offset o
+__device__ __forceinline__ void store_planes_ty(uint16_t* dst, int64_t tplane, int64_t o, const uint16_t (&h)[4][4],
+                                                int pstep, int valid) {
+  if (valid >= 4) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      if (p % pstep == 0)
+        *reinterpret_cast<ushort4*>(dst + p * tplane + o) = make_ushort4(h[p][0], h[p][1], h[p][2], h[p][3]);
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (r < valid)
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          if (p % pstep == 0) dst[p * tplane + o + r] = h[p][r];
+  }
+}
+
 template <bool SPLIT>
 __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args a) {
   using T = Ty<SPLIT>;
@@ -864,21 +952,27 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q = lane >> 4;
   const int64_t tplane = (int64_t)a.wcap * a.rpad * Kp;
   const int nch = Kp / BX_KC, nst = (int)ceil_div(nch, T::KS);
-  // direct global -> LDS staging: slot s of a plane region holds (row s / 4,
+  // direct global -> LDS staging: slot s of an A plane region holds (row s / 4,
   // segment (s % 4) ^ ((row / 4) % 4)), so each wave's loads fill 64
   // consecutive slots; every wave issues NP loads per operand and sub-chunk.
-  // The VHS tiles hold K chunk c of (w, cb) as one contiguous block.
+  // B: vector v of the chunk's B region is vector e = v % 32 of plane pi of
+  // atom slot (sj, sk) (v / 32 = (2 sj + sk) NP + pi), i.e. atom row e / 2,
+  // half (e % 2) ^ swizzle (atom_vec); the atom is (aj, ak) or its mirror.
+  const int NA = a.natom_side;
   const uint16_t* dla[T::NP];
-  const uint16_t* dlb[T::NP];
+  int baj[T::NP], bsk[T::NP], boff[T::NP];
 #pragma unroll
   for (int j = 0; j < T::NP; ++j) {
     const int sl = wave * T::NP * 64 + j * 64 + lane;
     const int row = sl >> 2, seg = (sl & 3) ^ lds_swz_ty(row);
     const int pi = row / TY_R, r = row % TY_R;
     dla[j] = a.tin + T::phys(pi) * tplane + ((int64_t)w * a.rpad + rb * TY_R + r) * Kp + seg * 8;
-    dlb[j] = a.vplanes + (int64_t)(w * ncb + cb) * (Kp / BX_KC) * TY_TILE + T::phys(pi) * (TY_C * BX_KC) + r * BX_KC +
-             seg * 8;
+    const int sp = sl >> 5, e = sl & 31, slot = sp / T::NP, bpi = sp % T::NP, ar = e >> 1;
+    baj[j] = 5 * cb + (slot >> 1);
+    bsk[j] = slot & 1;
+    boff[j] = T::phys(bpi) * 256 + ar * 16 + 8 * ((e & 1) ^ (((ar >> 2) ^ (ar >> 3)) & 1));
   }
+  const uint16_t* wat = a.vatoms + (int64_t)w * a.atom_stride;
   auto dl_issue_ty = [&](int t, int buf) __attribute__((always_inline)) {
     uint4* base = lds_u4 + buf * T::STAGE_VEC;
 #pragma unroll
@@ -890,8 +984,13 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
         __builtin_amdgcn_global_load_lds(
             (const void*)(dla[j] + c * BX_KC),
             (__attribute__((address_space(3))) void*)(base + sc * T::SUB_VEC + wave * T::NP * 64 + j * 64), 16, 0, 0);
+        // atom (aj, ak) as stored, or its mirror; outside the matrix the zero atom
+        const int aj = baj[j], ak = 2 * c + bsk[j];
+        const int lo = min(aj, ak), hi = max(aj, ak);
+        const uint16_t* src = (hi >= NA) ? a.zatom + boff[j]
+                                         : wat + (int64_t)(lo * NA - lo * (lo - 1) / 2 + hi - lo) * 1024 + boff[j];
         __builtin_amdgcn_global_load_lds(
-            (const void*)(dlb[j] + (int64_t)c * TY_TILE),
+            (const void*)src,
             (__attribute__((address_space(3))) void*)(base + sc * T::SUB_VEC + T::SUB_A + wave * T::NP * 64 + j * 64),
             16, 0, 0);
       }
@@ -911,13 +1010,19 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
 #pragma unroll
     for (int sc = 0; sc < T::KS; ++sc) {
       if (t * T::KS + sc >= nch) break;
-      const uint16_t* As = reinterpret_cast<const uint16_t*>(lds_u4 + (t & 1) * T::STAGE_VEC + sc * T::SUB_VEC);
-      const uint16_t* Bs = As + 8 * T::SUB_A;
-      const int bcol = wave * 16 + l16;
+      const uint4* stage = lds_u4 + (t & 1) * T::STAGE_VEC + sc * T::SUB_VEC;
+      const uint16_t* As = reinterpret_cast<const uint16_t*>(stage);
+      const uint4* Bq = stage + T::SUB_A;
+      // this wave's output atom aj against K atom ak of its lane group: stored or mirror
+      const int c = t * T::KS + sc, bslot = wave * 2 + (q >> 1);
+      const int aj = 5 * cb + wave, ak = 2 * c + (q >> 1);
+      const bool mirror = aj > ak && aj < NA;
       // operands swapped in every MFMA: the accumulators hold the transposed tiles
       if (SPLIT) {
-        const v8bf brh = lds_frag_ty(Bs, bcol, q), brl = lds_frag_ty(Bs, TY_C + bcol, q);
-        const v8bf bih = lds_frag_ty(Bs, 2 * TY_C + bcol, q), bil = lds_frag_ty(Bs, 3 * TY_C + bcol, q);
+        const v8bf brh = ty_bfrag(Bq, (bslot * T::NP + 0) * 32, l16, q, mirror);
+        const v8bf brl = ty_bfrag(Bq, (bslot * T::NP + 1) * 32, l16, q, mirror);
+        const v8bf bih = ty_bfrag(Bq, (bslot * T::NP + 2) * 32, l16, q, mirror);
+        const v8bf bil = ty_bfrag(Bq, (bslot * T::NP + 3) * 32, l16, q, mirror);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
           const int arow = i * 16 + l16;
@@ -938,7 +1043,8 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
           im[i] = mfma_bf16(brh, ail, im[i]);
         }
       } else {
-        const v8bf brh = lds_frag_ty(Bs, bcol, q), bih = lds_frag_ty(Bs, TY_C + bcol, q);
+        const v8bf brh = ty_bfrag(Bq, (bslot * T::NP + 0) * 32, l16, q, mirror);
+        const v8bf bih = ty_bfrag(Bq, (bslot * T::NP + 1) * 32, l16, q, mirror);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
           const int arow = i * 16 + l16;
@@ -990,25 +1096,10 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
       split_bf16(v.im, ht[2][r], ht[3][r]);
     }
     const int64_t o = ((int64_t)w * a.rpad + crow) * Kp + kc0;  // a multiple of 4: Kp % 32 == 0, kc0 % 4 == 0
-#pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
-      uint16_t* dst = pass == 0 ? a.tout : a.qout;
-      if (!dst) continue;
-      const uint16_t(&h)[4][4] = pass == 0 ? ht : hq;
-      // a next term with one product reads the hi planes (0 and 2) only
-      const int pstep = (pass == 0 && a.tout_hi_only) ? 2 : 1;
-      if (kc0 + 3 < N) {
-#pragma unroll
-        for (int p = 0; p < 4; p += pstep)
-          *reinterpret_cast<ushort4*>(dst + p * tplane + o) = make_ushort4(h[p][0], h[p][1], h[p][2], h[p][3]);
-      } else {
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (kc0 + r < N)
-#pragma unroll
-            for (int p = 0; p < 4; p += pstep) dst[p * tplane + o + r] = h[p][r];
-      }
-    }
+    // planes of out_k^T (a next term with one product reads the hi planes 0 and 2
+    // only) and, after the last term, of q^T
+    if (a.tout) store_planes_ty(a.tout, tplane, o, ht, a.tout_hi_only ? 2 : 1, N - kc0);
+    if (a.qout) store_planes_ty(a.qout, tplane, o, hq, 1, N - kc0);
   }
 }
 
@@ -1101,41 +1192,67 @@ hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream) {
 #endif
 constexpr bool kDl = AFQMC_BF16_DL != 0;
 
+template <bool SPLIT, bool TR, class TL>
+hipError_t launch_bf16_gemm_t(const Bf16GemmArgs& a, hipStream_t stream) {
+  const int64_t nb = ceil_div(a.nn, TL::C), mb = ceil_div(a.m, TL::R);
+  const int64_t mbg = (a.units && a.rgroup > 0 && a.rgroup < mb) ? ceil_div(mb, a.rgroup) * a.rgroup : mb;
+  const int64_t grid = mbg * (a.units ? (int64_t)a.nunits : nb * a.ksplit);
+  constexpr size_t lds = Bx<SPLIT, TL>::STAGES3;
+  hipError_t e = allow_lds((const void*)bf16_gemm_kernel<SPLIT, TR, kDl, TL>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((bf16_gemm_kernel<SPLIT, TR, kDl, TL>), dim3((unsigned)grid), dim3(TL::THREADS), lds, stream, a);
+  return hipGetLastError();
+}
+
+template <bool TR>
+hipError_t launch_bf16_gemm_shape(const Bf16GemmArgs& a, hipStream_t stream) {
+  switch (a.tile) {
+    case 1: return launch_bf16_gemm_t<true, TR, Tile1>(a, stream);
+    case 2: return launch_bf16_gemm_t<true, TR, Tile2>(a, stream);
+    case 3: return launch_bf16_gemm_t<true, TR, Tile3>(a, stream);
+    default: return launch_bf16_gemm_t<true, TR, Tile0>(a, stream);
+  }
+}
+
 hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream) {
   if (a.m < 1 || a.nn < 1 || a.kchunks < 1 || a.ksplit < 1 || a.lda % BX_KC != 0 || a.ldb % BX_KC != 0 ||
       (int64_t)a.kchunks * BX_KC > a.lda || (int64_t)a.kchunks * BX_KC > a.ldb)
     return hipErrorInvalidValue;
+  if (a.tile < 0 || a.tile >= BF16_TILES) return hipErrorInvalidValue;
+  // the quadratic form's unit tables and dot slots, and the single product, use shape 0
+  if (a.tile != 0 && (a.epilogue == BF16_EPI_QDOT || a.units || a.single)) return hipErrorInvalidValue;
   if (a.epilogue == BF16_EPI_VHS && (int64_t)a.nbasis * (a.nbasis + 1) / 2 != a.nn) return hipErrorInvalidValue;
+  if (a.epilogue == BF16_EPI_VATOM && (a.out16 == nullptr || a.nn % 256 != 0 || a.atom_stride < a.nn / 256 * 1024))
+    return hipErrorInvalidValue;
   if (a.epilogue == BF16_EPI_QDOT && (a.dg == nullptr || a.dkp < a.dn || a.dn < 1 || a.dstride < 1))
     return hipErrorInvalidValue;
   if (a.units && a.nunits < 1) return hipErrorInvalidValue;
   if ((a.a_gstride > 0 || a.epilogue == BF16_EPI_ADD) && a.group < 1) return hipErrorInvalidValue;
   if (a.epilogue == BF16_EPI_ADD && (a.base == nullptr || a.out == nullptr || a.single)) return hipErrorInvalidValue;
-  const int64_t nb = ceil_div(a.nn, BX_C);
-  const int64_t grid = ceil_div(a.m, BX_R) * (a.units ? (int64_t)a.nunits : nb * a.ksplit);
   if (a.single) {
-    if (a.epilogue == BF16_EPI_VHS) return hipErrorInvalidValue;
-    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<false, false, kDl>, Bx<false>::STAGES3);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((bf16_gemm_kernel<false, false, kDl>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<false>::STAGES3,
-                       stream, a);
-  } else if (a.epilogue == BF16_EPI_ADD) {
-    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true, true, kDl>, Bx<true>::STAGES3);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((bf16_gemm_kernel<true, true, kDl>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES3,
-                       stream, a);
-  } else {
-    hipError_t e = allow_lds((const void*)bf16_gemm_kernel<true, false, kDl>, Bx<true>::STAGES3);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((bf16_gemm_kernel<true, false, kDl>), dim3((unsigned)grid), dim3(BX_THREADS), Bx<true>::STAGES3,
-                       stream, a);
+    if (a.epilogue == BF16_EPI_VHS || a.epilogue == BF16_EPI_VATOM) return hipErrorInvalidValue;
+    return launch_bf16_gemm_t<false, false, Tile0>(a, stream);
   }
-  return hipGetLastError();
+  // ADD and VATOM store rows of consecutive columns: transposed accumulators
+  if (a.epilogue == BF16_EPI_ADD || a.epilogue == BF16_EPI_VATOM) return launch_bf16_gemm_shape<true>(a, stream);
+  return launch_bf16_gemm_shape<false>(a, stream);
+}
+
+bool bf16_tile_dims(int tile, int* rows, int* cols) {
+  switch (tile) {
+    case 0: *rows = Tile0::R; *cols = Tile0::C; return true;
+    case 1: *rows = Tile1::R; *cols = Tile1::C; return true;
+    case 2: *rows = Tile2::R; *cols = Tile2::C; return true;
+    case 3: *rows = Tile3::R; *cols = Tile3::C; return true;
+    default: return false;
+  }
 }
 
 hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream) {
   if (a.nwalkers < 1 || a.wcap < a.nwalkers || a.ncol < 1 || a.rpad % TY_R != 0 || a.rpad < a.ncol ||
       a.kpad % BX_KC != 0 || a.kpad < a.nbasis || a.init == nullptr || (a.last && a.q == nullptr) ||
+      a.vatoms == nullptr || a.zatom == nullptr || a.natom_side != vhs_atom_side(a.nbasis) ||
+      a.atom_stride < vhs_atom_count(a.nbasis) * 1024 ||
       (!(a.first && a.last) && a.dacc == nullptr))
     return hipErrorInvalidValue;
   const void* kern = a.split ? (const void*)taylor_bf16_kernel<true> : (const void*)taylor_bf16_kernel<false>;
@@ -1158,16 +1275,14 @@ hipError_t launch_split_transpose(const double* phi, int nwalkers, int wcap, int
   return hipGetLastError();
 }
 
-hipError_t launch_split_vhs_tiled(const float* x, int nwalkers, int N, int Kp, uint16_t* planes, hipStream_t stream) {
-  const int64_t rows = ceil_div(N, TY_C) * TY_C;
-  if (Kp % BX_KC != 0 || Kp < N) return hipErrorInvalidValue;
-  const int64_t grid = (int64_t)nwalkers * ceil_div(rows, 32) * (Kp / 32);
-  hipLaunchKernelGGL(split_vhs_tiled_kernel, dim3((unsigned)grid), dim3(256), 0, stream,
-                     reinterpret_cast<const float2*>(x), nwalkers, N, Kp, planes);
+hipError_t launch_split_chol_atoms(const double* chol_packed, int N, int X, int Xp, uint16_t* planes,
+                                   hipStream_t stream) {
+  if (N < 1 || X < 1 || Xp < X) return hipErrorInvalidValue;
+  const int64_t total = vhs_atom_count(N) * 256 * Xp;
+  hipLaunchKernelGGL(split_chol_atoms_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream, chol_packed,
+                     N, X, Xp, vhs_atom_side(N), planes);
   return hipGetLastError();
 }
-
-int64_t vhs_tiled_rows(int N) { return ceil_div(N, TY_C) * TY_C; }
 
 hipError_t launch_split_eye(const double* m, int N, int Kp, uint16_t* planes, hipStream_t stream) {
   const int64_t total = (int64_t)N * Kp;

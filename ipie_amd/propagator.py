@@ -153,11 +153,11 @@ class PhaselessGeneric:
         if self.surrogate:
             opts = dict(fraction=0.125, warmup=100, train_iters=4, lr=1e-3, seed=self.seed)
             opts.update(self.surrogate if isinstance(self.surrogate, dict) else {})
-        tuner = _autotune.as_tuner(self.autotune)
-        if tuner is not None:  # before the control variate is on: its step trains it
-            self.tuning = tuner.tune_walkers(eng.ctx, walkers, surrogate=opts is not None)
         if opts is not None:
             eng.ctx.surrogate_configure(True, **opts)
+        tuner = _autotune.as_tuner(self.autotune)
+        if tuner is not None:  # times state-free calls only: no step runs, the model is not trained
+            self.tuning = tuner.tune_walkers(eng.ctx, walkers, surrogate=opts is not None)
         walkers._engine = eng
         self._engine = eng
         self.timer.bind(eng.ctx)

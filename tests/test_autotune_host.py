@@ -23,8 +23,10 @@ class FakeCtx:
     dims = FakeDims()
 
     def __init__(self):
-        self.v = {"fb_split": 8, "quad_wgs": 1536, "exx_split": 0}
-        self.ranges = {"fb_split": (1, 12), "quad_wgs": (256, 8192), "exx_split": (0, 0)}
+        self.v = {"fb_split": 8, "quad_wgs": 1536, "exx_split": 0, "quad_rgroup": 1, "tile_fb": 0, "tile_vhs": 0,
+                  "tile_one_body": 0}
+        self.ranges = {"fb_split": (1, 12), "quad_wgs": (256, 8192), "exx_split": (0, 0), "quad_rgroup": (1, 1),
+                       "tile_fb": (0, 3), "tile_vhs": (0, 3), "tile_one_body": (0, 3)}
         self.calls = []
 
     def tune_get(self, knob):
@@ -37,8 +39,12 @@ class FakeCtx:
         self.v[knob] = value
 
     def cost(self, knob):
-        if knob == "fb_split":
-            return abs(self.v["fb_split"] - 3) + 1.0
+        if knob == "fb_split":  # the force bias: its split and its tile
+            return abs(self.v["fb_split"] - 3) + 1.0 + 0.5 * abs(self.v["tile_fb"] - 2)
+        if knob == "vhs":
+            return 1.0 + abs(self.v["tile_vhs"] - 1)
+        if knob == "one_body":
+            return 1.0 + abs(self.v["tile_one_body"] - 3)
         return abs(self.v["quad_wgs"] - 3072) / 1000.0 + 1.0
 
 
@@ -65,6 +71,9 @@ def _fake_tune(monkeypatch, tuner, ctx):
     # the knob's entry point returns which cost applies
     ctx.force_bias = lambda ga, gb, vb: "fb_split"
     ctx.local_energy = lambda ga, gb, en: "quad_wgs"
+    ctx.control_variate = lambda ga, gb, cv: "quad_wgs"
+    ctx.two_body = lambda *args, **kw: "vhs"
+    ctx.one_body = lambda pa, pb: "one_body"
 
 
 def test_tune_sets_the_fastest_and_caches_by_shape(tmp_path, monkeypatch):
@@ -74,9 +83,13 @@ def test_tune_sets_the_fastest_and_caches_by_shape(tmp_path, monkeypatch):
     ctx = FakeCtx()
     _fake_tune(monkeypatch, t, ctx)
     g = torch.zeros(32, 4, 40, dtype=torch.complex128)
-    chosen = t.tune(ctx, g, g)
-    assert chosen == {"fb_split": 3, "quad_wgs": 3072}
+    phi = torch.zeros(32, 40, 4, dtype=torch.complex128)
+    chosen = t.tune(ctx, g, g, phia=phi, phib=phi)
+    assert chosen == {"fb_split": 3, "quad_wgs": 3072, "quad_rgroup": 1, "tile_fb": 2, "tile_vhs": 1,
+                      "tile_one_body": 3}
     assert ctx.v["fb_split"] == 3 and ctx.v["quad_wgs"] == 3072
+    assert (ctx.v["tile_fb"], ctx.v["tile_vhs"], ctx.v["tile_one_body"]) == (2, 1, 3)
+    assert t.reports[autotune.shape_key(ctx)]["tile_vhs"]["ms"] == {"0": 2.0, "1": 1.0, "2": 2.0, "3": 3.0}
     key = autotune.shape_key(ctx)
     assert "N64=1" in key and "X256=1" in key and "rb=1" in key and "lib=" in key and key.endswith("mixed")
     assert autotune.shape_key(ctx, True) == key + "|surrogate"
@@ -115,15 +128,19 @@ def test_cached_values_outside_the_admissible_range_force_a_retune(tmp_path):
     path = tmp_path / "cache.json"
     ctx = FakeCtx()
     key = autotune.shape_key(ctx)
-    path.write_text(json.dumps({key: {"fb_split": 40, "quad_wgs": 3072}}))  # 40 > this context's 12
+    tiles = {"quad_rgroup": 1, "tile_fb": 1, "tile_vhs": 0, "tile_one_body": 2}
+    path.write_text(json.dumps({key: {"fb_split": 40, "quad_wgs": 3072, **tiles}}))  # 40 > this context's 12
     t = autotune.Autotuner(cache_path=str(path))
     assert t.apply(ctx) is None
-    assert ctx.v == {"fb_split": 8, "quad_wgs": 1536, "exx_split": 0}  # nothing was set
-    path.write_text(json.dumps({key: {"fb_split": 4}}))  # a knob missing from the entry
+    assert ctx.v == {"fb_split": 8, "quad_wgs": 1536, "exx_split": 0, "quad_rgroup": 1, "tile_fb": 0, "tile_vhs": 0,
+                     "tile_one_body": 0}  # nothing was set
+    path.write_text(json.dumps({key: {"fb_split": 4, "quad_wgs": 2048}}))  # tile knobs missing from the entry
     assert autotune.Autotuner(cache_path=str(path)).apply(ctx) is None
-    path.write_text(json.dumps({key: {"fb_split": 4, "quad_wgs": 2048}}))
-    assert autotune.Autotuner(cache_path=str(path)).apply(ctx) == {"fb_split": 4, "quad_wgs": 2048}
-    assert ctx.v["fb_split"] == 4 and ctx.v["quad_wgs"] == 2048
+    path.write_text(json.dumps({key: {"fb_split": 4, "quad_wgs": 2048, **tiles, "tile_vhs": 4}}))  # tile 4 > 3
+    assert autotune.Autotuner(cache_path=str(path)).apply(ctx) is None
+    path.write_text(json.dumps({key: {"fb_split": 4, "quad_wgs": 2048, **tiles}}))
+    assert autotune.Autotuner(cache_path=str(path)).apply(ctx) == {"fb_split": 4, "quad_wgs": 2048, **tiles}
+    assert ctx.v["fb_split"] == 4 and ctx.v["quad_wgs"] == 2048 and ctx.v["tile_one_body"] == 2
 
 
 def test_key_changes_with_the_library_build(monkeypatch):
@@ -142,12 +159,15 @@ def test_as_tuner_specs(tmp_path):
     assert autotune.as_tuner(True).cache_path == autotune.DEFAULT_CACHE
 
 
-def test_surrogate_contexts_tune_the_force_bias_only(monkeypatch):
+def test_surrogate_contexts_time_the_exchange_on_the_control_variate(monkeypatch):
     torch = pytest.importorskip("torch")
     t = autotune.Autotuner()
     ctx = FakeCtx()
     _fake_tune(monkeypatch, t, ctx)
     g = torch.zeros(32, 4, 40, dtype=torch.complex128)
-    assert t.tune(ctx, g, g, surrogate=True) == {"fb_split": 3}
-    assert ctx.v["quad_wgs"] == 1536
+    calls = []
+    ctx.local_energy = lambda ga, gb, en: calls.append("local_energy") or "quad_wgs"
+    # no walkers given: the tiles timed on walker copies are skipped
+    assert t.tune(ctx, g, g, surrogate=True) == {"fb_split": 3, "quad_wgs": 3072, "quad_rgroup": 1, "tile_fb": 2}
+    assert calls == []  # the exchange shapes were timed through the control variate
     assert autotune.shape_key(ctx, True) in t.cache and autotune.shape_key(ctx) not in t.cache

@@ -56,7 +56,7 @@ def test_every_knob_value_agrees_with_the_heuristic(case):
     tol = TOL[CASES[case][-1]]
     vb0, en0 = _outputs(p)
     applied = 0
-    for knob in ("fb_split", "quad_wgs", "exx_split"):
+    for knob in ("fb_split", "quad_wgs", "exx_split", "quad_rgroup"):
         value, lo, hi = ctx.tune_get(knob)
         if hi < 1:
             continue
@@ -76,6 +76,44 @@ def test_every_knob_value_agrees_with_the_heuristic(case):
     assert {k for k in ("fb_split", "quad_wgs", "exx_split") if ctx.tune_get(k)[2] >= 1} == expect
 
 
+@pytest.mark.parametrize("case", ["mixed_quad", "mixed_t"])
+def test_every_tile_shape_gives_the_same_products(case):
+    """The split-BF16 force-bias, VHS (two-body step) and one-body GEMMs in every
+    workgroup tile shape: the shape changes which workgroup computes an output,
+    not its K order, so the results equal those of shape 0."""
+    import torch
+
+    p = make(case)
+    ctx = p.walkers._engine.ctx
+    w = p.walkers
+    ga, gb = torch.empty_like(w.Ghalfa), torch.empty_like(w.Ghalfb)
+    ovlp = torch.empty_like(w.ovlp)
+    ctx.greens_function(w.phia, w.phib, ga, gb, ovlp)
+
+    def run():
+        vb = torch.empty(p.W, ctx.dims.nchol, dtype=torch.complex128, device=w.device)
+        ctx.force_bias(ga, gb, vb)
+        pa, pb = w.phia.clone(), w.phib.clone()
+        cmf = torch.empty(p.W, dtype=torch.complex128, device=w.device)
+        cfb = torch.empty_like(cmf)
+        ctx.two_body(pa, pb, ga, gb, cmf, cfb, seed=5, counter=3)
+        qa, qb = w.phia.clone(), w.phib.clone()
+        ctx.one_body(qa, qb)
+        torch.cuda.synchronize()
+        return [t.cpu().numpy() for t in (vb, pa, pb, cmf, cfb, qa, qb)]
+
+    base = run()
+    for knob in ("tile_fb", "tile_vhs", "tile_one_body"):
+        value, lo, hi = ctx.tune_get(knob)
+        assert (value, lo, hi) == (0, 0, 3)
+        for v in range(lo, hi + 1):
+            print(f"[case] {case} {knob}={v}", flush=True)
+            ctx.tune_set(knob, v)
+            for i, (a, b) in enumerate(zip(run(), base)):
+                assert rel_err(a, b) < 1e-13, (knob, v, i, rel_err(a, b))
+        ctx.tune_set(knob, 0)
+
+
 def test_bad_values_are_refused():
     import torch
 
@@ -90,6 +128,9 @@ def test_bad_values_are_refused():
         ctx.tune_set("fb_split", 0)
     with pytest.raises(AfqmcError):
         ctx.tune_set("quad_wgs", 1536)  # FP64 context: the knob does not apply
+    assert ctx.tune_get("tile_vhs")[2] == 0  # FP64 context: no split-BF16 GEMM tiles
+    with pytest.raises(AfqmcError):
+        ctx.tune_set("tile_vhs", 1)
     assert ctx.tune_get("fb_split")[0] == value
     e = torch.empty(p.W, 3, dtype=torch.complex128, device=p.walkers.device)
     p.prop.propagate_walkers(p.walkers, p.ham, p.trial, 0.0, energy_prev=e)
@@ -155,7 +196,7 @@ def test_autotuner_picks_caches_and_reapplies(tmp_path):
     p = make("mixed_quad", autotune=t1)
     ctx = p.walkers._engine.ctx
     chosen = p.prop.tuning
-    assert set(chosen) == {"fb_split", "quad_wgs"}
+    assert set(chosen) == {"fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body"}
     for knob, v in chosen.items():
         value, lo, hi = ctx.tune_get(knob)
         assert value == v and lo <= v <= hi
@@ -185,10 +226,11 @@ def test_autotuner_picks_caches_and_reapplies(tmp_path):
     assert rel_err(e1.cpu().numpy(), e2.cpu().numpy()) < 1e-4
 
 
-def test_surrogate_context_tunes_the_force_bias_only():
-    """A context running the learned control variate tunes fb_split alone (its
-    exchange runs inside the step); its run matches an untuned twin to the
-    mixed-precision tolerance, with the control variate trained alike."""
+def test_surrogate_context_tunes_on_its_control_variate():
+    """A context running the learned control variate times its exchange shapes
+    on the control variate (afqmc_control_variate; no step runs, the model is
+    not trained); its run matches an untuned twin to the mixed-precision
+    tolerance, with the control variate trained alike."""
     import torch
 
     from ipie_amd.autotune import Autotuner, shape_key
@@ -197,7 +239,7 @@ def test_surrogate_context_tunes_the_force_bias_only():
     t = Autotuner(cache_path=None, repeats=1, inner=1)
     p = make("mixed_quad", autotune=t, surrogate=sur)
     ctx = p.walkers._engine.ctx
-    assert set(p.prop.tuning) == {"fb_split"}
+    assert set(p.prop.tuning) == {"fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body"}
     assert shape_key(ctx, True) in t.cache and shape_key(ctx, False) not in t.cache
     assert p.prop.surrogate_stats()["steps"] == 0
     u = make("mixed_quad", surrogate=sur)
@@ -223,9 +265,9 @@ def test_shape_buckets_tune_once_and_batch():
     timed = []
     real_tune = t.tune
 
-    def counting_tune(ctx, ga, gb, surrogate=False):
+    def counting_tune(ctx, ga, gb, surrogate=False, **walkers):
         timed.append(ctx.describe())
-        return real_tune(ctx, ga, gb, surrogate)
+        return real_tune(ctx, ga, gb, surrogate, **walkers)
 
     t.tune = counting_tune
     cases = ["mixed_quad", "mixed_quad", "fp64_t"]

@@ -126,3 +126,58 @@ def test_unbiased_at_surrogate_workload():
     assert numpy.all((z < 6.0) | (err < tol_abs))
     pooled = (mean - ref[:, 2]).real.mean() / (numpy.sqrt((sem**2).sum()) / Wn)
     assert abs(pooled) < 5.0
+
+
+def _bf16(x):
+    """Round to BF16 (nearest even), as the planes' hi parts hold it."""
+    x = numpy.asarray(x, dtype=numpy.float32)
+    u = x.view(numpy.uint32).astype(numpy.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(numpy.uint32).view(numpy.float32).astype(numpy.float64)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_control_variate_entry_point_predicts_the_exchange_sum(precision):
+    """afqmc_control_variate gives S_T + 2 <K, dG> + Q(dG), the exchange sum S of a
+    walker (sum_x tr(T_x T_x) over both spins, local_energy_sd.py:263-303)
+    expanded about the trial's Green's function, dG = G - G_T.  S is a quadratic
+    form in G, so the expansion is exact and Q(dG) is S evaluated at dG; only
+    the single BF16 product of Q rounds.  Checks: every walker within 1% of the
+    largest |Q| of the exact S (a dropped, doubled or sign-flipped term misses
+    by |Q| or more); in FP64 contexts, whose Q forms every T_x from the BF16
+    hi parts of L and dG, within 0.05% of the largest |Q| of the same sum taken
+    from BF16-rounded operands on the CPU."""
+    import torch
+
+    from oracle import afqmc
+
+    p = make(precision=precision, warmup=2)
+    w = p.walkers
+    ga, gb = torch.empty_like(w.Ghalfa), torch.empty_like(w.Ghalfb)
+    ovlp = torch.empty_like(w.ovlp)
+    ctx = w._engine.ctx
+    ctx.greens_function(w.phia, w.phib, ga, gb, ovlp)
+    cv = torch.empty(W, dtype=torch.complex128, device=w.device)
+    ctx.control_variate(ga, gb, cv)
+    torch.cuda.synchronize()
+    got = cv.cpu().numpy()
+    o = p.orc
+    gan, gbn = ga.cpu().numpy(), gb.cpu().numpy()
+    ex = afqmc.exx_kernel_batch_real_rchol  # 1/2 sum_x tr(T_x T_x) per walker
+    ra, rb = o.trial._rchola, o.trial._rcholb
+    exact = 2.0 * (ex(ra, gan) + ex(rb, gbn))
+    # the trial's own half Green's function, (psi^T psi*)^-1 psi^T
+    psa, psb = o.trial.psi0a, o.trial.psi0b
+    gta = numpy.linalg.solve(psa.T @ psa.conj(), psa.T)[None]
+    gtb = numpy.linalg.solve(psb.T @ psb.conj(), psb.T)[None]
+    q = 2.0 * (ex(ra, gan - gta) + ex(rb, gbn - gtb))
+    qmax = numpy.abs(q).max()
+    err = numpy.abs(got - exact)
+    print(f"control variate {precision}: max err {err.max():.3e}, max |Q| {qmax:.3e}", flush=True)
+    assert qmax > 0.1
+    assert err.max() <= 1e-2 * qmax, err.max() / qmax
+    if precision == "fp64":
+        bc = lambda z: _bf16(z.real) + 1j * _bf16(z.imag)  # noqa: E731
+        q_bf = 2.0 * (ex(_bf16(ra), bc(gan - gta)) + ex(_bf16(rb), bc(gbn - gtb)))
+        emu = exact - q + q_bf
+        assert numpy.abs(got - emu).max() <= 5e-4 * qmax, numpy.abs(got - emu).max() / qmax
