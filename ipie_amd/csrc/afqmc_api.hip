@@ -82,6 +82,7 @@ struct afqmc_ctx {
   int quad_target = QUAD_TARGET_WGS;  // split-BF16 quadratic form: workgroups the column units aim for
   int tile_fb = 0, tile_vhs = 0, tile_ob = 0;  // split-BF16 GEMM tile shapes (force bias, VHS, one body)
   int quad_rgroup = 0;  // split-BF16 quadratic form: row blocks per workgroup group (0: all)
+  int taylor_k16 = 0;   // split-BF16 Taylor series: 16-wide K stages (four in LDS) instead of 32-wide (two)
   int64_t qy_cap = 0;                 // complex entries qy holds (split-BF16 quadratic form)
   int exx_split = 0;                  // T-form exchange: Cholesky-range split (0: heuristic)
   std::vector<void*> retired;         // buffers a retune replaced: graphs captured before still use them
@@ -777,6 +778,7 @@ int taylor_bf16(afqmc_ctx* c, int nw, const double** result, const uint16_t** re
     t.tout_hi_only = k + 1 > TAYLOR_SPLIT_TERMS;
     t.alpha = 1.0 / k;
     t.qout = k == c->d.exp_nmax ? c->tplanes[k & 1] : nullptr;
+    t.kstep16 = c->taylor_k16;
     HIP_TRY(c, launch_taylor_bf16(t, c->stream));
   }
   *result = Q;
@@ -2102,6 +2104,12 @@ int afqmc_tune_get(afqmc_ctx* c, int32_t knob, int32_t* value, int32_t* lo, int3
       *lo = 1;
       *hi = tune_exx_max(c);
       return AFQMC_OK;
+    case AFQMC_TUNE_TAYLOR_K16:
+      if (c->d.precision != AFQMC_PREC_MIXED || c->d.nchol < 1 || c->d.exp_nmax < 1) return AFQMC_OK;
+      *value = c->taylor_k16;
+      *lo = 0;
+      *hi = 1;
+      return AFQMC_OK;
     case AFQMC_TUNE_QUAD_RGROUP:
       if (!c->quad_on || c->d.precision != AFQMC_PREC_MIXED) return AFQMC_OK;
       *hi = (int32_t)ceil_div(c->d.nwalkers, 128);
@@ -2151,6 +2159,8 @@ int afqmc_tune_set(afqmc_ctx* c, int32_t knob, int32_t value) {
   } else if (knob == AFQMC_TUNE_QUAD_WGS) {
     c->quad_target = value;
     if (c->ws_ready) return build_quad_units(c);
+  } else if (knob == AFQMC_TUNE_TAYLOR_K16) {
+    c->taylor_k16 = value;
   } else if (knob == AFQMC_TUNE_QUAD_RGROUP) {
     c->quad_rgroup = value;
   } else if (knob == AFQMC_TUNE_TILE_FB) {

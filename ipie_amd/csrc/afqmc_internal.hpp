@@ -245,6 +245,7 @@ struct TaylorBf16Args {
   int tout_hi_only;         // 1: write the hi planes of out_k^T only (the next term uses one product)
   double alpha;
   uint16_t* qout;           // last term: planes of q^T (layout of tin), or null
+  int kstep16;              // 1: 16-wide K stages, four LDS stages (taylor16_kernel); 0: 32-wide, two
 };
 hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream);
 // Taylor terms k <= TAYLOR_SPLIT_TERMS use three BF16 products, later ones one.

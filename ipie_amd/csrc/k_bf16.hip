@@ -940,6 +940,55 @@ __device__ __forceinline__ void store_planes_ty(uint16_t* dst, int64_t tplane, i
   }
 }
 
+// Epilogue of a Taylor term from the accumulators (transposed tiles: element r
+// of tile i is row crow = i 16 + l16, a column of the walker matrix, and basis
+// index kcol = cb 80 + wave 16 + 4 q + r, so 16 lanes touch 16 consecutive
+// columns of q / dacc and each lane 4 consecutive K of a plane).
+__device__ __forceinline__ void ty_epilogue(const TaylorBf16Args& a, const v4f (&re)[5], const v4f (&im)[5], int w,
+                                            int rb, int cb, int wave, int l16, int q) {
+  const int N = a.nbasis, Kp = a.kpad;
+  const int64_t tplane = (int64_t)a.wcap * a.rpad * Kp;
+  const int kc0 = cb * TY_C + wave * 16 + 4 * q;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    const int crow = rb * TY_R + i * 16 + l16;
+    if (crow >= a.ncol || kc0 >= N) continue;
+    uint16_t ht[4][4], hq[4][4];  // [plane][r] of out_k and of q
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int kcol = kc0 + r;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) ht[p][r] = hq[p][r] = 0;
+      if (kcol >= N) continue;
+      const cplx v = cmake(a.alpha * re[i][r], a.alpha * im[i][r]);
+      const int64_t qi = ((int64_t)w * N + kcol) * a.ncol + crow;
+      if (a.last) {
+        cplx qv = cadd(ld(a.init, qi), v);
+        if (!a.first) {
+          const float2 d = reinterpret_cast<const float2*>(a.dacc)[qi];
+          qv.re += (double)d.x;
+          qv.im += (double)d.y;
+        }
+        st(a.q, qi, qv);
+        split_bf16(qv.re, hq[0][r], hq[1][r]);
+        split_bf16(qv.im, hq[2][r], hq[3][r]);
+      } else {
+        float2 d = a.first ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(a.dacc)[qi];
+        d.x = (float)((double)d.x + v.re);
+        d.y = (float)((double)d.y + v.im);
+        reinterpret_cast<float2*>(a.dacc)[qi] = d;
+      }
+      split_bf16(v.re, ht[0][r], ht[1][r]);
+      split_bf16(v.im, ht[2][r], ht[3][r]);
+    }
+    const int64_t o = ((int64_t)w * a.rpad + crow) * Kp + kc0;  // a multiple of 4: Kp % 32 == 0, kc0 % 4 == 0
+    // planes of out_k^T (a next term with one product reads the hi planes 0 and 2
+    // only) and, after the last term, of q^T
+    if (a.tout) store_planes_ty(a.tout, tplane, o, ht, a.tout_hi_only ? 2 : 1, N - kc0);
+    if (a.qout) store_planes_ty(a.qout, tplane, o, hq, 1, N - kc0);
+  }
+}
+
 template <bool SPLIT>
 __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args a) {
   using T = Ty<SPLIT>;
@@ -1059,48 +1108,166 @@ __global__ void __launch_bounds__(TY_THREADS) taylor_bf16_kernel(TaylorBf16Args 
     wait_vm<0>();
     dl_barrier();
   }
-  // epilogue: element r of tile i is row crow = i 16 + l16 (a column of the
-  // walker matrix) and basis index kcol = wave 16 + 4 q + r, so 16 lanes touch
-  // 16 consecutive columns of q / dacc and each lane 4 consecutive K of a plane
-  const int kc0 = cb * TY_C + wave * 16 + 4 * q;
+  ty_epilogue(a, re, im, w, rb, cb, wave, l16, q);
+}
+
+// The same term with 16-wide K stages and four LDS stages per workgroup, three
+// of them in flight while one is computed (v_mfma_f32_16x16x16_bf16: half the
+// MFMA rate, twice the loads in flight of the 32-wide form).  A stage holds the
+// 80 rows x 16 K of each A plane and the 5 atoms (5 cb + v, kc) of each B plane,
+// 2 KiB per (plane, 16 rows); every lane reads 4 consecutive K (8 bytes) per
+// fragment, a mirror atom's with one transposing load per plane.
+template <int NP>
+struct Ty16 {
+  static constexpr int A_VEC = NP * TY_R * 2, B_VEC = NP * 5 * 32, STAGE_VEC = A_VEC + B_VEC;
+  static constexpr int STAGES = 4;
+  static constexpr size_t LDS = 16 * (size_t)STAGES * STAGE_VEC;
+  static constexpr int A_INS = A_VEC / (64 * TY_WAVES), B_INS = B_VEC / (64 * TY_WAVES);  // per wave
+  static constexpr int INS = A_INS + B_INS;
+  static_assert(A_VEC % (64 * TY_WAVES) == 0 && B_VEC % (64 * TY_WAVES) == 0, "whole load instructions per wave");
+  static_assert(2 * INS <= 63, "vmcnt counts two stages of loads");
+};
+
+// 16-byte slot of half h of row r (A: of the 80 rows; B: of an atom's 16): the
+// halves of rows 8..15 (mod 16) are swapped, so the 8-byte reads of 32 lanes
+// (16 rows x two quarters) cover all 64 banks once
+__device__ __forceinline__ int ty16_vec(int r, int h) { return r * 2 + (h ^ ((r >> 3) & 1)); }
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ v4f mfma16_bf16(v4s a, v4s b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4s neg16(v4s x) {
+  return __builtin_bit_cast(v4s, __builtin_bit_cast(unsigned long long, x) ^ 0x8000800080008000ull);
+}
+// 4 consecutive K values (quarter g = 4 g .. 4 g + 3 of the 16) of row l16 at LDS vector base vb
+__device__ __forceinline__ v4s ty16_frag(const uint4* lds, int vb, int row, int g) {
+  const char* p = reinterpret_cast<const char*>(lds + vb + ty16_vec(row, g >> 1)) + 8 * (g & 1);
+  return *reinterpret_cast<const v4s*>(p);
+}
+// B fragment of an atom plane: stored atom row l16, or column l16 of the mirror
+// atom's rows 4 g .. 4 g + 3 by one transposing load (lane r' of the 16-lane
+// group addresses row 4 g + r' / 4, quarter r' % 4)
+__device__ __forceinline__ v4s ty16_bfrag(const uint4* lds, int vb, int l16, int g, bool mirror) {
+  if (!mirror) return ty16_frag(lds, vb, l16, g);
+  const int r = 4 * g + (l16 >> 2), qt = l16 & 3;
+  const char* p = reinterpret_cast<const char*>(lds + vb + ty16_vec(r, qt >> 1)) + 8 * (qt & 1);
+  return __builtin_bit_cast(v4s, __builtin_amdgcn_ds_read_tr16_b64_v4bf16((__attribute__((address_space(3))) v4bf*)p));
+}
+
+template <bool SPLIT>
+__global__ void __launch_bounds__(TY_THREADS) taylor16_kernel(TaylorBf16Args a) {
+  constexpr int NP = SPLIT ? 4 : 2;
+  using T = Ty16<NP>;
+  extern __shared__ uint4 lds_u4[];
+  const int N = a.nbasis, Kp = a.kpad;
+  const int nrb = a.rpad / TY_R, ncb = (int)ceil_div(N, TY_C);
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int cb = lin % ncb, rb = (lin / ncb) % nrb, w = lin / ncb / nrb;
+  if (w >= a.nwalkers) return;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, g = lane >> 4;
+  const int64_t tplane = (int64_t)a.wcap * a.rpad * Kp;
+  const int nst = Kp / 16, NA = a.natom_side;
+  auto phys = [](int pi) { return SPLIT ? pi : 2 * pi; };
+  // A: vector v of the stage's A region is plane v / 160, row (v % 160) / 2,
+  // half (v % 2) ^ swizzle; B: vector v of the B region is plane v / 160 of atom
+  // slot (v % 160) / 32, atom row (v % 32) / 2, half (v % 2) ^ swizzle
+  const uint16_t* asrc[T::A_INS];
+#pragma unroll
+  for (int j = 0; j < T::A_INS; ++j) {
+    const int v = wave * T::A_INS * 64 + j * 64 + lane;
+    const int pi = v / (TY_R * 2), rem = v % (TY_R * 2), row = rem >> 1;
+    const int h = (rem & 1) ^ ((row >> 3) & 1);
+    asrc[j] = a.tin + phys(pi) * tplane + ((int64_t)w * a.rpad + rb * TY_R + row) * Kp + h * 8;
+  }
+  int baj[T::B_INS], boff[T::B_INS];
+#pragma unroll
+  for (int j = 0; j < T::B_INS; ++j) {
+    const int v = wave * T::B_INS * 64 + j * 64 + lane;
+    const int pi = v / 160, rem = v % 160, slot = rem >> 5, e = rem & 31, ar = e >> 1;
+    baj[j] = 5 * cb + slot;
+    boff[j] = phys(pi) * 256 + ar * 16 + 8 * ((e & 1) ^ ((ar >> 3) & 1));
+  }
+  const uint16_t* wat = a.vatoms + (int64_t)w * a.atom_stride;
+  auto issue = [&](int t) __attribute__((always_inline)) {
+    uint4* base = lds_u4 + (t & 3) * T::STAGE_VEC;
+#pragma unroll
+    for (int j = 0; j < T::A_INS; ++j)
+      __builtin_amdgcn_global_load_lds((const void*)(asrc[j] + t * 16),
+                                       (__attribute__((address_space(3))) void*)(base + wave * T::A_INS * 64 + j * 64),
+                                       16, 0, 0);
+#pragma unroll
+    for (int j = 0; j < T::B_INS; ++j) {
+      // atom (aj, t) as stored, or its mirror; outside the matrix the zero atom
+      const int aj = baj[j], lo = min(aj, t), hi = max(aj, t);
+      const uint16_t* src = (hi >= NA) ? a.zatom + boff[j]
+                                       : wat + (int64_t)(lo * NA - lo * (lo - 1) / 2 + hi - lo) * 1024 + boff[j];
+      __builtin_amdgcn_global_load_lds(
+          (const void*)src, (__attribute__((address_space(3))) void*)(base + T::A_VEC + wave * T::B_INS * 64 + j * 64),
+          16, 0, 0);
+    }
+  };
+  v4f re[5], im[5];
 #pragma unroll
   for (int i = 0; i < 5; ++i) {
-    const int crow = rb * TY_R + i * 16 + l16;
-    if (crow >= a.ncol || kc0 >= N) continue;
-    uint16_t ht[4][4], hq[4][4];  // [plane][r] of out_k and of q
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int kcol = kc0 + r;
-#pragma unroll
-      for (int p = 0; p < 4; ++p) ht[p][r] = hq[p][r] = 0;
-      if (kcol >= N) continue;
-      const cplx v = cmake(a.alpha * re[i][r], a.alpha * im[i][r]);
-      const int64_t qi = ((int64_t)w * N + kcol) * a.ncol + crow;
-      if (a.last) {
-        cplx qv = cadd(ld(a.init, qi), v);
-        if (!a.first) {
-          const float2 d = reinterpret_cast<const float2*>(a.dacc)[qi];
-          qv.re += (double)d.x;
-          qv.im += (double)d.y;
-        }
-        st(a.q, qi, qv);
-        split_bf16(qv.re, hq[0][r], hq[1][r]);
-        split_bf16(qv.im, hq[2][r], hq[3][r]);
-      } else {
-        float2 d = a.first ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(a.dacc)[qi];
-        d.x = (float)((double)d.x + v.re);
-        d.y = (float)((double)d.y + v.im);
-        reinterpret_cast<float2*>(a.dacc)[qi] = d;
-      }
-      split_bf16(v.re, ht[0][r], ht[1][r]);
-      split_bf16(v.im, ht[2][r], ht[3][r]);
-    }
-    const int64_t o = ((int64_t)w * a.rpad + crow) * Kp + kc0;  // a multiple of 4: Kp % 32 == 0, kc0 % 4 == 0
-    // planes of out_k^T (a next term with one product reads the hi planes 0 and 2
-    // only) and, after the last term, of q^T
-    if (a.tout) store_planes_ty(a.tout, tplane, o, ht, a.tout_hi_only ? 2 : 1, N - kc0);
-    if (a.qout) store_planes_ty(a.qout, tplane, o, hq, 1, N - kc0);
+    re[i] = zero4f();
+    im[i] = zero4f();
   }
+  const int pre = min(3, nst);
+  for (int t = 0; t < pre; ++t) issue(t);
+  for (int t = 0; t < nst; ++t) {
+    // stage t has landed when at most the loads of the stages issued after it remain
+    const int later = min(2, nst - 1 - t);
+    if (later >= 2) wait_vm<2 * T::INS>();
+    else if (later == 1) wait_vm<T::INS>();
+    else wait_vm<0>();
+    dl_barrier();  // stage t is visible, and stage t - 1 (the buffer reused next) is read by every wave
+    if (t + 3 < nst) issue(t + 3);
+    const uint4* stage = lds_u4 + (t & 3) * T::STAGE_VEC;
+    const uint4* Bq = stage + T::A_VEC;
+    const int aj = 5 * cb + wave;
+    const bool mirror = aj > t && aj < NA;
+    const int bslot = wave * 32;
+    // operands swapped in every MFMA: the accumulators hold the transposed tiles
+    if (SPLIT) {
+      const v4s brh = ty16_bfrag(Bq, 0 * 160 + bslot, l16, g, mirror);
+      const v4s brl = ty16_bfrag(Bq, 1 * 160 + bslot, l16, g, mirror);
+      const v4s bih = ty16_bfrag(Bq, 2 * 160 + bslot, l16, g, mirror);
+      const v4s bil = ty16_bfrag(Bq, 3 * 160 + bslot, l16, g, mirror);
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        const int arow = i * 16 + l16;
+        const v4s arh = ty16_frag(stage, 0 * 160, arow, g), arl = ty16_frag(stage, 1 * 160, arow, g);
+        const v4s aih = ty16_frag(stage, 2 * 160, arow, g), ail = ty16_frag(stage, 3 * 160, arow, g);
+        const v4s naih = neg16(aih), nail = neg16(ail);
+        re[i] = mfma16_bf16(brh, arh, re[i]);
+        re[i] = mfma16_bf16(brl, arh, re[i]);
+        re[i] = mfma16_bf16(brh, arl, re[i]);
+        re[i] = mfma16_bf16(bih, naih, re[i]);
+        re[i] = mfma16_bf16(bil, naih, re[i]);
+        re[i] = mfma16_bf16(bih, nail, re[i]);
+        im[i] = mfma16_bf16(bih, arh, im[i]);
+        im[i] = mfma16_bf16(bil, arh, im[i]);
+        im[i] = mfma16_bf16(bih, arl, im[i]);
+        im[i] = mfma16_bf16(brh, aih, im[i]);
+        im[i] = mfma16_bf16(brl, aih, im[i]);
+        im[i] = mfma16_bf16(brh, ail, im[i]);
+      }
+    } else {
+      const v4s brh = ty16_bfrag(Bq, 0 * 160 + bslot, l16, g, mirror);
+      const v4s bih = ty16_bfrag(Bq, 1 * 160 + bslot, l16, g, mirror);
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        const int arow = i * 16 + l16;
+        const v4s arh = ty16_frag(stage, 0 * 160, arow, g), aih = ty16_frag(stage, 1 * 160, arow, g);
+        re[i] = mfma16_bf16(brh, arh, re[i]);
+        re[i] = mfma16_bf16(bih, neg16(aih), re[i]);
+        im[i] = mfma16_bf16(bih, arh, im[i]);
+        im[i] = mfma16_bf16(brh, aih, im[i]);
+      }
+    }
+  }
+  ty_epilogue(a, re, im, w, rb, cb, wave, l16, g);
 }
 
 // phi [W][N][ncol] complex -> transposed split planes [4][W][rpad][kpad] at
@@ -1255,11 +1422,22 @@ hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream) {
       a.atom_stride < vhs_atom_count(a.nbasis) * 1024 ||
       (!(a.first && a.last) && a.dacc == nullptr))
     return hipErrorInvalidValue;
+  const int64_t grid = (int64_t)a.nwalkers * (a.rpad / TY_R) * ceil_div(a.nbasis, TY_C);
+  if (a.kstep16) {
+    const void* kern = a.split ? (const void*)taylor16_kernel<true> : (const void*)taylor16_kernel<false>;
+    const size_t lds = a.split ? Ty16<4>::LDS : Ty16<2>::LDS;
+    hipError_t e = allow_lds(kern, lds);
+    if (e != hipSuccess) return e;
+    if (a.split)
+      hipLaunchKernelGGL(taylor16_kernel<true>, dim3((unsigned)grid), dim3(TY_THREADS), lds, stream, a);
+    else
+      hipLaunchKernelGGL(taylor16_kernel<false>, dim3((unsigned)grid), dim3(TY_THREADS), lds, stream, a);
+    return hipGetLastError();
+  }
   const void* kern = a.split ? (const void*)taylor_bf16_kernel<true> : (const void*)taylor_bf16_kernel<false>;
   const size_t lds = a.split ? Ty<true>::LDS : Ty<false>::LDS;
   hipError_t e = allow_lds(kern, lds);
   if (e != hipSuccess) return e;
-  const int64_t grid = (int64_t)a.nwalkers * (a.rpad / TY_R) * ceil_div(a.nbasis, TY_C);
   if (a.split)
     hipLaunchKernelGGL(taylor_bf16_kernel<true>, dim3((unsigned)grid), dim3(TY_THREADS), lds, stream, a);
   else

@@ -112,6 +112,14 @@ def test_every_tile_shape_gives_the_same_products(case):
             for i, (a, b) in enumerate(zip(run(), base)):
                 assert rel_err(a, b) < 1e-13, (knob, v, i, rel_err(a, b))
         ctx.tune_set(knob, 0)
+    # the Taylor series in 16-wide K stages: half-size MFMAs, so the FP32 sums
+    # are grouped differently (agreement to FP32 rounding of the terms)
+    assert ctx.tune_get("taylor_k16") == (0, 0, 1)
+    ctx.tune_set("taylor_k16", 1)
+    print(f"[case] {case} taylor_k16=1", flush=True)
+    for i, (a, b) in enumerate(zip(run(), base)):
+        assert rel_err(a, b) < 2e-6, ("taylor_k16", i, rel_err(a, b))
+    ctx.tune_set("taylor_k16", 0)
 
 
 def test_bad_values_are_refused():
@@ -196,7 +204,7 @@ def test_autotuner_picks_caches_and_reapplies(tmp_path):
     p = make("mixed_quad", autotune=t1)
     ctx = p.walkers._engine.ctx
     chosen = p.prop.tuning
-    assert set(chosen) == {"fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body"}
+    assert set(chosen) == {"fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body", "taylor_k16"}
     for knob, v in chosen.items():
         value, lo, hi = ctx.tune_get(knob)
         assert value == v and lo <= v <= hi
@@ -239,7 +247,8 @@ def test_surrogate_context_tunes_on_its_control_variate():
     t = Autotuner(cache_path=None, repeats=1, inner=1)
     p = make("mixed_quad", autotune=t, surrogate=sur)
     ctx = p.walkers._engine.ctx
-    assert set(p.prop.tuning) == {"fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body"}
+    assert set(p.prop.tuning) == {"fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body",
+                                  "taylor_k16"}
     assert shape_key(ctx, True) in t.cache and shape_key(ctx, False) not in t.cache
     assert p.prop.surrogate_stats()["steps"] == 0
     u = make("mixed_quad", surrogate=sur)
