@@ -350,15 +350,22 @@ int afqmc_tune_set(afqmc_ctx* ctx, int32_t knob, int32_t value);
  * restores the plain exact estimator.  afqmc_local_energy is always exact. */
 int afqmc_surrogate_configure(afqmc_ctx* ctx, int enable, double fraction, int warmup_steps, int train_iters,
                               double lr, uint64_t seed);
+/* enable = 1 (default): the control variate includes Q_bf16(G - G_T); 0: the
+ * predictor is E_T + 2 <K, G - G_T> (+ alpha m(f)) without the quadratic-form
+ * GEMM, the exact sample and the MLP term making up the rest (the estimator
+ * stays unbiased; its variance grows by the variance of Q).  Applies to steps
+ * enqueued or captured after the call. */
+int afqmc_surrogate_quadratic(afqmc_ctx* ctx, int enable);
 /* out[0..n-1], n >= 6: energy evaluations so far, warm-up length, last subset
  * size, last training loss per sample (normalised units), mean
  * |exact - predicted|^2 on the last subset (Ha^2), variance over the warm-up of
  * the MLP's target Re(exact - control variate), and (n >= 7) the current gain of
  * the MLP term.  Synchronises. */
 int afqmc_surrogate_stats(afqmc_ctx* ctx, double* out, int n);
-/* cv [W] complex = E_T + 2 <K, G - G_T> + Q_bf16(G - G_T) from Ghalf: the
- * physical control variate of each walker's exchange sum (the predictor above
- * without the MLP term), through the kernels afqmc_propagate runs for it.
+/* cv [W] complex = E_T + 2 <K, G - G_T> + Q_bf16(G - G_T) from Ghalf (without
+ * Q after afqmc_surrogate_quadratic(ctx, 0)): the physical control variate of
+ * each walker's exchange sum (the predictor above without the MLP term),
+ * through the kernels afqmc_propagate runs for it.
  * Needs afqmc_surrogate_configure(enable = 1); changes no walker or model
  * state.  No counterpart in ipie; it lets the autotuner time the control
  * variate's exchange. */

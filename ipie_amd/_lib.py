@@ -114,6 +114,7 @@ SIGNATURES = {
     "afqmc_batch_steps": (ctypes.c_int, [_dp, ctypes.POINTER(_i32)]),
     "afqmc_batch_launch": (ctypes.c_int, [_dp, _i32, ctypes.POINTER(_u64), ctypes.POINTER(_f64)]),
     "afqmc_control_variate": (ctypes.c_int, [_dp, _i32, _dp, _dp, _dp]),
+    "afqmc_surrogate_quadratic": (ctypes.c_int, [_dp, ctypes.c_int]),
     "afqmc_noalloc_scratch_bytes": (ctypes.c_size_t, [_i32, _i32, _i32]),
     "afqmc_noalloc_last_error": (ctypes.c_char_p, []),
     "afqmc_greens_function_noalloc": (
@@ -476,6 +477,10 @@ class Context:
     def surrogate_configure(self, enable=True, fraction=0.125, warmup=100, train_iters=4, lr=1e-3, seed=11):
         self._call("afqmc_surrogate_configure", int(bool(enable)), float(fraction), int(warmup), int(train_iters),
                    float(lr), int(seed))
+
+    def surrogate_quadratic(self, enable=True):
+        """Whether the control variate includes Q_bf16(G - G_T) (afqmc_surrogate_quadratic)."""
+        self._call("afqmc_surrogate_quadratic", int(bool(enable)))
 
     def exchange_form(self):
         """'quad' (g^T Wq g, Wq built once) or 't' (every T_x = L_x Ghalf^T, ipie's contraction)."""

@@ -107,7 +107,8 @@ struct afqmc_ctx {
     double* resid = nullptr;
     int* walker_of = nullptr;
     double* xbar = nullptr;  // [2, W]: mean, max
-    // physical control variate E_T + 2 <K, dG> + Q_bf16(dG)
+    // physical control variate E_T + 2 <K, dG> + Q_bf16(dG) (quadratic), or E_T + 2 <K, dG>
+    bool quadratic = true;
     bool cv_ready = false;   // gT, K, E_T hold the current trial / Hamiltonian
     double* gT = nullptr;    // [2][nmax, N] complex trial Ghalf per spin
     double* K = nullptr;     // [2][nmax, N] complex
@@ -1081,7 +1082,7 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   } else {
     HIP_TRY(c, hipMemsetAsync(S.pred, 0, 16 * (size_t)nw, c->stream));
   }
-  // physical part of the predictor: E_T + 2 <K, dG> + Q_bf16(dG), added to pred
+  // physical part of the predictor: E_T + 2 <K, dG> (+ Q_bf16(dG)), added to pred
   r = sur_prepare_cv(c);
   if (r) return r;
   r = ensure_planes(c);
@@ -1089,7 +1090,7 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   const int nspin = two_spins(c) ? 2 : 1;
   const int N = c->d.nbasis;
   const int64_t gstride = 2 * (int64_t)std::max(c->d.nalpha, c->d.nbeta) * N;
-  for (int s = 0; s < nspin; ++s) {
+  for (int s = 0; s < nspin && S.quadratic; ++s) {
     r = exchange_q_spin(c, nw, s, s == 0 ? ga : gb);
     if (r) return r;
   }
@@ -1103,7 +1104,7 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
     cva.gT[s] = S.gT + s * gstride;
     cva.K[s] = S.K + s * gstride;
     cva.part[s] = s == 0 ? c->exx_a : c->exx_b;
-    cva.xs[s] = c->xs_used[s];
+    cva.xs[s] = S.quadratic ? c->xs_used[s] : 0;  // partial sums of Q, when computed
   }
   cva.et = S.et;
   cva.cv = S.cv;
@@ -1801,7 +1802,7 @@ int afqmc_control_variate(afqmc_ctx* c, int32_t nw, const double* ga, const doub
   const int nspin = two_spins(c) ? 2 : 1;
   const int N = c->d.nbasis;
   const int64_t gstride = 2 * (int64_t)std::max(c->d.nalpha, c->d.nbeta) * N;
-  for (int s = 0; s < nspin; ++s) {
+  for (int s = 0; s < nspin && S.quadratic; ++s) {
     r = exchange_q_spin(c, nw, s, s == 0 ? ga : gb);
     if (r) return r;
   }
@@ -1815,7 +1816,7 @@ int afqmc_control_variate(afqmc_ctx* c, int32_t nw, const double* ga, const doub
     cva.gT[s] = S.gT + s * gstride;
     cva.K[s] = S.K + s * gstride;
     cva.part[s] = s == 0 ? c->exx_a : c->exx_b;
-    cva.xs[s] = c->xs_used[s];
+    cva.xs[s] = S.quadratic ? c->xs_used[s] : 0;  // partial sums of Q, when computed
   }
   cva.et = S.et;
   cva.cv = cv;
@@ -2182,6 +2183,14 @@ int afqmc_profile(afqmc_ctx* c, int enable) {
     c->prof_ms[i] = 0.0;
     c->prof_count[i] = 0;
   }
+  // phases recorded before the reset and not yet read are dropped with the
+  // totals (their events go back to the pool; recording an event again
+  // replaces its earlier record)
+  for (const PendingEvent& pe : c->ev_pending) {
+    c->ev_free.push_back(pe.start);
+    c->ev_free.push_back(pe.stop);
+  }
+  c->ev_pending.clear();
   return AFQMC_OK;
 }
 
@@ -2284,6 +2293,13 @@ int afqmc_surrogate_configure(afqmc_ctx* c, int enable, double fraction, int war
   HIP_TRY(c, hipMemsetAsync(S.state, 0, sizeof(SurrogateState), c->stream));
   HIP_TRY(c, hipMemsetAsync(c->dstate, 0, sizeof(StepState), c->stream));  // subset draws, Adam steps
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_surrogate_quadratic(afqmc_ctx* c, int enable) {
+  if (!c) return AFQMC_ERR_INVALID_ARG;
+  if (c->capturing) return refuse_in_capture(c, "afqmc_surrogate_quadratic");
+  c->sur.quadratic = enable != 0;
   return AFQMC_OK;
 }
 

@@ -16,7 +16,7 @@ the local energy as split-BF16 products with FP32 accumulation; Green's
 function, overlaps, weights, propagation and QR stay FP64).
 
 ``surrogate``: None, or True / a dict (fraction, warmup, train_iters, lr,
-seed) enabling the learned exchange control variate for the ``energy_prev``
+seed, quadratic) enabling the learned exchange control variate for the ``energy_prev``
 output of ``propagate_walkers`` (an unbiased estimate per walker; exact on a
 random subset of walkers, see afqmc_surrogate_configure).  ``local_energy_batch``
 is always exact.
@@ -154,7 +154,9 @@ class PhaselessGeneric:
             opts = dict(fraction=0.125, warmup=100, train_iters=4, lr=1e-3, seed=self.seed)
             opts.update(self.surrogate if isinstance(self.surrogate, dict) else {})
         if opts is not None:
+            quadratic = bool(opts.pop("quadratic", True))
             eng.ctx.surrogate_configure(True, **opts)
+            eng.ctx.surrogate_quadratic(quadratic)
         tuner = _autotune.as_tuner(self.autotune)
         if tuner is not None:  # times state-free calls only: no step runs, the model is not trained
             self.tuning = tuner.tune_walkers(eng.ctx, walkers, surrogate=opts is not None)

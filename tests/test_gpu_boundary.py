@@ -60,6 +60,14 @@ def test_timer_fields_filled_by_fused_step():
     assert t.tgf == 0.0
     p.prop.propagate_walkers(p.walkers, p.ham, p.trial, 0.0)
     assert 0.0 < t.tgf < vals["tgf"]
+    # a reset drops phases recorded before it that were never read
+    ctx = p.prop._engine.ctx
+    for _ in range(2):
+        p.prop.propagate_walkers(p.walkers, p.ham, p.trial, 0.0)
+    ctx.profile(True)
+    p.prop.propagate_walkers(p.walkers, p.ham, p.trial, 0.0)
+    prof = ctx.profile_read()
+    assert prof["gf"][1] == 1 and prof["overlap"][1] == 1 and prof["weight"][1] == 1, prof
 
 
 def test_strided_views_are_refused():

@@ -14,14 +14,14 @@ pytestmark = pytest.mark.gpu
 N, X, D, NA, NB, W = 48, 100, 30.0, 8, 6, 64
 
 
-def make(precision="fp64", warmup=3, fraction=0.125):
+def make(precision="fp64", warmup=3, fraction=0.125, quadratic=True):
     import torch  # noqa: F401
 
-    print(f"[case] surrogate N={N} X={X} na={NA} nb={NB} W={W} precision={precision}", flush=True)
+    print(f"[case] surrogate N={N} X={X} na={NA} nb={NB} W={W} precision={precision} quadratic={quadratic}", flush=True)
     h1e, chol, ecore, psi = synthetic_arrays(N, X, D, NA, NB, scale=(12.0 / (N * D)) ** 0.5)
     init = random_walkers(N, NA, NB, W, seed=8, amp=0.1, base=psi)
     return Pair(h1e, chol, psi, (NA, NB), W, init=init, precision=precision,
-                surrogate=dict(warmup=warmup, fraction=fraction, train_iters=3, lr=2e-3, seed=5))
+                surrogate=dict(warmup=warmup, fraction=fraction, train_iters=3, lr=2e-3, seed=5, quadratic=quadratic))
 
 
 def snapshot(w):
@@ -59,9 +59,9 @@ def test_warmup_steps_are_exact():
     assert p.prop.surrogate_stats()["last_k"] == W
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_estimator_is_unbiased(precision):
-    p = make(precision=precision, warmup=4)
+@pytest.mark.parametrize("precision,quadratic", [("fp64", True), ("mixed", True), ("mixed", False)])
+def test_estimator_is_unbiased(precision, quadratic):
+    p = make(precision=precision, warmup=4, quadratic=quadratic)
     for _ in range(4):  # warm-up (training on all walkers)
         fused_energy(p)
     snap = snapshot(p.walkers)
@@ -176,6 +176,13 @@ def test_control_variate_entry_point_predicts_the_exchange_sum(precision):
     print(f"control variate {precision}: max err {err.max():.3e}, max |Q| {qmax:.3e}", flush=True)
     assert qmax > 0.1
     assert err.max() <= 1e-2 * qmax, err.max() / qmax
+    # without its quadratic term the control variate is S_T + 2 <K, dG> = S - Q, in FP64
+    ctx.surrogate_quadratic(False)
+    ctx.control_variate(ga, gb, cv)
+    torch.cuda.synchronize()
+    lin = cv.cpu().numpy()
+    assert numpy.abs(lin - (exact - q)).max() <= 1e-10 * numpy.abs(exact).max()
+    ctx.surrogate_quadratic(True)
     if precision == "fp64":
         bc = lambda z: _bf16(z.real) + 1j * _bf16(z.imag)  # noqa: E731
         q_bf = 2.0 * (ex(_bf16(ra), bc(gan - gta)) + ex(_bf16(rb), bc(gbn - gtb)))
