@@ -305,10 +305,9 @@ int afqmc_profile_read(afqmc_ctx* ctx, double* ms, int64_t* launches);
  *                        quadratic form (mixed precision, quadratic form; the
  *                        default, all of them, puts every row block of a unit
  *                        on one XCD together)
- *   AFQMC_TUNE_TAYLOR_K16
- *                        1: the split-BF16 Taylor series stages 16-wide K
- *                        chunks, four in LDS (three loads in flight, half-rate
- *                        MFMAs); 0: 32-wide chunks, two in LDS
+ *   AFQMC_TUNE_TAYLOR    kernel of the split-BF16 Taylor series: 0 = 32-wide
+ *                        K chunks, two in LDS; 1 = 16-wide chunks, four in LDS
+ *                        (three loads in flight, half-rate MFMAs)
  *   AFQMC_TUNE_TILE_FB, AFQMC_TUNE_TILE_VHS, AFQMC_TUNE_TILE_ONE_BODY
  *                        workgroup tile of the split-BF16 force-bias, VHS and
  *                        one-body GEMMs (mixed precision): 0 = 128 x 160,
@@ -327,7 +326,7 @@ int afqmc_profile_read(afqmc_ctx* ctx, double* ms, int64_t* launches);
 #define AFQMC_TUNE_TILE_VHS 5
 #define AFQMC_TUNE_TILE_ONE_BODY 6
 #define AFQMC_TUNE_QUAD_RGROUP 7
-#define AFQMC_TUNE_TAYLOR_K16 8
+#define AFQMC_TUNE_TAYLOR 8
 int afqmc_tune_get(afqmc_ctx* ctx, int32_t knob, int32_t* value, int32_t* lo, int32_t* hi);
 int afqmc_tune_set(afqmc_ctx* ctx, int32_t knob, int32_t value);
 /* Learned control variate for the exchange energy of afqmc_propagate's

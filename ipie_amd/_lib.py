@@ -127,7 +127,7 @@ POP_METHODS = {"pair_branch": 0, "comb": 1, "stochastic_reconfiguration": 2, "re
 
 # launch-shape knobs of afqmc_tune_get / afqmc_tune_set
 TUNE_KNOBS = {"fb_split": 1, "quad_wgs": 2, "exx_split": 3, "tile_fb": 4, "tile_vhs": 5, "tile_one_body": 6,
-              "quad_rgroup": 7, "taylor_k16": 8}
+              "quad_rgroup": 7, "taylor": 8}
 
 PHASES = ["gf", "one_body", "force_bias", "fields", "vhs", "taylor", "overlap", "weight", "exchange", "energy",
           "reortho", "exchange_cv", "split"]

@@ -14,8 +14,8 @@ Eight launch shapes of the step depend on the system's dimensions:
   tile_one_body  workgroup tile of the split-BF16 one-body GEMM
                  (tiles: 0 = 128 x 160, 1 = 128 x 128, 2 = 128 x 80,
                  3 = 64 x 160 walker rows x output columns)
-  taylor_k16     split-BF16 Taylor series in 16-wide K stages, four in LDS
-                 (1), or 32-wide stages, two in LDS (0)
+  taylor         kernel of the split-BF16 Taylor series: 32-wide K stages, two
+                 in LDS (0); 16-wide stages, four in LDS (1)
 
 A context starts from heuristics (``afqmc_create``).  ``Autotuner.tune`` times
 every admissible candidate of each knob that applies on the context's own
@@ -24,7 +24,7 @@ shapes, through entry points that change no walker state
 exchange knobs, or ``afqmc_control_variate`` for them in a context that runs
 the learned control variate, whose step evaluates the single-product quadratic
 form of every walker) or act on scratch copies of the walkers
-(``afqmc_two_body`` for tile_vhs and taylor_k16, ``afqmc_one_body`` for
+(``afqmc_two_body`` for tile_vhs and taylor, ``afqmc_one_body`` for
 tile_one_body), and keeps the fastest unless the heuristic is within
 ``min_gain`` of it.  The choices are cached per device and shape: a multi-system batch whose
 systems share a shape tunes that shape once, and a JSON cache file carries the
@@ -46,12 +46,12 @@ import time
 
 import torch
 
-KNOBS = ("fb_split", "quad_wgs", "exx_split", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body", "taylor_k16")
+KNOBS = ("fb_split", "quad_wgs", "exx_split", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body", "taylor")
 # a context running the learned control variate: the quadratic-form knobs are
 # timed on its control variate (the T-form split is not used by it)
-SURROGATE_KNOBS = ("fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body", "taylor_k16")
+SURROGATE_KNOBS = ("fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body", "taylor")
 # knobs timed on scratch copies of the walkers
-PHI_KNOBS = ("tile_vhs", "tile_one_body", "taylor_k16")
+PHI_KNOBS = ("tile_vhs", "tile_one_body", "taylor")
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "ipie_amd", "autotune.json")
 
 _FB = (1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 24, 32)
@@ -105,7 +105,7 @@ def candidates(knob, value, lo, hi):
     if hi < 1:
         return []
     ladder = {"fb_split": _FB, "quad_wgs": _QUAD, "exx_split": _EXX, "tile_fb": _TILE, "tile_vhs": _TILE,
-              "tile_one_body": _TILE, "quad_rgroup": _RGROUP, "taylor_k16": (0, 1)}[knob]
+              "tile_one_body": _TILE, "quad_rgroup": _RGROUP, "taylor": (0, 1)}[knob]
     out = {v for v in ladder if lo <= v <= hi}
     if knob == "exx_split":
         out.add(0)  # the heuristic
@@ -219,7 +219,7 @@ class Autotuner:
             cfb = torch.empty_like(cmf)
             calls["tile_vhs"] = lambda: ctx.two_body(pa, pb, ghalfa, ghalfb, cmf, cfb, seed=1, counter=0)
             calls["tile_one_body"] = lambda: ctx.one_body(pa, pb)
-            calls["taylor_k16"] = calls["tile_vhs"]
+            calls["taylor"] = calls["tile_vhs"]
         chosen, report = {}, {}
         for knob in self.knobs:
             if surrogate and knob not in SURROGATE_KNOBS:

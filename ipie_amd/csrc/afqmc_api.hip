@@ -82,7 +82,7 @@ struct afqmc_ctx {
   int quad_target = QUAD_TARGET_WGS;  // split-BF16 quadratic form: workgroups the column units aim for
   int tile_fb = 0, tile_vhs = 0, tile_ob = 0;  // split-BF16 GEMM tile shapes (force bias, VHS, one body)
   int quad_rgroup = 0;  // split-BF16 quadratic form: row blocks per workgroup group (0: all)
-  int taylor_k16 = 0;   // split-BF16 Taylor series: 16-wide K stages (four in LDS) instead of 32-wide (two)
+  int taylor_variant = 0;  // split-BF16 Taylor kernel (TaylorBf16Args::variant)
   int64_t qy_cap = 0;                 // complex entries qy holds (split-BF16 quadratic form)
   int exx_split = 0;                  // T-form exchange: Cholesky-range split (0: heuristic)
   std::vector<void*> retired;         // buffers a retune replaced: graphs captured before still use them
@@ -779,7 +779,7 @@ int taylor_bf16(afqmc_ctx* c, int nw, const double** result, const uint16_t** re
     t.tout_hi_only = k + 1 > TAYLOR_SPLIT_TERMS;
     t.alpha = 1.0 / k;
     t.qout = k == c->d.exp_nmax ? c->tplanes[k & 1] : nullptr;
-    t.kstep16 = c->taylor_k16;
+    t.variant = c->taylor_variant;
     HIP_TRY(c, launch_taylor_bf16(t, c->stream));
   }
   *result = Q;
@@ -966,7 +966,7 @@ int do_exchange(afqmc_ctx* c, int nw, const double* ga, const double* gb, const 
 }
 
 int do_energy(afqmc_ctx* c, int nw, const double* ga, const double* gb, double* energy,
-              const double* exx_est = nullptr, double* e1_out = nullptr) {
+              const double* exx_est = nullptr, double* e1_out = nullptr, const double* e1_in = nullptr) {
   EnergyArgs a;
   a.nwalkers = nw;
   a.nbasis = c->d.nbasis;
@@ -986,6 +986,7 @@ int do_energy(afqmc_ctx* c, int nw, const double* ga, const double* gb, double* 
   a.energy = energy;
   a.exx_est = exx_est;
   a.e1_out = e1_out;
+  a.e1_in = e1_in;
   PhaseScope ps(c, AFQMC_PHASE_ENERGY);
   HIP_TRY(c, launch_energy(a, c->stream));
   return AFQMC_OK;
@@ -1155,7 +1156,8 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   if (iters > 0) HIP_TRY(c, launch_step_advance(c->dstate, 0, iters, c->stream));
   S.steps += 1;
   if (c->capturing) c->capture_sur += 1;
-  return do_energy(c, nw, ga, gb, energy, S.est, nullptr);
+  // the one-body energy of the first pass (S.e1) is reused: Ghalf is not read again
+  return do_energy(c, nw, ga, gb, energy, S.est, nullptr, S.e1);
 }
 
 int do_two_body_core(afqmc_ctx* c, int nw, const double* ga, const double* gb, const double* xi, uint64_t seed,
@@ -2105,9 +2107,9 @@ int afqmc_tune_get(afqmc_ctx* c, int32_t knob, int32_t* value, int32_t* lo, int3
       *lo = 1;
       *hi = tune_exx_max(c);
       return AFQMC_OK;
-    case AFQMC_TUNE_TAYLOR_K16:
+    case AFQMC_TUNE_TAYLOR:
       if (c->d.precision != AFQMC_PREC_MIXED || c->d.nchol < 1 || c->d.exp_nmax < 1) return AFQMC_OK;
-      *value = c->taylor_k16;
+      *value = c->taylor_variant;
       *lo = 0;
       *hi = 1;
       return AFQMC_OK;
@@ -2160,8 +2162,8 @@ int afqmc_tune_set(afqmc_ctx* c, int32_t knob, int32_t value) {
   } else if (knob == AFQMC_TUNE_QUAD_WGS) {
     c->quad_target = value;
     if (c->ws_ready) return build_quad_units(c);
-  } else if (knob == AFQMC_TUNE_TAYLOR_K16) {
-    c->taylor_k16 = value;
+  } else if (knob == AFQMC_TUNE_TAYLOR) {
+    c->taylor_variant = value;
   } else if (knob == AFQMC_TUNE_QUAD_RGROUP) {
     c->quad_rgroup = value;
   } else if (knob == AFQMC_TUNE_TILE_FB) {

@@ -245,7 +245,8 @@ struct TaylorBf16Args {
   int tout_hi_only;         // 1: write the hi planes of out_k^T only (the next term uses one product)
   double alpha;
   uint16_t* qout;           // last term: planes of q^T (layout of tin), or null
-  int kstep16;              // 1: 16-wide K stages, four LDS stages (taylor16_kernel); 0: 32-wide, two
+  int variant;              // 0: 32-wide K stages, two in LDS (taylor_bf16_kernel); 1: 16-wide, four
+                            // (taylor16_kernel)
 };
 hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream);
 // Taylor terms k <= TAYLOR_SPLIT_TERMS use three BF16 products, later ones one.
@@ -329,6 +330,7 @@ struct EnergyArgs {
   double* energy;         // [W, 3] complex
   const double* exx_est;  // optional [W] complex: exchange sum to use instead of the partials
   double* e1_out;         // optional [W] complex: one-body energy
+  const double* e1_in;    // optional [W] complex: one-body energy already computed (Ghalf not read)
 };
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t stream);
 

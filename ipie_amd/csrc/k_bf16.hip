@@ -1420,10 +1420,10 @@ hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream) {
       a.kpad % BX_KC != 0 || a.kpad < a.nbasis || a.init == nullptr || (a.last && a.q == nullptr) ||
       a.vatoms == nullptr || a.zatom == nullptr || a.natom_side != vhs_atom_side(a.nbasis) ||
       a.atom_stride < vhs_atom_count(a.nbasis) * 1024 ||
-      (!(a.first && a.last) && a.dacc == nullptr))
+      (!(a.first && a.last) && a.dacc == nullptr) || a.variant < 0 || a.variant > 1)
     return hipErrorInvalidValue;
   const int64_t grid = (int64_t)a.nwalkers * (a.rpad / TY_R) * ceil_div(a.nbasis, TY_C);
-  if (a.kstep16) {
+  if (a.variant == 1) {
     const void* kern = a.split ? (const void*)taylor16_kernel<true> : (const void*)taylor16_kernel<false>;
     const size_t lds = a.split ? Ty16<4>::LDS : Ty16<2>::LDS;
     hipError_t e = allow_lds(kern, lds);

@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(FLD_THREADS) energy_kernel(EnergyArgs a) {
   const int w = blockIdx.x;
   const int N = a.nbasis;
   double er = 0.0, ei = 0.0;
-  const int nspin = (a.rhf || a.nbeta == 0) ? 1 : 2;
+  const int nspin = a.e1_in ? 0 : (a.rhf || a.nbeta == 0) ? 1 : 2;
   for (int s = 0; s < nspin; ++s) {
     const int n = s == 0 ? a.nalpha : a.nbeta;
     const double* g = (s == 0 ? a.ghalfa : a.ghalfb) + 2 * (int64_t)w * n * N;
@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(FLD_THREADS) energy_kernel(EnergyArgs a) {
   ei = block_sum(ei, red);
   if (threadIdx.x == 0) {
     const double f = a.rhf ? 2.0 : 1.0;
-    const cplx e1 = cmake(f * er + a.ecore, f * ei);
+    const cplx e1 = a.e1_in ? ld(a.e1_in, w) : cmake(f * er + a.ecore, f * ei);
     if (a.e1_out) st(a.e1_out, w, e1);
     cplx ex = cmake(0.0, 0.0);
     if (a.exx_est) {

@@ -135,16 +135,19 @@ __global__ void __launch_bounds__(256) quad_reduce_kernel(const double* __restri
   if (threadIdx.x == 0) st(part, w, cmake(re, im));
 }
 
-// out[w] = sum_s part[s][w]: 16 walkers x 16 slot lanes per workgroup, the
-// slot lanes combined in a fixed order.
+// out[w] = sum_s part[s][w]: WL walkers x 256 / WL slot lanes per workgroup, the
+// slot lanes combined in a fixed order (few walkers, many slots: more lanes
+// per walker keep the serial chain of each lane short).
+template <int WL>
 __global__ void __launch_bounds__(256) quad_sum_kernel(const double* __restrict__ part, int nslots, int nwalkers,
                                                       double* __restrict__ out) {
-  __shared__ double sr[16][17], si[16][17];
-  const int wl = threadIdx.x & 15, sl = threadIdx.x >> 4;
-  const int w = blockIdx.x * 16 + wl;
+  constexpr int SL = 256 / WL;
+  __shared__ double sr[SL][WL + 1], si[SL][WL + 1];
+  const int wl = threadIdx.x % WL, sl = threadIdx.x / WL;
+  const int w = blockIdx.x * WL + wl;
   double re = 0.0, im = 0.0;
   if (w < nwalkers) {
-    for (int s = sl; s < nslots; s += 16) {
+    for (int s = sl; s < nslots; s += SL) {
       const cplx v = ld(part, (int64_t)s * nwalkers + w);
       re += v.re;
       im += v.im;
@@ -155,7 +158,7 @@ __global__ void __launch_bounds__(256) quad_sum_kernel(const double* __restrict_
   __syncthreads();
   if (sl == 0 && w < nwalkers) {
     double tr = 0.0, ti = 0.0;
-    for (int k = 0; k < 16; ++k) {
+    for (int k = 0; k < SL; ++k) {
       tr += sr[k][wl];
       ti += si[k][wl];
     }
@@ -167,8 +170,12 @@ __global__ void __launch_bounds__(256) quad_sum_kernel(const double* __restrict_
 
 hipError_t launch_quad_sum(const double* part, int nslots, int nwalkers, double* out, hipStream_t stream) {
   if (nslots < 1 || nwalkers < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(quad_sum_kernel, dim3((unsigned)ceil_div(nwalkers, 16)), dim3(256), 0, stream, part, nslots,
-                     nwalkers, out);
+  if (nwalkers >= 512)
+    hipLaunchKernelGGL(quad_sum_kernel<16>, dim3((unsigned)ceil_div(nwalkers, 16)), dim3(256), 0, stream, part,
+                       nslots, nwalkers, out);
+  else
+    hipLaunchKernelGGL(quad_sum_kernel<4>, dim3((unsigned)ceil_div(nwalkers, 4)), dim3(256), 0, stream, part,
+                       nslots, nwalkers, out);
   return hipGetLastError();
 }
 

@@ -38,6 +38,9 @@ constexpr int SG_F = SUR_FEATURES;
 // samples per gradient workgroup: each walks its samples serially, so few per
 // workgroup and many workgroups (partial gradients are summed in fixed order)
 constexpr int SG_SAMPLES_PER_WG = 4;
+// row stride of the H x H matrices in the gradient kernel's LDS: thread j walks
+// row j, so an odd stride puts the 64 lanes of a wave on distinct banks
+constexpr int SG_LD = SG_H + 1;
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
 __device__ __forceinline__ float dsilu(float z) {
@@ -162,9 +165,9 @@ __global__ void __launch_bounds__(SG_H) sur_forward_kernel(SurrogateForwardArgs 
 // Loss = mean_t sum_o (g_o(f_t) - y_t,o)^2 on normalised targets.
 __global__ void __launch_bounds__(SG_H) sur_grad_kernel(SurrogateTrainArgs a) {
   extern __shared__ float sm[];
-  float* W2s = sm;                    // [H][H] master weights
-  float* dW2 = W2s + SG_H * SG_H;     // [H][H] gradient accumulator
-  float* h1 = dW2 + SG_H * SG_H;      // [H]
+  float* W2s = sm;                    // [H][SG_LD] master weights
+  float* dW2 = W2s + SG_H * SG_LD;    // [H][SG_LD] gradient accumulator
+  float* h1 = dW2 + SG_H * SG_LD;     // [H]
   float* dz2 = h1 + SG_H;             // [H]
   float* xin = dz2 + SG_H;            // [F]
   __shared__ float red[2][SG_H / 64];
@@ -180,8 +183,8 @@ __global__ void __launch_bounds__(SG_H) sur_grad_kernel(SurrogateTrainArgs a) {
   const float* b3 = b2 + SG_H;               // [2]
   const SurrogateState* sst = a.state;
   for (int e = j; e < SG_H * SG_H; e += SG_H) {
-    W2s[e] = W2[e];
-    dW2[e] = 0.0f;
+    W2s[(e / SG_H) * SG_LD + e % SG_H] = W2[e];
+    dW2[(e / SG_H) * SG_LD + e % SG_H] = 0.0f;
   }
   float gW1[SG_F];
 #pragma unroll
@@ -205,7 +208,7 @@ __global__ void __launch_bounds__(SG_H) sur_grad_kernel(SurrogateTrainArgs a) {
     h1[j] = silu(z1);
     __syncthreads();
     float z2 = b2[j];
-    for (int k = 0; k < SG_H; ++k) z2 += W2s[j * SG_H + k] * h1[k];
+    for (int k = 0; k < SG_H; ++k) z2 += W2s[j * SG_LD + k] * h1[k];
     const float h2 = silu(z2);
     float o0 = w3a * h2, o1 = w3b * h2;
     for (int off = 32; off > 0; off >>= 1) {
@@ -240,10 +243,10 @@ __global__ void __launch_bounds__(SG_H) sur_grad_kernel(SurrogateTrainArgs a) {
     const float g2 = (w3a * d0 + w3b * d1) * dsilu(z2);
     dz2[j] = g2;
     gb2 += g2;
-    for (int k = 0; k < SG_H; ++k) dW2[j * SG_H + k] += g2 * h1[k];
+    for (int k = 0; k < SG_H; ++k) dW2[j * SG_LD + k] += g2 * h1[k];
     __syncthreads();
     float dh1 = 0.0f;
-    for (int r = 0; r < SG_H; ++r) dh1 += W2s[r * SG_H + j] * dz2[r];
+    for (int r = 0; r < SG_H; ++r) dh1 += W2s[r * SG_LD + j] * dz2[r];
     const float g1 = dh1 * dsilu(z1);
     gb1 += g1;
 #pragma unroll
@@ -253,7 +256,8 @@ __global__ void __launch_bounds__(SG_H) sur_grad_kernel(SurrogateTrainArgs a) {
   float* G = a.grad + (int64_t)blockIdx.x * SUR_NPARAM;
 #pragma unroll
   for (int q = 0; q < SG_F; ++q) G[j * SG_F + q] = gW1[q];
-  for (int k = 0; k < SG_H; ++k) G[SG_H * SG_F + j * SG_H + k] = dW2[j * SG_H + k];
+  __syncthreads();  // rows of dW2 were accumulated by their own threads; written out by columns
+  for (int r = 0; r < SG_H; ++r) G[SG_H * SG_F + r * SG_H + j] = dW2[r * SG_LD + j];
   G[SG_H * SG_F + SG_H * SG_H + j] = gW3a;
   G[SG_H * SG_F + SG_H * SG_H + SG_H + j] = gW3b;
   G[SG_H * SG_F + SG_H * SG_H + 2 * SG_H + j] = gb1;
@@ -532,7 +536,7 @@ hipError_t launch_sur_forward(const SurrogateForwardArgs& a, hipStream_t s) {
 }
 int sur_grad_groups(int count) { return (int)ceil_div(count, SG_SAMPLES_PER_WG); }
 hipError_t launch_sur_grad(const SurrogateTrainArgs& a, hipStream_t s) {
-  const size_t lds = sizeof(float) * (2 * SG_H * SG_H + 2 * SG_H + SG_F);
+  const size_t lds = sizeof(float) * (2 * SG_H * SG_LD + 2 * SG_H + SG_F);
   hipError_t e = allow_lds((const void*)sur_grad_kernel, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(sur_grad_kernel, dim3(sur_grad_groups(a.count)), dim3(SG_H), lds, s, a);

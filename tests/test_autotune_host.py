@@ -24,9 +24,9 @@ class FakeCtx:
 
     def __init__(self):
         self.v = {"fb_split": 8, "quad_wgs": 1536, "exx_split": 0, "quad_rgroup": 1, "tile_fb": 0, "tile_vhs": 0,
-                  "tile_one_body": 0, "taylor_k16": 0}
+                  "tile_one_body": 0, "taylor": 0}
         self.ranges = {"fb_split": (1, 12), "quad_wgs": (256, 8192), "exx_split": (0, 0), "quad_rgroup": (1, 1),
-                       "tile_fb": (0, 3), "tile_vhs": (0, 3), "tile_one_body": (0, 3), "taylor_k16": (0, 1)}
+                       "tile_fb": (0, 3), "tile_vhs": (0, 3), "tile_one_body": (0, 3), "taylor": (0, 1)}
         self.calls = []
 
     def tune_get(self, knob):
@@ -42,7 +42,7 @@ class FakeCtx:
         if knob == "fb_split":  # the force bias: its split and its tile
             return abs(self.v["fb_split"] - 3) + 1.0 + 0.5 * abs(self.v["tile_fb"] - 2)
         if knob == "vhs":  # the two-body step: the VHS tile and the Taylor stages
-            return 1.0 + abs(self.v["tile_vhs"] - 1) + 0.5 * (1 - self.v["taylor_k16"])
+            return 1.0 + abs(self.v["tile_vhs"] - 1) + 0.5 * (1 - self.v["taylor"])
         if knob == "one_body":
             return 1.0 + abs(self.v["tile_one_body"] - 3)
         return abs(self.v["quad_wgs"] - 3072) / 1000.0 + 1.0
@@ -86,7 +86,7 @@ def test_tune_sets_the_fastest_and_caches_by_shape(tmp_path, monkeypatch):
     phi = torch.zeros(32, 40, 4, dtype=torch.complex128)
     chosen = t.tune(ctx, g, g, phia=phi, phib=phi)
     assert chosen == {"fb_split": 3, "quad_wgs": 3072, "quad_rgroup": 1, "tile_fb": 2, "tile_vhs": 1,
-                      "tile_one_body": 3, "taylor_k16": 1}
+                      "tile_one_body": 3, "taylor": 1}
     assert ctx.v["fb_split"] == 3 and ctx.v["quad_wgs"] == 3072
     assert (ctx.v["tile_fb"], ctx.v["tile_vhs"], ctx.v["tile_one_body"]) == (2, 1, 3)
     assert t.reports[autotune.shape_key(ctx)]["tile_vhs"]["ms"] == {"0": 2.5, "1": 1.5, "2": 2.5, "3": 3.5}
@@ -128,12 +128,12 @@ def test_cached_values_outside_the_admissible_range_force_a_retune(tmp_path):
     path = tmp_path / "cache.json"
     ctx = FakeCtx()
     key = autotune.shape_key(ctx)
-    tiles = {"quad_rgroup": 1, "tile_fb": 1, "tile_vhs": 0, "tile_one_body": 2, "taylor_k16": 1}
+    tiles = {"quad_rgroup": 1, "tile_fb": 1, "tile_vhs": 0, "tile_one_body": 2, "taylor": 1}
     path.write_text(json.dumps({key: {"fb_split": 40, "quad_wgs": 3072, **tiles}}))  # 40 > this context's 12
     t = autotune.Autotuner(cache_path=str(path))
     assert t.apply(ctx) is None
     assert ctx.v == {"fb_split": 8, "quad_wgs": 1536, "exx_split": 0, "quad_rgroup": 1, "tile_fb": 0, "tile_vhs": 0,
-                     "tile_one_body": 0, "taylor_k16": 0}  # nothing was set
+                     "tile_one_body": 0, "taylor": 0}  # nothing was set
     path.write_text(json.dumps({key: {"fb_split": 4, "quad_wgs": 2048}}))  # tile knobs missing from the entry
     assert autotune.Autotuner(cache_path=str(path)).apply(ctx) is None
     path.write_text(json.dumps({key: {"fb_split": 4, "quad_wgs": 2048, **tiles, "tile_vhs": 4}}))  # tile 4 > 3

@@ -114,12 +114,13 @@ def test_every_tile_shape_gives_the_same_products(case):
         ctx.tune_set(knob, 0)
     # the Taylor series in 16-wide K stages: half-size MFMAs, so the FP32 sums
     # are grouped differently (agreement to FP32 rounding of the terms)
-    assert ctx.tune_get("taylor_k16") == (0, 0, 1)
-    ctx.tune_set("taylor_k16", 1)
-    print(f"[case] {case} taylor_k16=1", flush=True)
-    for i, (a, b) in enumerate(zip(run(), base)):
-        assert rel_err(a, b) < 2e-6, ("taylor_k16", i, rel_err(a, b))
-    ctx.tune_set("taylor_k16", 0)
+    assert ctx.tune_get("taylor") == (0, 0, 1)
+    for v, tol in ((1, 2e-6),):
+        ctx.tune_set("taylor", v)
+        print(f"[case] {case} taylor={v}", flush=True)
+        for i, (a, b) in enumerate(zip(run(), base)):
+            assert rel_err(a, b) < tol, ("taylor", v, i, rel_err(a, b))
+    ctx.tune_set("taylor", 0)
 
 
 def test_bad_values_are_refused():
@@ -204,7 +205,7 @@ def test_autotuner_picks_caches_and_reapplies(tmp_path):
     p = make("mixed_quad", autotune=t1)
     ctx = p.walkers._engine.ctx
     chosen = p.prop.tuning
-    assert set(chosen) == {"fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body", "taylor_k16"}
+    assert set(chosen) == {"fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body", "taylor"}
     for knob, v in chosen.items():
         value, lo, hi = ctx.tune_get(knob)
         assert value == v and lo <= v <= hi
@@ -248,7 +249,7 @@ def test_surrogate_context_tunes_on_its_control_variate():
     p = make("mixed_quad", autotune=t, surrogate=sur)
     ctx = p.walkers._engine.ctx
     assert set(p.prop.tuning) == {"fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body",
-                                  "taylor_k16"}
+                                  "taylor"}
     assert shape_key(ctx, True) in t.cache and shape_key(ctx, False) not in t.cache
     assert p.prop.surrogate_stats()["steps"] == 0
     u = make("mixed_quad", surrogate=sur)
