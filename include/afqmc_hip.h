@@ -84,7 +84,8 @@ typedef struct afqmc_dims {
   int32_t nalpha;    /* n_a, 1..64 */
   int32_t nbeta;     /* n_b, 0..64 */
   int32_t nwalkers;  /* maximum W per call */
-  int32_t exp_nmax;  /* Taylor order of exp(VHS), ipie default 6 */
+  int32_t exp_nmax;  /* Taylor order of exp(VHS), ipie default 6 (at most 24 in
+                        AFQMC_PREC_MIXED, which keeps every term's planes) */
   int32_t rhf;       /* walkers.rhf: propagate alpha only, beta == alpha */
   int32_t precision; /* AFQMC_PREC_* */
 } afqmc_dims;

@@ -62,7 +62,9 @@ struct afqmc_ctx {
   uint16_t* cplanes = nullptr;                // 2 x [NAT 256, xpad] (Cholesky rows in VHS atom order)
   uint16_t* vatoms = nullptr;                 // VHS split planes, upper atoms per walker ([W][NAT][4][256])
   uint16_t* zatom = nullptr;                  // one all-zero atom (4 x 256)
-  uint16_t* tplanes[2] = {nullptr, nullptr};  // 4 x [W, rpad, kpad] each (Taylor terms, transposed)
+  // 4 x [W, rpad, kpad] each, transposed: [0] the planes of P^T (input of the
+  // Taylor series) and, after it, of Q^T; [k] those of term k (max(2, exp_nmax) buffers)
+  uint16_t* tplanes[TAYLOR_MAX_TERMS] = {};
   uint16_t* dplanes = nullptr;                // split-BF16 expH1_s - I per spin, 2 x [N, kpad] each
   uint16_t* oplanes = nullptr;                // 4 x [W n_s, kpad]: transposed planes of phi_s (one body)
   bool ptp_ready = false;                     // tplanes[0] holds the planes of P^T from the one-body GEMM
@@ -351,7 +353,7 @@ int ensure_ws(afqmc_ctx* c) {
     ok = ok && dmalloc((void**)&c->vatoms, 2 * (size_t)W * NAT * 1024) == hipSuccess;
     ok = ok && dmalloc((void**)&c->zatom, 2 * (size_t)1024) == hipSuccess;
     ok = ok && hipMemset(c->zatom, 0, 2 * (size_t)1024) == hipSuccess;
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < std::max(2, d.exp_nmax); ++i) {
       ok = ok && dmalloc((void**)&c->tplanes[i], tbytes) == hipSuccess;
       ok = ok && hipMemset(c->tplanes[i], 0, tbytes) == hipSuccess;  // padding stays zero
     }
@@ -748,7 +750,10 @@ int scatter(afqmc_ctx* c, int nw, const double* src, double* phia, double* phib,
 
 // Taylor series of exp(VHS) applied to P; returns the buffer holding the result
 // and, in mixed precision, the transposed split planes of it (else null).
-// Split-BF16 Taylor series: Q = P + sum_k out_k with out_k^T = out_{k-1}^T VHS / k.
+// Split-BF16 Taylor series: Q = P + sum_k out_k with out_k^T = out_{k-1}^T VHS / k;
+// term k < n writes its planes to tplanes[k], the last term sums them into Q and
+// writes the planes of Q^T to tplanes[0] (tplanes[1] when n = 1, where tplanes[0]
+// is the last term's input).
 int taylor_bf16(afqmc_ctx* c, int nw, const double** result, const uint16_t** resultT) {
   const int N = c->d.nbasis, nc = c->ncol, rpad = taylor_rows_padded(nc);
   double* P = c->buf[0];
@@ -764,26 +769,29 @@ int taylor_bf16(afqmc_ctx* c, int nw, const double** result, const uint16_t** re
     t.ncol = nc;
     t.rpad = rpad;
     t.kpad = c->kpad;
-    t.tin = c->tplanes[(k - 1) & 1];
-    t.tout = k < c->d.exp_nmax ? c->tplanes[k & 1] : nullptr;
+    const int n = c->d.exp_nmax;
+    t.tin = c->tplanes[k - 1];
+    t.tout = k < n ? c->tplanes[k] : nullptr;
     t.vatoms = c->vatoms;
     t.zatom = c->zatom;
     t.natom_side = vhs_atom_side(N);
     t.atom_stride = vhs_atom_count(N) * 1024;
     t.init = P;
     t.q = Q;
-    t.dacc = reinterpret_cast<float*>(c->buf[1]);  // unused by the split-BF16 path otherwise
     t.first = k == 1;
     t.last = k == c->d.exp_nmax;
     t.split = k <= TAYLOR_SPLIT_TERMS;
-    t.tout_hi_only = k + 1 > TAYLOR_SPLIT_TERMS;
     t.alpha = 1.0 / k;
-    t.qout = k == c->d.exp_nmax ? c->tplanes[k & 1] : nullptr;
+    t.qout = k == n ? c->tplanes[n == 1 ? 1 : 0] : nullptr;
+    if (k == n) {
+      t.nterms = n - 1;
+      for (int j = 1; j < n; ++j) t.terms[j - 1] = c->tplanes[j];
+    }
     t.variant = c->taylor_variant;
     HIP_TRY(c, launch_taylor_bf16(t, c->stream));
   }
   *result = Q;
-  if (resultT) *resultT = c->tplanes[c->d.exp_nmax & 1];
+  if (resultT) *resultT = c->tplanes[c->d.exp_nmax == 1 ? 1 : 0];
   return AFQMC_OK;
 }
 
@@ -1196,6 +1204,8 @@ int afqmc_create(afqmc_ctx** out, int device, const afqmc_dims* dims) {
     return AFQMC_ERR_SHAPE;
   if (d.nalpha > 64 || d.nbeta > 64 || d.nalpha > d.nbasis || d.nbeta > d.nbasis) return AFQMC_ERR_SHAPE;
   if (d.precision != AFQMC_PREC_FP64 && d.precision != AFQMC_PREC_MIXED) return AFQMC_ERR_UNSUPPORTED;
+  // the split-BF16 Taylor series keeps every term's planes until the last term
+  if (d.precision == AFQMC_PREC_MIXED && d.exp_nmax > TAYLOR_MAX_TERMS) return AFQMC_ERR_SHAPE;
   if (hipSetDevice(device) != hipSuccess) return AFQMC_ERR_HIP;
   afqmc_ctx* c = new afqmc_ctx();
   c->d = d;
@@ -1261,13 +1271,14 @@ void afqmc_destroy(afqmc_ctx* c) {
   else (void)hipDeviceSynchronize();
   void* ptrs[] = {c->vbias_part, c->xshifted, c->wvec, c->exx_a, c->exx_b, c->vhs, c->buf[0], c->buf[1],
                   c->buf[2], c->buf[3], c->tri, c->flags, c->wflags, c->psi_flag, c->nfb, c->gplanes[0], c->gplanes[1], c->lplanes[0],
-                  c->lplanes[1], c->xplanes, c->cplanes, c->vatoms, c->zatom, c->tplanes[0], c->dplanes, c->oplanes,
-                  c->tplanes[1], c->gcomp, c->gf_o, c->gf_oinv, c->qw64[0], c->qw64[1], c->qwp[0],
+                  c->lplanes[1], c->xplanes, c->cplanes, c->vatoms, c->zatom, c->dplanes, c->oplanes, c->gcomp, c->gf_o, c->gf_oinv, c->qw64[0], c->qw64[1], c->qwp[0],
                   c->qwp[1], c->qy, c->qunits[0], c->qunits[1], c->sur.feat, c->sur.pred, c->sur.est, c->sur.target,
                   c->sur.resid, c->sur.walker_of, c->sur.xbar, c->sur.e1, c->sur.master, c->sur.m,
                   c->sur.v, c->sur.wbf, c->sur.bias, c->sur.grad, c->sur.loss, c->sur.state, c->sur.gT,
                   c->sur.K, c->sur.et, c->sur.cv, c->sur.mlp, c->sur.gtmp, c->dstate};
   for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (void* p : c->tplanes)
     if (p) (void)hipFree(p);
   for (void* p : c->retired) (void)hipFree(p);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);

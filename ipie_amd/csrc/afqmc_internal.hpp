@@ -229,6 +229,9 @@ hipError_t launch_quad_sum(const double* part, int nslots, int nwalkers, double*
 
 // One split-BF16 Taylor term per walker: out_k^T = alpha out_{k-1}^T VHS,
 // q = (init or q) + out_k, and (tout) the split planes of out_k^T.
+// most Taylor terms of the split-BF16 series (afqmc_create refuses a larger
+// exp_nmax in mixed precision: by then |VHS|^k / k! is far below FP64 rounding)
+constexpr int TAYLOR_MAX_TERMS = 24;
 struct TaylorBf16Args {
   int nwalkers, wcap, nbasis, ncol, rpad, kpad;
   const uint16_t* tin;      // [4][wcap][rpad][kpad] planes of out_{k-1}^T
@@ -239,10 +242,11 @@ struct TaylorBf16Args {
   int64_t atom_stride;      // bf16 values per walker in vatoms (NAT 1024)
   const double* init;       // [W][N][ncol] complex: the series' input (read by the last term)
   double* q;                // [W][N][ncol] complex: init + sum of the terms (written by the last term)
-  float* dacc;              // [W][N][ncol] complex FP32: sum of the terms so far
+  // last term: the planes of the earlier terms out_1 .. out_nterms (layout of tin), summed into q
+  const uint16_t* terms[TAYLOR_MAX_TERMS];
+  int nterms;
   int first, last;          // this term is the first / the last of the series
   int split;                // 1: three BF16 products (hi hi + hi lo + lo hi), 0: hi hi only
-  int tout_hi_only;         // 1: write the hi planes of out_k^T only (the next term uses one product)
   double alpha;
   uint16_t* qout;           // last term: planes of q^T (layout of tin), or null
   int variant;              // 0: 32-wide K stages, two in LDS (taylor_bf16_kernel); 1: 16-wide, four
@@ -250,6 +254,9 @@ struct TaylorBf16Args {
 };
 hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream);
 // Taylor terms k <= TAYLOR_SPLIT_TERMS use three BF16 products, later ones one.
+// Each term but the last writes its four split planes to a buffer of its own
+// (the next term's operand, hi planes alone for a one-product term) and the
+// last term adds them all (hi + lo: 16 significant bits) to P.
 // Term k is about |VHS|^k / k! of phi (|VHS|_2 ~ 0.35 at dt = 0.005 for the
 // normalised N = 400 Hamiltonian: terms 1e-1, 2e-2, 1e-3, 1e-4, 6e-6, 4e-7), so
 // one product's 2^-8 relative error on terms 4.. stays below the three-product

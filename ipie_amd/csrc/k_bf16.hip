@@ -866,12 +866,12 @@ size_t bf16_lds_bytes(int wb, int xb, int nocc) {
 // ds_read_b64_tr_b16, two per plane.  Each stored atom is thus read by the
 // workgroups of both its row and its column block, which run together on one
 // XCD, so the second read is served by L2.
-// Epilogue: the terms are summed in FP32 in dacc (the first term writes it,
-// later ones add to it) and the last term writes q = init + dacc + out_k (FP64)
-// with the split planes of q^T (qout); every term but the last writes the
-// split planes of out_k^T for the next term.  The out^T planes have a walker stride fixed by the
-// buffer capacity wcap; their padding rows and K entries are never written
-// and must be zero.
+// Epilogue: every term but the last writes the split planes of out_k^T to a
+// buffer of its own (the next term's operand); the last term writes
+// q = init + (the earlier terms, read back from their planes) + out_k in FP64,
+// with the split planes of q^T (qout).  No running sum is kept between terms.
+// The out^T planes have a walker stride fixed by the buffer capacity wcap;
+// their padding rows and K entries are never written and must be zero.
 constexpr int TY_WAVES = 5, TY_R = 80, TY_C = 80, TY_THREADS = 64 * TY_WAVES;
 static_assert(TY_C == 5 * 16 && BX_KC == 2 * 16, "a workgroup's B chunk is 5 x 2 atoms");
 
@@ -921,29 +921,27 @@ struct Ty {
   __device__ static int phys(int pi) { return SPLIT ? pi : 2 * pi; }
 };
 
-// 4 consecutive K entries (of which `valid` are < N) of planes 0, pstep, .. of a
+// 4 consecutive K entries (of which `valid` are < N) of the four planes of a
 // transposed Taylor operand at offset o
 __device__ __forceinline__ void store_planes_ty(uint16_t* dst, int64_t tplane, int64_t o, const uint16_t (&h)[4][4],
-                                                int pstep, int valid) {
+                                                int valid) {
   if (valid >= 4) {
 #pragma unroll
     for (int p = 0; p < 4; ++p)
-      if (p % pstep == 0)
-        *reinterpret_cast<ushort4*>(dst + p * tplane + o) = make_ushort4(h[p][0], h[p][1], h[p][2], h[p][3]);
+      *reinterpret_cast<ushort4*>(dst + p * tplane + o) = make_ushort4(h[p][0], h[p][1], h[p][2], h[p][3]);
   } else {
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       if (r < valid)
 #pragma unroll
-        for (int p = 0; p < 4; ++p)
-          if (p % pstep == 0) dst[p * tplane + o + r] = h[p][r];
+        for (int p = 0; p < 4; ++p) dst[p * tplane + o + r] = h[p][r];
   }
 }
 
 // Epilogue of a Taylor term from the accumulators (transposed tiles: element r
 // of tile i is row crow = i 16 + l16, a column of the walker matrix, and basis
 // index kcol = cb 80 + wave 16 + 4 q + r, so 16 lanes touch 16 consecutive
-// columns of q / dacc and each lane 4 consecutive K of a plane).
+// columns of q and each lane 4 consecutive K of a plane).
 __device__ __forceinline__ void ty_epilogue(const TaylorBf16Args& a, const v4f (&re)[5], const v4f (&im)[5], int w,
                                             int rb, int cb, int wave, int l16, int q) {
   const int N = a.nbasis, Kp = a.kpad;
@@ -953,6 +951,30 @@ __device__ __forceinline__ void ty_epilogue(const TaylorBf16Args& a, const v4f (
   for (int i = 0; i < 5; ++i) {
     const int crow = rb * TY_R + i * 16 + l16;
     if (crow >= a.ncol || kc0 >= N) continue;
+    // a multiple of 4: Kp % 32 == 0, kc0 % 4 == 0; kc0 + 3 < Kp, and the
+    // planes' K padding (kcol >= N) is zero
+    const int64_t o = ((int64_t)w * a.rpad + crow) * Kp + kc0;
+    // last term: the earlier terms at these 4 K entries, read back from their
+    // planes (hi + lo), smallest terms first, in FP64
+    double sre[4] = {0.0, 0.0, 0.0, 0.0}, sim[4] = {0.0, 0.0, 0.0, 0.0};
+    if (a.last) {
+      for (int t = a.nterms - 1; t >= 0; --t) {
+        const uint16_t* tp = a.terms[t] + o;
+        const ushort4 hr = *reinterpret_cast<const ushort4*>(tp);
+        const ushort4 hi = *reinterpret_cast<const ushort4*>(tp + 2 * tplane);
+        const ushort4 lr = *reinterpret_cast<const ushort4*>(tp + tplane);
+        const ushort4 li = *reinterpret_cast<const ushort4*>(tp + 3 * tplane);
+        const double vr[4] = {(double)bf2f(hr.x) + bf2f(lr.x), (double)bf2f(hr.y) + bf2f(lr.y),
+                              (double)bf2f(hr.z) + bf2f(lr.z), (double)bf2f(hr.w) + bf2f(lr.w)};
+        const double vi[4] = {(double)bf2f(hi.x) + bf2f(li.x), (double)bf2f(hi.y) + bf2f(li.y),
+                              (double)bf2f(hi.z) + bf2f(li.z), (double)bf2f(hi.w) + bf2f(li.w)};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          sre[r] += vr[r];
+          sim[r] += vi[r];
+        }
+      }
+    }
     uint16_t ht[4][4], hq[4][4];  // [plane][r] of out_k and of q
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -961,31 +983,20 @@ __device__ __forceinline__ void ty_epilogue(const TaylorBf16Args& a, const v4f (
       for (int p = 0; p < 4; ++p) ht[p][r] = hq[p][r] = 0;
       if (kcol >= N) continue;
       const cplx v = cmake(a.alpha * re[i][r], a.alpha * im[i][r]);
-      const int64_t qi = ((int64_t)w * N + kcol) * a.ncol + crow;
       if (a.last) {
-        cplx qv = cadd(ld(a.init, qi), v);
-        if (!a.first) {
-          const float2 d = reinterpret_cast<const float2*>(a.dacc)[qi];
-          qv.re += (double)d.x;
-          qv.im += (double)d.y;
-        }
+        const int64_t qi = ((int64_t)w * N + kcol) * a.ncol + crow;
+        const cplx qv = cadd(ld(a.init, qi), cmake(v.re + sre[r], v.im + sim[r]));
         st(a.q, qi, qv);
         split_bf16(qv.re, hq[0][r], hq[1][r]);
         split_bf16(qv.im, hq[2][r], hq[3][r]);
-      } else {
-        float2 d = a.first ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(a.dacc)[qi];
-        d.x = (float)((double)d.x + v.re);
-        d.y = (float)((double)d.y + v.im);
-        reinterpret_cast<float2*>(a.dacc)[qi] = d;
       }
       split_bf16(v.re, ht[0][r], ht[1][r]);
       split_bf16(v.im, ht[2][r], ht[3][r]);
     }
-    const int64_t o = ((int64_t)w * a.rpad + crow) * Kp + kc0;  // a multiple of 4: Kp % 32 == 0, kc0 % 4 == 0
-    // planes of out_k^T (a next term with one product reads the hi planes 0 and 2
-    // only) and, after the last term, of q^T
-    if (a.tout) store_planes_ty(a.tout, tplane, o, ht, a.tout_hi_only ? 2 : 1, N - kc0);
-    if (a.qout) store_planes_ty(a.qout, tplane, o, hq, 1, N - kc0);
+    // planes of out_k^T (all four: the last term sums hi + lo, a one-product
+    // next term reads the hi planes 0 and 2) and, after the last term, of q^T
+    if (a.tout) store_planes_ty(a.tout, tplane, o, ht, N - kc0);
+    if (a.qout) store_planes_ty(a.qout, tplane, o, hq, N - kc0);
   }
 }
 
@@ -1420,7 +1431,8 @@ hipError_t launch_taylor_bf16(const TaylorBf16Args& a, hipStream_t stream) {
       a.kpad % BX_KC != 0 || a.kpad < a.nbasis || a.init == nullptr || (a.last && a.q == nullptr) ||
       a.vatoms == nullptr || a.zatom == nullptr || a.natom_side != vhs_atom_side(a.nbasis) ||
       a.atom_stride < vhs_atom_count(a.nbasis) * 1024 ||
-      (!(a.first && a.last) && a.dacc == nullptr) || a.variant < 0 || a.variant > 1)
+      a.nterms < 0 || a.nterms >= TAYLOR_MAX_TERMS || (a.last && a.nterms != 0 && a.terms[a.nterms - 1] == nullptr) ||
+      a.variant < 0 || a.variant > 1)
     return hipErrorInvalidValue;
   const int64_t grid = (int64_t)a.nwalkers * (a.rpad / TY_R) * ceil_div(a.nbasis, TY_C);
   if (a.variant == 1) {

@@ -13,7 +13,8 @@ arithmetic itself rather than from the device's own output:
   force bias   vbias = L~ Ghalf per spin, split, FP32
   VHS          i sqrt(dt) sum_x xshifted_x L_x, split, kept in FP32
   Taylor       term k = VHS term_{k-1} / k: terms 1-3 split (three products),
-               4-6 hi * hi only; terms 1-5 summed in FP32; Q = P + sum + term 6 in FP64
+               4-6 hi * hi only; terms 1-5 kept as their split planes (hi + lo),
+               Q = P + their sum + term 6 in FP64
   exchange     g^T Wq g per spin (quadratic form, Wq = sum_x L_x (x) L_x) or
                sum_x tr(T_x T_x) (T-form, X < N), split, FP32 accumulation,
                the final dot / trace in FP64
@@ -123,18 +124,28 @@ def vhs(chol_packed, xshifted_wx, sqrt_dt, idx_i, idx_j, nbasis):
     return afqmc.unpack_VHS_batch(idx_i, idx_j, packed, nbasis)
 
 
+def planes_value(x):
+    """The value a term's split planes hold: hi + lo."""
+    if EXACT:
+        return numpy.asarray(x, dtype=numpy.float64)
+    hi, lo = split(x)
+    return hi.astype(numpy.float64) + lo.astype(numpy.float64)
+
+
 def taylor(phi, VHS, exp_nmax=6, split_terms=3):
     """P + sum_k term_k per walker (see the module docstring for the precision of each term)."""
     out = numpy.empty_like(phi)
     for w in range(phi.shape[0]):
         t = phi[w]
-        c32 = numpy.complex128 if EXACT else numpy.complex64
-        acc = numpy.zeros(phi[w].shape, dtype=c32)
+        terms = []
         for k in range(1, exp_nmax + 1):
             t = cmm(VHS[w], t, products=3 if k <= split_terms else 1) / k
             if k < exp_nmax:
-                acc = (acc + t.astype(c32)).astype(c32)
-        out[w] = phi[w] + acc.astype(numpy.complex128) + t
+                terms.append(planes_value(t.real) + 1j * planes_value(t.imag))
+        acc = numpy.zeros(phi[w].shape, dtype=numpy.complex128)
+        for term in reversed(terms):
+            acc = acc + term
+        out[w] = phi[w] + (t + acc)
     return out
 
 
