@@ -1,5 +1,5 @@
 """The C-ABI library loads and exports every symbol include/afqmc_hip.h declares
-(no GPU needed: nothing is called)."""
+(no GPU needed: nothing that reaches a device is called)."""
 
 import ctypes
 import os
@@ -40,3 +40,18 @@ def test_product_never_imports_the_oracle():
             if f.endswith(".py"):
                 src = open(os.path.join(dirpath, f)).read()
                 assert not re.search(r"^\s*(from|import)\s+oracle\b", src, flags=re.M), f
+
+
+def test_mixed_context_refuses_a_taylor_order_beyond_its_term_buffers():
+    """The split-BF16 Taylor series keeps every term's planes until the last
+    term (at most 24 terms): afqmc_create refuses a larger exp_nmax in mixed
+    precision with AFQMC_ERR_SHAPE, before it touches a device."""
+    from ipie_amd import _lib
+
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    lib.afqmc_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(_lib.Dims)]
+    lib.afqmc_create.restype = ctypes.c_int
+    ctx = ctypes.c_void_p()
+    dims = _lib.Dims(24, 72, 5, 5, 10, 25, 0, _lib.PRECISIONS["mixed"])
+    assert lib.afqmc_create(ctypes.byref(ctx), 0, ctypes.byref(dims)) == 2
+    assert ctx.value is None
