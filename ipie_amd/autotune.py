@@ -13,7 +13,7 @@ Eight launch shapes of the step depend on the system's dimensions:
   tile_vhs       workgroup tile of the split-BF16 VHS GEMM
   tile_one_body  workgroup tile of the split-BF16 one-body GEMM
                  (tiles: 0 = 128 x 160, 1 = 128 x 128, 2 = 128 x 80,
-                 3 = 64 x 160 walker rows x output columns)
+                 3 = 64 x 160, 4 = 128 x 320 walker rows x output columns)
   taylor         kernel of the split-BF16 Taylor series: 32-wide K stages, two
                  in LDS (0); 16-wide stages, four in LDS (1)
 
@@ -57,7 +57,7 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "ipie_amd", "aut
 _FB = (1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 24, 32)
 _QUAD = (512, 768, 1024, 1536, 2048, 3072, 4096, 6144)
 _EXX = (1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 192, 256)
-_TILE = (0, 1, 2, 3)
+_TILE = (0, 1, 2, 3, 4)
 _RGROUP = (1, 2, 4, 8, 16, 32)
 
 

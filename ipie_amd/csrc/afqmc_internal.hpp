@@ -201,7 +201,7 @@ struct Bf16GemmArgs {
   int rgroup;
 };
 hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream);
-constexpr int BF16_TILES = 4;
+constexpr int BF16_TILES = 5;
 // rows x columns of a workgroup tile shape (false for an unknown shape)
 bool bf16_tile_dims(int tile, int* rows, int* cols);
 

@@ -39,19 +39,22 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 // Workgroup tile of the BF16 GEMM: GR x GC waves, each owning MR x MC output
-// tiles of 16 x 16, so R = 16 GR MR rows by C = 16 GC MC columns.
-template <int GR_, int GC_, int MR_, int MC_>
+// tiles of 16 x 16, so R = 16 GR MR rows by C = 16 GC MC columns, with ST LDS
+// stages (ST - 1 in flight while one is computed).
+template <int GR_, int GC_, int MR_, int MC_, int ST_ = 3>
 struct Tile {
-  static constexpr int GR = GR_, GC = GC_, MR = MR_, MC = MC_;
+  static constexpr int GR = GR_, GC = GC_, MR = MR_, MC = MC_, ST = ST_;
   static constexpr int R = 16 * GR * MR, C = 16 * GC * MC, THREADS = 64 * GR * GC;
 };
 // The shapes bf16_gemm_kernel is built for (afqmc_tune_set picks one per
 // contraction; the exchange and the quadratic form use shape 0).  All have 8
-// waves and three LDS stages of the 3-product form within 160 KiB.
-using Tile0 = Tile<4, 2, 2, 5>;  // 128 x 160
-using Tile1 = Tile<4, 2, 2, 4>;  // 128 x 128
-using Tile2 = Tile<8, 1, 1, 5>;  // 128 x 80
-using Tile3 = Tile<4, 2, 1, 5>;  // 64 x 160
+// waves; shapes 0-3 three LDS stages of the 3-product form within 160 KiB,
+// shape 4 (64 x 80 per wave: 29% fewer LDS bytes per MFMA than 32 x 80) two.
+using Tile0 = Tile<4, 2, 2, 5>;     // 128 x 160
+using Tile1 = Tile<4, 2, 2, 4>;     // 128 x 128
+using Tile2 = Tile<8, 1, 1, 5>;     // 128 x 80
+using Tile3 = Tile<4, 2, 1, 5>;     // 64 x 160
+using Tile4 = Tile<2, 4, 4, 5, 2>;  // 128 x 320
 
 // 8 waves in a BX_GR x BX_GC grid, each owning BX_MR x BX_MC tiles of 16 x 16 (shape 0)
 constexpr int BX_GR = Tile0::GR, BX_GC = Tile0::GC, BX_MR = Tile0::MR, BX_MC = Tile0::MC;
@@ -76,15 +79,16 @@ struct Bx {
   static constexpr int STAGE_VEC = KS * SUB_VEC;
   static constexpr int A_PER = A_VEC / TL::THREADS;
   static constexpr int B_PER = (B_VEC + TL::THREADS - 1) / TL::THREADS;
-  static constexpr size_t STAGES = 2 * 16 * (size_t)STAGE_VEC;  // double-buffered, bytes
-  static constexpr size_t STAGES3 = 3 * 16 * (size_t)STAGE_VEC;  // triple-buffered (bf16_gemm_kernel)
+  static constexpr size_t STAGES = 2 * 16 * (size_t)STAGE_VEC;  // double-buffered, bytes (exchange_bf16_kernel)
+  static constexpr size_t STAGES_ALL = TL::ST * 16 * (size_t)STAGE_VEC;  // the tile's stages (bf16_gemm_kernel)
   static_assert(A_VEC % TL::THREADS == 0, "A staging is split evenly over the threads");
   static_assert((B_VEC % TL::THREADS) % 64 == 0, "the partial B vector round is wave-uniform");
-  static_assert(STAGES <= BX_LDS_MAX, "staging fits the workgroup LDS");
-  static_assert(STAGES3 <= 163840, "three stages fit the 160 KiB of LDS");
+  static_assert(STAGES_ALL <= 163840, "the stages fit the 160 KiB of LDS");
   __device__ static int a_plane(int p) { return SPLIT ? p : 2 * p; }
   __device__ static int b_plane(int p) { return SPLIT ? p : 0; }
 };
+
+static_assert(Bx<true>::STAGES <= BX_LDS_MAX, "two stages of the exchange kernel fit its LDS budget");
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
   uint32_t u = __float_as_uint(f);
@@ -640,33 +644,37 @@ __global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) 
         im[i][j] = zero4f();
       }
     // LDS stages of up to KS sub-chunks of 32 K values over this split's range
-    // [c0, c1), three buffers: stage t + 2 is staged while t is computed, from
-    // registers loaded during stage t - 1 (two stages of latency cover)
+    // [c0, c1), TL::ST buffers: stage t + ST - 1 is staged while t is computed
+    // (three stages: from registers loaded during stage t - 1 without DL)
     const int nst = (int)ceil_div(c1 - c0, C::KS);
     auto nsub_of = [&](int t) { return min(C::KS, c1 - (c0 + t * C::KS)); };
     if constexpr (DL) {
-      // direct loads: stage t + 2 is issued before stage t is computed; the
-      // wait after it leaves stage t + 2 in flight and lands stage t + 1
+      // direct loads: stage t + ST - 1 is issued before stage t is computed; the
+      // wait after it leaves that stage in flight and lands stage t + 1
+      constexpr int ST = TL::ST;
+      static_assert(ST == 2 || ST == 3, "two or three LDS stages");
       int64_t da[(Dl<SPLIT, TL>::A_INS)], db[(Dl<SPLIT, TL>::B_INS)];
       dl_sources<SPLIT, TL>(da, db, a, rb, cb, wave, lane);
       if (nst > 0) dl_issue<SPLIT, TL>(lds_u4, 0, nsub_of(0), a.a, a.b, da, db, c0, wave, lane);
-      if (nst > 1) dl_issue<SPLIT, TL>(lds_u4, 1, nsub_of(1), a.a, a.b, da, db, c0 + C::KS, wave, lane);
-      dl_wait<SPLIT, TL>(nst > 1 ? nsub_of(1) : 0);
+      if (ST == 3 && nst > 1) dl_issue<SPLIT, TL>(lds_u4, 1, nsub_of(1), a.a, a.b, da, db, c0 + C::KS, wave, lane);
+      dl_wait<SPLIT, TL>(ST == 3 && nst > 1 ? nsub_of(1) : 0);
       dl_barrier();
       int buf = 0;
       for (int t = 0; t < nst; ++t) {
         int pend = 0;
-        if (t + 2 < nst) {
-          dl_issue<SPLIT, TL>(lds_u4, buf == 0 ? 2 : buf - 1, nsub_of(t + 2), a.a, a.b, da, db, c0 + (t + 2) * C::KS,
-                          wave, lane);
-          pend = nsub_of(t + 2);
+        if (t + ST - 1 < nst) {
+          dl_issue<SPLIT, TL>(lds_u4, buf == 0 ? ST - 1 : buf - 1, nsub_of(t + ST - 1), a.a, a.b, da, db,
+                              c0 + (t + ST - 1) * C::KS, wave, lane);
+          pend = nsub_of(t + ST - 1);
         }
         bx_chunk<SPLIT, TR, TL>(lds_u4 + buf * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub_of(t));
-        dl_wait<SPLIT, TL>(pend);
+        // three stages: chunk t + 2 stays in flight; two: chunk t + 1 must land
+        dl_wait<SPLIT, TL>(ST == 3 ? pend : 0);
         dl_barrier();
-        buf = buf == 2 ? 0 : buf + 1;
+        buf = buf == ST - 1 ? 0 : buf + 1;
       }
     } else {
+      static_assert(TL::ST == 3, "register staging uses three stages");
     v4u pa[C::KS][C::A_PER], pb[C::KS][C::B_PER];
     if (nst > 0) {
       bx_fetch<SPLIT, TL>(pa, pb, a.a, a.b, a_src, b_src, c0, nsub_of(0));
@@ -1375,7 +1383,7 @@ hipError_t launch_bf16_gemm_t(const Bf16GemmArgs& a, hipStream_t stream) {
   const int64_t nb = ceil_div(a.nn, TL::C), mb = ceil_div(a.m, TL::R);
   const int64_t mbg = (a.units && a.rgroup > 0 && a.rgroup < mb) ? ceil_div(mb, a.rgroup) * a.rgroup : mb;
   const int64_t grid = mbg * (a.units ? (int64_t)a.nunits : nb * a.ksplit);
-  constexpr size_t lds = Bx<SPLIT, TL>::STAGES3;
+  constexpr size_t lds = Bx<SPLIT, TL>::STAGES_ALL;
   hipError_t e = allow_lds((const void*)bf16_gemm_kernel<SPLIT, TR, kDl, TL>, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((bf16_gemm_kernel<SPLIT, TR, kDl, TL>), dim3((unsigned)grid), dim3(TL::THREADS), lds, stream, a);
@@ -1388,6 +1396,7 @@ hipError_t launch_bf16_gemm_shape(const Bf16GemmArgs& a, hipStream_t stream) {
     case 1: return launch_bf16_gemm_t<true, TR, Tile1>(a, stream);
     case 2: return launch_bf16_gemm_t<true, TR, Tile2>(a, stream);
     case 3: return launch_bf16_gemm_t<true, TR, Tile3>(a, stream);
+    case 4: return launch_bf16_gemm_t<true, TR, Tile4>(a, stream);
     default: return launch_bf16_gemm_t<true, TR, Tile0>(a, stream);
   }
 }
@@ -1422,6 +1431,7 @@ bool bf16_tile_dims(int tile, int* rows, int* cols) {
     case 1: *rows = Tile1::R; *cols = Tile1::C; return true;
     case 2: *rows = Tile2::R; *cols = Tile2::C; return true;
     case 3: *rows = Tile3::R; *cols = Tile3::C; return true;
+    case 4: *rows = Tile4::R; *cols = Tile4::C; return true;
     default: return false;
   }
 }
