@@ -40,10 +40,11 @@ typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 // Workgroup tile of the BF16 GEMM: GR x GC waves, each owning MR x MC output
 // tiles of 16 x 16, so R = 16 GR MR rows by C = 16 GC MC columns, with ST LDS
-// stages (ST - 1 in flight while one is computed).
-template <int GR_, int GC_, int MR_, int MC_, int ST_ = 3>
+// stages (ST - 1 in flight while one is computed) of KS1 sub-chunks of 32 K
+// values each in the 1-product form (one in the 3-product form).
+template <int GR_, int GC_, int MR_, int MC_, int ST_ = 3, int KS1_ = 2>
 struct Tile {
-  static constexpr int GR = GR_, GC = GC_, MR = MR_, MC = MC_, ST = ST_;
+  static constexpr int GR = GR_, GC = GC_, MR = MR_, MC = MC_, ST = ST_, KS1 = KS1_;
   static constexpr int R = 16 * GR * MR, C = 16 * GC * MC, THREADS = 64 * GR * GC;
 };
 // The shapes bf16_gemm_kernel is built for (afqmc_tune_set picks one per
@@ -68,12 +69,13 @@ constexpr size_t BX_LDS_MAX = 106496;  // LDS of a workgroup: two stages of the 
 // Staging, in 16-byte vectors.  SPLIT (3 products): A planes re_hi, re_lo,
 // im_hi, im_lo and B planes hi, lo.  Otherwise (1 product, hi * hi): A planes
 // re_hi, im_hi (physical planes 0 and 2) and B plane hi.  An LDS stage holds
-// KS sub-chunks of 32 K values (the 1-product form uses two, halving the
-// barriers per unit of MFMA work; the 3-product stage already fills the LDS).
+// KS sub-chunks of 32 K values (the 1-product form uses the tile's KS1, two by
+// default, halving the barriers per unit of MFMA work; the 3-product stage
+// already fills the LDS).
 template <bool SPLIT, class TL = Tile0>
 struct Bx {
   static constexpr int NA = SPLIT ? 4 : 2, NB = SPLIT ? 2 : 1;
-  static constexpr int KS = SPLIT ? 1 : 2;
+  static constexpr int KS = SPLIT ? 1 : TL::KS1;
   static constexpr int A_VEC = NA * TL::R * 4, B_VEC = NB * TL::C * 4;  // per sub-chunk
   static constexpr int SUB_VEC = A_VEC + B_VEC;
   static constexpr int STAGE_VEC = KS * SUB_VEC;
@@ -351,7 +353,9 @@ __device__ __forceinline__ void dl_barrier() { asm volatile("s_waitcnt lgkmcnt(0
 template <bool SPLIT, class TL = Tile0>
 __device__ __forceinline__ void dl_wait(int pend) {
   constexpr int I = Dl<SPLIT, TL>::INS_SUB;
-  if (pend >= 2) wait_vm<2 * I>();
+  static_assert(3 * I <= 63, "vmcnt counts three sub-chunks of loads");
+  if (pend >= 3) wait_vm<3 * I>();
+  else if (pend == 2) wait_vm<2 * I>();
   else if (pend == 1) wait_vm<I>();
   else wait_vm<0>();
 }
@@ -406,12 +410,20 @@ __device__ __forceinline__ void bx_sub(const uint4* sub, int wr, int wc, int l16
   }
 }
 
-// The nsub sub-chunks of LDS stage `stage`.
+// The nsub sub-chunks of LDS stage `stage`.  A full stage is one straight
+// block, so that the LDS reads of a sub-chunk can be scheduled under the
+// MFMAs of the one before.
 template <bool SPLIT, bool TR = false, class TL = Tile0>
 __device__ __forceinline__ void bx_chunk(const uint4* stage, int wr, int wc, int l16, int q,
                                          v4f (&re)[(TL::MR)][(TL::MC)], v4f (&im)[(TL::MR)][(TL::MC)], int nsub) {
+  constexpr int KS = Bx<SPLIT, TL>::KS;
+  if (nsub == KS) {
 #pragma unroll
-  for (int sc = 0; sc < Bx<SPLIT, TL>::KS; ++sc) {
+    for (int sc = 0; sc < KS; ++sc) bx_sub<SPLIT, TR, TL>(stage + sc * Bx<SPLIT, TL>::SUB_VEC, wr, wc, l16, q, re, im);
+    return;
+  }
+#pragma unroll
+  for (int sc = 0; sc < KS; ++sc) {
     if (sc >= nsub) break;
     bx_sub<SPLIT, TR, TL>(stage + sc * Bx<SPLIT, TL>::SUB_VEC, wr, wc, l16, q, re, im);
   }
@@ -652,24 +664,31 @@ __global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) 
       // direct loads: stage t + ST - 1 is issued before stage t is computed; the
       // wait after it leaves that stage in flight and lands stage t + 1
       constexpr int ST = TL::ST;
-      static_assert(ST == 2 || ST == 3, "two or three LDS stages");
+      static_assert(ST >= 2 && ST <= 4 && (ST - 2) * C::KS <= 3, "two to four LDS stages, at most 3 sub-chunks pending");
       int64_t da[(Dl<SPLIT, TL>::A_INS)], db[(Dl<SPLIT, TL>::B_INS)];
       dl_sources<SPLIT, TL>(da, db, a, rb, cb, wave, lane);
-      if (nst > 0) dl_issue<SPLIT, TL>(lds_u4, 0, nsub_of(0), a.a, a.b, da, db, c0, wave, lane);
-      if (ST == 3 && nst > 1) dl_issue<SPLIT, TL>(lds_u4, 1, nsub_of(1), a.a, a.b, da, db, c0 + C::KS, wave, lane);
-      dl_wait<SPLIT, TL>(ST == 3 && nst > 1 ? nsub_of(1) : 0);
+      // prologue: stages 0 .. ST - 2 issued; stage 0 landed, the rest may stay in flight
+      int pend0 = 0;
+#pragma unroll
+      for (int s = 0; s < ST - 1; ++s)
+        if (s < nst) {
+          dl_issue<SPLIT, TL>(lds_u4, s, nsub_of(s), a.a, a.b, da, db, c0 + s * C::KS, wave, lane);
+          if (s > 0) pend0 += nsub_of(s);
+        }
+      dl_wait<SPLIT, TL>(pend0);
       dl_barrier();
       int buf = 0;
       for (int t = 0; t < nst; ++t) {
-        int pend = 0;
-        if (t + ST - 1 < nst) {
+        if (t + ST - 1 < nst)
           dl_issue<SPLIT, TL>(lds_u4, buf == 0 ? ST - 1 : buf - 1, nsub_of(t + ST - 1), a.a, a.b, da, db,
                               c0 + (t + ST - 1) * C::KS, wave, lane);
-          pend = nsub_of(t + ST - 1);
-        }
         bx_chunk<SPLIT, TR, TL>(lds_u4 + buf * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub_of(t));
-        // three stages: chunk t + 2 stays in flight; two: chunk t + 1 must land
-        dl_wait<SPLIT, TL>(ST == 3 ? pend : 0);
+        // stage t + 1 must land; the ST - 2 stages after it may stay in flight
+        int pend = 0;
+#pragma unroll
+        for (int s = 2; s < ST; ++s)
+          if (t + s < nst) pend += nsub_of(t + s);
+        dl_wait<SPLIT, TL>(pend);
         dl_barrier();
         buf = buf == ST - 1 ? 0 : buf + 1;
       }
@@ -697,38 +716,61 @@ __global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) 
     }
     }
     if (a.epilogue == BF16_EPI_QDOT) {
+      // (the quadratic form runs with tile shape 0 and plain accumulators only:
+      // launch_bf16_gemm; each epilogue is compiled only where it can run)
+      if constexpr (!TR && std::is_same<TL, Tile0>::value) {
       // per-row dot with d over this wave's 80 columns, then across the 16 lanes
-      // of a row; lane l16 = 4 i + r keeps the sum of row (i, r)
+      // of a row; lane l16 = 4 i + r keeps the sum of row (i, r).  The lane's
+      // columns and d_ref are set up once, and the next row's d values are
+      // loaded before this row's dot and cross-lane sum (one row of loads in
+      // flight under the arithmetic; same sums in the same order as row by row).
+      int64_t offj[(TL::MC)];
+      bool okj[(TL::MC)];
+      cplx refj[(TL::MC)];
+#pragma unroll
+      for (int j = 0; j < TL::MC; ++j) {
+        const int64_t col = (int64_t)cb * TL::C + (wc * TL::MC + j) * 16 + l16;
+        const int io = (int)(col / a.dkp), mm = (int)(col % a.dkp);
+        okj[j] = col < a.nn && mm < a.dn;
+        offj[j] = okj[j] ? (int64_t)io * a.dn + mm : 0;  // offset 0 of the row outside the matrix
+        refj[j] = (a.dref && okj[j]) ? ld(a.dref, offj[j]) : cmake(0.0, 0.0);
+      }
+      auto load_row = [&](int i, int r, cplx (&d)[(TL::MC)]) __attribute__((always_inline)) {
+        int64_t row = (int64_t)rb * TL::R + (wr * TL::MR + i) * 16 + 4 * q + r;
+        if (row > a.m - 1) row = a.m - 1;  // clamped rows are computed, never stored
+        const double* drow = a.dg + 2 * (a.walker_of ? (int64_t)a.walker_of[row] : row) * a.dstride;
+#pragma unroll
+        for (int j = 0; j < TL::MC; ++j) d[j] = ld(drow, offj[j]);
+      };
       double myr = 0.0, myi = 0.0;
+      cplx dnext[(TL::MC)];
+      load_row(0, 0, dnext);
 #pragma unroll
       for (int i = 0; i < TL::MR; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int64_t row = (int64_t)rb * TL::R + (wr * TL::MR + i) * 16 + 4 * q + r;
-          if (row > a.m - 1) row = a.m - 1;  // clamped rows are computed, never stored
-          const double* drow = a.dg + 2 * (a.walker_of ? (int64_t)a.walker_of[row] : row) * a.dstride;
-          double er = 0.0, ei = 0.0;
+          cplx d[(TL::MC)];
+#pragma unroll
+          for (int j = 0; j < TL::MC; ++j) d[j] = dnext[j];
+          if (i * 4 + r + 1 < 4 * TL::MR) load_row((i * 4 + r + 1) / 4, (i * 4 + r + 1) % 4, dnext);
+          double sr = 0.0, si = 0.0;
 #pragma unroll
           for (int j = 0; j < TL::MC; ++j) {
-            const int64_t col = (int64_t)cb * TL::C + (wc * TL::MC + j) * 16 + l16;
-            const int io = (int)(col / a.dkp), mm = (int)(col % a.dkp);
-            if (col < a.nn && mm < a.dn) {
-              const int64_t off = (int64_t)io * a.dn + mm;
-              cplx d = ld(drow, off);
-              if (a.dref) d = csub(d, ld(a.dref, off));
+            if (okj[j]) {
+              const cplx dd = csub(d[j], refj[j]);
               const double yr = re[i][j][r], yi = im[i][j][r];
-              er += d.re * yr - d.im * yi;
-              ei += d.re * yi + d.im * yr;
+              sr += dd.re * yr - dd.im * yi;
+              si += dd.re * yi + dd.im * yr;
             }
           }
 #pragma unroll
           for (int o = 8; o >= 1; o >>= 1) {
-            er += __shfl_xor(er, o, 16);
-            ei += __shfl_xor(ei, o, 16);
+            sr += __shfl_xor(sr, o, 16);
+            si += __shfl_xor(si, o, 16);
           }
           if (l16 == 4 * i + r) {
-            myr = er;
-            myi = ei;
+            myr = sr;
+            myi = si;
           }
         }
       if (l16 < 4 * TL::MR) {
@@ -737,9 +779,14 @@ __global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) 
         const int64_t slot = (a.units ? (int64_t)cu : (int64_t)cb * a.ksplit + z) * TL::GC + wc;
         if (row < a.m) st(a.out, slot * a.m + row, cmake(a.scale * myr, a.scale * myi));
       }
+      }
     } else if (a.epilogue == BF16_EPI_ADD) {
+      if constexpr (TR) {
       // transposed accumulators (TR): lane l16 walks the rows, so the FP64
       // reads and writes of 16 lanes are consecutive columns c of one walker
+      const double* __restrict__ bsrc = a.base;
+      double* __restrict__ odst = a.out;
+      uint16_t* __restrict__ tpd = a.tp;
 #pragma unroll
       for (int i = 0; i < TL::MR; ++i) {
         const int64_t row = (int64_t)rb * TL::R + (wr * TL::MR + i) * 16 + l16;
@@ -749,42 +796,52 @@ __global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) 
 #pragma unroll
         for (int j = 0; j < TL::MC; ++j) {
           const int64_t col0 = (int64_t)cb * TL::C + (wc * TL::MC + j) * 16 + 4 * q;
+          // the 4 base values are loaded before any store (columns past the end
+          // read the last column), so the loads are in flight together; base
+          // and out are distinct buffers or the same one indexed alike
+          cplx base[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int64_t col = col0 + r > a.nn - 1 ? a.nn - 1 : col0 + r;
+            base[r] = ld(bsrc, w * a.b_ws + col * a.b_ld + a.b_off + c);
+          }
           uint16_t h[4][4];  // [plane][r]: re_hi, re_lo, im_hi, im_lo
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int64_t col = col0 + r;
             h[0][r] = h[1][r] = h[2][r] = h[3][r] = 0;
             if (col >= a.nn) continue;
-            const cplx base = ld(a.base, w * a.b_ws + col * a.b_ld + a.b_off + c);
-            const cplx v = cmake(base.re + a.scale * re[i][j][r], base.im + a.scale * im[i][j][r]);
-            st(a.out, w * a.o_ws + col * a.o_ld + a.o_off + c, v);
+            const cplx v = cmake(base[r].re + a.scale * re[i][j][r], base[r].im + a.scale * im[i][j][r]);
+            st(odst, w * a.o_ws + col * a.o_ld + a.o_off + c, v);
             split_bf16(v.re, h[0][r], h[1][r]);
             split_bf16(v.im, h[2][r], h[3][r]);
           }
-          if (a.tp && col0 < a.nn) {
+          if (tpd && col0 < a.nn) {
             const int64_t o = (w * a.tp_rows + a.tp_off + c) * a.tp_ld + col0;
             if (col0 + 3 < a.nn && (o & 3) == 0) {
 #pragma unroll
               for (int p = 0; p < 4; ++p)
-                *reinterpret_cast<ushort4*>(a.tp + p * a.tp_plane + o) =
+                *reinterpret_cast<ushort4*>(tpd + p * a.tp_plane + o) =
                     make_ushort4(h[p][0], h[p][1], h[p][2], h[p][3]);
             } else {
 #pragma unroll
               for (int r = 0; r < 4; ++r)
                 if (col0 + r < a.nn)
 #pragma unroll
-                  for (int p = 0; p < 4; ++p) a.tp[p * a.tp_plane + o + r] = h[p][r];
+                  for (int p = 0; p < 4; ++p) tpd[p * a.tp_plane + o + r] = h[p][r];
             }
           }
         }
       }
+      }
     } else if (a.epilogue == BF16_EPI_VATOM) {
+      if constexpr (TR) {
       // transposed accumulators (TR): each lane holds 4 consecutive columns of one
       // row, so each plane takes one 8-byte store (4 entries of an atom row)
 #pragma unroll
       for (int i = 0; i < (TL::MR); ++i) {
         const int64_t row = (int64_t)rb * TL::R + (wr * TL::MR + i) * 16 + l16;
-        if (!TR || row >= a.m) continue;
+        if (row >= a.m) continue;
         uint16_t* drow = a.out16 + row * a.atom_stride;
 #pragma unroll
         for (int j = 0; j < (TL::MC); ++j) {
@@ -804,7 +861,8 @@ __global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) 
             *reinterpret_cast<ushort4*>(dst + 256 * p) = make_ushort4(h[p][0], h[p][1], h[p][2], h[p][3]);
         }
       }
-    } else {
+      }
+    } else if constexpr (!TR) {
       const int64_t NN = (int64_t)a.nbasis * a.nbasis;
 #pragma unroll
       for (int j = 0; j < TL::MC; ++j) {
