@@ -24,7 +24,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "walker-steps/s at N=400,X=2400: FP64 vs mixed+surrogate at <=0.1 mHa bias"
-PEAK = {"f64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
+PEAK = {"f64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops": 5000.0, "hbm_gbs": 8000.0}
 REORTHO_EVERY = 5
 SURROGATE_WARMUP = 100
 BIAS_BUDGET_HA = 1e-4
@@ -33,13 +33,19 @@ BIAS_BUDGET_HA = 1e-4
 VARIANCE_BUDGET_FRAC = 0.01
 # modes run by default per workload: the headline compares the FP64 step with the
 # mixed-precision modes; the surrogate workload compares the control variate
-# (with and without its MLP term, and without its quadratic term) with the
-# all-exact mixed-precision estimator ("mixed+surrogate-linear" also runs at
-# n400 when named with --modes)
+# (with and without its MLP term, with its quadratic term in BF16 instead of
+# FP8, and without it) with the all-exact mixed-precision estimator (the
+# ablations also run at n400 when named with --modes)
 DEFAULT_MODES = {
     "n400": ("fp64", "mixed", "mixed+surrogate"),
-    "n300": ("mixed", "mixed+surrogate", "mixed+surrogate-nomlp", "mixed+surrogate-linear"),
+    "n300": ("mixed", "mixed+surrogate", "mixed+surrogate-bf16q", "mixed+surrogate-nomlp", "mixed+surrogate-linear"),
 }
+
+
+def quadratic_mode(mode):
+    """The control variate's quadratic term for a surrogate mode: "none"
+    (-linear), "bf16" (-bf16q) or "fp8"."""
+    return "none" if mode.endswith("-linear") else ("bf16" if mode.endswith("-bf16q") else "fp8")
 
 
 def log(*a):
@@ -110,9 +116,9 @@ def build_device(host, w, mode, tuner=None):
     surrogate = None
     if "surrogate" in mode:
         # "-nomlp": the physical control variate alone (train_iters = 0 turns the MLP term off);
-        # "-linear": the control variate without its quadratic term Q_bf16(dG)
+        # "-linear": without its quadratic term Q(dG); "-bf16q": Q in BF16 instead of FP8
         surrogate = dict(fraction=0.125, warmup=SURROGATE_WARMUP, train_iters=0 if mode.endswith("-nomlp") else 4,
-                         lr=1e-3, quadratic=not mode.endswith("-linear"))
+                         lr=1e-3, quadratic=quadratic_mode(mode))
     t0 = time.time()
     walkers = UHFWalkers(host["psi"], w.nalpha, w.nbeta, w.nbasis, w.nwalkers)
     walkers.build(host["trial"])
@@ -594,10 +600,10 @@ def run_modes(args, w, host, mode_list, K, warmup, tuner):
                 "fraction_exact": 0.125,
                 "warmup_steps": SURROGATE_WARMUP,
                 "last_subset": int(ss["last_k"]),
-                "quadratic_term": not mode.endswith("-linear"),
+                "quadratic_term": quadratic_mode(mode),
                 # mean |exact - g|^2 on the last subset (a constant offset in it cancels in the estimate)
                 "mean_sq_residual_Ha2": ss["resid2"],
-                # warm-up variance of exact - (E_T + 2<K,dG> [+ Q_bf16]), the MLP's target
+                # warm-up variance of exact - (E_T + 2<K,dG> [+ Q]), the MLP's target
                 "control_variate_residual_var_Ha2": ss["target_var"],
                 "mlp_gain": ss["gain"],
             }
@@ -606,13 +612,22 @@ def run_modes(args, w, host, mode_list, K, warmup, tuner):
             # control variate over all walkers (its own profiling phase)
             cv_ms, cv_n = prof["exchange_cv"]
             cv_avg_s = cv_ms / max(cv_n, 1) / 1e3
-            cv_kernel = ("Q = dG^T Wq dG over block-triangular W', single BF16 product (bf16_gemm_kernel<false, "
-                         "false, true>, dG.Y in the epilogue) + quad_sum_kernel" if form == "quad_sym"
-                         else "exchange_bf16_kernel<false> (single BF16 product, every dT_x formed)")
+            fp8 = form == "quad_sym" and quadratic_mode(mode) == "fp8"
+            if fp8:
+                cv_kernel = ("Q = dG^T Wq dG over block-triangular W', single FP8 (E4M3) product, dG scaled per "
+                             "walker and W' per spin by powers of two (bf16_gemm_kernel<false, false, true, Tile0, "
+                             "true>, dG.Y in the epilogue) + quad_sum_kernel")
+            elif form == "quad_sym":
+                cv_kernel = ("Q = dG^T Wq dG over block-triangular W', single BF16 product (bf16_gemm_kernel<false, "
+                             "false, true>, dG.Y in the epilogue) + quad_sum_kernel")
+            else:
+                cv_kernel = "exchange_bf16_kernel<false> (single BF16 product, every dT_x formed)"
             rec["roofline_kernel"] = dict(
                 kernel="control-variate exchange of every walker, one launch per spin: " + cv_kernel,
                 avg_launch_ms=cv_avg_s * 1e3, flops_per_launch=fl, achieved=fl / cv_avg_s / 1e12,
-                peak=PEAK["bf16_mfma_tflops"], peak_note="dense BF16 MFMA 2.5 PFLOP/s, one product per flop")
+                peak=PEAK["fp8_mfma_tflops"] if fp8 else PEAK["bf16_mfma_tflops"],
+                peak_note=("dense FP8 MFMA 5.0 PFLOP/s, one product per flop" if fp8
+                           else "dense BF16 MFMA 2.5 PFLOP/s, one product per flop"))
         modes[mode] = rec
         del st
         torch.cuda.empty_cache()
@@ -714,12 +729,16 @@ def main():
                       "(fp32 accumulate)",
              "mixed+surrogate": "f64 (Green's function, weights, one-body, QR) + bf16x3 force bias, VHS, Taylor (fp32 "
                                 "accumulate); exchange estimated per walker as g + [sampled] 8 (exact - g) with "
-                                "g = E_T + 2<K,dG> (f64) + Q(dG) (bf16 x1) + gain * MLP (bf16 weights), exact "
-                                "bf16x3 exchange on a random 1/8 of walkers",
+                                "g = E_T + 2<K,dG> (f64) + Q(dG) (fp8 e4m3 x1, scaled) + gain * MLP (bf16 weights), "
+                                "exact bf16x3 exchange on a random 1/8 of walkers",
+             "mixed+surrogate-bf16q": "f64 (Green's function, weights, one-body, QR) + bf16x3 force bias, VHS, Taylor "
+                                      "(fp32 accumulate); exchange estimated per walker as g + [sampled] 8 "
+                                      "(exact - g) with g = E_T + 2<K,dG> (f64) + Q(dG) (bf16 x1) + gain * MLP (bf16 "
+                                      "weights), exact bf16x3 exchange on a random 1/8 of walkers",
              "mixed+surrogate-nomlp": "f64 (Green's function, weights, one-body, QR) + bf16x3 force bias, VHS, Taylor "
                                       "(fp32 accumulate); exchange estimated per walker as g + [sampled] 8 "
-                                      "(exact - g) with g = E_T + 2<K,dG> (f64) + Q(dG) (bf16 x1), exact bf16x3 "
-                                      "exchange on a random 1/8 of walkers",
+                                      "(exact - g) with g = E_T + 2<K,dG> (f64) + Q(dG) (fp8 e4m3 x1, scaled), exact "
+                                      "bf16x3 exchange on a random 1/8 of walkers",
              "mixed+surrogate-linear": "f64 (Green's function, weights, one-body, QR) + bf16x3 force bias, VHS, "
                                        "Taylor (fp32 accumulate); exchange estimated per walker as g + [sampled] 8 "
                                        "(exact - g) with g = E_T + 2<K,dG> (f64) + gain * MLP (bf16 weights), exact "

@@ -333,9 +333,10 @@ int afqmc_tune_get(afqmc_ctx* ctx, int32_t knob, int32_t* value, int32_t* lo, in
 int afqmc_tune_set(afqmc_ctx* ctx, int32_t knob, int32_t value);
 /* Learned control variate for the exchange energy of afqmc_propagate's
  * energy_prev output.  The predictor of each walker's exchange sum is
- *   g = E_T + 2 <K, G - G_T> + Q_bf16(G - G_T) + alpha m(f):
+ *   g = E_T + 2 <K, G - G_T> + Q(G - G_T) + alpha m(f):
  * the exchange expanded about the trial's Green's function G_T (E_T and K in
- * FP64, computed once), its quadratic remainder Q in single BF16 products, and
+ * FP64, computed once), its quadratic remainder Q in single FP8 or BF16
+ * products (afqmc_surrogate_quadratic), and
  * an MLP m (2 x 128 SiLU, BF16 weights, FP32 accumulation) over the one-body
  * and Coulomb energies, mean / max |xbar|, log|overlap| and the overlap phase,
  * trained on the residual, with a gain alpha in [0, 1] fitted on earlier
@@ -351,20 +352,29 @@ int afqmc_tune_set(afqmc_ctx* ctx, int32_t knob, int32_t value);
  * restores the plain exact estimator.  afqmc_local_energy is always exact. */
 int afqmc_surrogate_configure(afqmc_ctx* ctx, int enable, double fraction, int warmup_steps, int train_iters,
                               double lr, uint64_t seed);
-/* enable = 1 (default): the control variate includes Q_bf16(G - G_T); 0: the
- * predictor is E_T + 2 <K, G - G_T> (+ alpha m(f)) without the quadratic-form
- * GEMM, the exact sample and the MLP term making up the rest (the estimator
- * stays unbiased; its variance grows by the variance of Q).  Applies to steps
- * enqueued or captured after the call. */
-int afqmc_surrogate_quadratic(afqmc_ctx* ctx, int enable);
+/* The quadratic term Q(G - G_T) of the control variate:
+ *   AFQMC_SURROGATE_Q_FP8 (default): one FP8 (E4M3) product, d and W' scaled
+ *     by powers of two (per walker for d), FP32 accumulation, where the
+ *     quadratic-form planes exist (mixed precision, N < X); elsewhere BF16;
+ *   AFQMC_SURROGATE_Q_BF16: one BF16 product;
+ *   AFQMC_SURROGATE_Q_NONE: no Q; the predictor is E_T + 2 <K, G - G_T>
+ *     (+ alpha m(f)), the exact sample and the MLP term making up the rest.
+ * The estimator stays unbiased in every mode (the rounding of Q changes only
+ * how well g predicts, i.e. the variance).  Applies to steps enqueued or
+ * captured after the call; the first FP8 evaluation (which builds the FP8 copy
+ * of W') must not be inside a capture. */
+#define AFQMC_SURROGATE_Q_NONE 0
+#define AFQMC_SURROGATE_Q_BF16 1
+#define AFQMC_SURROGATE_Q_FP8 2
+int afqmc_surrogate_quadratic(afqmc_ctx* ctx, int mode);
 /* out[0..n-1], n >= 6: energy evaluations so far, warm-up length, last subset
  * size, last training loss per sample (normalised units), mean
  * |exact - predicted|^2 on the last subset (Ha^2), variance over the warm-up of
  * the MLP's target Re(exact - control variate), and (n >= 7) the current gain of
  * the MLP term.  Synchronises. */
 int afqmc_surrogate_stats(afqmc_ctx* ctx, double* out, int n);
-/* cv [W] complex = E_T + 2 <K, G - G_T> + Q_bf16(G - G_T) from Ghalf (without
- * Q after afqmc_surrogate_quadratic(ctx, 0)): the physical control variate of
+/* cv [W] complex = E_T + 2 <K, G - G_T> + Q(G - G_T) from Ghalf, Q in the mode
+ * of afqmc_surrogate_quadratic (none after AFQMC_SURROGATE_Q_NONE): the physical control variate of
  * each walker's exchange sum (the predictor above without the MLP term),
  * through the kernels afqmc_propagate runs for it.
  * Needs afqmc_surrogate_configure(enable = 1); changes no walker or model

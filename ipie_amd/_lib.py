@@ -478,9 +478,13 @@ class Context:
         self._call("afqmc_surrogate_configure", int(bool(enable)), float(fraction), int(warmup), int(train_iters),
                    float(lr), int(seed))
 
-    def surrogate_quadratic(self, enable=True):
-        """Whether the control variate includes Q_bf16(G - G_T) (afqmc_surrogate_quadratic)."""
-        self._call("afqmc_surrogate_quadratic", int(bool(enable)))
+    def surrogate_quadratic(self, mode="fp8"):
+        """The control variate's quadratic term Q(G - G_T) (afqmc_surrogate_quadratic):
+        "fp8" (or True: the default), "bf16", or "none" (or False)."""
+        modes = {"none": 0, "bf16": 1, "fp8": 2, False: 0, True: 2}
+        if mode not in modes:
+            raise ValueError(f"quadratic mode {mode!r}: one of 'fp8', 'bf16', 'none'")
+        self._call("afqmc_surrogate_quadratic", modes[mode])
 
     def exchange_form(self):
         """'quad' (g^T Wq g, Wq built once) or 't' (every T_x = L_x Ghalf^T, ipie's contraction)."""

@@ -26,7 +26,8 @@ the learned control variate, whose step evaluates the single-product quadratic
 form of every walker) or act on scratch copies of the walkers
 (``afqmc_two_body`` for tile_vhs and taylor, ``afqmc_one_body`` for
 tile_one_body), and keeps the fastest unless the heuristic is within
-``min_gain`` of it.  The choices are cached per device and shape: a multi-system batch whose
+``min_gain`` of it.  Knobs that shape one launch (fb_split with tile_fb,
+quad_wgs with quad_rgroup) are timed over every pair of candidates.  The choices are cached per device and shape: a multi-system batch whose
 systems share a shape tunes that shape once, and a JSON cache file carries the
 choices to the next run.  The knobs change launch shapes and the grouping of
 partial sums only, so results agree with the untuned context to rounding.
@@ -40,6 +41,7 @@ keyed apart).  ``PhaselessGeneric(autotune=...)`` tunes at build time.
 No counterpart in ipie, whose GPU kernels take CuPy's and numba's shapes.
 """
 
+import itertools
 import json
 import os
 import time
@@ -52,6 +54,10 @@ KNOBS = ("fb_split", "quad_wgs", "exx_split", "quad_rgroup", "tile_fb", "tile_vh
 SURROGATE_KNOBS = ("fb_split", "quad_wgs", "quad_rgroup", "tile_fb", "tile_vhs", "tile_one_body", "taylor")
 # knobs timed on scratch copies of the walkers
 PHI_KNOBS = ("tile_vhs", "tile_one_body", "taylor")
+# knobs that shape the same launch and are timed together, every pair of
+# candidates (the force-bias split that suits one tile shape can leave another
+# with too few workgroups)
+JOINT = (("fb_split", "tile_fb"), ("quad_wgs", "quad_rgroup"))
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "ipie_amd", "autotune.json")
 
 _FB = (1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 24, 32)
@@ -221,23 +227,40 @@ class Autotuner:
             calls["tile_one_body"] = lambda: ctx.one_body(pa, pb)
             calls["taylor"] = calls["tile_vhs"]
         chosen, report = {}, {}
-        for knob in self.knobs:
-            if surrogate and knob not in SURROGATE_KNOBS:
-                continue
-            if knob not in calls:
-                continue
+
+        def tunable(knob):
+            if (surrogate and knob not in SURROGATE_KNOBS) or knob not in calls:
+                return None
             value, lo, hi = ctx.tune_get(knob)
             cands = candidates(knob, value, lo, hi)
-            if not cands:
+            return (value, cands) if cands else None
+
+        done = set()
+        for knob in self.knobs:
+            if knob in done:
                 continue
+            pair = next((g for g in JOINT if knob in g and all(k in self.knobs for k in g)), (knob,))
+            group = [k for k in pair if k not in done and tunable(k) is not None]
+            done.update(pair)
+            if not group:
+                continue
+            found = {k: tunable(k) for k in group}
+            default = tuple(found[k][0] for k in group)
             times = {}
-            for v in cands:
-                ctx.tune_set(knob, v)
-                times[v] = self._time(ctx, calls[knob])
-            pick = choose(times, value, self.min_gain)
-            ctx.tune_set(knob, pick)
-            chosen[knob] = pick
-            report[knob] = {"default": value, "chosen": pick, "ms": {str(k): round(t, 4) for k, t in times.items()}}
+            for combo in itertools.product(*(found[k][1] for k in group)):
+                for k, v in zip(group, combo):
+                    ctx.tune_set(k, v)
+                times[combo] = self._time(ctx, calls[group[0]])
+            pick = choose(times, default, self.min_gain)
+            for i, k in enumerate(group):
+                ctx.tune_set(k, pick[i])
+                chosen[k] = pick[i]
+                # the knob's candidates with the others at their chosen values
+                line = {c[i]: t for c, t in times.items() if all(c[j] == pick[j] for j in range(len(group)) if j != i)}
+                report[k] = {"default": default[i], "chosen": pick[i],
+                             "ms": {str(v): round(t, 4) for v, t in sorted(line.items())}}
+            if len(group) > 1:
+                report[group[0]]["grid"] = {",".join(map(str, c)): round(t, 4) for c, t in times.items()}
         key = shape_key(ctx, bool(surrogate))
         self.cache[key] = chosen
         self.reports[key] = report

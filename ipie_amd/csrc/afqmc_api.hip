@@ -74,6 +74,18 @@ struct afqmc_ctx {
   bool quad_ready = false;                    // Wq holds the current static tensors
   double* qw64[2] = {nullptr, nullptr};       // FP64 Wq per spin [n_s N, n_s N]
   uint16_t* qwp[2] = {nullptr, nullptr};      // split-BF16 Wq per spin 2 x [n_s Kp, n_s Kp]
+  // FP8 quadratic form of the control variate (made on first use): W' per spin
+  // [q8_ld][q8_ld] bytes, q8_ld = n_s Kp rounded up to 64; the scales s_w (FP32,
+  // two) and their maximum scratch (two uints) on the device; the FP8 planes of dG
+  // [2][W][max q8_ld] and each walker's inverse scale; column units in 64-value chunks
+  uint8_t* qw8[2] = {nullptr, nullptr};
+  int64_t q8_ld[2] = {0, 0};
+  float* q8_scale = nullptr;
+  uint8_t* g8 = nullptr;
+  double* q8_rowinv = nullptr;
+  bool q8_ready = false;
+  int4* qunits8[2] = {nullptr, nullptr};
+  std::vector<int> qunit8_off[2], qunit8_n[2];
   double* qy = nullptr;                       // FP64: Y = G Wq [W, n N]; BF16: dot partials [slots, W] (complex)
   int4* qunits[2] = {nullptr, nullptr};       // BF16 column units per spin, one table per row-block count
   std::vector<int> qunit_off[2], qunit_n[2];  // [mb - 1]: offset and length of the table for mb row blocks
@@ -109,8 +121,9 @@ struct afqmc_ctx {
     double* resid = nullptr;
     int* walker_of = nullptr;
     double* xbar = nullptr;  // [2, W]: mean, max
-    // physical control variate E_T + 2 <K, dG> + Q_bf16(dG) (quadratic), or E_T + 2 <K, dG>
-    bool quadratic = true;
+    // physical control variate E_T + 2 <K, dG> + Q(dG) with Q in FP8 (qmode 2, where the
+    // quadratic-form planes exist; BF16 otherwise) or BF16 (1), or E_T + 2 <K, dG> (0)
+    int qmode = AFQMC_SURROGATE_Q_FP8;
     bool cv_ready = false;   // gT, K, E_T hold the current trial / Hamiltonian
     double* gT = nullptr;    // [2][nmax, N] complex trial Ghalf per spin
     double* K = nullptr;     // [2][nmax, N] complex
@@ -240,23 +253,25 @@ int refuse_in_capture(afqmc_ctx* c, const char* what) {
   return fail(c, AFQMC_ERR_INVALID_ARG, "%s is not allowed while a step graph is being captured", what);
 }
 
-// Column units of the block-triangular quadratic-form GEMM with mb row blocks:
-// column tile t sums the K chunks [0, min(5 (t + 1), kchunks)), cut at the
-// multiples of T chunks (T chosen for about target_wgs workgroups, at least 32), and
-// the units are listed by K block, then tile.  The grid takes the row blocks
+// Column units of the block-triangular quadratic-form GEMM with mb row blocks
+// and K chunks of cw values: column tile t sums the chunks
+// [0, min(ceil(QUAD_TILE (t + 1) / cw), kchunks)), cut at the multiples of T
+// chunks (T chosen for about target_wgs workgroups, at least 1024 / cw), and
+// the units are listed by K block, then tile (chunks past a tile's own rows
+// cover zeros of the block-triangular W').  The grid takes the row blocks
 // fastest, so the workgroups that run together on one XCD (consecutive logical
 // indices) are a few tiles of the same K block: they read the same d rows and
 // the same K range, which L2 then serves once.
-std::vector<int4> quad_units(int64_t nb, int kchunks, int64_t mb, int target_wgs) {
-  const int per_tile = QUAD_TILE / 32;
+std::vector<int4> quad_units(int64_t nb, int kchunks, int64_t mb, int target_wgs, int cw = 32) {
+  auto extent = [&](int64_t t) { return std::min<int64_t>(ceil_div((int64_t)QUAD_TILE * (t + 1), cw), kchunks); };
   int64_t total = 0;
-  for (int64_t t = 0; t < nb; ++t) total += std::min<int64_t>(per_tile * (t + 1), kchunks);
+  for (int64_t t = 0; t < nb; ++t) total += extent(t);
   const int64_t target = std::max<int64_t>(ceil_div(target_wgs, mb), 1);
-  const int64_t T = std::max<int64_t>(ceil_div(total, target), 32);
+  const int64_t T = std::max<int64_t>(ceil_div(total, target), 1024 / cw);
   std::vector<int4> u;
   for (int64_t z = 0; z * T < kchunks; ++z)
     for (int64_t t = 0; t < nb; ++t) {
-      const int64_t kc = std::min<int64_t>(per_tile * (t + 1), kchunks);
+      const int64_t kc = extent(t);
       if (z * T >= kc) continue;
       int4 e;
       e.x = (int)t;
@@ -289,17 +304,22 @@ int alloc_exchange_planes(afqmc_ctx* c) {
 int build_quad_units(afqmc_ctx* c) {
   const int64_t W = c->d.nwalkers;
   int64_t max_units = 0;
-  std::vector<int4> all[2];
-  std::vector<int> off[2], cnt[2];
+  // [0, 1]: BF16 tables (32-value chunks) per spin; [2, 3]: FP8 (64-value chunks)
+  std::vector<int4> all[4];
+  std::vector<int> off[4], cnt[4];
   for (int s = 0; s < 2; ++s) {
     if (!c->qwp[s]) continue;
     const int64_t ldn = (int64_t)(s == 0 ? c->d.nalpha : c->d.nbeta) * c->kpad;
-    for (int64_t mb = 1; mb <= ceil_div(W, 128); ++mb) {
-      std::vector<int4> u = quad_units(ceil_div(ldn, QUAD_TILE), (int)(ldn / 32), mb, c->quad_target);
-      off[s].push_back((int)all[s].size());
-      cnt[s].push_back((int)u.size());
-      max_units = std::max<int64_t>(max_units, (int64_t)u.size());
-      all[s].insert(all[s].end(), u.begin(), u.end());
+    for (int f = 0; f < 2; ++f) {
+      const int t = 2 * f + s, cw = f ? 64 : 32;
+      const int kch = (int)(f ? ceil_div(ldn, 64) : ldn / 32);
+      for (int64_t mb = 1; mb <= ceil_div(W, 128); ++mb) {
+        std::vector<int4> u = quad_units(ceil_div(ldn, QUAD_TILE), kch, mb, c->quad_target, cw);
+        off[t].push_back((int)all[t].size());
+        cnt[t].push_back((int)u.size());
+        max_units = std::max<int64_t>(max_units, (int64_t)u.size());
+        all[t].insert(all[t].end(), u.begin(), u.end());
+      }
     }
   }
   if (max_units == 0) return AFQMC_OK;
@@ -312,16 +332,17 @@ int build_quad_units(afqmc_ctx* c) {
     c->qy = qy;
     c->qy_cap = need;
   }
-  for (int s = 0; s < 2; ++s) {
-    if (all[s].empty()) continue;
+  for (int k = 0; k < 4; ++k) {
+    if (all[k].empty()) continue;
     int4* t = nullptr;
-    if (dmalloc((void**)&t, sizeof(int4) * all[s].size()) != hipSuccess)
+    if (dmalloc((void**)&t, sizeof(int4) * all[k].size()) != hipSuccess)
       return fail(c, AFQMC_ERR_ALLOC, "quadratic-form unit table allocation failed (%s)", c->describe_dims);
-    HIP_TRY(c, hipMemcpy(t, all[s].data(), sizeof(int4) * all[s].size(), hipMemcpyHostToDevice));
-    if (c->qunits[s]) c->retired.push_back(c->qunits[s]);
-    c->qunits[s] = t;
-    c->qunit_off[s] = off[s];
-    c->qunit_n[s] = cnt[s];
+    HIP_TRY(c, hipMemcpy(t, all[k].data(), sizeof(int4) * all[k].size(), hipMemcpyHostToDevice));
+    int4*& slot = k < 2 ? c->qunits[k] : c->qunits8[k - 2];
+    if (slot) c->retired.push_back(slot);
+    slot = t;
+    (k < 2 ? c->qunit_off[k] : c->qunit8_off[k - 2]) = off[k];
+    (k < 2 ? c->qunit_n[k] : c->qunit8_n[k - 2]) = cnt[k];
   }
   return AFQMC_OK;
 }
@@ -832,6 +853,88 @@ int ensure_quad(afqmc_ctx* c) {
   return AFQMC_OK;
 }
 
+// FP8 copy of the split-BF16 W' per spin for the control variate's quadratic
+// form (buffers allocated on first use, outside graph capture).
+int ensure_quad8(afqmc_ctx* c) {
+  int r = ensure_quad(c);
+  if (r) return r;
+  if (c->q8_ready) return AFQMC_OK;
+  if (c->capturing) return refuse_in_capture(c, "the first FP8 control-variate evaluation");
+  const int64_t W = c->d.nwalkers;
+  if (!c->q8_scale && dmalloc((void**)&c->q8_scale, 4 * sizeof(float)) != hipSuccess)
+    return fail(c, AFQMC_ERR_ALLOC, "FP8 scale allocation failed");
+  int64_t ldmax = 0;
+  for (int s = 0; s < 2; ++s) {
+    if (!c->qwp[s]) continue;
+    const int64_t ldn = (int64_t)(s == 0 ? c->d.nalpha : c->d.nbeta) * c->kpad;
+    c->q8_ld[s] = ceil_div(ldn, 64) * 64;
+    ldmax = std::max(ldmax, c->q8_ld[s]);
+    if (!c->qw8[s] && dmalloc((void**)&c->qw8[s], (size_t)c->q8_ld[s] * c->q8_ld[s]) != hipSuccess)
+      return fail(c, AFQMC_ERR_ALLOC, "FP8 quadratic-form allocation failed (%s)", c->describe_dims);
+  }
+  if (!c->g8 && dmalloc((void**)&c->g8, 2 * (size_t)W * ldmax) != hipSuccess)
+    return fail(c, AFQMC_ERR_ALLOC, "FP8 plane allocation failed (%s)", c->describe_dims);
+  if (!c->q8_rowinv && dmalloc((void**)&c->q8_rowinv, sizeof(double) * (size_t)W) != hipSuccess)
+    return fail(c, AFQMC_ERR_ALLOC, "FP8 scale allocation failed");
+  unsigned int* absmax = reinterpret_cast<unsigned int*>(c->q8_scale + 2);
+  HIP_TRY(c, hipMemsetAsync(absmax, 0, 2 * sizeof(unsigned int), c->stream));
+  for (int s = 0; s < 2; ++s) {
+    if (!c->qwp[s]) continue;
+    const int64_t ldn = (int64_t)(s == 0 ? c->d.nalpha : c->d.nbeta) * c->kpad;
+    HIP_TRY(c, launch_quad_w8(c->qwp[s], ldn, c->q8_ld[s], c->qw8[s], c->q8_scale + s, absmax + s, c->stream));
+  }
+  c->q8_ready = true;
+  return AFQMC_OK;
+}
+
+// FP8 quadratic form of nw walkers for one spin, d^T Wq d with d = G - gref:
+// d in per-walker scaled FP8 planes, Y = d8 W8 over the FP8 column units, the
+// dot products with d (FP64) in the GEMM's epilogue, rescaled per walker, their
+// sum per walker into part.  The split is timed as AFQMC_PHASE_SPLIT.
+int quad_fp8(afqmc_ctx* c, int nw, int spin, const double* g, const double* gref, double* part, int phase) {
+  const int N = c->d.nbasis;
+  const int n = spin == 0 ? c->d.nalpha : c->d.nbeta;
+  const int64_t ld8 = c->q8_ld[spin];
+  {
+    PhaseScope ps(c, AFQMC_PHASE_SPLIT);
+    HIP_TRY(c, launch_split_fp8_rows(g, gref, nw, n, N, c->kpad, ld8, c->g8, c->q8_scale + spin, c->q8_rowinv,
+                                     c->stream));
+  }
+  PhaseScope ps(c, phase);
+  const int64_t mb = ceil_div(nw, 128);
+  if (mb < 1 || mb > (int64_t)c->qunit8_n[spin].size()) return fail(c, AFQMC_ERR_INVALID_ARG, "walker count");
+  const int nunits = c->qunit8_n[spin][mb - 1];
+  Bf16GemmArgs b{};
+  // byte arrays in 16-bit units: the im plane starts nw ld8 bytes in (A plane 2)
+  b.a = reinterpret_cast<const uint16_t*>(c->g8);
+  b.a_plane = (int64_t)nw * ld8 / 4;
+  b.lda = ld8 / 2;
+  b.m = nw;
+  b.b = reinterpret_cast<const uint16_t*>(c->qw8[spin]);
+  b.b_plane = 0;
+  b.ldb = ld8 / 2;
+  b.nn = (int64_t)n * c->kpad;
+  b.kchunks = (int)(ld8 / 64);
+  b.ksplit = 1;
+  b.single = 1;
+  b.fp8 = 1;
+  b.epilogue = BF16_EPI_QDOT;
+  b.out = c->qy;
+  b.scale = 1.0;
+  b.units = c->qunits8[spin] + c->qunit8_off[spin][mb - 1];
+  b.nunits = nunits;
+  b.rgroup = c->quad_rgroup;
+  b.dg = g;
+  b.dref = gref;
+  b.dstride = (int64_t)n * N;
+  b.dn = N;
+  b.dkp = c->kpad;
+  b.row_scale = c->q8_rowinv;
+  HIP_TRY(c, launch_bf16_gemm(b, c->stream));
+  HIP_TRY(c, launch_quad_sum(c->qy, nunits * QUAD_SLOTS, nw, part, c->stream));
+  return AFQMC_OK;
+}
+
 // Split-BF16 quadratic form of nw walkers for one spin, d^T Wq d with d = G - gref
 // (gref may be null; walker_of selects the walkers): Y = d W' over the column
 // units of the block-triangular planes, the per-tile dot products in the GEMM's
@@ -1043,9 +1146,16 @@ int exchange_q_spin(afqmc_ctx* c, int nw, int spin, const double* g) {
   const double* gT = c->sur.gT + spin * gstride;
   double* part = spin == 0 ? c->exx_a : c->exx_b;
   if (c->qwp[spin]) {
-    int r = ensure_quad(c);
-    if (r) return r;
-    r = quad_bf16(c, nw, spin, g, nullptr, gT, true, part, AFQMC_PHASE_EXCHANGE_CV);
+    int r;
+    if (c->sur.qmode == AFQMC_SURROGATE_Q_FP8) {
+      r = ensure_quad8(c);
+      if (r) return r;
+      r = quad_fp8(c, nw, spin, g, gT, part, AFQMC_PHASE_EXCHANGE_CV);
+    } else {
+      r = ensure_quad(c);
+      if (r) return r;
+      r = quad_bf16(c, nw, spin, g, nullptr, gT, true, part, AFQMC_PHASE_EXCHANGE_CV);
+    }
     if (r) return r;
     c->xs_used[spin] = 1;
     return AFQMC_OK;
@@ -1099,7 +1209,7 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   const int nspin = two_spins(c) ? 2 : 1;
   const int N = c->d.nbasis;
   const int64_t gstride = 2 * (int64_t)std::max(c->d.nalpha, c->d.nbeta) * N;
-  for (int s = 0; s < nspin && S.quadratic; ++s) {
+  for (int s = 0; s < nspin && S.qmode != AFQMC_SURROGATE_Q_NONE; ++s) {
     r = exchange_q_spin(c, nw, s, s == 0 ? ga : gb);
     if (r) return r;
   }
@@ -1113,7 +1223,7 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
     cva.gT[s] = S.gT + s * gstride;
     cva.K[s] = S.K + s * gstride;
     cva.part[s] = s == 0 ? c->exx_a : c->exx_b;
-    cva.xs[s] = S.quadratic ? c->xs_used[s] : 0;  // partial sums of Q, when computed
+    cva.xs[s] = S.qmode != AFQMC_SURROGATE_Q_NONE ? c->xs_used[s] : 0;  // partial sums of Q, when computed
   }
   cva.et = S.et;
   cva.cv = S.cv;
@@ -1280,6 +1390,9 @@ void afqmc_destroy(afqmc_ctx* c) {
     if (p) (void)hipFree(p);
   for (void* p : c->tplanes)
     if (p) (void)hipFree(p);
+  void* p8[] = {c->qw8[0], c->qw8[1], c->q8_scale, c->g8, c->q8_rowinv, c->qunits8[0], c->qunits8[1]};
+  for (void* p : p8)
+    if (p) (void)hipFree(p);
   for (void* p : c->retired) (void)hipFree(p);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   for (hipEvent_t e : c->ev_free) (void)hipEventDestroy(e);
@@ -1320,6 +1433,7 @@ int afqmc_set_static(afqmc_ctx* c, const afqmc_static* s) {
   c->psi_real = any == 0;
   c->planes_ready = false;
   c->quad_ready = false;
+  c->q8_ready = false;
   c->sur.cv_ready = false;
   return AFQMC_OK;
 }
@@ -1815,7 +1929,7 @@ int afqmc_control_variate(afqmc_ctx* c, int32_t nw, const double* ga, const doub
   const int nspin = two_spins(c) ? 2 : 1;
   const int N = c->d.nbasis;
   const int64_t gstride = 2 * (int64_t)std::max(c->d.nalpha, c->d.nbeta) * N;
-  for (int s = 0; s < nspin && S.quadratic; ++s) {
+  for (int s = 0; s < nspin && S.qmode != AFQMC_SURROGATE_Q_NONE; ++s) {
     r = exchange_q_spin(c, nw, s, s == 0 ? ga : gb);
     if (r) return r;
   }
@@ -1829,7 +1943,7 @@ int afqmc_control_variate(afqmc_ctx* c, int32_t nw, const double* ga, const doub
     cva.gT[s] = S.gT + s * gstride;
     cva.K[s] = S.K + s * gstride;
     cva.part[s] = s == 0 ? c->exx_a : c->exx_b;
-    cva.xs[s] = S.quadratic ? c->xs_used[s] : 0;  // partial sums of Q, when computed
+    cva.xs[s] = S.qmode != AFQMC_SURROGATE_Q_NONE ? c->xs_used[s] : 0;  // partial sums of Q, when computed
   }
   cva.et = S.et;
   cva.cv = cv;
@@ -2309,10 +2423,10 @@ int afqmc_surrogate_configure(afqmc_ctx* c, int enable, double fraction, int war
   return AFQMC_OK;
 }
 
-int afqmc_surrogate_quadratic(afqmc_ctx* c, int enable) {
-  if (!c) return AFQMC_ERR_INVALID_ARG;
+int afqmc_surrogate_quadratic(afqmc_ctx* c, int mode) {
+  if (!c || mode < AFQMC_SURROGATE_Q_NONE || mode > AFQMC_SURROGATE_Q_FP8) return AFQMC_ERR_INVALID_ARG;
   if (c->capturing) return refuse_in_capture(c, "afqmc_surrogate_quadratic");
-  c->sur.quadratic = enable != 0;
+  c->sur.qmode = mode;
   return AFQMC_OK;
 }
 

@@ -199,6 +199,12 @@ struct Bf16GemmArgs {
   // (row blocks fastest), group after group of row blocks; 0 or >= the row-block
   // count: all row blocks x units
   int rgroup;
+  // single product in FP8 (E4M3): the operands are byte arrays addressed in
+  // 16-bit units (a K chunk of BX_KC units holds 64 FP8 values), each 16-byte
+  // fragment feeding two 16x16x32 FP8 MFMAs; QDOT only.  row_scale[row]
+  // multiplies each row's dot (the inverse of the operands' scales).
+  int fp8;
+  const double* row_scale;
 };
 hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream);
 constexpr int BF16_TILES = 5;
@@ -226,6 +232,19 @@ hipError_t launch_quad_reduce(const double* y, int ksplit, int nwalkers, int n, 
                               const double* gref = nullptr);
 // out[w] = sum_{s < nslots} part[s][w] (complex, fixed order)
 hipError_t launch_quad_sum(const double* part, int nslots, int nwalkers, double* out, hipStream_t stream);
+// FP8 (E4M3) copy of the block-triangular W' for the control variate's single
+// product: w8[r][k] (rows and columns padded to ld8, a multiple of 64, with
+// zeros) = fp8(s_w (hi + lo)[r][k]) from the split-BF16 planes [2][ldn][ldn];
+// s_w = 2^e with max |W'| s_w <= 224, written to scale[0] (FP32, on the device).
+// absmax (one unsigned int of device scratch) must hold 0 on entry.
+hipError_t launch_quad_w8(const uint16_t* wplanes, int64_t ldn, int64_t ld8, uint8_t* w8, float* scale,
+                          unsigned int* absmax, hipStream_t stream);
+// FP8 planes of d = g - gref per walker for the FP8 quadratic form: a8[p][w][k]
+// (p = re, im; k = i Kp + m, zero for m >= N and k >= n Kp, row stride ld8)
+// = fp8(s_row d), s_row = 2^e with max |d_w| s_row <= 224 (per walker), and
+// row_inv[w] = 1 / (s_row s_w) with s_w = wscale[0].
+hipError_t launch_split_fp8_rows(const double* g, const double* gref, int nwalkers, int n, int N, int Kp,
+                                 int64_t ld8, uint8_t* a8, const float* wscale, double* row_inv, hipStream_t stream);
 
 // One split-BF16 Taylor term per walker: out_k^T = alpha out_{k-1}^T VHS,
 // q = (init or q) + out_k, and (tout) the split planes of out_k^T.

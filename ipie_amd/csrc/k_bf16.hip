@@ -369,9 +369,65 @@ __device__ __forceinline__ v4f mfma_ab(v8bf a, v8bf b, v4f c, bool tr) {
   return tr ? mfma_bf16(b, a, c) : mfma_bf16(a, b, c);
 }
 
-template <bool SPLIT, bool TR = false, class TL = Tile0>
+// FP8 (F8, single product only): each 16-byte fragment holds 16 FP8 values of
+// K, fed to two 16x16x32 FP8 MFMAs (bytes 0-7 with 0-7, 8-15 with 8-15: A and
+// B take the same K positions, so each K is summed once).
+__device__ __forceinline__ v4f mfma_fp8_pair(v8bf a, v8bf b, v4f c) {
+  const v4u ua = __builtin_bit_cast(v4u, a), ub = __builtin_bit_cast(v4u, b);
+  const long a0 = (long)(((unsigned long)ua[1] << 32) | ua[0]), a1 = (long)(((unsigned long)ua[3] << 32) | ua[2]);
+  const long b0 = (long)(((unsigned long)ub[1] << 32) | ub[0]), b1 = (long)(((unsigned long)ub[3] << 32) | ub[2]);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a0, b0, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a1, b1, c, 0, 0, 0);
+}
+
+// FP8 over two sub-chunks at once: lane (l16, q) gives 16 bytes of each
+// sub-chunk (128 K values of a row in all) to one 16x16x128 MFMA, twice the
+// rate of the 16x16x32 FP8 form.  A and B give the same bytes of both
+// sub-chunks, so each K is summed once whatever order the instruction takes
+// the 32 bytes of a lane in.  Block scales 127 (E8M0 for 2^0): the operands
+// carry their power-of-two scales already.
+__device__ __forceinline__ v4f mfma_fp8_x128(v4u a0, v4u a1, v4u b0, v4u b1, v4f c) {
+  typedef int v8i __attribute__((ext_vector_type(8)));
+  const v8i a = {(int)a0[0], (int)a0[1], (int)a0[2], (int)a0[3], (int)a1[0], (int)a1[1], (int)a1[2], (int)a1[3]};
+  const v8i b = {(int)b0[0], (int)b0[1], (int)b0[2], (int)b0[3], (int)b1[0], (int)b1[1], (int)b1[2], (int)b1[3]};
+  return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, c, 0, 0, 0, 127, 0, 127);
+}
+
+template <class TL>
+__device__ __forceinline__ void bx_sub_f8x2(const uint4* sub, int wr, int wc, int l16, int q,
+                                            v4f (&re)[(TL::MR)][(TL::MC)], v4f (&im)[(TL::MR)][(TL::MC)]) {
+  using B = Bx<false, TL>;
+  const uint16_t* A0 = reinterpret_cast<const uint16_t*>(sub);
+  const uint16_t* A1 = A0 + 8 * B::SUB_VEC;
+  const uint16_t* B0 = A0 + 8 * B::A_VEC;
+  const uint16_t* B1 = A1 + 8 * B::A_VEC;
+  auto frag = [](const uint16_t* base, int row, int seg) {
+    return reinterpret_cast<const v4u*>(base)[lds_vec(row, seg)];
+  };
+  v4u b0[(TL::MC)], b1[(TL::MC)];
+#pragma unroll
+  for (int j = 0; j < TL::MC; ++j) {
+    const int col = (wc * TL::MC + j) * 16 + l16;
+    b0[j] = frag(B0, col, q);
+    b1[j] = frag(B1, col, q);
+  }
+#pragma unroll
+  for (int i = 0; i < TL::MR; ++i) {
+    const int row = (wr * TL::MR + i) * 16 + l16;
+    const v4u ar0 = frag(A0, row, q), ar1 = frag(A1, row, q);
+    const v4u ai0 = frag(A0, TL::R + row, q), ai1 = frag(A1, TL::R + row, q);
+#pragma unroll
+    for (int j = 0; j < TL::MC; ++j) {
+      re[i][j] = mfma_fp8_x128(ar0, ar1, b0[j], b1[j], re[i][j]);
+      im[i][j] = mfma_fp8_x128(ai0, ai1, b0[j], b1[j], im[i][j]);
+    }
+  }
+}
+
+template <bool SPLIT, bool TR = false, class TL = Tile0, bool F8 = false>
 __device__ __forceinline__ void bx_sub(const uint4* sub, int wr, int wc, int l16, int q,
                                        v4f (&re)[(TL::MR)][(TL::MC)], v4f (&im)[(TL::MR)][(TL::MC)]) {
+  static_assert(!F8 || (!SPLIT && !TR), "FP8 operands: single product, plain accumulators");
   const uint16_t* As = reinterpret_cast<const uint16_t*>(sub);
   const uint16_t* Bs = As + 8 * Bx<SPLIT, TL>::A_VEC;
   v8bf bh[(TL::MC)], bl[(TL::MC)];
@@ -403,8 +459,13 @@ __device__ __forceinline__ void bx_sub(const uint4* sub, int wr, int wc, int l16
       const v8bf aih = lds_frag(As, TL::R + row, q);
 #pragma unroll
       for (int j = 0; j < TL::MC; ++j) {
-        re[i][j] = mfma_ab(arh, bh[j], re[i][j], TR);
-        im[i][j] = mfma_ab(aih, bh[j], im[i][j], TR);
+        if constexpr (F8) {
+          re[i][j] = mfma_fp8_pair(arh, bh[j], re[i][j]);
+          im[i][j] = mfma_fp8_pair(aih, bh[j], im[i][j]);
+        } else {
+          re[i][j] = mfma_ab(arh, bh[j], re[i][j], TR);
+          im[i][j] = mfma_ab(aih, bh[j], im[i][j], TR);
+        }
       }
     }
   }
@@ -413,19 +474,27 @@ __device__ __forceinline__ void bx_sub(const uint4* sub, int wr, int wc, int l16
 // The nsub sub-chunks of LDS stage `stage`.  A full stage is one straight
 // block, so that the LDS reads of a sub-chunk can be scheduled under the
 // MFMAs of the one before.
-template <bool SPLIT, bool TR = false, class TL = Tile0>
+template <bool SPLIT, bool TR = false, class TL = Tile0, bool F8 = false>
 __device__ __forceinline__ void bx_chunk(const uint4* stage, int wr, int wc, int l16, int q,
                                          v4f (&re)[(TL::MR)][(TL::MC)], v4f (&im)[(TL::MR)][(TL::MC)], int nsub) {
   constexpr int KS = Bx<SPLIT, TL>::KS;
+  if constexpr (F8 && KS % 2 == 0) {
+    if (nsub == KS) {
+#pragma unroll
+      for (int sc = 0; sc < KS; sc += 2) bx_sub_f8x2<TL>(stage + sc * Bx<SPLIT, TL>::SUB_VEC, wr, wc, l16, q, re, im);
+      return;
+    }
+  }
   if (nsub == KS) {
 #pragma unroll
-    for (int sc = 0; sc < KS; ++sc) bx_sub<SPLIT, TR, TL>(stage + sc * Bx<SPLIT, TL>::SUB_VEC, wr, wc, l16, q, re, im);
+    for (int sc = 0; sc < KS; ++sc)
+      bx_sub<SPLIT, TR, TL, F8>(stage + sc * Bx<SPLIT, TL>::SUB_VEC, wr, wc, l16, q, re, im);
     return;
   }
 #pragma unroll
   for (int sc = 0; sc < KS; ++sc) {
     if (sc >= nsub) break;
-    bx_sub<SPLIT, TR, TL>(stage + sc * Bx<SPLIT, TL>::SUB_VEC, wr, wc, l16, q, re, im);
+    bx_sub<SPLIT, TR, TL, F8>(stage + sc * Bx<SPLIT, TL>::SUB_VEC, wr, wc, l16, q, re, im);
   }
 }
 
@@ -596,7 +665,7 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
 // QDOT slot is u BX_GC + wc; otherwise column tile cb and split z of ksplit.
 // TR (transposed accumulators, see bx_sub) is used with the ADD and VATOM epilogues only.
 // Workgroups that run together share the B tile (row blocks vary fastest).
-template <bool SPLIT, bool TR, bool DL, class TL>
+template <bool SPLIT, bool TR, bool DL, class TL, bool F8 = false>
 __global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
   using C = Bx<SPLIT, TL>;
   extern __shared__ uint4 lds_u4[];
@@ -682,7 +751,7 @@ __global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) 
         if (t + ST - 1 < nst)
           dl_issue<SPLIT, TL>(lds_u4, buf == 0 ? ST - 1 : buf - 1, nsub_of(t + ST - 1), a.a, a.b, da, db,
                               c0 + (t + ST - 1) * C::KS, wave, lane);
-        bx_chunk<SPLIT, TR, TL>(lds_u4 + buf * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub_of(t));
+        bx_chunk<SPLIT, TR, TL, F8>(lds_u4 + buf * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub_of(t));
         // stage t + 1 must land; the ST - 2 stages after it may stay in flight
         int pend = 0;
 #pragma unroll
@@ -707,7 +776,7 @@ __global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) 
     __syncthreads();
     int buf = 0;
     for (int t = 0; t < nst; ++t) {
-      bx_chunk<SPLIT, TR, TL>(lds_u4 + buf * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub_of(t));
+      bx_chunk<SPLIT, TR, TL, F8>(lds_u4 + buf * C::STAGE_VEC, wr, wc, l16, q, re, im, nsub_of(t));
       const int nbuf = buf == 0 ? 2 : buf - 1;  // (t + 2) % 3: the buffer computed at t - 1
       if (t + 2 < nst) bx_stash<SPLIT, TL>(lds_u4, pa, pb, tid, nbuf, nsub_of(t + 2));
       if (t + 3 < nst) bx_fetch<SPLIT, TL>(pa, pb, a.a, a.b, a_src, b_src, c0 + (t + 3) * C::KS, nsub_of(t + 3));
@@ -777,7 +846,10 @@ __global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) 
         const int i = l16 >> 2, r = l16 & 3;
         const int64_t row = (int64_t)rb * TL::R + (wr * TL::MR + i) * 16 + 4 * q + r;
         const int64_t slot = (a.units ? (int64_t)cu : (int64_t)cb * a.ksplit + z) * TL::GC + wc;
-        if (row < a.m) st(a.out, slot * a.m + row, cmake(a.scale * myr, a.scale * myi));
+        if (row < a.m) {
+          const double f = a.scale * (a.row_scale ? a.row_scale[row] : 1.0);
+          st(a.out, slot * a.m + row, cmake(f * myr, f * myi));
+        }
       }
       }
     } else if (a.epilogue == BF16_EPI_ADD) {
@@ -1436,15 +1508,16 @@ hipError_t launch_exchange_bf16(const ExchangeBf16Args& a, hipStream_t stream) {
 #endif
 constexpr bool kDl = AFQMC_BF16_DL != 0;
 
-template <bool SPLIT, bool TR, class TL>
+template <bool SPLIT, bool TR, class TL, bool F8 = false>
 hipError_t launch_bf16_gemm_t(const Bf16GemmArgs& a, hipStream_t stream) {
   const int64_t nb = ceil_div(a.nn, TL::C), mb = ceil_div(a.m, TL::R);
   const int64_t mbg = (a.units && a.rgroup > 0 && a.rgroup < mb) ? ceil_div(mb, a.rgroup) * a.rgroup : mb;
   const int64_t grid = mbg * (a.units ? (int64_t)a.nunits : nb * a.ksplit);
   constexpr size_t lds = Bx<SPLIT, TL>::STAGES_ALL;
-  hipError_t e = allow_lds((const void*)bf16_gemm_kernel<SPLIT, TR, kDl, TL>, lds);
+  hipError_t e = allow_lds((const void*)bf16_gemm_kernel<SPLIT, TR, kDl, TL, F8>, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((bf16_gemm_kernel<SPLIT, TR, kDl, TL>), dim3((unsigned)grid), dim3(TL::THREADS), lds, stream, a);
+  hipLaunchKernelGGL((bf16_gemm_kernel<SPLIT, TR, kDl, TL, F8>), dim3((unsigned)grid), dim3(TL::THREADS), lds, stream,
+                     a);
   return hipGetLastError();
 }
 
@@ -1474,6 +1547,11 @@ hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream) {
   if (a.units && a.nunits < 1) return hipErrorInvalidValue;
   if ((a.a_gstride > 0 || a.epilogue == BF16_EPI_ADD) && a.group < 1) return hipErrorInvalidValue;
   if (a.epilogue == BF16_EPI_ADD && (a.base == nullptr || a.out == nullptr || a.single)) return hipErrorInvalidValue;
+  if (a.fp8) {
+    // FP8 operands: the quadratic form's single product
+    if (!a.single || a.epilogue != BF16_EPI_QDOT || !kDl) return hipErrorInvalidValue;
+    return launch_bf16_gemm_t<false, false, Tile0, true>(a, stream);
+  }
   if (a.single) {
     if (a.epilogue == BF16_EPI_VHS || a.epilogue == BF16_EPI_VATOM) return hipErrorInvalidValue;
     return launch_bf16_gemm_t<false, false, Tile0>(a, stream);

@@ -166,7 +166,115 @@ __global__ void __launch_bounds__(256) quad_sum_kernel(const double* __restrict_
   }
 }
 
+// ---------------------------------------------------------------- FP8 operands
+// FP8 (E4M3, round to nearest even) of four values, packed little-endian
+__device__ __forceinline__ uint32_t pack_fp8x4(float v0, float v1, float v2, float v3) {
+  int p = 0;
+  p = __builtin_amdgcn_cvt_pk_fp8_f32(v0, v1, p, false);
+  p = __builtin_amdgcn_cvt_pk_fp8_f32(v2, v3, p, true);
+  return (uint32_t)p;
+}
+
+// power-of-two scale putting the largest magnitude at most at 224 (half the
+// E4M3 maximum: the lo part of a split value and rounding stay in range)
+__device__ __forceinline__ double fp8_scale(double amax) {
+  return amax > 0.0 ? exp2(floor(log2(224.0 / amax))) : 1.0;
+}
+
+// max |x| over n BF16 values into *absmax (FP32 bits; non-negative floats
+// order like their bit patterns, so the result does not depend on timing)
+__global__ void __launch_bounds__(256) quad_absmax_kernel(const uint16_t* __restrict__ x, int64_t n,
+                                                        unsigned int* __restrict__ absmax) {
+  float m = 0.0f;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(qbf2f(x[e])));
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(absmax, __float_as_uint(m));
+}
+
+__global__ void quad_w8_scale_kernel(const unsigned int* __restrict__ absmax, float* __restrict__ scale) {
+  if (threadIdx.x == 0) scale[0] = (float)fp8_scale((double)__uint_as_float(absmax[0]));
+}
+
+// w8[r][k], 4 consecutive k per thread, zero outside ldn x ldn
+__global__ void __launch_bounds__(256) quad_w8_kernel(const uint16_t* __restrict__ wp, int64_t ldn, int64_t ld8,
+                                                    uint8_t* __restrict__ w8, const float* __restrict__ scale) {
+  const int64_t q4 = ld8 / 4;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ld8 * q4) return;
+  const int64_t r = idx / q4, k0 = (idx % q4) * 4;
+  const float s = scale[0];
+  float v[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int64_t k = k0 + j;
+    v[j] = (r < ldn && k < ldn) ? (qbf2f(wp[r * ldn + k]) + qbf2f(wp[ldn * ldn + r * ldn + k])) * s : 0.0f;
+  }
+  *reinterpret_cast<uint32_t*>(w8 + r * ld8 + k0) = pack_fp8x4(v[0], v[1], v[2], v[3]);
+}
+
+// one workgroup per walker: the largest |re|, |im| of d_w, then its scaled FP8 planes
+__global__ void __launch_bounds__(256) split_fp8_rows_kernel(const double* __restrict__ g,
+                                                           const double* __restrict__ gref, int nwalkers, int n,
+                                                           int N, int Kp, int64_t ld8, uint8_t* __restrict__ a8,
+                                                           const float* __restrict__ wscale,
+                                                           double* __restrict__ row_inv) {
+  __shared__ double red[256 / 64];
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const double* gw = g + 2 * (int64_t)w * n * N;
+  double m = 0.0;
+  for (int e = tid; e < n * N; e += 256) {
+    cplx d = ld(gw, e);
+    if (gref) d = csub(d, ld(gref, e));
+    m = fmax(m, fmax(fabs(d.re), fabs(d.im)));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((tid & 63) == 0) red[tid >> 6] = m;
+  __syncthreads();
+  m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  const double s = fp8_scale(m);
+  uint8_t* pre = a8 + (int64_t)w * ld8;
+  uint8_t* pim = a8 + ((int64_t)nwalkers + w) * ld8;
+  for (int64_t k0 = (int64_t)tid * 4; k0 < ld8; k0 += 4 * 256) {
+    float vr[4], vi[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t k = k0 + j;
+      const int i = (int)(k / Kp), mm = (int)(k % Kp);
+      vr[j] = vi[j] = 0.0f;
+      if (i < n && mm < N) {
+        cplx d = ld(gw, (int64_t)i * N + mm);
+        if (gref) d = csub(d, ld(gref, (int64_t)i * N + mm));
+        vr[j] = (float)(d.re * s);
+        vi[j] = (float)(d.im * s);
+      }
+    }
+    *reinterpret_cast<uint32_t*>(pre + k0) = pack_fp8x4(vr[0], vr[1], vr[2], vr[3]);
+    *reinterpret_cast<uint32_t*>(pim + k0) = pack_fp8x4(vi[0], vi[1], vi[2], vi[3]);
+  }
+  if (tid == 0) row_inv[w] = 1.0 / (s * (double)wscale[0]);
+}
+
 }  // namespace
+
+hipError_t launch_quad_w8(const uint16_t* wplanes, int64_t ldn, int64_t ld8, uint8_t* w8, float* scale,
+                          unsigned int* absmax, hipStream_t stream) {
+  if (ldn < 1 || ld8 < ldn || ld8 % 64 != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(quad_absmax_kernel, dim3(2048), dim3(256), 0, stream, wplanes, ldn * ldn, absmax);
+  hipLaunchKernelGGL(quad_w8_scale_kernel, dim3(1), dim3(64), 0, stream, absmax, scale);
+  const int64_t total = ld8 * (ld8 / 4);
+  hipLaunchKernelGGL(quad_w8_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream, wplanes, ldn, ld8,
+                     w8, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_fp8_rows(const double* g, const double* gref, int nwalkers, int n, int N, int Kp,
+                                 int64_t ld8, uint8_t* a8, const float* wscale, double* row_inv, hipStream_t stream) {
+  if (nwalkers < 1 || n < 1 || Kp < N || ld8 < (int64_t)n * Kp || ld8 % 64 != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(split_fp8_rows_kernel, dim3((unsigned)nwalkers), dim3(256), 0, stream, g, gref, nwalkers, n, N,
+                     Kp, ld8, a8, wscale, row_inv);
+  return hipGetLastError();
+}
 
 hipError_t launch_quad_sum(const double* part, int nslots, int nwalkers, double* out, hipStream_t stream) {
   if (nslots < 1 || nwalkers < 1) return hipErrorInvalidValue;

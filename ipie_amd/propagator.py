@@ -154,7 +154,7 @@ class PhaselessGeneric:
             opts = dict(fraction=0.125, warmup=100, train_iters=4, lr=1e-3, seed=self.seed)
             opts.update(self.surrogate if isinstance(self.surrogate, dict) else {})
         if opts is not None:
-            quadratic = bool(opts.pop("quadratic", True))
+            quadratic = opts.pop("quadratic", "fp8")  # "fp8", "bf16" or "none" (True / False)
             eng.ctx.surrogate_configure(True, **opts)
             eng.ctx.surrogate_quadratic(quadratic)
         tuner = _autotune.as_tuner(self.autotune)

@@ -171,3 +171,26 @@ def test_surrogate_contexts_time_the_exchange_on_the_control_variate(monkeypatch
     assert t.tune(ctx, g, g, surrogate=True) == {"fb_split": 3, "quad_wgs": 3072, "quad_rgroup": 1, "tile_fb": 2}
     assert calls == []  # the exchange shapes were timed through the control variate
     assert autotune.shape_key(ctx, True) in t.cache and autotune.shape_key(ctx) not in t.cache
+
+
+def test_coupled_knobs_are_tuned_jointly(monkeypatch):
+    """The force-bias split and tile shape: split 2 is best with tile 0, but
+    tile 3 needs split 4 (a knob-by-knob search stops at (2, 0))."""
+    torch = pytest.importorskip("torch")
+    t = autotune.Autotuner(cache_path=None)
+    ctx = FakeCtx()
+    _fake_tune(monkeypatch, t, ctx)
+    cost = {(2, 0): 1.0, (4, 0): 1.02, (2, 3): 1.5, (4, 3): 0.9}
+
+    def fb_cost(knob, base=ctx.cost):
+        if knob != "fb_split":
+            return base(knob)
+        return cost.get((ctx.v["fb_split"], ctx.v["tile_fb"]), 2.0)
+
+    ctx.cost = fb_cost
+    g = torch.zeros(32, 4, 40, dtype=torch.complex128)
+    chosen = t.tune(ctx, g, g)
+    assert (chosen["fb_split"], chosen["tile_fb"]) == (4, 3)
+    rep = t.reports[autotune.shape_key(ctx)]
+    assert rep["fb_split"]["ms"]["2"] == 1.5 and rep["tile_fb"]["ms"]["0"] == 1.02  # lines through the choice
+    assert rep["fb_split"]["grid"]["4,3"] == 0.9

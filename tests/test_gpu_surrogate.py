@@ -59,7 +59,7 @@ def test_warmup_steps_are_exact():
     assert p.prop.surrogate_stats()["last_k"] == W
 
 
-@pytest.mark.parametrize("precision,quadratic", [("fp64", True), ("mixed", True), ("mixed", False)])
+@pytest.mark.parametrize("precision,quadratic", [("fp64", True), ("mixed", True), ("mixed", "bf16"), ("mixed", False)])
 def test_estimator_is_unbiased(precision, quadratic):
     p = make(precision=precision, warmup=4, quadratic=quadratic)
     for _ in range(4):  # warm-up (training on all walkers)
@@ -91,41 +91,64 @@ def test_estimator_is_unbiased(precision, quadratic):
 @pytest.mark.timeout(900)
 def test_unbiased_at_surrogate_workload():
     """N=300, X=1800, n=30, W=2048 (the surrogate workload: 16 walker row blocks
-    in the quadratic-form GEMMs, a subset of k = 256 exact walkers): after a
-    short warm-up, R = 48 re-evaluations of one state; every walker's mean
-    estimate within 6 standard errors of its exact exchange, the pooled mean
-    within 5, weights finite."""
+    in the quadratic-form GEMMs, a subset of k = 256 exact walkers), with the
+    physical control variate alone (no MLP, so the predictor g of a walker is the
+    same in every draw): R = 64 re-evaluations of one state after a short
+    warm-up.  Each walker's estimate then takes two values, g when it is not
+    sampled and g + (W / k)(x - g) when it is, so the estimate is unbiased exactly
+    when every sampled value implies the exact x (checked against the exact local
+    energy to the mixed-precision exchange error) and walkers are sampled with
+    frequency k / W (each walker's count of sampling draws against Binomial(R,
+    k / W)); the pooled mean is checked too, weights finite."""
     import torch
 
     from ipie_amd.estimators import local_energy_batch
     from tests._big import device_state
 
     system, ham, trial, walkers, prop = device_state(
-        "n300", precision="mixed", surrogate=dict(warmup=6, fraction=0.125, train_iters=3, lr=2e-3, seed=5))
+        "n300", precision="mixed", surrogate=dict(warmup=6, fraction=0.125, train_iters=0, lr=2e-3, seed=5))
     Wn = walkers.nwalkers
+    k = round(Wn / 8)
     e = torch.empty(Wn, 3, dtype=torch.complex128, device=walkers.device)
     for _ in range(6):
         prop.propagate_walkers(walkers, ham, trial, 0.0, energy_prev=e)
     snap = snapshot(walkers)
     trial.calc_greens_function(walkers)
     ref = local_energy_batch(system, ham, walkers, trial).cpu().numpy()
-    R = 48
+    R = 64
     est = numpy.empty((R, Wn), dtype=numpy.complex128)
     for r in range(R):
         restore(walkers, snap)
         prop.propagate_walkers(walkers, ham, trial, 0.0, energy_prev=e)
         est[r] = e[:, 2].cpu().numpy()
-    assert prop.surrogate_stats()["last_k"] == round(Wn / 8)
+    assert prop.surrogate_stats()["last_k"] == k
     assert bool(torch.isfinite(walkers.weight).all())
-    mean = est.mean(axis=0)
-    sem = est.std(axis=0) / numpy.sqrt(R) + 1e-12 * numpy.abs(ref[:, 2])
-    err = numpy.abs(mean - ref[:, 2])
-    z = err / sem
-    tol_abs = 1e-5 * numpy.abs(ref[:, 2]).max()  # mixed-precision exchange error of the exact samples
-    print(f"max z {z.max():.2f} mean z {z.mean():.2f} stats {prop.surrogate_stats()}", flush=True)
-    assert numpy.all((z < 6.0) | (err < tol_abs))
-    pooled = (mean - ref[:, 2]).real.mean() / (numpy.sqrt((sem**2).sum()) / Wn)
-    assert abs(pooled) < 5.0
+    x = ref[:, 2]
+    scale = numpy.abs(x).max()
+    tol_abs = 1e-5 * scale  # mixed-precision exchange error of the exact samples
+    # g per walker: the median draw (7 of 8 draws leave a walker unsampled)
+    g = est[numpy.argsort(est.real, axis=0)[R // 2], numpy.arange(Wn)]
+    hit = numpy.abs(est - g) > 1e-9 * scale
+    implied = g + (est - g) * (k / Wn)
+    print(f"sampled fraction {hit.mean():.4f}, max |x - g| {numpy.abs(x - g).max():.3e}, "
+          f"max implied error {numpy.abs(implied - x)[hit].max(initial=0.0):.3e}", flush=True)
+    assert numpy.abs(implied - x)[hit].max(initial=0.0) <= tol_abs
+    # walkers whose sampling shows: the sampled value differs from g by far more
+    # than the detection threshold; each one's count of draws that sample it is
+    # Binomial(R, k / W) when the subsets are uniform and independent
+    res = numpy.abs(x - g) * (Wn / k) > 1e-7 * scale
+    p = k / Wn
+    counts = hit[:, res].sum(axis=0)
+    sd = numpy.sqrt(R * p * (1 - p))
+    print(f"{res.sum()} walkers show their sampling: mean count {counts.mean():.3f} (want {R * p:.3f}), "
+          f"sd {counts.std():.3f} (want {sd:.3f})", flush=True)
+    assert res.sum() > Wn // 2
+    assert abs(counts.mean() - R * p) < 5.0 * sd / numpy.sqrt(res.sum())
+    assert 0.8 < counts.std() / sd < 1.2
+    # the pooled mean of the estimates against the binomial spread of the draws
+    dev = (est.mean(axis=0) - x).real
+    sem = numpy.abs(x - g) * (Wn / k) * numpy.sqrt(p * (1 - p) / R)
+    assert abs(dev.mean()) < 5.0 * numpy.sqrt((sem**2).sum()) / Wn + tol_abs
 
 
 def _bf16(x):
@@ -136,17 +159,74 @@ def _bf16(x):
     return u.astype(numpy.uint32).view(numpy.float32).astype(numpy.float64)
 
 
+def _fp8(x):
+    """Round to FP8 E4M3 (OCP: 3 mantissa bits, exponent bias 7, subnormals below
+    2^-6, |x| <= 448 assumed), nearest even, as float64 values."""
+    x = numpy.asarray(x, dtype=numpy.float64)
+    a = numpy.abs(x)
+    e = numpy.floor(numpy.log2(numpy.where(a > 0, a, 1.0)))
+    step = 2.0 ** (numpy.maximum(e, -6.0) - 3.0)
+    return numpy.sign(x) * numpy.round(a / step) * step  # numpy.round: halves to even
+
+
+def _pow2_scale(amax):
+    return 2.0 ** numpy.floor(numpy.log2(224.0 / amax)) if amax > 0 else 1.0
+
+
+def _q_fp8(rchol, G, GT, N, Kp):
+    """Q per walker of one spin as the FP8 control variate forms it: the
+    block-triangular W' (2 Wq left of the diagonal 160-column tiles, Wq on them,
+    0 right of them, in the Kp-padded index i Kp + m) rounded to BF16 hi + lo, then
+    to FP8 after a power-of-two scale; dG = G - G_T rounded to FP32 and FP8 after a
+    per-walker power-of-two scale; Q = sum_b dG_b Y_b with Y = dG8 W8 unscaled."""
+    from tests import _mixed_emu as emu
+
+    n = G.shape[1]
+    Wq = emu.quad_matrix(rchol, n, N)
+    ldn = n * Kp
+    idx = (numpy.arange(n)[:, None] * Kp + numpy.arange(N)[None, :]).ravel()
+    Wp = numpy.zeros((ldn, ldn))
+    Wp[numpy.ix_(idx, idx)] = Wq
+    tile = numpy.arange(ldn) // 160
+    # plane row b, column a: 2 Wq_ab if tile(a) < tile(b), Wq_ab on the diagonal tiles, else 0
+    Wd = numpy.where(tile[None, :] < tile[:, None], 2.0 * Wp, numpy.where(tile[None, :] == tile[:, None], Wp, 0.0))
+    hi = _bf16(Wd)
+    lo = _bf16(Wd - hi)
+    v32 = (hi + lo).astype(numpy.float32).astype(numpy.float64)
+    sw = _pow2_scale(numpy.abs(hi).max())
+    W8 = _fp8(v32 * sw)
+    d = numpy.zeros((G.shape[0], ldn), dtype=numpy.complex128)
+    d[:, idx] = (G - GT).reshape(G.shape[0], -1)
+    out = numpy.empty(G.shape[0], dtype=numpy.complex128)
+    for w in range(G.shape[0]):
+        s = _pow2_scale(max(numpy.abs(d[w].real).max(), numpy.abs(d[w].imag).max()))
+        ar = _fp8((d[w].real * s).astype(numpy.float32).astype(numpy.float64))
+        ai = _fp8((d[w].imag * s).astype(numpy.float32).astype(numpy.float64))
+        y = (W8 @ ar + 1j * (W8 @ ai)) / (s * sw)
+        out[w] = numpy.sum(d[w] * y)
+    return out
+
+
+def test_fp8_rounding_known_values():
+    # the encodings the device's conversion gives (1, 448, 2^-6, 2^-9 and a tie)
+    x = numpy.array([1.0, 448.0, 2.0**-6, 2.0**-9, 1.0 + 2.0**-4, 1.0 + 3 * 2.0**-4, 0.3])
+    want = numpy.array([1.0, 448.0, 2.0**-6, 2.0**-9, 1.0, 1.25, 0.3125])
+    assert numpy.array_equal(_fp8(x), want)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_control_variate_entry_point_predicts_the_exchange_sum(precision):
     """afqmc_control_variate gives S_T + 2 <K, dG> + Q(dG), the exchange sum S of a
     walker (sum_x tr(T_x T_x) over both spins, local_energy_sd.py:263-303)
     expanded about the trial's Green's function, dG = G - G_T.  S is a quadratic
     form in G, so the expansion is exact and Q(dG) is S evaluated at dG; only
-    the single BF16 product of Q rounds.  Checks: every walker within 1% of the
-    largest |Q| of the exact S (a dropped, doubled or sign-flipped term misses
-    by |Q| or more); in FP64 contexts, whose Q forms every T_x from the BF16
-    hi parts of L and dG, within 0.05% of the largest |Q| of the same sum taken
-    from BF16-rounded operands on the CPU."""
+    the single product of Q rounds.  Checks: with Q in BF16, every walker within
+    1% of the largest |Q| of the exact S (a dropped, doubled or sign-flipped term
+    misses by |Q| or more); in FP64 contexts, whose Q forms every T_x from the
+    BF16 hi parts of L and dG, within 0.05% of the largest |Q| of the same sum
+    taken from BF16-rounded operands on the CPU; in mixed contexts with Q in FP8
+    (the default there), within 0.05% of the largest |Q| of the same FP8
+    rounding emulated on the CPU, and within 5% of the largest |Q| of S."""
     import torch
 
     from oracle import afqmc
@@ -158,6 +238,7 @@ def test_control_variate_entry_point_predicts_the_exchange_sum(precision):
     ctx = w._engine.ctx
     ctx.greens_function(w.phia, w.phib, ga, gb, ovlp)
     cv = torch.empty(W, dtype=torch.complex128, device=w.device)
+    ctx.surrogate_quadratic("bf16")
     ctx.control_variate(ga, gb, cv)
     torch.cuda.synchronize()
     got = cv.cpu().numpy()
@@ -182,7 +263,19 @@ def test_control_variate_entry_point_predicts_the_exchange_sum(precision):
     torch.cuda.synchronize()
     lin = cv.cpu().numpy()
     assert numpy.abs(lin - (exact - q)).max() <= 1e-10 * numpy.abs(exact).max()
-    ctx.surrogate_quadratic(True)
+    if precision == "mixed":
+        ctx.surrogate_quadratic("fp8")
+        ctx.control_variate(ga, gb, cv)
+        torch.cuda.synchronize()
+        got8 = cv.cpu().numpy()
+        Kp = ((N + 31) // 32) * 32
+        q8 = _q_fp8(ra, gan, gta[0], N, Kp) + _q_fp8(rb, gbn, gtb[0], N, Kp)
+        emu8 = exact - q + q8
+        print(f"fp8 control variate: max err vs exact {numpy.abs(got8 - exact).max() / qmax:.3e} of max |Q|, "
+              f"vs emulation {numpy.abs(got8 - emu8).max() / qmax:.3e}", flush=True)
+        assert numpy.abs(got8 - emu8).max() <= 5e-4 * qmax
+        assert numpy.abs(got8 - exact).max() <= 5e-2 * qmax
+    ctx.surrogate_quadratic("bf16")
     if precision == "fp64":
         bc = lambda z: _bf16(z.real) + 1j * _bf16(z.imag)  # noqa: E731
         q_bf = 2.0 * (ex(_bf16(ra), bc(gan - gta)) + ex(_bf16(rb), bc(gbn - gtb)))
