@@ -255,6 +255,58 @@ __global__ void __launch_bounds__(256) split_fp8_rows_kernel(const double* __res
   if (tid == 0) row_inv[w] = 1.0 / (s * (double)wscale[0]);
 }
 
+// The same planes with the walker's dG held in registers between the maximum
+// and the conversion (G read once): one walker per workgroup of 1024 threads,
+// J chunks of 4 consecutive padded K positions per thread (ld8 <= 4096 J;
+// Kp a multiple of 4, so a chunk never crosses a row of Kp).
+template <int J>
+__global__ void __launch_bounds__(1024) split_fp8_rows_reg_kernel(const double* __restrict__ g,
+                                                                 const double* __restrict__ gref, int nwalkers,
+                                                                 int n, int N, int Kp, int ld8,
+                                                                 uint8_t* __restrict__ a8,
+                                                                 const float* __restrict__ wscale,
+                                                                 double* __restrict__ row_inv) {
+  __shared__ double red[16];
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const double* gw = g + 2 * (int64_t)w * n * N;
+  cplx v[J][4];
+  double m = 0.0;
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int k0 = 4 * (tid + 1024 * j);
+    const int i = k0 / Kp, m0 = k0 - i * Kp;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      v[j][r] = cplx{0.0, 0.0};
+      if (k0 < ld8 && i < n && m0 + r < N) {
+        cplx d = ld(gw, i * N + m0 + r);
+        if (gref) d = csub(d, ld(gref, i * N + m0 + r));
+        v[j][r] = d;
+        m = fmax(m, fmax(fabs(d.re), fabs(d.im)));
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((tid & 63) == 0) red[tid >> 6] = m;
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 16; ++t) m = fmax(m, red[t]);
+  const double s = fp8_scale(m);
+  uint8_t* pre = a8 + (int64_t)w * ld8;
+  uint8_t* pim = a8 + ((int64_t)nwalkers + w) * ld8;
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int k0 = 4 * (tid + 1024 * j);
+    if (k0 < ld8) {
+      *reinterpret_cast<uint32_t*>(pre + k0) =
+          pack_fp8x4((float)(v[j][0].re * s), (float)(v[j][1].re * s), (float)(v[j][2].re * s), (float)(v[j][3].re * s));
+      *reinterpret_cast<uint32_t*>(pim + k0) =
+          pack_fp8x4((float)(v[j][0].im * s), (float)(v[j][1].im * s), (float)(v[j][2].im * s), (float)(v[j][3].im * s));
+    }
+  }
+  if (tid == 0) row_inv[w] = 1.0 / (s * (double)wscale[0]);
+}
+
 }  // namespace
 
 hipError_t launch_quad_w8(const uint16_t* wplanes, int64_t ldn, int64_t ld8, uint8_t* w8, float* scale,
@@ -271,6 +323,19 @@ hipError_t launch_quad_w8(const uint16_t* wplanes, int64_t ldn, int64_t ld8, uin
 hipError_t launch_split_fp8_rows(const double* g, const double* gref, int nwalkers, int n, int N, int Kp,
                                  int64_t ld8, uint8_t* a8, const float* wscale, double* row_inv, hipStream_t stream) {
   if (nwalkers < 1 || n < 1 || Kp < N || ld8 < (int64_t)n * Kp || ld8 % 64 != 0) return hipErrorInvalidValue;
+  if (Kp % 4 == 0 && ld8 <= 5 * 4096) {
+    const int l = (int)ld8;
+    if (ld8 <= 2 * 4096)
+      hipLaunchKernelGGL(split_fp8_rows_reg_kernel<2>, dim3((unsigned)nwalkers), dim3(1024), 0, stream, g, gref,
+                         nwalkers, n, N, Kp, l, a8, wscale, row_inv);
+    else if (ld8 <= 3 * 4096)
+      hipLaunchKernelGGL(split_fp8_rows_reg_kernel<3>, dim3((unsigned)nwalkers), dim3(1024), 0, stream, g, gref,
+                         nwalkers, n, N, Kp, l, a8, wscale, row_inv);
+    else
+      hipLaunchKernelGGL(split_fp8_rows_reg_kernel<5>, dim3((unsigned)nwalkers), dim3(1024), 0, stream, g, gref,
+                         nwalkers, n, N, Kp, l, a8, wscale, row_inv);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(split_fp8_rows_kernel, dim3((unsigned)nwalkers), dim3(256), 0, stream, g, gref, nwalkers, n, N,
                      Kp, ld8, a8, wscale, row_inv);
   return hipGetLastError();
