@@ -313,8 +313,8 @@ int afqmc_profile_read(afqmc_ctx* ctx, double* ms, int64_t* launches);
  *                        workgroup tile of the split-BF16 force-bias, VHS and
  *                        one-body GEMMs (mixed precision): 0 = 128 x 160,
  *                        1 = 128 x 128, 2 = 128 x 80, 3 = 64 x 160 (three LDS
- *                        stages), 4 = 128 x 320 (two LDS stages; walker rows
- *                        x output columns, 8 waves each)
+ *                        stages), 4 = 128 x 320, 5 = 64 x 160 (two LDS
+ *                        stages; walker rows x output columns, 8 waves each)
  * afqmc_tune_get returns the current value and the admissible range [lo, hi];
  * hi = 0 when the knob does not apply to this context.  afqmc_tune_set
  * changes only launch shapes and summation grouping (results agree to

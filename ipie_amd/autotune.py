@@ -63,7 +63,7 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "ipie_amd", "aut
 _FB = (1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 24, 32)
 _QUAD = (512, 768, 1024, 1536, 2048, 3072, 4096, 6144)
 _EXX = (1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128, 192, 256)
-_TILE = (0, 1, 2, 3, 4)
+_TILE = (0, 1, 2, 3, 4, 5)
 _RGROUP = (1, 2, 4, 8, 16, 32)
 
 

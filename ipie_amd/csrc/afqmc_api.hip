@@ -666,6 +666,7 @@ int one_body_bf16(afqmc_ctx* c, int nw, int s, const uint16_t* aplanes, int64_t 
   b.ksplit = 1;
   b.epilogue = BF16_EPI_ADD;
   b.tile = c->tile_ob;
+  b.cols_fast = 1;  // a few column blocks over the walkers' large A planes
   b.scale = 1.0;
   b.group = n;
   b.a_gstride = a_gstride;

@@ -200,14 +200,19 @@ struct Bf16GemmArgs {
   // count: all row blocks x units
   int rgroup;
   // single product in FP8 (E4M3): the operands are byte arrays addressed in
-  // 16-bit units (a K chunk of BX_KC units holds 64 FP8 values), each 16-byte
-  // fragment feeding two 16x16x32 FP8 MFMAs; QDOT only.  row_scale[row]
+  // 16-bit units (a K chunk of BX_KC units holds 64 FP8 values), the 16-byte
+  // fragments of two chunks feeding one 16x16x128 FP8 MFMA; QDOT only.  row_scale[row]
   // multiplies each row's dot (the inverse of the operands' scales).
   int fp8;
   const double* row_scale;
+  // without a unit table: column blocks vary fastest over the grid, so the
+  // workgroups of one row block run together on one XCD and share its A tile
+  // in L2 (for a large A and a few column blocks, as in the one body);
+  // otherwise row blocks vary fastest and share the B tile
+  int cols_fast;
 };
 hipError_t launch_bf16_gemm(const Bf16GemmArgs& a, hipStream_t stream);
-constexpr int BF16_TILES = 5;
+constexpr int BF16_TILES = 6;
 // rows x columns of a workgroup tile shape (false for an unknown shape)
 bool bf16_tile_dims(int tile, int* rows, int* cols);
 

@@ -50,12 +50,15 @@ struct Tile {
 // The shapes bf16_gemm_kernel is built for (afqmc_tune_set picks one per
 // contraction; the exchange and the quadratic form use shape 0).  All have 8
 // waves; shapes 0-3 three LDS stages of the 3-product form within 160 KiB,
-// shape 4 (64 x 80 per wave: 29% fewer LDS bytes per MFMA than 32 x 80) two.
+// shape 4 (64 x 80 per wave: 29% fewer LDS bytes per MFMA than 32 x 80) two,
+// shape 5 two stages of 36 KiB, so that two workgroups share a CU (one can
+// run its epilogue while the other computes: short K, heavy epilogues).
 using Tile0 = Tile<4, 2, 2, 5>;     // 128 x 160
 using Tile1 = Tile<4, 2, 2, 4>;     // 128 x 128
 using Tile2 = Tile<8, 1, 1, 5>;     // 128 x 80
 using Tile3 = Tile<4, 2, 1, 5>;     // 64 x 160
 using Tile4 = Tile<2, 4, 4, 5, 2>;  // 128 x 320
+using Tile5 = Tile<4, 2, 1, 5, 2>;  // 64 x 160
 
 // 8 waves in a BX_GR x BX_GC grid, each owning BX_MR x BX_MC tiles of 16 x 16 (shape 0)
 constexpr int BX_GR = Tile0::GR, BX_GC = Tile0::GC, BX_MR = Tile0::MR, BX_MC = Tile0::MC;
@@ -664,7 +667,8 @@ __global__ void __launch_bounds__(BX_THREADS) exchange_bf16_kernel(ExchangeBf16A
 // (column tile, K chunk range) = (units[u].x, [units[u].y, units[u].z)) and the
 // QDOT slot is u BX_GC + wc; otherwise column tile cb and split z of ksplit.
 // TR (transposed accumulators, see bx_sub) is used with the ADD and VATOM epilogues only.
-// Workgroups that run together share the B tile (row blocks vary fastest).
+// Workgroups that run together share the B tile (row blocks vary fastest), or
+// with cols_fast the A tile (column blocks vary fastest).
 template <bool SPLIT, bool TR, bool DL, class TL, bool F8 = false>
 __global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) {
   using C = Bx<SPLIT, TL>;
@@ -673,6 +677,11 @@ __global__ void __launch_bounds__(TL::THREADS) bf16_gemm_kernel(Bf16GemmArgs a) 
   const int ncu = a.units ? a.nunits : nb;  // column units of the grid
   const int lin = xcd_remap(blockIdx.x, gridDim.x);
   int rb = lin % mb, cu = (lin / mb) % ncu, z = lin / mb / ncu;
+  if (a.cols_fast && !a.units) {
+    cu = lin % ncu;
+    rb = (lin / ncu) % mb;
+    z = lin / ncu / mb;
+  }
   if (a.units && a.rgroup > 0 && a.rgroup < mb) {
     // groups of rgroup row blocks, each over every unit (the workgroups that run
     // together on one XCD: rgroup row blocks x several units of one K block)
@@ -1528,6 +1537,7 @@ hipError_t launch_bf16_gemm_shape(const Bf16GemmArgs& a, hipStream_t stream) {
     case 2: return launch_bf16_gemm_t<true, TR, Tile2>(a, stream);
     case 3: return launch_bf16_gemm_t<true, TR, Tile3>(a, stream);
     case 4: return launch_bf16_gemm_t<true, TR, Tile4>(a, stream);
+    case 5: return launch_bf16_gemm_t<true, TR, Tile5>(a, stream);
     default: return launch_bf16_gemm_t<true, TR, Tile0>(a, stream);
   }
 }
@@ -1568,6 +1578,7 @@ bool bf16_tile_dims(int tile, int* rows, int* cols) {
     case 2: *rows = Tile2::R; *cols = Tile2::C; return true;
     case 3: *rows = Tile3::R; *cols = Tile3::C; return true;
     case 4: *rows = Tile4::R; *cols = Tile4::C; return true;
+    case 5: *rows = Tile5::R; *cols = Tile5::C; return true;
     default: return false;
   }
 }

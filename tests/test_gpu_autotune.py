@@ -105,7 +105,7 @@ def test_every_tile_shape_gives_the_same_products(case):
     base = run()
     for knob in ("tile_fb", "tile_vhs", "tile_one_body"):
         value, lo, hi = ctx.tune_get(knob)
-        assert (value, lo, hi) == (0, 0, 4)
+        assert (value, lo, hi) == (0, 0, 5)
         for v in range(lo, hi + 1):
             print(f"[case] {case} {knob}={v}", flush=True)
             ctx.tune_set(knob, v)
