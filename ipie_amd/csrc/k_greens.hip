@@ -4,8 +4,9 @@
 //   gf_lu_kernel      P O_s = L U (partial pivoting on |re| + |im|, the first
 //                     index on ties, LAPACK's izamax rule); log|det O_s| =
 //                     sum log|U_kk|, phase = prod U_kk / |U_kk| * (-1)^swaps;
-//                     optionally O_s^{-1}.  One wave per walker and spin,
-//                     lane = row (factorisation) or column (inverse).
+//                     optionally O_s^{-1}, in place (U^{-1}, then inv(O) L =
+//                     U^{-1}, then the pivots' column interchanges: one matrix
+//                     of LDS).  One wave per walker and spin, lane = row.
 //   gf_combine_kernel ovlp = phase_a phase_b exp(logdet_a + logdet_b - log_shift)
 //   gf_ghalf_kernel   Ghalf_s = O_s^{-1} phi_s^T            [n, N]
 //                     (FP64 MFMA, one wave per 16-row tile of Ghalf)
@@ -145,9 +146,9 @@ __device__ __forceinline__ double wave_max(double v) {
 __global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __restrict__ obuf,
                                                   double* __restrict__ oinv, int nmax, double* __restrict__ res) {
   extern __shared__ cplx smem[];
-  const int lr = nmax + 1;  // LDS row stride
-  cplx* LU = smem;             // [nmax][lr]
-  cplx* X = smem + nmax * lr;  // [nmax][lr], columns of the inverse (LDS only when oinv is set)
+  const int lr = nmax + 1;        // LDS row stride
+  cplx* LU = smem;                // [nmax][lr]; with oinv, then inverted in place
+  cplx* work = smem + nmax * lr;  // [nmax], a column of L (oinv only)
   __shared__ int piv[64];
   const int lane = threadIdx.x;
   const int nspin = gf_nspin(a);
@@ -237,31 +238,42 @@ __global__ void __launch_bounds__(64) gf_lu_kernel(GfArgs a, const double* __res
     logdet = lg;
     phase = ph;
     if (oinv) {
-      // column j of O^{-1} solves L U x = P e_j (lane j)
-      const int j = lane;
-      if (j < n) {
-        for (int i = 0; i < n; ++i) X[i * lr + j] = cmake(i == j ? 1.0 : 0.0, 0.0);
-        for (int k = 0; k < n; ++k) {
-          const int p = piv[k];
-          if (p != k) {
-            const cplx t = X[k * lr + j];
-            X[k * lr + j] = X[p * lr + j];
-            X[p * lr + j] = t;
-          }
+      // O^{-1} in place from the factors, one matrix of LDS (lane = row i):
+      // U^{-1} column by column, then inv(O) L = U^{-1} for the columns from
+      // the right (each column of L saved to `work` first), then the column
+      // interchanges of the pivots in reverse.  Every sum runs over k in a fixed
+      // order (lu_dot).
+      const int i = lane;
+      for (int j = 0; j < n; ++j) {
+        // column j of U^{-1}: x_i = sum_{i <= k < j} Uinv[i][k] U[k][j], times -1 / U[j][j]
+        const cplx t = i < j ? lu_dot(LU + i * lr, LU + j, lr, i, j) : cmake(0.0, 0.0);
+        const cplx djj = cdiv(cmake(1.0, 0.0), LU[j * lr + j]);
+        __syncthreads();
+        if (i < j) LU[i * lr + j] = cmul(t, cmake(-djj.re, -djj.im));
+        if (i == j) LU[j * lr + j] = djj;
+        __syncthreads();
+      }
+      for (int j = n - 2; j >= 0; --j) {
+        if (i > j && i < n) {
+          work[i] = LU[i * lr + j];
+          LU[i * lr + j] = cmake(0.0, 0.0);
         }
-        // substitutions: dot products with four independent partial sums, so the
-        // LDS loads of four terms are in flight together
-        for (int i = 0; i < n; ++i) {
-          X[i * lr + j] = csub(X[i * lr + j], lu_dot(LU + i * lr, X + j, lr, 0, i));
-        }
-        for (int i = n - 1; i >= 0; --i) {
-          X[i * lr + j] = cdiv(csub(X[i * lr + j], lu_dot(LU + i * lr, X + j, lr, i + 1, n)), LU[i * lr + i]);
+        __syncthreads();
+        if (i < n) LU[i * lr + j] = csub(LU[i * lr + j], lu_dot(LU + i * lr, work, 1, j + 1, n));
+        __syncthreads();
+      }
+      for (int j = n - 2; j >= 0; --j) {
+        const int p = piv[j];
+        if (p != j && i < n) {
+          const cplx t = LU[i * lr + j];
+          LU[i * lr + j] = LU[i * lr + p];
+          LU[i * lr + p] = t;
         }
       }
       __syncthreads();
       double* oi = oinv + 2 * ((int64_t)w * 2 + s) * nmax * nmax;
-      for (int i = 0; i < n; ++i)
-        if (lane < n) st(oi, (int64_t)i * nmax + lane, X[i * lr + lane]);
+      for (int r = 0; r < n; ++r)
+        if (lane < n) st(oi, (int64_t)r * nmax + lane, LU[r * lr + lane]);
     }
     __syncthreads();
   }
@@ -405,7 +417,7 @@ hipError_t launch_imag_any(const double* z, int64_t n, int* flag, hipStream_t st
 
 size_t gf_lds_bytes(int nalpha, int nbeta) {
   const int nmax = nalpha > nbeta ? nalpha : nbeta;
-  return sizeof(cplx) * 2 * (size_t)nmax * (nmax + 1);
+  return sizeof(cplx) * ((size_t)nmax * (nmax + 1) + nmax);
 }
 
 // O matrices [W, 2, nmax, nmax] complex, then the per-(walker, spin) LU results
@@ -440,8 +452,8 @@ hipError_t launch_gf(const GfArgs& a, bool want_ghalf, double* obuf, double* oin
 #undef AFQMC_OBUILD
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  // the inverse's columns need a second matrix of LDS; the overlap alone does not
-  const size_t lds = gf_lds_bytes(a.nalpha, a.nbeta) / (want_ghalf ? 1 : 2);
+  // the inverse needs a column of L besides the factors; the overlap alone does not
+  const size_t lds = gf_lds_bytes(a.nalpha, a.nbeta) - (want_ghalf ? 0 : sizeof(cplx) * (size_t)nmax);
   e = allow_lds((const void*)gf_lu_kernel, lds);
   if (e != hipSuccess) return e;
   double* res = obuf + 2 * (int64_t)a.nwalkers * 2 * nmax * nmax;
