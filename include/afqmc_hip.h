@@ -348,7 +348,7 @@ int afqmc_tune_set(afqmc_ctx* ctx, int32_t knob, int32_t value);
  * all exact and train the model (train_iters Adam steps each, learning rate
  * lr); afterwards it keeps learning from the sampled walkers.  train_iters = 0
  * turns the MLP term off (gain held at 0, no training): the predictor is then
- * the physical control variate E_T + 2 <K, G - G_T> + Q_bf16 alone.  enable = 0
+ * the physical control variate E_T + 2 <K, G - G_T> + Q alone.  enable = 0
  * restores the plain exact estimator.  afqmc_local_energy is always exact. */
 int afqmc_surrogate_configure(afqmc_ctx* ctx, int enable, double fraction, int warmup_steps, int train_iters,
                               double lr, uint64_t seed);

@@ -13,7 +13,8 @@ Eight launch shapes of the step depend on the system's dimensions:
   tile_vhs       workgroup tile of the split-BF16 VHS GEMM
   tile_one_body  workgroup tile of the split-BF16 one-body GEMM
                  (tiles: 0 = 128 x 160, 1 = 128 x 128, 2 = 128 x 80,
-                 3 = 64 x 160, 4 = 128 x 320 walker rows x output columns)
+                 3 = 64 x 160, 4 = 128 x 320, 5 = 64 x 160 with two LDS
+                 stages; walker rows x output columns)
   taylor         kernel of the split-BF16 Taylor series: 32-wide K stages, two
                  in LDS (0); 16-wide stages, four in LDS (1)
 
@@ -27,7 +28,8 @@ form of every walker) or act on scratch copies of the walkers
 (``afqmc_two_body`` for tile_vhs and taylor, ``afqmc_one_body`` for
 tile_one_body), and keeps the fastest unless the heuristic is within
 ``min_gain`` of it.  Knobs that shape one launch (fb_split with tile_fb,
-quad_wgs with quad_rgroup) are timed over every pair of candidates.  The choices are cached per device and shape: a multi-system batch whose
+quad_wgs with quad_rgroup) are timed over every pair of candidates.  The
+choices are cached per device and shape: a multi-system batch whose
 systems share a shape tunes that shape once, and a JSON cache file carries the
 choices to the next run.  The knobs change launch shapes and the grouping of
 partial sums only, so results agree with the untuned context to rounding.
