@@ -363,10 +363,12 @@ __global__ void __launch_bounds__(64 * GF_WAVES) gf_ghalf_kernel(GfArgs a, const
       v4d re = zero4d(), im = zero4d();
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
-        re = mfma64(af[ks].re, bv[u][ks].re, re);
-        re = mfma64(-af[ks].im, bv[u][ks].im, re);
-        im = mfma64(af[ks].re, bv[u][ks].im, im);
-        im = mfma64(af[ks].im, bv[u][ks].re, im);
+        if (ks * 4 < n) {  // K steps past n have zero operands (n = 40 takes 10 of 12); a branch keeps the unroll
+          re = mfma64(af[ks].re, bv[u][ks].re, re);
+          re = mfma64(-af[ks].im, bv[u][ks].im, re);
+          im = mfma64(af[ks].re, bv[u][ks].im, im);
+          im = mfma64(af[ks].im, bv[u][ks].re, im);
+        }
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
