@@ -92,3 +92,49 @@ def test_fp64_workload_as_specified_generator():
         _check_step(p, e_dev, e_orc, step, TOL, TOL)
         nrm = numpy.abs(p.rows("phia")).max()
         print(f"step {step}: max |phi| {nrm:.3e}", flush=True)
+
+
+@pytest.mark.timeout(900)
+def test_headline_shape_control_variate():
+    """The control variate's physical part at the headline shape (104 column tiles
+    of the block-triangular W' per spin, 1024-walker grids, the FP8 split with
+    each walker's dG in registers): afqmc_control_variate gives S_T + 2 <K, dG> +
+    Q(dG) for every walker, S the exchange sum (local_energy_sd.py:263-303).  On
+    the oracle's subset: with Q in BF16 within 1% of the largest |Q| of S, in FP8
+    within 5% (three mantissa bits), and without Q S - Q to 1e-10 of max |S| (the
+    expansion about G_T is exact, so the FP64 terms carry no rounding of Q)."""
+    import torch
+
+    from oracle import afqmc
+
+    p = SubsetPair("n400", precision="mixed", surrogate=dict(warmup=1, fraction=0.125, train_iters=0, lr=1e-3, seed=2))
+    w = p.walkers
+    ctx = w._engine.ctx
+    ga, gb = torch.empty_like(w.Ghalfa), torch.empty_like(w.Ghalfb)
+    ovlp = torch.empty_like(w.ovlp)
+    ctx.greens_function(w.phia, w.phib, ga, gb, ovlp)
+    cv = torch.empty(p.W, dtype=torch.complex128, device=w.device)
+    idx = torch.tensor(p.subset, device=w.device)
+    got = {}
+    for mode in ("bf16", "fp8", "none"):
+        ctx.surrogate_quadratic(mode)
+        ctx.control_variate(ga, gb, cv)
+        torch.cuda.synchronize()
+        assert bool(torch.isfinite(cv).all()), mode
+        got[mode] = cv.index_select(0, idx).cpu().numpy()
+    ctx.surrogate_quadratic("fp8")
+    gan = ga.index_select(0, idx).cpu().numpy()
+    gbn = gb.index_select(0, idx).cpu().numpy()
+    ex = afqmc.exx_kernel_batch_real_rchol  # 1/2 sum_x tr(T_x T_x) per walker
+    ra, rb = p.orc.trial._rchola, p.orc.trial._rcholb
+    psa, psb = p.orc.trial.psi0a, p.orc.trial.psi0b
+    gta = numpy.linalg.solve(psa.T @ psa.conj(), psa.T)[None]
+    gtb = numpy.linalg.solve(psb.T @ psb.conj(), psb.T)[None]
+    exact = 2.0 * (ex(ra, gan) + ex(rb, gbn))
+    q = 2.0 * (ex(ra, gan - gta) + ex(rb, gbn - gtb))
+    qmax = numpy.abs(q).max()
+    for mode, bound in (("bf16", 1e-2), ("fp8", 5e-2)):
+        err = numpy.abs(got[mode] - exact).max() / qmax
+        print(f"headline control variate {mode}: max err {err:.3e} of max |Q| = {qmax:.3e}", flush=True)
+        assert err <= bound, (mode, err)
+    assert numpy.abs(got["none"] - (exact - q)).max() <= 1e-10 * numpy.abs(exact).max()
