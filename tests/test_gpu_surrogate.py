@@ -281,3 +281,39 @@ def test_control_variate_entry_point_predicts_the_exchange_sum(precision):
         q_bf = 2.0 * (ex(_bf16(ra), bc(gan - gta)) + ex(_bf16(rb), bc(gbn - gtb)))
         emu = exact - q + q_bf
         assert numpy.abs(got - emu).max() <= 5e-4 * qmax, numpy.abs(got - emu).max() / qmax
+
+
+def test_fp8_split_of_long_walker_rows():
+    """The FP8 split of dG keeps a walker's values in registers up to 20480
+    padded values (n Kp); longer rows (here n = 64, Kp = 352: 22528) take the
+    two-pass split.  Its control variate agrees with the BF16 one to the FP8
+    rounding of Q (5% of the largest |Q|; the BF16 form is pinned against the
+    exact exchange sum in the tests above)."""
+    import torch
+
+    Nl, Xl, nl, Wl = 336, 400, 64, 128
+    print(f"[case] long-row FP8 split N={Nl} X={Xl} n={nl} W={Wl}", flush=True)
+    h1e, chol, ecore, psi = synthetic_arrays(Nl, Xl, 0.75 * Nl, nl, nl, scale=(12.0 / (Nl * 0.75 * Nl)) ** 0.5)
+    init = random_walkers(Nl, nl, nl, Wl, seed=4, amp=0.1, base=psi)
+    p = Pair(h1e, chol, psi, (nl, nl), Wl, init=init, precision="mixed",
+             surrogate=dict(warmup=1, fraction=0.125, train_iters=0, lr=1e-3, seed=5))
+    w = p.walkers
+    ctx = w._engine.ctx
+    assert ctx.exchange_form() == "quad"
+    ga, gb = torch.empty_like(w.Ghalfa), torch.empty_like(w.Ghalfb)
+    ovlp = torch.empty_like(w.ovlp)
+    ctx.greens_function(w.phia, w.phib, ga, gb, ovlp)
+    cv = torch.empty(Wl, dtype=torch.complex128, device=w.device)
+    got = {}
+    for mode in ("bf16", "none", "fp8"):
+        ctx.surrogate_quadratic(mode)
+        ctx.control_variate(ga, gb, cv)
+        torch.cuda.synchronize()
+        got[mode] = cv.cpu().numpy()
+    q16 = got["bf16"] - got["none"]
+    q8 = got["fp8"] - got["none"]
+    qmax = numpy.abs(q16).max()
+    err = numpy.abs(q8 - q16).max() / qmax
+    print(f"long rows: fp8 vs bf16 Q max err {err:.3e} of max |Q| = {qmax:.3e}", flush=True)
+    assert numpy.all(numpy.isfinite(q8)) and qmax > 0.0
+    assert err <= 5e-2
