@@ -256,7 +256,11 @@ int refuse_in_capture(afqmc_ctx* c, const char* what) {
 // Column units of the block-triangular quadratic-form GEMM with mb row blocks
 // and K chunks of cw values: column tile t sums the chunks
 // [0, min(ceil(QUAD_TILE (t + 1) / cw), kchunks)), cut at the multiples of T
-// chunks (T chosen for about target_wgs workgroups, at least 1024 / cw), and
+// chunks (T chosen for about target_wgs workgroups, at least 1024 / cw, and
+// at least 2048 / cw for a single row block that still makes 256 units or
+// more: the control variate's exact sample at n400 ran 10% faster in 64-chunk
+// units than in 32, while small systems need the shorter units to fill the
+// GPU), and
 // the units are listed by K block, then tile (chunks past a tile's own rows
 // cover zeros of the block-triangular W').  The grid takes the row blocks
 // fastest, so the workgroups that run together on one XCD (consecutive logical
@@ -267,7 +271,8 @@ std::vector<int4> quad_units(int64_t nb, int kchunks, int64_t mb, int target_wgs
   int64_t total = 0;
   for (int64_t t = 0; t < nb; ++t) total += extent(t);
   const int64_t target = std::max<int64_t>(ceil_div(target_wgs, mb), 1);
-  const int64_t T = std::max<int64_t>(ceil_div(total, target), 1024 / cw);
+  int64_t T = std::max<int64_t>(ceil_div(total, target), 1024 / cw);
+  if (mb == 1 && ceil_div(total, 2048 / cw) >= 256) T = std::max<int64_t>(T, 2048 / cw);
   std::vector<int4> u;
   for (int64_t z = 0; z * T < kchunks; ++z)
     for (int64_t t = 0; t < nb; ++t) {
