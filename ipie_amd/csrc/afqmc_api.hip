@@ -1196,19 +1196,9 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   const bool warm = S.steps < S.warmup;
   if (c->capturing && (warm || S.steps == 0))
     return fail(c, AFQMC_ERR_INVALID_ARG, "the control variate's warm-up steps run eagerly, before a capture");
-  // one-body energy for the features (the energy output is rewritten below)
-  int r = do_energy(c, nw, ga, gb, energy, S.pred, S.e1);
-  if (r) return r;
-  SurrogateFeatureArgs fa{nw, S.e1, wv(c, WV_ECOUL), S.xbar, S.xbar + c->d.nwalkers, ovlp, S.feat};
-  HIP_TRY(c, launch_sur_features(fa, c->stream));
-  if (S.steps > 0) {
-    SurrogateForwardArgs f{nw, S.feat, S.wbf, S.bias, S.state, S.pred};
-    HIP_TRY(c, launch_sur_forward(f, c->stream));
-  } else {
-    HIP_TRY(c, hipMemsetAsync(S.pred, 0, 16 * (size_t)nw, c->stream));
-  }
-  // physical part of the predictor: E_T + 2 <K, dG> (+ Q_bf16(dG)), added to pred
-  r = sur_prepare_cv(c);
+  // physical part of the predictor: E_T + 2 <K, dG> (+ Q(dG)), with the one-body
+  // energy for the features from the same pass over Ghalf
+  int r = sur_prepare_cv(c);
   if (r) return r;
   r = ensure_planes(c);
   if (r) return r;
@@ -1236,7 +1226,26 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   cva.pred = S.pred;
   cva.mlp = S.mlp;
   cva.state = S.state;
-  HIP_TRY(c, launch_sur_cv(cva, c->stream));
+  cva.fold = 0;
+  cva.rH1[0] = c->s.rH1a;
+  cva.rH1[1] = c->s.rH1b;
+  cva.e1 = S.e1;
+  cva.e1_factor = c->d.rhf ? 2.0 : 1.0;
+  cva.ecore = c->s.ecore;
+  {
+    PhaseScope ps(c, AFQMC_PHASE_ENERGY);
+    HIP_TRY(c, launch_sur_cv(cva, c->stream));
+  }
+  // the MLP term from the features, folded into the predictor
+  SurrogateFeatureArgs fa{nw, S.e1, wv(c, WV_ECOUL), S.xbar, S.xbar + c->d.nwalkers, ovlp, S.feat};
+  HIP_TRY(c, launch_sur_features(fa, c->stream));
+  if (S.steps > 0) {
+    SurrogateForwardArgs f{nw, S.feat, S.wbf, S.bias, S.state, S.pred};
+    HIP_TRY(c, launch_sur_forward(f, c->stream));
+  } else {
+    HIP_TRY(c, hipMemsetAsync(S.pred, 0, 16 * (size_t)nw, c->stream));
+  }
+  HIP_TRY(c, launch_sur_fold(nw, S.cv, S.pred, S.mlp, S.state, c->stream));
   // exact exchange on the sampled subset (permutation drawn on the device)
   int k = nw;
   if (!warm) {
@@ -1280,7 +1289,7 @@ int do_energy_surrogate(afqmc_ctx* c, int nw, const double* ga, const double* gb
   if (iters > 0) HIP_TRY(c, launch_step_advance(c->dstate, 0, iters, c->stream));
   S.steps += 1;
   if (c->capturing) c->capture_sur += 1;
-  // the one-body energy of the first pass (S.e1) is reused: Ghalf is not read again
+  // the one-body energy of the control-variate pass (S.e1) is reused: Ghalf is not read again
   return do_energy(c, nw, ga, gb, energy, S.est, nullptr, S.e1);
 }
 
@@ -1958,6 +1967,7 @@ int afqmc_control_variate(afqmc_ctx* c, int32_t nw, const double* ga, const doub
   cva.pred = S.pred;
   cva.mlp = S.mlp;
   cva.state = S.state;
+  cva.fold = 1;
   HIP_TRY(c, launch_sur_cv(cva, c->stream));
   return AFQMC_OK;
 }

@@ -451,7 +451,18 @@ struct SurrogateCvArgs {
   double* pred;            // [W] complex: MLP output in, cv + gain * MLP out
   double* mlp;             // [W] complex copy of the MLP output
   const SurrogateState* state;
+  // fold = 0: cv only, pred and mlp untouched (launch_sur_fold folds the MLP
+  // term in once it is known)
+  int fold;
+  // optional, in the same pass over Ghalf: e1[w] = e1_factor sum_s <rH1_s, Ghalf_s> + ecore
+  // (the one-body energy, as energy_kernel forms it)
+  const double* rH1[2];
+  double* e1;
+  double e1_factor, ecore;
 };
+// pred = cv + gain * pred, mlp = the incoming pred (the MLP output), per walker
+hipError_t launch_sur_fold(int nwalkers, const double* cv, double* pred, double* mlp, const SurrogateState* state,
+                           hipStream_t s);
 struct SurrogateGainArgs {
   int k;
   const int* walker_of;    // [k]

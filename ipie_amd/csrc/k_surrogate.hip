@@ -460,26 +460,50 @@ __global__ void __launch_bounds__(256) sur_cv_kernel(SurrogateCvArgs a) {
   __shared__ double red[256 / 64];
   const int w = blockIdx.x;
   cplx cv = cmake(0.0, 0.0);
+  double hr = 0.0, hi = 0.0;  // <rH1, Ghalf> over the spins
   for (int s = 0; s < a.nspin; ++s) {
     const int64_t len = (int64_t)a.nocc[s] * a.nbasis;
     const double* g = a.g[s] + 2 * (int64_t)w * len;
     double re = 0.0, im = 0.0;
     for (int64_t e = threadIdx.x; e < len; e += blockDim.x) {
-      const cplx k = ld(a.K[s], e), d = csub(ld(g, e), ld(a.gT[s], e));
+      const cplx gv = ld(g, e);
+      const cplx k = ld(a.K[s], e), d = csub(gv, ld(a.gT[s], e));
       re += k.re * d.re - k.im * d.im;
       im += k.re * d.im + k.im * d.re;
+      if (a.e1) {
+        const cplx h = ld(a.rH1[s], e);
+        hr += gv.re * h.re - gv.im * h.im;
+        hi += gv.re * h.im + gv.im * h.re;
+      }
     }
     re = block_sum(re, red);
     im = block_sum(im, red);
     cv = cadd(cv, cadd(ld(a.et, s), cmake(2.0 * re, 2.0 * im)));
     for (int z = 0; z < a.xs[s]; ++z) cv = cadd(cv, ld(a.part[s], (int64_t)z * a.nwalkers + w));
   }
-  if (threadIdx.x == 0) {
-    const cplx m = ld(a.pred, w);
-    st(a.cv, w, cv);
-    st(a.mlp, w, m);
-    st(a.pred, w, cadd(cv, cscale(m, a.state->gain)));
+  if (a.e1) {
+    hr = block_sum(hr, red);
+    hi = block_sum(hi, red);
   }
+  if (threadIdx.x == 0) {
+    st(a.cv, w, cv);
+    if (a.e1) st(a.e1, w, cmake(a.e1_factor * hr + a.ecore, a.e1_factor * hi));
+    if (a.fold) {
+      const cplx m = ld(a.pred, w);
+      st(a.mlp, w, m);
+      st(a.pred, w, cadd(cv, cscale(m, a.state->gain)));
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) sur_fold_kernel(int nwalkers, const double* __restrict__ cv,
+                                                       double* __restrict__ pred, double* __restrict__ mlp,
+                                                       const SurrogateState* __restrict__ state) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwalkers) return;
+  const cplx m = ld(pred, w);
+  st(mlp, w, m);
+  st(pred, w, cadd(ld(cv, w), cscale(m, state->gain)));
 }
 
 // Least-squares gain of the MLP term from this step's exact samples, for the
@@ -578,7 +602,16 @@ hipError_t launch_cv_static(const double* L, const double* gT, int X, int n, int
 }
 
 hipError_t launch_sur_cv(const SurrogateCvArgs& a, hipStream_t s) {
+  if (a.e1 && (a.rH1[0] == nullptr || (a.nspin > 1 && a.rH1[1] == nullptr))) return hipErrorInvalidValue;
   hipLaunchKernelGGL(sur_cv_kernel, dim3(a.nwalkers), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_sur_fold(int nwalkers, const double* cv, double* pred, double* mlp, const SurrogateState* state,
+                           hipStream_t s) {
+  if (nwalkers < 1) return hipSuccess;
+  hipLaunchKernelGGL(sur_fold_kernel, dim3((unsigned)ceil_div(nwalkers, 256)), dim3(256), 0, s, nwalkers, cv, pred,
+                     mlp, state);
   return hipGetLastError();
 }
 
