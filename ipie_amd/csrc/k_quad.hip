@@ -239,8 +239,9 @@ __global__ void __launch_bounds__(256) split_fp8_rows_kernel(const double* __res
     float vr[4], vi[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int64_t k = k0 + j;
-      const int i = (int)(k / Kp), mm = (int)(k % Kp);
+      // a walker row holds n Kp < 2^31 values: 32-bit row and column
+      const int k = (int)(k0 + j);
+      const int i = k / Kp, mm = k - i * Kp;
       vr[j] = vi[j] = 0.0f;
       if (i < n && mm < N) {
         cplx d = ld(gw, (int64_t)i * N + mm);
@@ -322,7 +323,8 @@ hipError_t launch_quad_w8(const uint16_t* wplanes, int64_t ldn, int64_t ld8, uin
 
 hipError_t launch_split_fp8_rows(const double* g, const double* gref, int nwalkers, int n, int N, int Kp,
                                  int64_t ld8, uint8_t* a8, const float* wscale, double* row_inv, hipStream_t stream) {
-  if (nwalkers < 1 || n < 1 || Kp < N || ld8 < (int64_t)n * Kp || ld8 % 64 != 0) return hipErrorInvalidValue;
+  if (nwalkers < 1 || n < 1 || Kp < N || ld8 < (int64_t)n * Kp || ld8 % 64 != 0 || ld8 > INT32_MAX)
+    return hipErrorInvalidValue;
   if (Kp % 4 == 0 && ld8 <= 5 * 4096) {
     const int l = (int)ld8;
     if (ld8 <= 2 * 4096)
